@@ -1,0 +1,179 @@
+/*
+ * shmap_hip -- C ABI of the MI355X (gfx950) per-read mapping library.
+ *
+ * The entry points mirror shmap-rs's library surface for the mapping path:
+ * the FracMinHash sketcher, the reference SketchIndex and the Mapper plugin
+ * (reference paths are relative to the shmap-rs crate root).  All calls
+ * return 0 on success and a nonzero shm_status otherwise; shm_last_error()
+ * returns a thread-local message for the last failure.  Callers own every
+ * host buffer they pass; handles own their device memory.
+ */
+#ifndef SHMAP_HIP_H
+#define SHMAP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum shm_status {
+    SHM_OK = 0,
+    SHM_ERR_ARG = 1,     /* invalid argument (the reference's Params::validate) */
+    SHM_ERR_HIP = 2,     /* HIP runtime failure */
+    SHM_ERR_NOMEM = 3,   /* device or host allocation failed */
+    SHM_ERR_CAPACITY = 4 /* caller buffer too small; the needed size is returned */
+};
+
+/* Metric -- src/types.rs:300-322 */
+enum shm_metric { SHM_CONTAINMENT = 0, SHM_JACCARD = 1, SHM_BUCKET_SH = 2, SHM_BUCKET_LCS = 3 };
+
+/* Kmer{r, h, strand} -- src/types.rs:24-28.  strand: 0 forward, 1 reverse. */
+typedef struct {
+    uint64_t h;
+    int32_t r;
+    uint8_t strand;
+    uint8_t pad[3];
+} shm_kmer;
+
+/* Hit{r, tpos, strand, segm_id} -- src/types.rs:44-51 */
+typedef struct {
+    int32_t r;
+    int32_t tpos;
+    int32_t strand;
+    int32_t segm_id;
+} shm_hit;
+
+/* One reference segment (FASTA record) handed to the index build. */
+typedef struct {
+    const char *name;
+    const uint8_t *seq;
+    int64_t len;
+} shm_segment;
+
+/* One read handed to the mapper (host memory). */
+typedef struct {
+    const uint8_t *seq;
+    int32_t len;
+} shm_read;
+
+/* The parameters map_read consumes -- src/params.rs:14-186.  max_matches is
+ * the index's -M and is given to shm_index_build instead. */
+typedef struct {
+    int32_t k;
+    double h_frac;
+    double theta;            /* -t */
+    double min_diff;         /* -d */
+    double max_overlap;      /* -o */
+    int32_t metric;          /* -m, enum shm_metric */
+    int32_t no_bucket_pruning; /* -P */
+    int32_t one_sweep;       /* -B */
+    int32_t abs_pos;         /* -F */
+} shm_map_params;
+
+/* Everything map_read produces for one read (src/shmap/mod.rs:653-971): the
+ * fields of the PAF record (src/mapping.rs:12-247) and the per-read counters
+ * printed in it or in --per-read-stats (src/shmap/mod.rs:280-327). */
+typedef struct {
+    int32_t read_len;
+    int32_t m;                /* sketch size, p:i: */
+    int32_t u;                /* distinct k-mers */
+    int32_t S;                /* seeds:i: */
+    int32_t mapped;
+    int32_t unmappable_short; /* sketch below MIN_HALFLEN */
+    int32_t possible_matches;
+    int32_t seed_matches;
+    int32_t total_matches;
+    int32_t max_seed_matches;
+    int32_t seeded_buckets;
+    int32_t refined_buckets;
+    int32_t refine_memo_hits;
+    int32_t final_buckets;
+    int32_t p_start, p_end;
+    int32_t strand;           /* '+' or '-' */
+    int32_t segm_id;
+    int32_t t_l, t_r;
+    int32_t mapq;
+    int32_t s_sz;
+    int32_t intersection, intersection2;
+    int32_t same_strand;
+    int32_t b_segm, b_b, b2_segm, b2_b;
+    int32_t status;           /* 0 = ok; nonzero = this read failed alone */
+    int32_t has_second;       /* a second-best mapping was found */
+    double J, J2, sh, sigmas, match_inefficiency;
+} shm_read_result;
+
+typedef struct shm_sketcher shm_sketcher;
+typedef struct shm_index shm_index;
+typedef struct shm_mapper shm_mapper;
+
+const char *shm_last_error(void);
+
+/* FracMinHash::new -- src/sketch.rs:58-109 */
+int shm_sketcher_create(int32_t k, double h_frac, shm_sketcher **out);
+void shm_sketcher_free(shm_sketcher *sk);
+
+/* FracMinHash::sketch_slice_into -- src/sketch.rs:167-209 (sketch/sketch_into
+ * are offset 0).  Runs on the GPU.  *n receives the number of k-mers; when it
+ * exceeds cap nothing is written and SHM_ERR_CAPACITY is returned. */
+int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t len, int32_t offset,
+               shm_kmer *out, int64_t cap, int64_t *n);
+
+/* SketchIndex::build_index -- src/index.rs:384-663.  Builds the whole index on
+ * the GPU and keeps it resident.  max_matches <= 0 means None (-M unset). */
+int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, int32_t n_segments,
+                    int32_t max_matches, int32_t device_id, shm_index **out);
+void shm_index_free(shm_index *idx);
+
+/* SketchIndex::count -- src/index.rs:204-213 */
+int shm_index_count(const shm_index *idx, uint64_t h, int32_t *count);
+/* SketchIndex::single_hit / multi_hits -- src/index.rs:192-201: every hit of h
+ * sorted by (segm_id, r).  *n receives the count; SHM_ERR_CAPACITY if > cap. */
+int shm_index_hits(const shm_index *idx, uint64_t h, shm_hit *out, int32_t cap, int32_t *n);
+/* SketchIndex::get_segment -- src/index.rs:184-186 (name, sz, kmers.len()) */
+int shm_index_n_segments(const shm_index *idx, int32_t *n);
+int shm_index_segment(const shm_index *idx, int32_t segm_id, const char **name, int32_t *sz,
+                      int64_t *n_kmers);
+/* get_kmer_stats counters -- src/index.rs:343-376 */
+int shm_index_stats(const shm_index *idx, int64_t *indexed_kmers, int64_t *indexed_hits,
+                    int64_t *highest_freq, int64_t *blacklisted_kmers, int64_t *blacklisted_hits);
+
+/* create_mapper -- src/mapper.rs:25-39: a mapper over a built index, holding
+ * the device scratch for batches of reads. */
+int shm_mapper_create(const shm_index *idx, const shm_map_params *params, shm_mapper **out);
+void shm_mapper_free(shm_mapper *mp);
+
+/* Mapper::map_reads for a batch of host reads -- src/shmap/mod.rs:374-650,
+ * with map_read (:653-971) for each.  out[i] is read i's result, in input
+ * order.  A read that fails is reported in out[i].status alone. */
+int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out);
+
+/* The same over reads already resident in device memory: bases d_seq with
+ * n+1 offsets d_off (device pointers), results copied to host out. */
+int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,
+                         shm_read_result *out);
+
+/* Per-read record formatting shared by every caller: the PAF line of a mapped
+ * read (src/mapping.rs:87-107, 161-182, 228-247; no newline) with map_time as
+ * t:f:, or the unmapped line (src/mapping.rs:82-84).  Returns the length
+ * written (snprintf semantics). */
+int shm_format_paf(const shm_index *idx, const shm_map_params *params, const char *query_id,
+                   const shm_read_result *r, double map_time, char *buf, size_t cap);
+int shm_format_unmapped(const char *query_id, int32_t read_len, char *buf, size_t cap);
+
+/* Device kernel timing of the last batch on this mapper, in milliseconds:
+ * total GPU time and the time of each stage (HIP events on the mapper's own
+ * stream).  stage_ms receives up to n_stages values; names via
+ * shm_mapper_stage_name. */
+int shm_mapper_last_timing(const shm_mapper *mp, double *total_ms, double *stage_ms,
+                           int32_t n_stages, int32_t *n_written);
+const char *shm_mapper_stage_name(int32_t stage);
+/* Turns per-stage event timing on or off (off by default: it adds a stream
+ * synchronisation per stage group). */
+int shm_mapper_set_timing(shm_mapper *mp, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

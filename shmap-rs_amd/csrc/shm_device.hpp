@@ -1,0 +1,215 @@
+// Device-side building blocks for the gfx950 mapping path: the resident index
+// view, ntHash rolling, wave/workgroup scans and a workgroup bitonic sort.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shm {
+
+constexpr int WG = 256;      // threads per workgroup (4 wave64s)
+constexpr int WAVE = 64;
+constexpr int NWAVES = WG / WAVE;
+constexpr uint32_t EMPTY_KEY = 0xffffffffu;
+
+// Resident reference index (built once on the device).
+//   t_h/t_r/t_s   reference sketch of every segment, concatenated; segment s
+//                 owns global k-mer ids [kstart[s], kstart[s+1])
+//   hit_*         every indexed k-mer occurrence grouped by hash, each group in
+//                 (segm_id, r) order: global k-mer id, r | strand << 31, segment
+//   strand_pre    exclusive prefix count of reverse-strand hits over hit_*
+//   table         8 shards (by h & 7) of open-addressing slots
+//                 {key lo, key hi, hit offset, count}; count 0 = empty slot
+struct DevIndex {
+    const uint64_t *t_h;
+    const int32_t *t_r;
+    const uint8_t *t_s;
+    const int64_t *kstart;
+    int32_t nseg;
+    int64_t n_kmers;
+    const uint32_t *hit_g;
+    const uint32_t *hit_r;
+    const uint32_t *hit_seg;
+    const uint32_t *strand_pre;
+    const uint4 *table;
+    uint64_t shard_base[8];
+    uint32_t shard_bits[8];
+};
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t n) {
+    n &= 63u;
+    return n ? (x << n) | (x >> (64u - n)) : x;
+}
+__device__ __forceinline__ uint64_t rotr64(uint64_t x, uint32_t n) {
+    n &= 63u;
+    return n ? (x >> n) | (x << (64u - n)) : x;
+}
+
+// ntHash seed of a base, forward strand; 0 for anything but ACGTacgt.
+__device__ __forceinline__ uint64_t nt_fw(uint8_t c) {
+    const uint8_t l = c | 0x20;
+    return l == 'a' ? 0x3c8bfbb395c60474ull
+         : l == 'c' ? 0x3193c18562a02b4cull
+         : l == 'g' ? 0x20323ed082572324ull
+         : l == 't' ? 0x295549f54be24456ull : 0ull;
+}
+// Reverse-complement strand seed: the forward seed of the complement.
+__device__ __forceinline__ uint64_t nt_rc(uint8_t c) {
+    const uint8_t l = c | 0x20;
+    return l == 'a' ? 0x295549f54be24456ull
+         : l == 'c' ? 0x20323ed082572324ull
+         : l == 'g' ? 0x3193c18562a02b4cull
+         : l == 't' ? 0x3c8bfbb395c60474ull : 0ull;
+}
+
+__device__ __forceinline__ uint32_t table_pos(uint64_t h, uint32_t bits) {
+    return (uint32_t)(((h >> 3) * 0x9E3779B97F4A7C15ull) >> (64u - bits));
+}
+
+// SketchIndex::count + hit offset (src/index.rs:192-213).
+__device__ __forceinline__ uint32_t index_lookup(const DevIndex &I, uint64_t h, uint32_t &off) {
+    const uint32_t sh = (uint32_t)(h & 7u);
+    const uint32_t bits = I.shard_bits[sh];
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint4 *T = I.table + I.shard_base[sh];
+    uint32_t pos = table_pos(h, bits);
+    const uint32_t lo = (uint32_t)h, hi = (uint32_t)(h >> 32);
+    for (;;) {
+        const uint4 s = T[pos];
+        if (s.w == 0) {
+            off = 0;
+            return 0;
+        }
+        if (s.x == lo && s.y == hi) {
+            off = s.z;
+            return s.w;
+        }
+        pos = (pos + 1u) & mask;
+    }
+}
+
+// ---- wave / workgroup scans --------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint64_t t = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the workgroup; returns this thread's exclusive prefix and
+// writes the total to *total.  `tmp` is NWAVES+1 words of LDS.
+__device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *tmp, uint32_t *total) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == WAVE - 1) tmp[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NWAVES; i++) {
+        const uint32_t t = tmp[i];
+        if (i < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+__device__ __forceinline__ uint64_t wg_excl_scan64(uint64_t v, uint64_t *tmp, uint64_t *total) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const uint64_t inc = wave_incl_scan64(v);
+    if (lane == WAVE - 1) tmp[w] = inc;
+    __syncthreads();
+    uint64_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NWAVES; i++) {
+        const uint64_t t = tmp[i];
+        if (i < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+// ---- workgroup bitonic sort of (key, val) pairs ------------------------------
+//
+// Sorts n pairs ascending by (key, val).  All merge steps run ascending (the
+// first step of each merge compares mirrored positions), so positions >= n act
+// as +infinity and are simply skipped: no padding storage is needed, and the
+// same loop runs on LDS copies (n <= cap) or directly on the global arrays.
+
+template <typename K, typename V>
+__device__ __forceinline__ void cswap(K *k, V *v, uint32_t a, uint32_t b) {
+    const K ka = k[a], kb = k[b];
+    const V va = v[a], vb = v[b];
+    if (kb < ka || (kb == ka && vb < va)) {
+        k[a] = kb;
+        k[b] = ka;
+        v[a] = vb;
+        v[b] = va;
+    }
+}
+
+template <typename K, typename V>
+__device__ void bitonic_core(K *k, V *v, uint32_t n) {
+    uint32_t N = 1;
+    while (N < n) N <<= 1;
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        const uint32_t half = size >> 1;
+        for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
+            const uint32_t blk = i / half, p = i % half;
+            const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
+            if (b < n) cswap(k, v, a, b);
+        }
+        __syncthreads();
+        for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
+                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                if (b < n) cswap(k, v, a, b);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Sorts global (gk, gv)[0, n) using LDS scratch (sk, sv) of `cap` entries when
+// it fits.  Ends with a barrier; results are visible to the whole workgroup.
+template <typename K, typename V>
+__device__ void wg_sort_pairs(K *gk, V *gv, uint32_t n, K *sk, V *sv, uint32_t cap) {
+    if (n <= 1) {
+        __syncthreads();
+        return;
+    }
+    if (n <= cap) {
+        for (uint32_t i = threadIdx.x; i < n; i += WG) {
+            sk[i] = gk[i];
+            sv[i] = gv[i];
+        }
+        __syncthreads();
+        bitonic_core(sk, sv, n);
+        for (uint32_t i = threadIdx.x; i < n; i += WG) {
+            gk[i] = sk[i];
+            gv[i] = sv[i];
+        }
+        __syncthreads();
+    } else {
+        bitonic_core(gk, gv, n);
+    }
+}
+
+}  // namespace shm

@@ -1,0 +1,1169 @@
+// gfx950 kernels of the mapping path.
+//
+// Sketching (src/sketch.rs:167-219) runs one wave64 per 4096-window chunk of a
+// sequence with a k-1 base halo staged in LDS; the index build
+// (src/index.rs:384-663) is a device radix sort of (hash, k-mer id) followed by
+// run-length encoding into an 8-shard open-addressing table; per-read mapping
+// (src/shmap/mod.rs:653-971) is a sequence of persistent kernels over a batch of
+// reads, each read's intermediate data living in flat arrays at per-read
+// offsets.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shm_device.hpp"
+#include "shm_kernels.hpp"
+
+namespace shm {
+
+// ============================================================================
+// Device-wide exclusive scan (three phases over a fixed grid).  n is read from
+// device memory so no host round trip is needed to launch it.
+// ============================================================================
+
+__global__ void __launch_bounds__(WG) k_scan_reduce(const uint64_t *in, const uint64_t *n_ptr,
+                                                    uint64_t n_static, uint64_t *part) {
+    const uint64_t n = n_ptr ? *n_ptr : n_static;
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = (uint64_t)blockIdx.x * per, e = b + per < n ? b + per : n;
+    uint64_t s = 0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += WG) s += in[i];
+    __shared__ uint64_t tmp[NWAVES + 1];
+    uint64_t tot;
+    wg_excl_scan64(s, tmp, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(WG) k_scan_top(uint64_t *part, uint32_t nparts) {
+    __shared__ uint64_t tmp[NWAVES + 1];
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += WG) {
+        const uint32_t i = base + threadIdx.x;
+        const uint64_t v = i < nparts ? part[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan64(v, tmp, &tot);
+        if (i < nparts) part[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) part[nparts] = carry;
+}
+
+__global__ void __launch_bounds__(WG) k_scan_down(const uint64_t *in, uint64_t *out,
+                                                  const uint64_t *n_ptr, uint64_t n_static,
+                                                  const uint64_t *part) {
+    const uint64_t n = n_ptr ? *n_ptr : n_static;
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = (uint64_t)blockIdx.x * per, e = b + per < n ? b + per : n;
+    __shared__ uint64_t tmp[NWAVES + 1];
+    uint64_t carry = part[blockIdx.x];
+    for (uint64_t base = b; base < e; base += WG) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = i < e ? in[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan64(v, tmp, &tot);
+        if (i < e) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = part[gridDim.x];
+}
+
+// ============================================================================
+// Sketching -- FracMinHash::sketch_slice_into (src/sketch.rs:167-219)
+// ============================================================================
+
+__device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(WG) k_chunk_counts(SeqSet S, int32_t k, uint64_t *nch) {
+    for (int32_t i = blockIdx.x * WG + threadIdx.x; i < S.n; i += gridDim.x * WG) {
+        const int64_t len = S.off[i + 1] - S.off[i];
+        const int64_t nw = (len >= k && len >= 1) ? len - k + 1 : 0;
+        nch[i] = (uint64_t)((nw + CHUNK_W - 1) / CHUNK_W);
+    }
+}
+
+// One wave per chunk of CHUNK_W windows; the chunk's bases plus the k-1 halo
+// are staged in LDS.  Pass 0 counts kept k-mers per chunk; pass 1 writes them
+// at the scanned chunk offsets, in position order (lanes cover consecutive
+// window runs and write at their exclusive prefix).
+template <int PASS>
+__global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thres,
+                                               const uint64_t *chunk_start, uint64_t *chunk_val,
+                                               uint64_t *out_h, int32_t *out_r, uint8_t *out_s) {
+    extern __shared__ uint8_t lds_bases[];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const uint32_t span = (CHUNK_W + k - 1 + 15) & ~15u;
+    uint8_t *buf = lds_bases + (size_t)w * span;
+    const uint64_t total = chunk_start[S.n];
+    const uint32_t rk = (uint32_t)k, rk1 = (uint32_t)(k - 1);
+    for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
+        const uint64_t c = cb + w;
+        const bool valid = c < total;
+        int64_t w0 = 0, w1 = 0;
+        if (valid) {
+            const uint32_t seq = upper_bound_u64(chunk_start, (uint32_t)S.n + 1, c) - 1;
+            const int64_t len = S.off[seq + 1] - S.off[seq];
+            const int64_t nw = len - k + 1;
+            w0 = (int64_t)(c - chunk_start[seq]) * CHUNK_W;
+            w1 = w0 + CHUNK_W < nw ? w0 + CHUNK_W : nw;
+            const int64_t nb = w1 - w0 + k - 1;
+            const uint8_t *src = S.bases + S.off[seq] + w0;
+            for (int64_t i = lane; i < nb; i += WAVE) buf[i] = src[i];
+        }
+        __syncthreads();
+        if (valid) {
+            const int64_t nwin = w1 - w0;
+            const int64_t lw0 = (int64_t)lane * LANE_W;
+            const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
+            uint64_t base_out = 0;
+            for (int phase = 0; phase < (PASS == 1 ? 2 : 1); phase++) {
+                uint32_t cnt = 0;
+                if (lw0 < lw1) {
+                    uint64_t hf = 0, hr = 0;
+                    for (int32_t i = 0; i < k; i++) {
+                        const uint8_t ch = buf[lw0 + i];
+                        hf ^= rotl64(nt_fw(ch), (uint32_t)(k - i - 1));
+                        hr ^= rotl64(nt_rc(ch), (uint32_t)i);
+                    }
+                    for (int64_t wi = lw0;; wi++) {
+                        const uint64_t h = hr ^ hf;
+                        if (h <= thres) {
+                            if (PASS == 1 && phase == 1) {
+                                const uint64_t o = base_out + cnt;
+                                out_h[o] = h;
+                                out_r[o] = (int32_t)(w0 + wi + k - 1);
+                                out_s[o] = hf > hr;
+                            }
+                            cnt++;
+                        }
+                        if (wi + 1 >= lw1) break;
+                        const uint8_t in = buf[wi + k], out = buf[wi];
+                        hf = rotl64(hf, 1) ^ rotl64(nt_fw(out), rk) ^ nt_fw(in);
+                        hr = rotr64(hr, 1) ^ rotr64(nt_rc(out), 1) ^ rotl64(nt_rc(in), rk1);
+                    }
+                }
+                if (PASS == 0) {
+                    uint32_t s = cnt;
+                    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_xor(s, d);
+                    if (lane == 0) chunk_val[c] = s;
+                } else if (phase == 0) {
+                    const uint32_t inc = wave_incl_scan(cnt);
+                    base_out = chunk_val[c] + (inc - cnt);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]].
+__global__ void __launch_bounds__(WG) k_seq_koff(int32_t n, const uint64_t *chunk_start,
+                                                 const uint64_t *chunk_koff, uint64_t *koff) {
+    for (int32_t i = blockIdx.x * WG + threadIdx.x; i <= n; i += gridDim.x * WG)
+        koff[i] = chunk_koff[chunk_start[i]];
+}
+
+// ============================================================================
+// Index build (src/index.rs:233-376)
+// ============================================================================
+
+__global__ void __launch_bounds__(WG) k_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys,
+                                                   uint32_t *vals) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
+        keys[i] = t_h[i];
+        vals[i] = (uint32_t)i;
+    }
+}
+
+// Stable LSD radix sort pass over 8-bit digits: per-tile digit histograms ...
+__global__ void __launch_bounds__(WG) k_rs_hist(const uint64_t *keys, uint64_t n, uint32_t shift,
+                                                uint64_t *hist, uint32_t ntiles) {
+    __shared__ uint32_t cnt[256];
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t t0 = (uint64_t)tile * RS_TILE;
+        for (int it = 0; it < RS_ITEMS; it++) {
+            const uint64_t e = t0 + (uint64_t)it * WG + threadIdx.x;
+            if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 0xff], 1u);
+        }
+        __syncthreads();
+        hist[(uint64_t)threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// ... then a stable scatter: within a tile, keys are visited in index order in
+// rounds of WG, ranked inside each wave by ballot matching on the digit, and
+// across waves through LDS counters.
+__global__ void __launch_bounds__(WG) k_rs_scatter(const uint64_t *kin, const uint32_t *vin,
+                                                   uint64_t *kout, uint32_t *vout, uint64_t n,
+                                                   uint32_t shift, const uint64_t *offs,
+                                                   uint32_t ntiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[NWAVES][256];
+    __shared__ uint64_t gbase[256];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        run[threadIdx.x] = 0;
+        for (int i = 0; i < NWAVES; i++) wcnt[i][threadIdx.x] = 0;
+        gbase[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + tile];
+        __syncthreads();
+        const uint64_t t0 = (uint64_t)tile * RS_TILE;
+        for (int it = 0; it < RS_ITEMS; it++) {
+            const uint64_t e = t0 + (uint64_t)it * WG + threadIdx.x;
+            const bool valid = e < n;
+            uint64_t key = 0;
+            uint32_t val = 0;
+            uint32_t d = 256;
+            if (valid) {
+                key = kin[e];
+                val = vin[e];
+                d = (uint32_t)((key >> shift) & 0xff);
+            }
+            uint64_t peers = ~0ull;
+#pragma unroll
+            for (int b = 0; b < 9; b++) {
+                const uint64_t bal = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? bal : ~bal;
+            }
+            const uint32_t rank = __popcll(peers & lt);
+            const uint32_t cntw = __popcll(peers);
+            if (valid && rank == 0) wcnt[w][d] = cntw;
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = run[d] + rank;
+                for (int i = 0; i < w; i++) pos += wcnt[i][d];
+                const uint64_t o = gbase[d] + pos;
+                kout[o] = key;
+                vout[o] = val;
+            }
+            __syncthreads();
+            if (valid && rank == 0) {
+                atomicAdd(&run[d], cntw);
+                wcnt[w][d] = 0;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_mark_heads(const uint64_t *keys, uint64_t n, uint64_t *flag) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG)
+        flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(WG) k_emit_keys(const uint64_t *keys, uint64_t n, const uint64_t *flag,
+                                                  const uint64_t *kid, uint64_t *uk_h, uint64_t *uk_off) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
+        if (flag[i]) {
+            uk_h[kid[i]] = keys[i];
+            uk_off[kid[i]] = i;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) uk_off[kid[n]] = n;
+}
+
+__device__ __forceinline__ uint32_t seg_of(const int64_t *kstart, int32_t nseg, uint64_t g) {
+    int32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)kstart[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
+__global__ void __launch_bounds__(WG) k_fill_hits(const uint32_t *vals, uint64_t n, const int32_t *t_r,
+                                                  const uint8_t *t_s, const int64_t *kstart,
+                                                  int32_t nseg, uint32_t *hit_g, uint32_t *hit_r,
+                                                  uint32_t *hit_seg, uint64_t *sbits) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
+        const uint32_t g = vals[i];
+        hit_g[i] = g;
+        hit_r[i] = (uint32_t)t_r[g] | ((uint32_t)t_s[g] << 31);
+        hit_seg[i] = seg_of(kstart, nseg, g);
+        sbits[i] = t_s[g];
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_narrow_prefix(const uint64_t *in, uint64_t n, uint32_t *out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * WG)
+        out[i] = (uint32_t)in[i];
+}
+
+// The -M closed form (src/index.rs:264-291, :295-315, :360-376): a hash whose
+// total count exceeds max_matches is erased; a single-hit hash never is.
+__device__ __forceinline__ bool key_kept(uint64_t cnt, int32_t max_matches) {
+    return !(max_matches > 0 && cnt > 1 && cnt > (uint64_t)max_matches);
+}
+
+__global__ void __launch_bounds__(WG) k_shard_count(const uint64_t *uk_h, const uint64_t *uk_off,
+                                                    uint64_t nk, int32_t max_matches,
+                                                    unsigned long long *shard_n, unsigned long long *stats) {
+    __shared__ unsigned long long sc[8 + 4];
+    if (threadIdx.x < 12) sc[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < nk; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t cnt = uk_off[i + 1] - uk_off[i];
+        if (key_kept(cnt, max_matches)) {
+            atomicAdd(&sc[uk_h[i] & 7], 1ull);
+        } else {
+            atomicAdd(&sc[8], 1ull);
+            atomicAdd(&sc[9], (unsigned long long)cnt);
+        }
+        if (cnt > 1) atomicMax(&sc[10], (unsigned long long)cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) atomicAdd(&shard_n[threadIdx.x], sc[threadIdx.x]);
+    if (threadIdx.x == 8) atomicAdd(&stats[0], sc[8]);
+    if (threadIdx.x == 9) atomicAdd(&stats[1], sc[9]);
+    if (threadIdx.x == 10) atomicMax(&stats[2], sc[10]);
+}
+
+__global__ void __launch_bounds__(WG) k_insert(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk,
+                                               int32_t max_matches, uint4 *table, ShardGeom geom) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < nk; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t cnt = uk_off[i + 1] - uk_off[i];
+        if (!key_kept(cnt, max_matches)) continue;
+        const uint64_t h = uk_h[i];
+        const uint32_t sh = (uint32_t)(h & 7u);
+        const uint32_t bits = geom.bits[sh];
+        const uint32_t mask = (1u << bits) - 1u;
+        uint4 *T = table + geom.base[sh];
+        uint32_t pos = table_pos(h, bits);
+        for (;;) {
+            const uint32_t old = atomicCAS(&T[pos].w, 0u, (uint32_t)cnt);
+            if (old == 0u) {
+                T[pos].x = (uint32_t)h;
+                T[pos].y = (uint32_t)(h >> 32);
+                T[pos].z = (uint32_t)uk_off[i];
+                break;
+            }
+            pos = (pos + 1u) & mask;
+        }
+    }
+}
+
+// ============================================================================
+// Per-read mapping stages (src/shmap/mod.rs:653-971).  Seeding kernels cover
+// the whole batch; bucketing kernels cover one group of reads [r0, r0+RG)
+// whose tables fit the device budget.
+// ============================================================================
+
+// Sort one read's k-mers by (hash, position): the order of
+// unique_elements_with_info (src/shmap/seeding.rs:28-34) with every run kept
+// in increasing r, so its first element is the smallest-r occurrence whose
+// strand the seed keeps and pmatches is the run read backwards.
+__global__ void __launch_bounds__(WG) k_sort_kmers(MapArgs A) {
+    extern __shared__ uint8_t lds[];
+    uint64_t *sk = (uint64_t *)lds;
+    uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
+    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
+        const uint64_t o = A.koff[rd];
+        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        for (uint32_t i = threadIdx.x; i < m; i += WG) {
+            A.ps_h[o + i] = A.k_h[o + i];
+            A.ps_v[o + i] = i;
+        }
+        __syncthreads();
+        wg_sort_pairs(A.ps_h + o, A.ps_v + o, m, sk, sv, SORT_CAP);
+    }
+}
+
+// Group runs into seeds (hash order), probe the index (SketchIndex::count), and
+// key the rarest-first order: the stable sort by hits_in_t of
+// src/shmap/seeding.rs:71 is a sort by (hits_in_t, seed_num).
+__global__ void __launch_bounds__(WG) k_group_seeds(MapArgs A, DevIndex I) {
+    __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_u, s_pm;
+    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
+        const uint64_t o = A.koff[rd];
+        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        if (threadIdx.x == 0) {
+            s_u = 0;
+            s_pm = 0;
+        }
+        __syncthreads();
+        for (uint32_t base = 0; base < m; base += WG) {
+            const uint32_t i = base + threadIdx.x;
+            const bool head = i < m && (i == 0 || A.ps_h[o + i] != A.ps_h[o + i - 1]);
+            uint32_t tot;
+            const uint32_t ex = wg_excl_scan(head ? 1u : 0u, tmp, &tot);
+            const uint32_t u0 = s_u;
+            if (head) A.sd_first[o + u0 + ex] = i;
+            __syncthreads();
+            if (threadIdx.x == 0) s_u = u0 + tot;
+            __syncthreads();
+        }
+        const uint32_t u = s_u;
+        uint32_t pm = 0;
+        for (uint32_t sid = threadIdx.x; sid < u; sid += WG) {
+            const uint32_t first = A.sd_first[o + sid];
+            const uint32_t next = sid + 1 < u ? A.sd_first[o + sid + 1] : m;
+            const uint64_t h = A.ps_h[o + first];
+            uint32_t hoff;
+            const uint32_t cnt = index_lookup(I, h, hoff);
+            A.sd_h[o + sid] = h;
+            A.sd_occs[o + sid] = next - first;
+            A.sd_strand[o + sid] = A.k_s[o + A.ps_v[o + first]];
+            A.sd_cnt[o + sid] = cnt;
+            A.sd_hoff[o + sid] = hoff;
+            A.sd_key[o + sid] = ((uint64_t)cnt << 32) | sid;
+            A.sd_ord[o + sid] = sid;
+            pm += cnt;
+        }
+        for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
+        if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_pm, pm);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            A.rd_u[rd] = (int32_t)u;
+            A.rd_possible[rd] = (int32_t)s_pm;  // an i32 sum (src/shmap/mod.rs:689)
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_sort_seeds(MapArgs A) {
+    extern __shared__ uint8_t lds[];
+    uint64_t *sk = (uint64_t *)lds;
+    uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
+    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
+        const uint64_t o = A.koff[rd];
+        const uint32_t u = (uint32_t)A.rd_u[rd];
+        wg_sort_pairs(A.sd_key + o, A.sd_ord + o, u, sk, sv, SORT_CAP);
+    }
+}
+
+// Seed budget S and halflen (src/shmap/mod.rs:726-742), the consumed prefix of
+// match_seeds (src/shmap/seeding.rs:101-102: seeds are taken while fewer than S
+// occurrences have been consumed, a zero-hit seed included) and its counters
+// (:103-106, :207-209).
+__global__ void __launch_bounds__(WG) k_budget(MapArgs A) {
+    __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_c, s_seeds, s_max, s_occ_carry, s_cnt_carry;
+    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
+        const uint64_t o = A.koff[rd];
+        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        const int32_t u = A.rd_u[rd];
+        const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
+        const double theta2 = A.theta - A.min_diff;
+        const int32_t S = A.one_sweep ? u : (int32_t)((1.0 - theta2) * (double)m) + 1;
+        const int32_t hl = A.abs_pos ? len : m;
+        const bool ok = hl >= MIN_HALFLEN;
+        if (threadIdx.x == 0) {
+            s_c = 0;
+            s_seeds = 0;
+            s_max = 0;
+            s_occ_carry = 0;
+            s_cnt_carry = 0;
+        }
+        __syncthreads();
+        if (ok) {
+            for (int32_t base = 0; base < u; base += WG) {
+                const int32_t q = base + threadIdx.x;
+                uint32_t occs = 0, cnt = 0;
+                if (q < u) {
+                    const uint32_t sid = A.sd_ord[o + q];
+                    occs = A.sd_occs[o + sid];
+                    cnt = A.sd_cnt[o + sid];
+                }
+                uint32_t tot;
+                const uint32_t ex = wg_excl_scan(occs, tmp, &tot);
+                const uint32_t pre = s_occ_carry + ex;
+                const bool consumed = q < u && (int64_t)pre < (int64_t)S;
+                uint32_t ctot;
+                const uint32_t cex = wg_excl_scan(consumed ? cnt : 0u, tmp, &ctot);
+                if (consumed) {
+                    A.sd_epre[o + q] = s_cnt_carry + cex;
+                    atomicAdd(&s_c, 1u);
+                    atomicAdd(&s_seeds, occs);
+                    atomicMax(&s_max, cnt);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    s_occ_carry += tot;
+                    s_cnt_carry += ctot;
+                }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) {
+            const uint32_t E = s_cnt_carry;
+            A.rd_S[rd] = S;
+            A.rd_hl[rd] = hl;
+            A.rd_ok[rd] = ok ? 1 : 0;
+            A.rd_c[rd] = (int32_t)s_c;
+            A.rd_bseeds[rd] = (int32_t)s_seeds;
+            A.rd_max_seed[rd] = (int32_t)s_max;
+            A.rd_seed_matches[rd] = (int32_t)E;
+            A.rd_E[rd] = E;
+            uint64_t cap = 0;
+            if (E > 0) {
+                cap = 64;
+                while (cap < 2ull * E) cap <<= 1;
+            }
+            A.rd_cap[rd] = cap;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_tab_init(MapArgs A) {
+    const uint64_t n = A.tab_base[A.RG];
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
+        A.tb_key[i] = EMPTY_KEY;
+        A.tb_mc[i] = 0;
+        A.tb_min[i] = INT32_MAX;
+        A.tb_max[i] = -1;
+    }
+}
+
+// Bucket geometry.  A hit at pos (tpos, or r with abs_pos) lies in half-bucket
+// pos / hl; bucket b spans half-buckets b and b+1 (src/buckets.rs:512-518).
+// g = ext_start[seg] / hl + 2*seg + b numbers buckets in (segm_id, b) order
+// without gaps shared between segments.
+__device__ __forceinline__ int32_t hit_pos(const DevIndex &I, uint32_t idx, uint32_t seg, int ap) {
+    return ap ? (int32_t)(I.hit_r[idx] & 0x7fffffffu)
+              : (int32_t)((int64_t)I.hit_g[idx] - I.kstart[seg]);
+}
+
+__device__ __forceinline__ uint64_t bucket_id(const MapArgs &A, uint32_t seg, int32_t b, int32_t hl) {
+    const int64_t ext = A.abs_pos ? A.seg_cum[seg] : A.kstart[seg];
+    return (uint64_t)(ext / hl) + 2ull * seg + (uint64_t)b;
+}
+
+__device__ __forceinline__ void decode_bucket(const MapArgs &A, uint32_t g, int32_t hl, uint32_t &seg,
+                                              int32_t &b) {
+    int32_t lo = 0, hi = A.nseg - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (bucket_id(A, (uint32_t)mid, 0, hl) <= g) lo = mid; else hi = mid - 1;
+    }
+    seg = (uint32_t)lo;
+    b = (int32_t)(g - bucket_id(A, seg, 0, hl));
+}
+
+// First hit index in [lo, hi) at or after (seg, pos) -- the partition_point of
+// src/shmap/pruning.rs:52-61.
+__device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t lo, uint32_t hi,
+                                                     uint32_t seg, int64_t pos, int ap) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t s = I.hit_seg[mid];
+        const bool before = s != seg ? s < seg : (int64_t)hit_pos(I, mid, s, ap) < pos;
+        if (before) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void tab_add(const MapArgs &A, uint64_t base, uint64_t cap, uint32_t g,
+                                        uint32_t matches, int32_t codir, int32_t rmin, int32_t rmax) {
+    const uint32_t lg = 63 - __clzll(cap);
+    const uint32_t mask = (uint32_t)(cap - 1);
+    uint32_t pos = ((g * 0x9E3779B1u) >> (32 - lg)) & mask;
+    for (;;) {
+        const uint32_t old = atomicCAS(&A.tb_key[base + pos], EMPTY_KEY, g);
+        if (old == EMPTY_KEY || old == g) break;
+        pos = (pos + 1u) & mask;
+    }
+    atomicAdd((unsigned long long *)&A.tb_mc[base + pos],
+              (unsigned long long)(((uint64_t)matches << 32) + (uint64_t)(int64_t)codir));
+    atomicMin(&A.tb_min[base + pos], rmin);
+    atomicMax(&A.tb_max[base + pos], rmax);
+}
+
+// match_seeds (src/shmap/seeding.rs:98-212) as one flat pass over every hit of
+// every consumed seed of the group.  A seed's hits are in (segm_id, pos) order,
+// so the first hit of each (segment, half-bucket) run emits the clamped
+// per-(seed, bucket) contribution of bucket b (runs b and b+1) and -- unless the
+// previous run, at b-1, already did -- of bucket b-1 (run b alone).  Counts come
+// from binary searches, strand agreement from the strand prefix.
+__global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
+    const uint64_t E = A.hit_base[A.RG];
+    const int ap = A.abs_pos;
+    for (uint64_t f = (uint64_t)blockIdx.x * WG + threadIdx.x; f < E; f += (uint64_t)gridDim.x * WG) {
+        const uint32_t lr = upper_bound_u64(A.hit_base, (uint32_t)A.RG + 1, f) - 1;
+        const int32_t rd = A.r0 + (int32_t)lr;
+        const uint32_t local = (uint32_t)(f - A.hit_base[lr]);
+        const uint64_t o = A.koff[rd];
+        uint32_t lo = 0, hi = (uint32_t)A.rd_c[rd];
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (A.sd_epre[o + mid] <= local) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t q = lo - 1;
+        const uint32_t sid = A.sd_ord[o + q];
+        const uint32_t j = local - A.sd_epre[o + q];
+        const uint32_t hoff = A.sd_hoff[o + sid], cnt = A.sd_cnt[o + sid];
+        const uint32_t occs = A.sd_occs[o + sid];
+        const uint32_t sstrand = A.sd_strand[o + sid];
+        const int32_t hl = A.rd_hl[rd];
+        const uint32_t idx = hoff + j;
+        const uint32_t seg = I.hit_seg[idx];
+        const int32_t b = hit_pos(I, idx, seg, ap) / hl;
+        bool prev_bm1 = false;
+        if (j > 0) {
+            const uint32_t seg0 = I.hit_seg[idx - 1];
+            if (seg0 == seg) {
+                const int32_t b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
+                if (b0 == b) continue;  // inside a run
+                prev_bm1 = b0 == b - 1;
+            }
+        }
+        const uint64_t tbase = A.tab_base[lr], tcap = A.rd_cap[rd];
+        const int32_t rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
+        for (int side = 0; side < 2; side++) {
+            if (side == 1 && (b == 0 || prev_bm1)) break;
+            const int32_t bb = side == 0 ? b : b - 1;
+            const uint32_t e = hits_lower_bound(I, idx, hoff + cnt, seg, (int64_t)(bb + 2) * hl, ap);
+            const uint32_t n = e - idx;
+            const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx];
+            const uint32_t same = sstrand ? ones : n - ones;
+            tab_add(A, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
+                    (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_count_buckets(MapArgs A) {
+    __shared__ uint32_t s_n;
+    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+        const int32_t rd = A.r0 + lr;
+        const uint64_t base = A.tab_base[lr], cap = A.rd_cap[rd];
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        uint32_t n = 0;
+        for (uint64_t i = threadIdx.x; i < cap; i += WG) n += A.tb_key[base + i] != EMPTY_KEY;
+        for (int d = WAVE / 2; d > 0; d >>= 1) n += __shfl_xor(n, d);
+        if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_n, n);
+        __syncthreads();
+        if (threadIdx.x == 0) A.rd_nb[rd] = (int32_t)s_n;
+        __syncthreads();
+    }
+}
+
+// propagate_seeds_to_buckets + get_sorted_buckets (src/buckets.rs:603-640,
+// 685-724): a read's buckets in bucket-id order, then ordered by (matches
+// descending, bucket-id rank ascending).
+__global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
+    extern __shared__ uint8_t lds[];
+    uint64_t *sk = (uint64_t *)lds;
+    uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
+    __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_n;
+    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+        const int32_t rd = A.r0 + lr;
+        const uint64_t tb = A.tab_base[lr], cap = A.rd_cap[rd], base = A.bk_base[lr];
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        for (uint64_t b0 = 0; b0 < cap; b0 += WG) {
+            const uint64_t i = b0 + threadIdx.x;
+            const bool used = i < cap && A.tb_key[tb + i] != EMPTY_KEY;
+            uint32_t tot;
+            const uint32_t ex = wg_excl_scan(used ? 1u : 0u, tmp, &tot);
+            const uint32_t n0 = s_n;
+            if (used) {
+                A.bk_key[base + n0 + ex] = A.tb_key[tb + i];
+                A.bk_val[base + n0 + ex] = (uint32_t)i;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_n = n0 + tot;
+            __syncthreads();
+        }
+        const uint32_t nb = s_n;
+        wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
+        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
+            const uint32_t slot = A.bk_val[base + i];
+            const uint64_t mc = A.tb_mc[tb + slot];
+            const int32_t codir = (int32_t)(uint32_t)mc;
+            const uint32_t matches = (uint32_t)((mc - (uint64_t)(int64_t)codir) >> 32);
+            A.tmp_g[base + i] = (uint32_t)A.bk_key[base + i];
+            A.tmp_m[base + i] = (int32_t)matches;
+            A.tmp_c[base + i] = codir;
+            A.tmp_min[base + i] = A.tb_min[tb + slot];
+            A.tmp_max[base + i] = A.tb_max[tb + slot];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
+            A.bk_key[base + i] = ((uint64_t)(0xffffffffu - (uint32_t)A.tmp_m[base + i]) << 32) | i;
+            A.bk_val[base + i] = i;
+        }
+        __syncthreads();
+        wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
+        for (uint32_t p = threadIdx.x; p < nb; p += WG) {
+            const uint32_t i = A.bk_val[base + p];
+            A.bs_g[base + p] = A.tmp_g[base + i];
+            A.bs_m[base + p] = A.tmp_m[base + i];
+            A.bs_c[base + p] = A.tmp_c[base + i];
+            A.bs_min[base + p] = A.tmp_min[base + i];
+            A.bs_max[base + p] = A.tmp_max[base + i];
+        }
+        __syncthreads();
+    }
+}
+
+// seed_heuristic_pass + matches_in_bucket (src/shmap/pruning.rs:10-119) run
+// to completion for every bucket.  Each extension adds occs to seeds and at most
+// occs to matches, so sh never increases and the fully extended sh decides
+// survival for any threshold; extension stops once sh falls below
+// theta*(1-min_diff), under every threshold either match_rest pass can use.
+__global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
+    const int ap = A.abs_pos;
+    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+        const int32_t rd = A.r0 + lr;
+        const uint64_t base = A.bk_base[lr], o = A.koff[rd];
+        const int32_t nb = A.rd_nb[rd];
+        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
+        const int32_t seeds0 = A.rd_bseeds[rd];
+        for (int32_t p = threadIdx.x; p < nb; p += WG) {
+            int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
+            int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
+            uint8_t alive = 1;
+            double sh = 1.0;
+            if (!A.no_bucket_pruning) {
+                uint32_t seg;
+                int32_t b;
+                decode_bucket(A, A.bs_g[base + p], hl, seg, b);
+                const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
+                int32_t seeds = seeds0;
+                for (int32_t q = c;; q++) {
+                    sh = 1.0 - (double)(seeds - matches) / (double)m;
+                    if (sh < A.thr_min) {
+                        alive = 0;
+                        break;
+                    }
+                    if (q >= u) break;
+                    const uint32_t sid = A.sd_ord[o + q];
+                    const uint32_t occs = A.sd_occs[o + sid];
+                    const uint32_t cnt = A.sd_cnt[o + sid];
+                    seeds += (int32_t)occs;
+                    if (cnt == 0) continue;
+                    const uint32_t hoff = A.sd_hoff[o + sid];
+                    const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
+                    const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
+                    const uint32_t n = e - lo;
+                    if (n == 0) continue;
+                    const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
+                    const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
+                    codir += (int32_t)(2 * same) - (int32_t)n;
+                    const int32_t r0 = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
+                    const int32_t r1 = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
+                    if (r0 < rmin) rmin = r0;
+                    if (r1 > rmax) rmax = r1;
+                    matches += (int32_t)(n < occs ? n : occs);
+                }
+            }
+            A.bs_m[base + p] = matches;
+            A.bs_c[base + p] = codir;
+            A.bs_min[base + p] = rmin;
+            A.bs_max[base + p] = rmax;
+            A.bs_alive[base + p] = alive;
+            A.bs_sh[base + p] = sh;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int32_t read_seed_of(const MapArgs &A, uint64_t o, int32_t u, uint64_t h) {
+    int32_t lo = 0, hi = u;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (A.sd_h[o + mid] < h) lo = mid + 1; else hi = mid;
+    }
+    return (lo < u && A.sd_h[o + lo] == h) ? lo : -1;
+}
+
+// find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket that can
+// survive either match_rest pass: the best_fixed_length window sweep
+// (:189-291), the LCS of :114-170, or the bucket_SH record.  The workgroup
+// resolves the read seed of every reference k-mer of the bucket's span; one
+// lane then runs the sequential sweep over LDS.
+__global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
+    extern __shared__ uint8_t lds[];
+    int32_t *l_sid = (int32_t *)lds;
+    int32_t *l_diff = (int32_t *)(lds + (size_t)REFINE_CAP * 4);
+    const int ap = A.abs_pos;
+    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+        const int32_t rd = A.r0 + lr;
+        const uint64_t base = A.bk_base[lr], o = A.koff[rd];
+        const int32_t nb = A.rd_nb[rd];
+        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        const int32_t u = A.rd_u[rd], hl = A.rd_hl[rd];
+        const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
+        int32_t *diff = (u <= REFINE_CAP) ? l_diff : A.g_diff + o;
+        for (int32_t s = threadIdx.x; s < u; s += WG) diff[s] = (int32_t)A.sd_occs[o + s];
+        __syncthreads();
+        for (int32_t p = 0; p < nb; p++) {
+            if (!A.bs_alive[base + p]) continue;
+            uint32_t seg;
+            int32_t b;
+            decode_bucket(A, A.bs_g[base + p], hl, seg, b);
+            const double sh = A.no_bucket_pruning ? 1.0 : A.bs_sh[base + p];
+            const int64_t ks = I.kstart[seg];
+            const int64_t tlen = I.kstart[seg + 1] - ks;
+            const int64_t bb = (int64_t)b * hl, be = (int64_t)(b + 2) * hl;
+            const int64_t l0 = bb > 0 ? bb : 0;
+            const int64_t end = be < tlen ? be : tlen;
+            const int64_t span = end > l0 ? end - l0 : 0;
+            if (A.metric == METRIC_BUCKET_SH) {
+                if (threadIdx.x == 0) {
+                    const int32_t tl = A.bs_min[base + p], tr = A.bs_max[base + p];
+                    A.sc_j[base + p] = sh;
+                    A.sc_i[base + p] = A.bs_m[base + p];
+                    A.sc_same[base + p] = A.bs_c[base + p];
+                    A.sc_tl[base + p] = tl;
+                    A.sc_tr[base + p] = tr;
+                    A.sc_ssz[base + p] = tr - tl;
+                    A.sc_pend[base + p] = len - 1;
+                    A.sc_seg[base + p] = (int32_t)seg;
+                }
+                continue;
+            }
+            int32_t *sid_arr = span <= REFINE_CAP ? l_sid : A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
+            for (int64_t t = l0 + threadIdx.x; t < end; t += WG) {
+                const int32_t s = read_seed_of(A, o, u, I.t_h[ks + t]);
+                int32_t v = -1;
+                if (s >= 0) v = s | ((A.sd_strand[o + s] == I.t_s[ks + t]) ? 0 : 0x40000000);
+                sid_arr[t - l0] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double J = -1.0;
+                int32_t inter = -1, same = -1, tl = -1, tr = -1, ssz = -1, pend = -1, mseg = -1;
+                if (A.metric == METRIC_BUCKET_LCS) {
+                    // Patience LIS of the pmatches sequence and of its reverse;
+                    // tails hold distinct read positions, so at most m of them.
+                    int32_t *tails = A.g_tails + o;
+                    int32_t best = 0;
+                    for (int dir = 0; dir < 2; dir++) {
+                        int32_t nt = 0;
+                        for (int64_t ii = 0; ii < span; ii++) {
+                            const int64_t t = dir == 0 ? ii : span - 1 - ii;
+                            const int32_t v = sid_arr[t];
+                            if (v < 0) continue;
+                            const int32_t s = v & 0x3fffffff;
+                            const uint32_t first = A.sd_first[o + s], occs = A.sd_occs[o + s];
+                            for (uint32_t x = 0; x < occs; x++) {
+                                const uint32_t at = dir == 0 ? first + occs - 1 - x : first + x;
+                                const int32_t val = A.k_r[o + A.ps_v[o + at]];
+                                int32_t lo2 = -1, hi2 = nt;
+                                while (lo2 + 1 < hi2) {
+                                    const int32_t mid = (lo2 + hi2) / 2;
+                                    if (tails[mid] < val) lo2 = mid; else hi2 = mid;
+                                }
+                                if (hi2 == nt) tails[nt++] = val; else tails[hi2] = val;
+                            }
+                        }
+                        if (nt > best) best = nt;
+                    }
+                    J = (double)best / (double)m;
+                    inter = A.bs_m[base + p];
+                    same = A.bs_c[base + p];
+                    tl = A.bs_min[base + p];
+                    tr = A.bs_max[base + p];
+                    ssz = tr - tl;
+                    pend = len - 1;
+                    mseg = (int32_t)seg;
+                } else {
+                    const bool jac = A.metric == METRIC_JACCARD;
+                    const int32_t p_sz = len - A.k;
+                    int64_t l = l0, r = l0;
+                    int32_t I2 = 0, ss = 0;
+                    while (l < end) {
+                        const int32_t tlr = I.t_r[ks + l];
+                        while (r < end) {
+                            const bool in_w = ap ? I.t_r[ks + r] < tlr + m : r < l + m;
+                            if (!in_w) break;
+                            const int32_t v = sid_arr[r - l0];
+                            if (v >= 0) {
+                                ss += (v & 0x40000000) ? -1 : 1;
+                                if (--diff[v & 0x3fffffff] >= 0) I2++;
+                            }
+                            r++;
+                        }
+                        const int32_t s_kmers = (int32_t)(r - l);
+                        const double score = jac ? (double)I2 / (double)(m + s_kmers - I2)
+                                                 : (double)I2 / (double)m;
+                        if (l < r && score > J) {
+                            J = score;
+                            inter = I2;
+                            same = ss;
+                            tl = I.t_r[ks + l];
+                            tr = I.t_r[ks + r - 1];
+                            ssz = tr - tl + 1;
+                            pend = p_sz - 1;
+                            mseg = (int32_t)seg;
+                        }
+                        const int32_t v = sid_arr[l - l0];
+                        if (v >= 0) {
+                            ss -= (v & 0x40000000) ? -1 : 1;
+                            if (++diff[v & 0x3fffffff] >= 1) I2--;
+                        }
+                        l++;
+                    }
+                }
+                A.sc_j[base + p] = J;
+                A.sc_i[base + p] = inter;
+                A.sc_same[base + p] = same;
+                A.sc_tl[base + p] = tl;
+                A.sc_tr[base + p] = tr;
+                A.sc_ssz[base + p] = ssz;
+                A.sc_pend[base + p] = pend;
+                A.sc_seg[base + p] = mseg;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double overlap_of(int32_t sa, int32_t la, int32_t ra, int32_t sb, int32_t lb,
+                                             int32_t rb) {
+    if (sa != sb) return -0.0;
+    int32_t cap = (ra < rb ? ra : rb) - (la > lb ? la : lb);
+    if (cap < 0) cap = 0;
+    const int32_t cup = (ra > rb ? ra : rb) - (la < lb ? la : lb);
+    return (double)cap / (double)cup;
+}
+
+// match_rest, both passes (src/shmap/scoring.rs:367-489 called from
+// src/shmap/mod.rs:809-861), over the precomputed survival and scores; the
+// refine memo's counters follow :38-109 (a Containment/Jaccard bucket refined in
+// pass 1 is a memo hit in pass 2).
+__global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
+    for (int32_t lr = blockIdx.x * WG + threadIdx.x; lr < A.RG; lr += gridDim.x * WG) {
+        const int32_t rd = A.r0 + lr;
+        shm_read_result r = {};
+        const uint64_t base = A.bk_base[lr], o = A.koff[rd];
+        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        r.read_len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
+        r.m = m;
+        r.u = A.rd_u[rd];
+        r.S = A.rd_S[rd];
+        r.possible_matches = A.rd_possible[rd];
+        r.b_segm = r.b_b = r.b2_segm = r.b2_b = -1;
+        r.intersection2 = -1;
+        r.J2 = -1.0;
+        if (!A.rd_ok[rd]) {
+            r.unmappable_short = 1;
+            out[rd] = r;
+            continue;
+        }
+        r.seed_matches = A.rd_seed_matches[rd];
+        r.total_matches = A.rd_seed_matches[rd];
+        r.max_seed_matches = A.rd_max_seed[rd];
+        const int32_t nb = A.rd_nb[rd];
+        const int32_t hl = A.rd_hl[rd];
+        r.seeded_buckets = nb;
+        const bool nbp = A.no_bucket_pruning;
+        const bool memoizable = A.metric == METRIC_CONTAINMENT || A.metric == METRIC_JACCARD;
+        int32_t refined = 0, memo = 0, fin = 0;
+        double thr = A.theta;
+        int32_t best = -1;
+        for (int32_t p = 0; p < nb; p++) {
+            if (!A.bs_alive[base + p]) continue;
+            if (!nbp && A.bs_sh[base + p] < thr) continue;
+            refined++;
+            A.bs_alive[base + p] = 2;  // refined in pass 1: a memo entry
+            const double sc = A.sc_j[base + p];
+            if (sc > thr) {
+                fin++;
+                if (best < 0 || sc > A.sc_j[base + best]) {
+                    thr = sc;
+                    best = p;
+                }
+            }
+        }
+        int32_t best2 = -1;
+        if (best >= 0) {
+            double thr2 = A.sc_j[base + best] * (1.0 - A.min_diff);
+            const int32_t fs = A.sc_seg[base + best], fl = A.sc_tl[base + best], fr = A.sc_tr[base + best];
+            for (int32_t p = 0; p < nb; p++) {
+                const uint8_t al = A.bs_alive[base + p];
+                if (!al) continue;
+                if (!nbp && A.bs_sh[base + p] < thr2) continue;
+                if (memoizable && al == 2) memo++; else refined++;
+                const double sc = A.sc_j[base + p];
+                if (sc > thr2) {
+                    fin++;
+                    const double ov =
+                        overlap_of(A.sc_seg[base + p], A.sc_tl[base + p], A.sc_tr[base + p], fs, fl, fr);
+                    if (ov < A.max_overlap && (best2 < 0 || sc > A.sc_j[base + best2])) {
+                        if (sc > thr2) thr2 = sc;
+                        best2 = p;
+                    }
+                }
+            }
+        }
+        r.refined_buckets = refined;
+        r.refine_memo_hits = memo;
+        r.final_buckets = fin;
+        if (best >= 0) {
+            const uint64_t bp = base + best;
+            uint32_t seg;
+            int32_t b;
+            r.mapped = 1;
+            r.p_start = 0;
+            r.p_end = A.sc_pend[bp];
+            r.strand = A.sc_same[bp] > 0 ? '+' : '-';
+            r.segm_id = A.sc_seg[bp];
+            r.t_l = A.sc_tl[bp];
+            r.t_r = A.sc_tr[bp];
+            r.s_sz = A.sc_ssz[bp];
+            r.intersection = A.sc_i[bp];
+            r.same_strand = A.sc_same[bp];
+            r.J = A.sc_j[bp];
+            r.sh = nbp ? 1.0 : A.bs_sh[bp];
+            decode_bucket(A, A.bs_g[bp], hl, seg, b);
+            r.b_segm = (int32_t)seg;
+            r.b_b = b;
+            if (best2 >= 0) {
+                const uint64_t bq = base + best2;
+                r.has_second = 1;
+                r.intersection2 = A.sc_i[bq];
+                r.J2 = A.sc_j[bq];
+                decode_bucket(A, A.bs_g[bq], hl, seg, b);
+                r.b2_segm = (int32_t)seg;
+                r.b2_b = b;
+            }
+        }
+        out[rd] = r;
+    }
+}
+
+}  // namespace shm
+
+// ============================================================================
+// Host launch wrappers
+// ============================================================================
+namespace shm {
+
+static inline uint32_t flat_grid(uint64_t n, uint32_t cap) {
+    uint64_t g = (n + WG - 1) / WG;
+    if (g < 1) g = 1;
+    return (uint32_t)(g < cap ? g : cap);
+}
+
+void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint,
+                 uint64_t *part, hipStream_t st) {
+    // n_hint bounds the grid; the kernels read the true length from n_dev.
+    const uint32_t G = flat_grid(n_hint, SCAN_PARTS);
+    k_scan_reduce<<<G, WG, 0, st>>>(in, n_dev, n_hint, part);
+    k_scan_top<<<1, WG, 0, st>>>(part, G);
+    k_scan_down<<<G, WG, 0, st>>>(in, out, n_dev, n_hint, part);
+}
+
+void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
+    k_chunk_counts<<<flat_grid((uint64_t)S.n, 1024), WG, 0, st>>>(S, k, nch);
+}
+
+size_t sketch_lds(int32_t k) { return (size_t)NWAVES * ((CHUNK_W + k - 1 + 15) & ~15u); }
+
+void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
+                   uint64_t *chunk_val, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
+                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+    uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    if (pass == 0)
+        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, out_h,
+                                                             out_r, out_s);
+    else
+        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, out_h,
+                                                             out_r, out_s);
+}
+
+void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
+                     hipStream_t st) {
+    k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, chunk_start, chunk_koff, koff);
+}
+
+void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st) {
+    k_fill_pairs<<<flat_grid(n, 8192), WG, 0, st>>>(t_h, n, keys, vals);
+}
+
+void launch_radix_pass(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t n,
+                       uint32_t shift, uint64_t *hist, uint64_t *offs, uint64_t *part, hipStream_t st) {
+    const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+    const uint32_t g = ntiles < 4096 ? ntiles : 4096;
+    k_rs_hist<<<g, WG, 0, st>>>(kin, n, shift, hist, ntiles);
+    launch_scan(hist, offs, nullptr, (uint64_t)ntiles * 256, part, st);
+    k_rs_scatter<<<g, WG, 0, st>>>(kin, vin, kout, vout, n, shift, offs, ntiles);
+}
+
+void launch_mark_heads(const uint64_t *keys, uint64_t n, uint64_t *flag, hipStream_t st) {
+    k_mark_heads<<<flat_grid(n, 8192), WG, 0, st>>>(keys, n, flag);
+}
+
+void launch_emit_keys(const uint64_t *keys, uint64_t n, const uint64_t *flag, const uint64_t *kid,
+                      uint64_t *uk_h, uint64_t *uk_off, hipStream_t st) {
+    k_emit_keys<<<flat_grid(n, 8192), WG, 0, st>>>(keys, n, flag, kid, uk_h, uk_off);
+}
+
+void launch_fill_hits(const uint32_t *vals, uint64_t n, const int32_t *t_r, const uint8_t *t_s,
+                      const int64_t *kstart, int32_t nseg, uint32_t *hit_g, uint32_t *hit_r,
+                      uint32_t *hit_seg, uint64_t *sbits, hipStream_t st) {
+    k_fill_hits<<<flat_grid(n, 8192), WG, 0, st>>>(vals, n, t_r, t_s, kstart, nseg, hit_g, hit_r, hit_seg,
+                                                    sbits);
+}
+
+void launch_narrow_prefix(const uint64_t *in, uint64_t n, uint32_t *out, hipStream_t st) {
+    k_narrow_prefix<<<flat_grid(n + 1, 8192), WG, 0, st>>>(in, n, out);
+}
+
+void launch_shard_count(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk, int32_t max_matches,
+                        unsigned long long *shard_n, unsigned long long *stats, hipStream_t st) {
+    k_shard_count<<<flat_grid(nk, 4096), WG, 0, st>>>(uk_h, uk_off, nk, max_matches, shard_n, stats);
+}
+
+void launch_insert(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk, int32_t max_matches,
+                   uint4 *table, ShardGeom geom, hipStream_t st) {
+    k_insert<<<flat_grid(nk, 8192), WG, 0, st>>>(uk_h, uk_off, nk, max_matches, table, geom);
+}
+
+static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
+    return (uint32_t)(n < 1 ? 1 : ((uint32_t)n < max_grid ? (uint32_t)n : max_grid));
+}
+
+static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
+static const size_t REFINE_LDS = (size_t)REFINE_CAP * 8;
+
+// One wrapper per per-read stage kernel so the driver can time each launch.
+void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out,
+                  uint64_t flat_n, uint32_t max_grid, hipStream_t st) {
+    const uint32_t gb = read_grid(A.R, max_grid);   // batch-wide per-read kernels
+    const uint32_t gg = read_grid(A.RG, max_grid);  // group per-read kernels
+    switch (stage) {
+    case ST_SORT_KMERS: k_sort_kmers<<<gb, WG, SORT_LDS, st>>>(A); break;
+    case ST_GROUP_SEEDS: k_group_seeds<<<gb, WG, 0, st>>>(A, I); break;
+    case ST_SORT_SEEDS: k_sort_seeds<<<gb, WG, SORT_LDS, st>>>(A); break;
+    case ST_BUDGET: k_budget<<<gb, WG, 0, st>>>(A); break;
+    case ST_TAB_INIT: k_tab_init<<<flat_grid(flat_n, 8192), WG, 0, st>>>(A); break;
+    case ST_CONTRIB: k_contrib<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I); break;
+    case ST_COUNT_BUCKETS: k_count_buckets<<<gg, WG, 0, st>>>(A); break;
+    case ST_EXTRACT: k_extract<<<gg, WG, SORT_LDS, st>>>(A); break;
+    case ST_PRUNE: k_prune<<<gg, WG, 0, st>>>(A, I); break;
+    case ST_REFINE: k_refine<<<gg, WG, REFINE_LDS, st>>>(A, I); break;
+    case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;
+    default: break;
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_lookup(DevIndex I, const uint64_t *hs, uint32_t n, uint32_t *cnt,
+                                               uint32_t *off) {
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) cnt[i] = index_lookup(I, hs[i], off[i]);
+}
+
+void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,
+                   hipStream_t st) {
+    k_lookup<<<flat_grid(n, 1024), WG, 0, st>>>(I, hs, n, cnt, off);
+}
+
+}  // namespace shm

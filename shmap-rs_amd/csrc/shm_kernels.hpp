@@ -1,0 +1,138 @@
+// Kernel argument structures and launch constants shared by the gfx950 kernels
+// and the host driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/shmap_hip.h"
+
+namespace shm {
+
+constexpr int CHUNK_W = 4096;             // windows per wave-chunk when sketching
+constexpr int LANE_W = CHUNK_W / 64;      // windows per lane
+constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
+constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
+constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
+constexpr int32_t REFINE_CAP = 6144;      // int32 entries per LDS array in refine
+constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
+constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
+
+enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
+
+// A set of sequences: sequence i is bases[off[i], off[i+1]).
+struct SeqSet {
+    const uint8_t *bases;
+    const int64_t *off;
+    int32_t n;
+};
+
+struct ShardGeom {
+    uint64_t base[8];
+    uint32_t bits[8];
+};
+
+// Everything the per-read stages read and write.  Arrays indexed by k-mer
+// ("koff") belong to the whole batch; arrays indexed by "tab"/"bk" offsets
+// belong to the current group of reads [r0, r0 + RG).
+struct MapArgs {
+    int32_t R;          // reads in the batch
+    int32_t r0, RG;     // current group
+    int32_t k;
+    double theta, min_diff, max_overlap, thr_min;
+    int32_t metric, no_bucket_pruning, one_sweep, abs_pos;
+    const int64_t *roff;   // R+1 read offsets into the bases
+    const int64_t *kstart; // reference segment k-mer starts (nseg+1)
+    const int64_t *seg_cum;// reference segment base starts (nseg+1)
+    int32_t nseg;
+    // sketch of every read, at koff[rd]
+    const uint64_t *koff;
+    uint64_t *k_h;
+    int32_t *k_r;
+    uint8_t *k_s;
+    uint64_t *ps_h;
+    uint32_t *ps_v;
+    // seeds, hash order, at koff[rd]
+    uint64_t *sd_h;
+    uint32_t *sd_first, *sd_occs, *sd_cnt, *sd_hoff, *sd_ord, *sd_epre;
+    uint8_t *sd_strand;
+    uint64_t *sd_key;
+    // per read (whole batch)
+    int32_t *rd_u, *rd_possible, *rd_S, *rd_hl, *rd_ok, *rd_c, *rd_bseeds, *rd_max_seed,
+        *rd_seed_matches, *rd_nb;
+    uint64_t *rd_E, *rd_cap;
+    // per group read (local index), RG+1 entries
+    const uint64_t *hit_base, *tab_base, *bk_base;
+    // per-read bucket tables (group)
+    uint32_t *tb_key;
+    uint64_t *tb_mc;
+    int32_t *tb_min, *tb_max;
+    // buckets (group), at bk_base
+    uint64_t *bk_key;
+    uint32_t *bk_val, *tmp_g;
+    int32_t *tmp_m, *tmp_c, *tmp_min, *tmp_max;
+    uint32_t *bs_g;
+    int32_t *bs_m, *bs_c, *bs_min, *bs_max;
+    uint8_t *bs_alive;
+    double *bs_sh;
+    double *sc_j;
+    int32_t *sc_i, *sc_same, *sc_tl, *sc_tr, *sc_ssz, *sc_pend, *sc_seg;
+    // scratch
+    int32_t *g_sid;        // per workgroup, sid_stride entries
+    uint64_t sid_stride;
+    int32_t *g_diff, *g_tails; // at koff
+};
+
+}  // namespace shm
+
+namespace shm {
+
+struct DevIndex;
+
+enum Stage {
+    ST_SKETCH_COUNT = 0,
+    ST_SKETCH_WRITE,
+    ST_SORT_KMERS,
+    ST_GROUP_SEEDS,
+    ST_SORT_SEEDS,
+    ST_BUDGET,
+    ST_TAB_INIT,
+    ST_CONTRIB,
+    ST_COUNT_BUCKETS,
+    ST_EXTRACT,
+    ST_PRUNE,
+    ST_REFINE,
+    ST_FINAL,
+    ST_COUNT
+};
+
+// host launch wrappers (shm_kernels.hip)
+void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint, uint64_t *part,
+                 hipStream_t st);
+void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
+size_t sketch_lds(int32_t k);
+void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
+                   uint64_t *chunk_val, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint64_t n_chunks,
+                   uint32_t max_grid, hipStream_t st);
+void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
+                     hipStream_t st);
+void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);
+void launch_radix_pass(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t n,
+                       uint32_t shift, uint64_t *hist, uint64_t *offs, uint64_t *part, hipStream_t st);
+void launch_mark_heads(const uint64_t *keys, uint64_t n, uint64_t *flag, hipStream_t st);
+void launch_emit_keys(const uint64_t *keys, uint64_t n, const uint64_t *flag, const uint64_t *kid,
+                      uint64_t *uk_h, uint64_t *uk_off, hipStream_t st);
+void launch_fill_hits(const uint32_t *vals, uint64_t n, const int32_t *t_r, const uint8_t *t_s,
+                      const int64_t *kstart, int32_t nseg, uint32_t *hit_g, uint32_t *hit_r, uint32_t *hit_seg,
+                      uint64_t *sbits, hipStream_t st);
+void launch_narrow_prefix(const uint64_t *in, uint64_t n, uint32_t *out, hipStream_t st);
+void launch_shard_count(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk, int32_t max_matches,
+                        unsigned long long *shard_n, unsigned long long *stats, hipStream_t st);
+void launch_insert(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk, int32_t max_matches,
+                   uint4 *table, ShardGeom geom, hipStream_t st);
+void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out, uint64_t flat_n,
+                  uint32_t max_grid, hipStream_t st);
+void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,
+                   hipStream_t st);
+
+}  // namespace shm

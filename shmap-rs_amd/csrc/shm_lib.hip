@@ -1,0 +1,916 @@
+// Host side of the gfx950 mapping library: the C ABI of include/shmap_hip.h.
+//
+// Mirrors shmap-rs's library surface for the mapping path -- FracMinHash
+// (src/sketch.rs), SketchIndex (src/index.rs), create_mapper/Mapper
+// (src/mapper.rs) -- with the work done by the kernels of shm_kernels.hip.
+// PAF text follows src/mapping.rs.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/shmap_hip.h"
+#include "shm_device.hpp"
+#include "shm_kernels.hpp"
+
+using namespace shm;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? SHM_ERR_NOMEM : SHM_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                  \
+    } while (0)
+
+#define RCHK(expr)                \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != SHM_OK) return rc_; \
+    } while (0)
+
+extern "C" const char *shm_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= cap && p) return SHM_OK;
+        release();
+        size_t c = n < 16 ? 16 : n + n / 4;
+        hipError_t e = hipMalloc((void **)&p, c * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SHM_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", c * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        cap = c;
+        return SHM_OK;
+    }
+};
+
+// FracMinHash threshold (src/sketch.rs:126-132): (h_frac * u64::MAX as f64) as
+// u64 below 1, where u64::MAX as f64 is 2^64; u64::MAX otherwise.
+static uint64_t hash_threshold(double h_frac) {
+    if (h_frac < 1.0) {
+        const double v = h_frac * 18446744073709551616.0;
+        if (!(v > 0.0)) return 0;
+        if (v >= 18446744073709551616.0) return UINT64_MAX;
+        return (uint64_t)v;
+    }
+    return UINT64_MAX;
+}
+
+static uint32_t device_grid(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+    const uint32_t g = (uint32_t)prop.multiProcessorCount * 8;
+    return g < 64 ? 64 : g;
+}
+
+// ---------------------------------------------------------------------------
+// sketcher
+// ---------------------------------------------------------------------------
+
+struct shm_sketcher {
+    int32_t k;
+    double h_frac;
+    uint64_t thres;
+};
+
+extern "C" int shm_sketcher_create(int32_t k, double h_frac, shm_sketcher **out) {
+    if (!out) return fail(SHM_ERR_ARG, "null output");
+    if (k <= 0) return fail(SHM_ERR_ARG, "K-mer length (-k) should be positive. You provided %d.", k);
+    if (k > 4096) return fail(SHM_ERR_ARG, "K-mer length (-k) above 4096 is not supported.");
+    if (!(h_frac > 0.0) || h_frac > 1.0)
+        return fail(SHM_ERR_ARG, "Given hash ratio (-r) should be between 0 and 1. You provided %g.", h_frac);
+    shm_sketcher *s = new shm_sketcher;
+    s->k = k;
+    s->h_frac = h_frac;
+    s->thres = hash_threshold(h_frac);
+    *out = s;
+    return SHM_OK;
+}
+
+extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
+
+// Sketches a set of sequences already on the device: chunk table on the host,
+// count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
+struct SketchOut {
+    DevBuf<uint64_t> chunk_start, chunk_cnt, chunk_koff, koff, part;
+    std::vector<uint64_t> h_koff;
+};
+
+static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off,
+                         const std::vector<int64_t> &h_off, int32_t n, SketchOut &so, DevBuf<uint64_t> &out_h,
+                         DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s, uint32_t max_grid, hipStream_t st,
+                         hipEvent_t *ev_count, hipEvent_t *ev_write) {
+    std::vector<uint64_t> cs(n + 1);
+    cs[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const int64_t len = h_off[i + 1] - h_off[i];
+        const int64_t nw = len >= sk->k ? len - sk->k + 1 : 0;
+        cs[i + 1] = cs[i] + (uint64_t)((nw + CHUNK_W - 1) / CHUNK_W);
+    }
+    const uint64_t C = cs[n];
+    RCHK(so.chunk_start.ensure(n + 1));
+    RCHK(so.chunk_cnt.ensure(C + 1));
+    RCHK(so.chunk_koff.ensure(C + 1));
+    RCHK(so.koff.ensure(n + 1));
+    RCHK(so.part.ensure(SCAN_PARTS + 1));
+    HIPCHK(hipMemcpyAsync(so.chunk_start.p, cs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+    SeqSet S{d_bases, d_off, n};
+    if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
+    if (C > 0)
+        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_cnt.p, nullptr, nullptr, nullptr, C,
+                      max_grid, st);
+    if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
+    launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
+    launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, st);
+    so.h_koff.resize(n + 1);
+    HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint64_t M = so.h_koff[n];
+    RCHK(out_h.ensure(M + 1));
+    RCHK(out_r.ensure(M + 1));
+    RCHK(out_s.ensure(M + 1));
+    if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
+    if (C > 0)
+        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_koff.p, out_h.p, out_r.p, out_s.p, C,
+                      max_grid, st);
+    if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
+    HIPCHK(hipGetLastError());
+    return SHM_OK;
+}
+
+extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t len, int32_t offset,
+                          shm_kmer *out, int64_t cap, int64_t *n) {
+    if (!sk || (!seq && len > 0) || !n || len < 0) return fail(SHM_ERR_ARG, "invalid argument");
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    DevBuf<uint8_t> bases;
+    DevBuf<int64_t> off;
+    RCHK(bases.ensure(len + 1));
+    RCHK(off.ensure(2));
+    std::vector<int64_t> h_off = {0, len};
+    if (len > 0) HIPCHK(hipMemcpyAsync(bases.p, seq, len, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(off.p, h_off.data(), 16, hipMemcpyHostToDevice, st));
+    SketchOut so;
+    DevBuf<uint64_t> kh;
+    DevBuf<int32_t> kr;
+    DevBuf<uint8_t> ks;
+    // (s.len() as RPos) < k yields nothing (src/sketch.rs:170-173)
+    const bool empty = (int32_t)len < sk->k;
+    if (!empty) RCHK(sketch_device(sk, bases.p, off.p, h_off, 1, so, kh, kr, ks, 1024, st, nullptr, nullptr));
+    const int64_t M = empty ? 0 : (int64_t)so.h_koff[1];
+    *n = M;
+    if (M > cap) {
+        (void)hipStreamDestroy(st);
+        return fail(SHM_ERR_CAPACITY, "sketch has %lld k-mers, buffer holds %lld", (long long)M, (long long)cap);
+    }
+    if (M > 0) {
+        std::vector<uint64_t> h(M);
+        std::vector<int32_t> r(M);
+        std::vector<uint8_t> s(M);
+        HIPCHK(hipMemcpyAsync(h.data(), kh.p, M * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(r.data(), kr.p, M * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(s.data(), ks.p, M, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < M; i++) {
+            out[i].h = h[i];
+            out[i].r = r[i] + offset;
+            out[i].strand = s[i];
+            out[i].pad[0] = out[i].pad[1] = out[i].pad[2] = 0;
+        }
+    }
+    HIPCHK(hipStreamDestroy(st));
+    return SHM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// index
+// ---------------------------------------------------------------------------
+
+struct shm_index {
+    int device = 0;
+    int32_t k = 0;
+    double h_frac = 0;
+    int32_t nseg = 0;
+    std::vector<std::string> names;
+    std::vector<int32_t> sz;
+    std::vector<int64_t> h_kstart, h_segcum;
+    int64_t n_kmers = 0;
+    int64_t indexed_kmers = 0, indexed_hits = 0, highest = 0, black_kmers = 0, black_hits = 0;
+    DevBuf<uint64_t> t_h;
+    DevBuf<int32_t> t_r;
+    DevBuf<uint8_t> t_s;
+    DevBuf<int64_t> kstart, segcum;
+    DevBuf<uint32_t> hit_g, hit_r, hit_seg, strand_pre;
+    DevBuf<uint4> table;
+    ShardGeom geom;
+    DevIndex view() const {
+        DevIndex I;
+        I.t_h = t_h.p;
+        I.t_r = t_r.p;
+        I.t_s = t_s.p;
+        I.kstart = kstart.p;
+        I.nseg = nseg;
+        I.n_kmers = n_kmers;
+        I.hit_g = hit_g.p;
+        I.hit_r = hit_r.p;
+        I.hit_seg = hit_seg.p;
+        I.strand_pre = strand_pre.p;
+        I.table = table.p;
+        for (int i = 0; i < 8; i++) {
+            I.shard_base[i] = geom.base[i];
+            I.shard_bits[i] = geom.bits[i];
+        }
+        return I;
+    }
+};
+
+// SketchIndex::build_index (src/index.rs:384-663), on the device: sketch every
+// segment, radix-sort (hash, k-mer id), run-length encode, and insert each kept
+// hash into its shard's open-addressing table.
+extern "C" int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, int32_t n_segments,
+                               int32_t max_matches, int32_t device_id, shm_index **out) {
+    if (!sk || !out || n_segments < 0 || (n_segments > 0 && !segs)) return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(device_id));
+    shm_index *idx = new shm_index;
+    idx->device = device_id;
+    idx->k = sk->k;
+    idx->h_frac = sk->h_frac;
+    idx->nseg = n_segments;
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    const uint32_t max_grid = device_grid(device_id);
+
+    std::vector<int64_t> h_off(n_segments + 1, 0);
+    for (int32_t s = 0; s < n_segments; s++) {
+        idx->names.push_back(segs[s].name ? segs[s].name : "");
+        idx->sz.push_back((int32_t)segs[s].len);
+        h_off[s + 1] = h_off[s] + segs[s].len;
+    }
+    idx->h_segcum = h_off;
+    const int64_t total_bases = h_off[n_segments];
+    DevBuf<uint8_t> bases;
+    DevBuf<int64_t> d_off;
+    int rc = SHM_OK;
+    auto cleanup = [&](int code) {
+        (void)hipStreamDestroy(st);
+        if (code != SHM_OK) delete idx;
+        return code;
+    };
+    if ((rc = bases.ensure(total_bases + 64)) != SHM_OK) return cleanup(rc);
+    if ((rc = d_off.ensure(n_segments + 1)) != SHM_OK) return cleanup(rc);
+    for (int32_t s = 0; s < n_segments; s++)
+        if (segs[s].len > 0 &&
+            hipMemcpyAsync(bases.p + h_off[s], segs[s].seq, segs[s].len, hipMemcpyHostToDevice, st) != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "copying segment %d to the device failed", s));
+    if (hipMemcpyAsync(d_off.p, h_off.data(), (n_segments + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+        return cleanup(fail(SHM_ERR_HIP, "copy failed"));
+
+    // 1. sketch every segment (segments shorter than k contribute nothing)
+    SketchOut so;
+    if ((rc = sketch_device(sk, bases.p, d_off.p, h_off, n_segments, so, idx->t_h, idx->t_r, idx->t_s, max_grid, st,
+                            nullptr, nullptr)) != SHM_OK)
+        return cleanup(rc);
+    const uint64_t N = so.h_koff[n_segments];
+    idx->n_kmers = (int64_t)N;
+    idx->h_kstart.assign(so.h_koff.begin(), so.h_koff.end());
+    if ((rc = idx->kstart.ensure(n_segments + 1)) != SHM_OK) return cleanup(rc);
+    if ((rc = idx->segcum.ensure(n_segments + 1)) != SHM_OK) return cleanup(rc);
+    if (hipMemcpyAsync(idx->kstart.p, idx->h_kstart.data(), (n_segments + 1) * 8, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(idx->segcum.p, idx->h_segcum.data(), (n_segments + 1) * 8, hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+        return cleanup(fail(SHM_ERR_HIP, "copy failed"));
+    bases.release();
+    if (N >= 0xffffffffull) return cleanup(fail(SHM_ERR_CAPACITY, "more than 2^32-1 indexed k-mers"));
+
+    // 2. stable radix sort of (hash, k-mer id) by hash
+    {
+        DevBuf<uint64_t> k0, k1, hist, offs, part;
+        DevBuf<uint32_t> v0, v1;
+        if ((rc = k0.ensure(N + 1)) || (rc = k1.ensure(N + 1)) || (rc = v0.ensure(N + 1)) ||
+            (rc = v1.ensure(N + 1)) || (rc = part.ensure(SCAN_PARTS + 1)))
+            return cleanup(rc);
+        const uint64_t ntiles = (N + RS_TILE - 1) / RS_TILE;
+        if ((rc = hist.ensure(ntiles * 256 + 1)) || (rc = offs.ensure(ntiles * 256 + 1))) return cleanup(rc);
+        if (N > 0) launch_fill_pairs(idx->t_h.p, N, k0.p, v0.p, st);
+        const uint64_t thres = sk->thres;
+        const int key_bits = thres == 0 ? 1 : 64 - __builtin_clzll(thres);
+        uint64_t *ka = k0.p, *kb = k1.p;
+        uint32_t *va = v0.p, *vb = v1.p;
+        for (int shift = 0; shift < key_bits && N > 1; shift += 8) {
+            launch_radix_pass(ka, va, kb, vb, N, (uint32_t)shift, hist.p, offs.p, part.p, st);
+            std::swap(ka, kb);
+            std::swap(va, vb);
+        }
+        // 3. run-length encode: key ids, per-key offsets
+        DevBuf<uint64_t> flag, kid, uk_h, uk_off;
+        if ((rc = flag.ensure(N + 1)) || (rc = kid.ensure(N + 1))) return cleanup(rc);
+        if (N > 0) launch_mark_heads(ka, N, flag.p, st);
+        launch_scan(flag.p, kid.p, nullptr, N, part.p, st);
+        uint64_t U = 0;
+        if (hipMemcpyAsync(&U, kid.p + N, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "sync failed"));
+        if ((rc = uk_h.ensure(U + 1)) || (rc = uk_off.ensure(U + 1))) return cleanup(rc);
+        if (N > 0) launch_emit_keys(ka, N, flag.p, kid.p, uk_h.p, uk_off.p, st);
+        else if (hipMemsetAsync(uk_off.p, 0, 8, st) != hipSuccess) return cleanup(fail(SHM_ERR_HIP, "memset"));
+        // 4. hit arrays in (hash, segm_id, r) order and the strand prefix
+        if ((rc = idx->hit_g.ensure(N + 1)) || (rc = idx->hit_r.ensure(N + 1)) || (rc = idx->hit_seg.ensure(N + 1)) ||
+            (rc = idx->strand_pre.ensure(N + 1)))
+            return cleanup(rc);
+        DevBuf<uint64_t> sbits, spre;
+        if ((rc = sbits.ensure(N + 1)) || (rc = spre.ensure(N + 1))) return cleanup(rc);
+        if (N > 0)
+            launch_fill_hits(va, N, idx->t_r.p, idx->t_s.p, idx->kstart.p, n_segments, idx->hit_g.p, idx->hit_r.p,
+                             idx->hit_seg.p, sbits.p, st);
+        launch_scan(sbits.p, spre.p, nullptr, N, part.p, st);
+        launch_narrow_prefix(spre.p, N, idx->strand_pre.p, st);
+        // 5. shards: count kept keys per shard (and -M statistics)
+        DevBuf<unsigned long long> shard_n, stats;
+        if ((rc = shard_n.ensure(8)) || (rc = stats.ensure(4))) return cleanup(rc);
+        if (hipMemsetAsync(shard_n.p, 0, 8 * 8, st) != hipSuccess || hipMemsetAsync(stats.p, 0, 4 * 8, st) != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "memset"));
+        if (U > 0) launch_shard_count(uk_h.p, uk_off.p, U, max_matches, shard_n.p, stats.p, st);
+        unsigned long long hn[8], hs[4];
+        if (hipMemcpyAsync(hn, shard_n.p, 64, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hs, stats.p, 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "sync failed"));
+        uint64_t tot = 0;
+        for (int s = 0; s < 8; s++) {
+            uint32_t bits = 6;
+            while ((1ull << bits) < 2ull * hn[s] + 1) bits++;
+            idx->geom.bits[s] = bits;
+            idx->geom.base[s] = tot;
+            tot += 1ull << bits;
+        }
+        if ((rc = idx->table.ensure(tot)) != SHM_OK) return cleanup(rc);
+        if (hipMemsetAsync(idx->table.p, 0, tot * sizeof(uint4), st) != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "memset"));
+        if (U > 0) launch_insert(uk_h.p, uk_off.p, U, max_matches, idx->table.p, idx->geom, st);
+        if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
+            return cleanup(fail(SHM_ERR_HIP, "index build kernels failed"));
+        // get_kmer_stats + erase_frequent_kmers counters (src/index.rs:343-376)
+        idx->indexed_kmers = (int64_t)U;
+        idx->indexed_hits = (int64_t)N;
+        idx->highest = (int64_t)hs[2];
+        idx->black_kmers = (int64_t)hs[0];
+        idx->black_hits = (int64_t)hs[1];
+    }
+    *out = idx;
+    (void)hipStreamDestroy(st);
+    return SHM_OK;
+}
+
+extern "C" void shm_index_free(shm_index *idx) { delete idx; }
+
+static int lookup_one(const shm_index *idx, uint64_t h, uint32_t *cnt, uint32_t *off) {
+    HIPCHK(hipSetDevice(idx->device));
+    DevBuf<uint64_t> dh;
+    DevBuf<uint32_t> dc, dofs;
+    RCHK(dh.ensure(1));
+    RCHK(dc.ensure(1));
+    RCHK(dofs.ensure(1));
+    HIPCHK(hipMemcpy(dh.p, &h, 8, hipMemcpyHostToDevice));
+    launch_lookup(idx->view(), dh.p, 1, dc.p, dofs.p, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(cnt, dc.p, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(off, dofs.p, 4, hipMemcpyDeviceToHost));
+    return SHM_OK;
+}
+
+extern "C" int shm_index_count(const shm_index *idx, uint64_t h, int32_t *count) {
+    if (!idx || !count) return fail(SHM_ERR_ARG, "invalid argument");
+    uint32_t c, o;
+    RCHK(lookup_one(idx, h, &c, &o));
+    *count = (int32_t)c;
+    return SHM_OK;
+}
+
+extern "C" int shm_index_hits(const shm_index *idx, uint64_t h, shm_hit *out, int32_t cap, int32_t *n) {
+    if (!idx || !n) return fail(SHM_ERR_ARG, "invalid argument");
+    uint32_t c, o;
+    RCHK(lookup_one(idx, h, &c, &o));
+    *n = (int32_t)c;
+    if ((int32_t)c > cap) return fail(SHM_ERR_CAPACITY, "%u hits, buffer holds %d", c, cap);
+    if (c == 0) return SHM_OK;
+    std::vector<uint32_t> g(c), r(c), s(c);
+    HIPCHK(hipMemcpy(g.data(), idx->hit_g.p + o, c * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(r.data(), idx->hit_r.p + o, c * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(s.data(), idx->hit_seg.p + o, c * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < c; i++) {
+        out[i].r = (int32_t)(r[i] & 0x7fffffffu);
+        out[i].strand = (int32_t)(r[i] >> 31);
+        out[i].segm_id = (int32_t)s[i];
+        out[i].tpos = (int32_t)((int64_t)g[i] - idx->h_kstart[s[i]]);
+    }
+    return SHM_OK;
+}
+
+extern "C" int shm_index_n_segments(const shm_index *idx, int32_t *n) {
+    if (!idx || !n) return fail(SHM_ERR_ARG, "invalid argument");
+    *n = idx->nseg;
+    return SHM_OK;
+}
+
+extern "C" int shm_index_segment(const shm_index *idx, int32_t s, const char **name, int32_t *sz,
+                                 int64_t *n_kmers) {
+    if (!idx || s < 0 || s >= idx->nseg) return fail(SHM_ERR_ARG, "segment id out of range");
+    if (name) *name = idx->names[s].c_str();
+    if (sz) *sz = idx->sz[s];
+    if (n_kmers) *n_kmers = idx->h_kstart[s + 1] - idx->h_kstart[s];
+    return SHM_OK;
+}
+
+extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, int64_t *hf, int64_t *bk,
+                               int64_t *bh) {
+    if (!idx) return fail(SHM_ERR_ARG, "invalid argument");
+    if (ik) *ik = idx->indexed_kmers;
+    if (ih) *ih = idx->indexed_hits;
+    if (hf) *hf = idx->highest;
+    if (bk) *bk = idx->black_kmers;
+    if (bh) *bh = idx->black_hits;
+    return SHM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mapper
+// ---------------------------------------------------------------------------
+
+static const char *kStageNames[ST_COUNT] = {"sketch_count", "sketch_write", "sort_kmers", "group_seeds",
+                                            "sort_seeds",   "budget",       "tab_init",   "contrib",
+                                            "count_buckets", "extract",     "prune",      "refine",
+                                            "final"};
+
+struct shm_mapper {
+    const shm_index *idx = nullptr;
+    shm_map_params prm{};
+    shm_sketcher sk{};
+    hipStream_t st = nullptr;
+    uint32_t max_grid = 1024;
+    uint64_t tab_budget = 1ull << 27;  // table slots per group (20 B each)
+    bool timing = false;
+    hipEvent_t ev[ST_COUNT][2];
+    double stage_ms[ST_COUNT] = {};
+    double total_ms = 0;
+    // batch buffers
+    DevBuf<uint8_t> bases;
+    DevBuf<int64_t> roff;
+    SketchOut so;
+    DevBuf<uint64_t> k_h, ps_h, sd_h, sd_key;
+    DevBuf<int32_t> k_r;
+    DevBuf<uint8_t> k_s, sd_strand;
+    DevBuf<uint32_t> ps_v, sd_first, sd_occs, sd_cnt, sd_hoff, sd_ord, sd_epre;
+    DevBuf<int32_t> rd_i32;  // 10 per-read int arrays
+    DevBuf<uint64_t> rd_u64; // 2 per-read u64 arrays
+    DevBuf<uint64_t> grp_base; // 3 x (RG+1)
+    DevBuf<uint32_t> tb_key;
+    DevBuf<uint64_t> tb_mc;
+    DevBuf<int32_t> tb_min, tb_max;
+    DevBuf<uint64_t> bk_key;
+    DevBuf<uint32_t> bk_val, tmp_g, bs_g;
+    DevBuf<int32_t> bk_i32;  // tmp_m,c,min,max + bs_m,c,min,max + sc_* (7)
+    DevBuf<uint8_t> bs_alive;
+    DevBuf<double> bs_sh, sc_j;
+    DevBuf<int32_t> g_sid, g_diff, g_tails;
+    DevBuf<shm_read_result> d_out;
+};
+
+extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, shm_mapper **out) {
+    if (!idx || !p || !out) return fail(SHM_ERR_ARG, "invalid argument");
+    if (p->k != idx->k || p->h_frac != idx->h_frac)
+        return fail(SHM_ERR_ARG, "mapper k/h_frac must match the index (k=%d r=%g)", idx->k, idx->h_frac);
+    if (p->theta < 0.0 || p->theta > 1.0)
+        return fail(SHM_ERR_ARG, "The threshold (-t) should be between 0 and 1. You provided %g.", p->theta);
+    if (p->min_diff < 0.0)
+        return fail(SHM_ERR_ARG, "The minimum difference (-d) should be >= 0. You provided %g.", p->min_diff);
+    if (p->max_overlap < 0.0 || p->max_overlap > 1.0)
+        return fail(SHM_ERR_ARG, "The maximum overlap (-o) should be between 0 and 1. You provided %g.",
+                    p->max_overlap);
+    if (p->metric < 0 || p->metric > 3) return fail(SHM_ERR_ARG, "invalid metric %d", p->metric);
+    HIPCHK(hipSetDevice(idx->device));
+    shm_mapper *mp = new shm_mapper;
+    mp->idx = idx;
+    mp->prm = *p;
+    mp->sk.k = p->k;
+    mp->sk.h_frac = p->h_frac;
+    mp->sk.thres = hash_threshold(p->h_frac);
+    mp->max_grid = device_grid(idx->device);
+    if (const char *e = getenv("SHM_TAB_BUDGET")) mp->tab_budget = strtoull(e, nullptr, 10);
+    if (hipStreamCreate(&mp->st) != hipSuccess) {
+        delete mp;
+        return fail(SHM_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (int s = 0; s < ST_COUNT; s++)
+        for (int j = 0; j < 2; j++) (void)hipEventCreate(&mp->ev[s][j]);
+    *out = mp;
+    return SHM_OK;
+}
+
+extern "C" void shm_mapper_free(shm_mapper *mp) {
+    if (!mp) return;
+    for (int s = 0; s < ST_COUNT; s++)
+        for (int j = 0; j < 2; j++) (void)hipEventDestroy(mp->ev[s][j]);
+    (void)hipStreamDestroy(mp->st);
+    delete mp;
+}
+
+extern "C" int shm_mapper_set_timing(shm_mapper *mp, int32_t on) {
+    if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
+    mp->timing = on != 0;
+    return SHM_OK;
+}
+
+extern "C" const char *shm_mapper_stage_name(int32_t s) {
+    return (s >= 0 && s < ST_COUNT) ? kStageNames[s] : "";
+}
+
+extern "C" int shm_mapper_last_timing(const shm_mapper *mp, double *total_ms, double *stage_ms, int32_t n,
+                                      int32_t *n_written) {
+    if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
+    if (total_ms) *total_ms = mp->total_ms;
+    int32_t w = 0;
+    for (; w < n && w < ST_COUNT; w++) stage_ms[w] = mp->stage_ms[w];
+    if (n_written) *n_written = w;
+    return SHM_OK;
+}
+
+// src/mapping.rs:341-361, 370-408 -- second best, MAPQ, printed J2 and the
+// match_inefficiency of the reported mapping.
+static void finalize_result(const shm_map_params &p, shm_read_result &r) {
+    if (!r.mapped) return;
+    const double theta2 = p.theta - p.min_diff;
+    double j2 = -1.0, sig = -1.0;
+    int32_t i2 = -1;
+    if (r.has_second) {
+        i2 = r.intersection2;
+        j2 = r.J2;
+        const double x = i2 == -1 ? 0.0 : (double)i2;
+        const double y = r.intersection == -1 ? 0.0 : (double)r.intersection;
+        sig = fabs(x - y) / sqrt(x + y);
+    } else {
+        r.b2_segm = r.b2_b = -1;
+    }
+    if (j2 < theta2) j2 = theta2;
+    const double frac = 1.0 - j2 / r.J;
+    const int32_t strand5 = abs(r.same_strand) < r.intersection / 2 ? 5 : 60;
+    r.mapq = frac > p.min_diff ? (60 < strand5 ? 60 : strand5) : 0;
+    r.intersection2 = i2;
+    r.J2 = j2;
+    r.sigmas = sig;
+    r.match_inefficiency = (double)r.total_matches / (double)r.intersection;
+}
+
+#define TIME_BEGIN(s) \
+    if (mp->timing) HIPCHK(hipEventRecord(mp->ev[s][0], st))
+#define TIME_END(s)                                   \
+    if (mp->timing) {                                 \
+        HIPCHK(hipEventRecord(mp->ev[s][1], st));     \
+        pending.push_back(s);                         \
+    }
+
+static int flush_timing(shm_mapper *mp, std::vector<int> &pending) {
+    if (!mp->timing) return SHM_OK;
+    HIPCHK(hipStreamSynchronize(mp->st));
+    for (int s : pending) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, mp->ev[s][0], mp->ev[s][1]));
+        mp->stage_ms[s] += ms;
+    }
+    pending.clear();
+    return SHM_OK;
+}
+
+// Maps R reads whose bases are on the device.  h_off are the host copies of
+// the read offsets.
+static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, const std::vector<int64_t> &h_off,
+                      int32_t R, shm_read_result *out) {
+    hipStream_t st = mp->st;
+    const shm_index *ix = mp->idx;
+    const DevIndex I = ix->view();
+    for (int s = 0; s < ST_COUNT; s++) mp->stage_ms[s] = 0;
+    std::vector<int> pending;
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventRecord(t0, st));
+    if (R == 0) {
+        mp->total_ms = 0;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        return SHM_OK;
+    }
+
+    // sketch
+    RCHK(sketch_device(&mp->sk, d_seq, d_off, h_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
+                       mp->timing ? mp->ev[ST_SKETCH_COUNT] : nullptr, mp->timing ? mp->ev[ST_SKETCH_WRITE] : nullptr));
+    if (mp->timing) {
+        pending.push_back(ST_SKETCH_COUNT);
+        pending.push_back(ST_SKETCH_WRITE);
+    }
+    const uint64_t M = mp->so.h_koff[R];
+    RCHK(mp->ps_h.ensure(M + 1));
+    RCHK(mp->ps_v.ensure(M + 1));
+    RCHK(mp->sd_h.ensure(M + 1));
+    RCHK(mp->sd_key.ensure(M + 1));
+    RCHK(mp->sd_strand.ensure(M + 1));
+    RCHK(mp->sd_first.ensure(M + 1));
+    RCHK(mp->sd_occs.ensure(M + 1));
+    RCHK(mp->sd_cnt.ensure(M + 1));
+    RCHK(mp->sd_hoff.ensure(M + 1));
+    RCHK(mp->sd_ord.ensure(M + 1));
+    RCHK(mp->sd_epre.ensure(M + 1));
+    RCHK(mp->g_diff.ensure(M + 1));
+    RCHK(mp->g_tails.ensure(M + 1));
+    RCHK(mp->rd_i32.ensure((size_t)R * 10 + 1));
+    RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
+    RCHK(mp->d_out.ensure(R));
+
+    MapArgs A{};
+    A.R = R;
+    A.k = mp->prm.k;
+    A.theta = mp->prm.theta;
+    A.min_diff = mp->prm.min_diff;
+    A.max_overlap = mp->prm.max_overlap;
+    A.thr_min = mp->prm.theta * (1.0 - mp->prm.min_diff);
+    A.metric = mp->prm.metric;
+    A.no_bucket_pruning = mp->prm.no_bucket_pruning;
+    A.one_sweep = mp->prm.one_sweep;
+    A.abs_pos = mp->prm.abs_pos;
+    A.roff = d_off;
+    A.kstart = ix->kstart.p;
+    A.seg_cum = ix->segcum.p;
+    A.nseg = ix->nseg;
+    A.koff = mp->so.koff.p;
+    A.k_h = mp->k_h.p;
+    A.k_r = mp->k_r.p;
+    A.k_s = mp->k_s.p;
+    A.ps_h = mp->ps_h.p;
+    A.ps_v = mp->ps_v.p;
+    A.sd_h = mp->sd_h.p;
+    A.sd_first = mp->sd_first.p;
+    A.sd_occs = mp->sd_occs.p;
+    A.sd_cnt = mp->sd_cnt.p;
+    A.sd_hoff = mp->sd_hoff.p;
+    A.sd_ord = mp->sd_ord.p;
+    A.sd_epre = mp->sd_epre.p;
+    A.sd_strand = mp->sd_strand.p;
+    A.sd_key = mp->sd_key.p;
+    int32_t *ri = mp->rd_i32.p;
+    A.rd_u = ri + 0 * (size_t)R;
+    A.rd_possible = ri + 1 * (size_t)R;
+    A.rd_S = ri + 2 * (size_t)R;
+    A.rd_hl = ri + 3 * (size_t)R;
+    A.rd_ok = ri + 4 * (size_t)R;
+    A.rd_c = ri + 5 * (size_t)R;
+    A.rd_bseeds = ri + 6 * (size_t)R;
+    A.rd_max_seed = ri + 7 * (size_t)R;
+    A.rd_seed_matches = ri + 8 * (size_t)R;
+    A.rd_nb = ri + 9 * (size_t)R;
+    A.rd_E = mp->rd_u64.p;
+    A.rd_cap = mp->rd_u64.p + R;
+    A.g_diff = mp->g_diff.p;
+    A.g_tails = mp->g_tails.p;
+    A.r0 = 0;
+    A.RG = R;
+
+    for (Stage s : {ST_SORT_KMERS, ST_GROUP_SEEDS, ST_SORT_SEEDS, ST_BUDGET}) {
+        TIME_BEGIN(s);
+        launch_stage(s, A, I, nullptr, 0, mp->max_grid, st);
+        TIME_END(s);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<uint64_t> E(R), cap(R);
+    std::vector<int32_t> hl(R);
+    HIPCHK(hipMemcpyAsync(E.data(), A.rd_E, R * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cap.data(), A.rd_cap, R * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hl.data(), A.rd_hl, R * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    RCHK(flush_timing(mp, pending));
+
+    // bucketing, in groups of reads whose tables fit the budget
+    int32_t r0 = 0;
+    while (r0 < R) {
+        int32_t r1 = r0;
+        uint64_t tsum = 0;
+        while (r1 < R && (r1 == r0 || tsum + cap[r1] <= mp->tab_budget)) tsum += cap[r1++];
+        const int32_t RG = r1 - r0;
+        std::vector<uint64_t> hb(RG + 1), tb(RG + 1);
+        hb[0] = tb[0] = 0;
+        int32_t max_hl = 0;
+        for (int32_t i = 0; i < RG; i++) {
+            hb[i + 1] = hb[i] + E[r0 + i];
+            tb[i + 1] = tb[i] + cap[r0 + i];
+            max_hl = std::max(max_hl, hl[r0 + i]);
+        }
+        RCHK(mp->grp_base.ensure(3 * (size_t)(RG + 1)));
+        HIPCHK(hipMemcpyAsync(mp->grp_base.p, hb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(mp->grp_base.p + (RG + 1), tb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
+        const uint64_t T = tb[RG];
+        RCHK(mp->tb_key.ensure(T + 1));
+        RCHK(mp->tb_mc.ensure(T + 1));
+        RCHK(mp->tb_min.ensure(T + 1));
+        RCHK(mp->tb_max.ensure(T + 1));
+        A.r0 = r0;
+        A.RG = RG;
+        A.hit_base = mp->grp_base.p;
+        A.tab_base = mp->grp_base.p + (RG + 1);
+        A.bk_base = mp->grp_base.p + 2 * (size_t)(RG + 1);
+        A.tb_key = mp->tb_key.p;
+        A.tb_mc = mp->tb_mc.p;
+        A.tb_min = mp->tb_min.p;
+        A.tb_max = mp->tb_max.p;
+        TIME_BEGIN(ST_TAB_INIT);
+        launch_stage(ST_TAB_INIT, A, I, nullptr, T, mp->max_grid, st);
+        TIME_END(ST_TAB_INIT);
+        TIME_BEGIN(ST_CONTRIB);
+        launch_stage(ST_CONTRIB, A, I, nullptr, hb[RG], mp->max_grid, st);
+        TIME_END(ST_CONTRIB);
+        TIME_BEGIN(ST_COUNT_BUCKETS);
+        launch_stage(ST_COUNT_BUCKETS, A, I, nullptr, 0, mp->max_grid, st);
+        TIME_END(ST_COUNT_BUCKETS);
+        HIPCHK(hipGetLastError());
+        std::vector<int32_t> nb(RG);
+        HIPCHK(hipMemcpyAsync(nb.data(), A.rd_nb + r0, RG * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        RCHK(flush_timing(mp, pending));
+        std::vector<uint64_t> bb(RG + 1);
+        bb[0] = 0;
+        for (int32_t i = 0; i < RG; i++) bb[i + 1] = bb[i] + (uint64_t)nb[i];
+        const uint64_t B = bb[RG];
+        HIPCHK(hipMemcpyAsync(mp->grp_base.p + 2 * (size_t)(RG + 1), bb.data(), (RG + 1) * 8, hipMemcpyHostToDevice,
+                              st));
+        RCHK(mp->bk_key.ensure(B + 1));
+        RCHK(mp->bk_val.ensure(B + 1));
+        RCHK(mp->tmp_g.ensure(B + 1));
+        RCHK(mp->bs_g.ensure(B + 1));
+        RCHK(mp->bk_i32.ensure(15 * (B + 1)));
+        RCHK(mp->bs_alive.ensure(B + 1));
+        RCHK(mp->bs_sh.ensure(B + 1));
+        RCHK(mp->sc_j.ensure(B + 1));
+        const uint64_t stride = 2ull * (uint64_t)max_hl + 2;
+        RCHK(mp->g_sid.ensure(stride * mp->max_grid + 1));
+        A.bk_key = mp->bk_key.p;
+        A.bk_val = mp->bk_val.p;
+        A.tmp_g = mp->tmp_g.p;
+        A.bs_g = mp->bs_g.p;
+        int32_t *bi = mp->bk_i32.p;
+        const size_t Bs = B + 1;
+        A.tmp_m = bi + 0 * Bs;
+        A.tmp_c = bi + 1 * Bs;
+        A.tmp_min = bi + 2 * Bs;
+        A.tmp_max = bi + 3 * Bs;
+        A.bs_m = bi + 4 * Bs;
+        A.bs_c = bi + 5 * Bs;
+        A.bs_min = bi + 6 * Bs;
+        A.bs_max = bi + 7 * Bs;
+        A.sc_i = bi + 8 * Bs;
+        A.sc_same = bi + 9 * Bs;
+        A.sc_tl = bi + 10 * Bs;
+        A.sc_tr = bi + 11 * Bs;
+        A.sc_ssz = bi + 12 * Bs;
+        A.sc_pend = bi + 13 * Bs;
+        A.sc_seg = bi + 14 * Bs;
+        A.bs_alive = mp->bs_alive.p;
+        A.bs_sh = mp->bs_sh.p;
+        A.sc_j = mp->sc_j.p;
+        A.g_sid = mp->g_sid.p;
+        A.sid_stride = stride;
+        for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
+            TIME_BEGIN(s);
+            launch_stage(s, A, I, mp->d_out.p, 0, mp->max_grid, st);
+            TIME_END(s);
+        }
+        HIPCHK(hipGetLastError());
+        r0 = r1;
+    }
+    HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(t1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    RCHK(flush_timing(mp, pending));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    mp->total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    for (int32_t i = 0; i < R; i++) finalize_result(mp->prm, out[i]);
+    return SHM_OK;
+}
+
+extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out) {
+    if (!mp || n < 0 || (n > 0 && (!reads || !out))) return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(mp->idx->device));
+    std::vector<int64_t> off(n + 1, 0);
+    for (int32_t i = 0; i < n; i++) {
+        if (reads[i].len < 0 || (reads[i].len > 0 && !reads[i].seq)) return fail(SHM_ERR_ARG, "read %d invalid", i);
+        off[i + 1] = off[i] + reads[i].len;
+    }
+    RCHK(mp->bases.ensure(off[n] + 64));
+    RCHK(mp->roff.ensure(n + 1));
+    // stage through one pinned-free host copy: reads are separate allocations
+    std::vector<uint8_t> h(off[n] > 0 ? off[n] : 1);
+    for (int32_t i = 0; i < n; i++)
+        if (reads[i].len) memcpy(h.data() + off[i], reads[i].seq, reads[i].len);
+    HIPCHK(hipMemcpyAsync(mp->bases.p, h.data(), off[n], hipMemcpyHostToDevice, mp->st));
+    HIPCHK(hipMemcpyAsync(mp->roff.p, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
+    HIPCHK(hipStreamSynchronize(mp->st));
+    return map_device(mp, mp->bases.p, mp->roff.p, off, n, out);
+}
+
+extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,
+                                    shm_read_result *out) {
+    if (!mp || n < 0 || (n > 0 && (!d_seq || !d_off || !out))) return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(mp->idx->device));
+    std::vector<int64_t> off(n + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), d_off, (n + 1) * 8, hipMemcpyDeviceToHost, mp->st));
+    HIPCHK(hipStreamSynchronize(mp->st));
+    return map_device(mp, d_seq, d_off, off, n, out);
+}
+
+// ---------------------------------------------------------------------------
+// PAF text (src/mapping.rs:87-107, 161-182, 228-247)
+// ---------------------------------------------------------------------------
+
+// Rust's {:.N} on f64 prints the exact binary value rounded half to even, as
+// glibc printf does in the default rounding mode, but spells NaN and the
+// infinities "NaN", "inf" and "-inf".
+static void rust_fixed(char *buf, size_t cap, double v, int prec) {
+    if (isnan(v)) snprintf(buf, cap, "NaN");
+    else if (isinf(v)) snprintf(buf, cap, v > 0 ? "inf" : "-inf");
+    else snprintf(buf, cap, "%.*f", prec, v);
+}
+
+extern "C" int shm_format_paf(const shm_index *idx, const shm_map_params *p, const char *qid,
+                              const shm_read_result *r, double map_time, char *buf, size_t cap) {
+    if (!idx || !p || !qid || !r || !r->mapped || r->segm_id < 0 || r->segm_id >= idx->nseg) {
+        fail(SHM_ERR_ARG, "not a mapped record");
+        return -1;
+    }
+    char isd[48], j[48], j2[48], jd[48], sh[48], t[48], mi[48];
+    rust_fixed(isd, sizeof isd, r->sigmas, 5);
+    rust_fixed(j, sizeof j, r->J, 5);
+    rust_fixed(j2, sizeof j2, r->J2, 5);
+    rust_fixed(jd, sizeof jd, r->J - r->J2, 5);
+    rust_fixed(sh, sizeof sh, r->sh, 5);
+    rust_fixed(t, sizeof t, map_time, 5);
+    rust_fixed(mi, sizeof mi, r->match_inefficiency, 3);
+    return snprintf(buf, cap,
+                    // MappingPaf: 12 mandatory columns; 10 and 11 repeat the read length
+                    "%s\t%d\t%d\t%d\t%c\t%s\t%d\t%d\t%d\t%d\t%d\t%d"
+                    // LocalMappingStats
+                    "\tp:i:%d\ts:i:%d\tI:i:%d\tI2:i:%d\tIdiff:i:%d\tIsdiff:f:%s\tJ:f:%s\tJ2:f:%s"
+                    "\tJdiff:f:%s\tsh:f:%s\tstrand:i:%d\tt:f:%s\tb:s:(%d,%d)\tb2:s:(%d,%d)"
+                    // GlobalMappingStats
+                    "\tk:i:%d\tgt_J:f:-1.000\tgt_C:f:-1.000\tgt_C_bucket:f:-1.000\tseeds:i:%d"
+                    "\tmax_seed_matches:i:%d\tseed_matches:i:%d\ttotal_matches:i:%d"
+                    "\tmatch_inefficiency:f:%s\tseeded_buckets:i:%d\tfinal_buckets:i:%d\tFPTP:f:-1.000",
+                    qid, r->read_len, r->p_start, r->p_end, (char)r->strand, idx->names[r->segm_id].c_str(),
+                    idx->sz[r->segm_id], r->t_l, r->t_r, r->read_len, r->read_len, r->mapq, r->m, r->s_sz,
+                    r->intersection, r->intersection2, r->intersection - r->intersection2, isd, j, j2, jd, sh,
+                    r->same_strand, t, r->b_segm, r->b_b, r->b2_segm, r->b2_b, p->k, r->S, r->max_seed_matches,
+                    r->seed_matches, r->total_matches, mi, r->seeded_buckets, r->final_buckets);
+}
+
+extern "C" int shm_format_unmapped(const char *qid, int32_t read_len, char *buf, size_t cap) {
+    return snprintf(buf, cap, "%s\t%d\t0\t0\t*\t*\t0\t0\t0\t0\t0\t0", qid ? qid : "", read_len);
+}
