@@ -1,0 +1,301 @@
+#!/usr/bin/env python3
+"""Mapping throughput of the gfx950 per-read mapping path on one MI355X.
+
+Workload (default ``genome``): a 3.1 Gbp synthetic reference of 24 contigs
+(50-250 Mbp) with 8% satellite arrays (171 bp units, 1-3% copy divergence),
+92,000 ONT-like reads (lognormal mean 24 kb / sd 12 kb, 7% errors split
+40/30/30 sub/ins/del, both strands), k=25 r=0.01 t=0.4 -d 0.075 -o 0.3,
+Containment.  The reference index is built on the GPU and stays resident; the
+reads are uploaded to HBM before the timed region.  One step maps every read
+(batched, persistent kernels) and copies the per-read records back.
+
+Also reported, outside the timed region: the plain one-read-per-launch form on
+a sample, per-stage kernel times with the dominant kernel's roofline, and the
+CPU baseline (the C oracle on the host cores, -@1 and all cores) on a bounded
+sample whose records are checked bit-for-bit against the GPU's.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+G = _load("shmap_amd", os.path.join(ROOT, "shmap-rs_amd", "shmap_amd.py"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# workloads (synthetic and seeded)
+# ---------------------------------------------------------------------------
+
+def genome_contigs(total=3_100_000_000, n=24, lo=50_000_000, hi=250_000_000):
+    f = lambda p: np.array([lo + (hi - lo) * (i / (n - 1)) ** p for i in range(n)])
+    a, b = 0.3, 5.0
+    for _ in range(80):
+        mid = (a + b) / 2
+        if f(mid).sum() > total:
+            a = mid
+        else:
+            b = mid
+    lens = np.round(f(mid)).astype(np.int64)
+    lens[n // 2] += total - lens.sum()
+    return lens
+
+
+WORKLOADS = {
+    # 10 Mbp uniform, 2,000 x 10 kb reads, 0.5% substitutions
+    "plain_small": dict(lens=[10_000_000], repeat=0.0, n_reads=2000, length=(10_000, 0), lognormal=False,
+                        clip=(1, 10_000), errors=((0.005, 1.0, 0.0, 0.0),), t=0.4),
+    # 100 Mbp, 20% tandem arrays (171 bp, 2%), 20,000 HiFi-like reads
+    "batched_uneven": dict(lens=[100_000_000], repeat=0.20, div=(0.02, 0.02), n_units=16, n_reads=20_000,
+                           length=(20_000, 8_000), lognormal=True, clip=(1_000, 80_000),
+                           errors=((0.001, 0.6, 0.2, 0.2),), t=0.4),
+    # 3.1 Gbp, 24 contigs, 8% satellite 1-3%, 92,000 ONT-like reads, 7% errors
+    "genome": dict(lens=None, repeat=0.08, div=(0.01, 0.03), n_units=64, n_reads=92_000,
+                   length=(24_000, 12_000), lognormal=True, clip=(1_000, 200_000),
+                   errors=((0.07, 0.4, 0.3, 0.3),), t=0.4),
+}
+
+
+def make_workload(name, seed=1, n_reads=None):
+    w = dict(WORKLOADS[name])
+    lens = genome_contigs() if w["lens"] is None else np.asarray(w["lens"], dtype=np.int64)
+    t0 = time.time()
+    div = w.get("div", (0.02, 0.02))
+    ref = G.synth_reference(seed, lens, repeat_frac=w["repeat"], unit_len=171, div_min=div[0], div_max=div[1],
+                            n_units=w.get("n_units", 16), array_len=200_000)
+    n = n_reads or w["n_reads"]
+    bases, off, truth = G.synth_reads(seed + 1, ref, lens, n, w["length"], lognormal=w["lognormal"], clip=w["clip"],
+                                      errors=w["errors"], both_strands=True)
+    log(f"[bench] workload {name}: ref {lens.sum() / 1e9:.3f} Gbp in {len(lens)} contigs, {n} reads "
+        f"({(off[-1]) / 1e9:.3f} Gbp), generated in {time.time() - t0:.1f}s")
+    return lens, ref, bases, off, truth, w
+
+
+# ---------------------------------------------------------------------------
+# device buffers straight from the HIP runtime (inputs resident in HBM)
+# ---------------------------------------------------------------------------
+
+class Hip:
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+
+    def upload(self, arr: np.ndarray) -> int:
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(arr.nbytes, 64)) == 0
+        assert self.L.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0
+        return p.value
+
+    def free(self, p: int) -> None:
+        self.L.hipFree(C.c_void_p(p))
+
+    def sync(self) -> None:
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+# ---------------------------------------------------------------------------
+# roofline: algorithmic bytes of each stage (documented in DESIGN.md)
+# ---------------------------------------------------------------------------
+
+def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
+    m = res["m"].astype(np.int64).sum()
+    u = res["u"].astype(np.int64).sum()
+    E = res["seed_matches"].astype(np.int64).clip(min=0).sum()
+    return {
+        "sketch_count": bases_total,                       # every base read once
+        "sketch_write": bases_total + 13 * m,              # bases + (h, r, strand) per kept k-mer
+        "sort_kmers": 8 * m + 2 * 12 * m,                  # hashes in, (hash, index) pairs out
+        "group_seeds": 12 * m + 16 * u + 37 * u,           # sorted pairs, one table slot per probe, seed rows
+        "sort_seeds": 2 * 12 * u,
+        "budget": 12 * u,
+        "contrib": 12 * E,                                 # one packed hit record per examined hit
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="genome", choices=sorted(WORKLOADS))
+    ap.add_argument("--metric", default="Containment")
+    ap.add_argument("--reads", type=int, default=None, help="override the read count")
+    ap.add_argument("--plain-reads", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
+              f"of this build: reads shard across GPUs with a replicated index, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    lens, ref, bases, off, truth, w = make_workload(args.workload, n_reads=args.reads)
+    n_reads = len(off) - 1
+    k, h_frac, theta, d, o = 25, 0.01, w["t"], 0.075, 0.3
+    names, at = [], 0
+    segs = []
+    for i, L in enumerate(lens):
+        segs.append((f"chr{i + 1}", ref[at:at + int(L)]))
+        at += int(L)
+
+    hip = Hip()
+    t0 = time.time()
+    sk = G.FracMinHash(k, h_frac)
+    idx = G.SketchIndex(sk, segs, None, device=0)
+    hip.sync()
+    t_index = time.time() - t0
+    st = idx.stats()
+    log(f"[bench] GPU index: {st['indexed_kmers']} kmers, {st['indexed_hits']} hits in {t_index:.2f}s")
+
+    prm = G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
+    mp = G.Mapper(idx, prm)
+    d_bases = hip.upload(bases)
+    d_off = hip.upload(off)
+
+    for i in range(args.warmup):
+        res = mp.map_device(d_bases, d_off, n_reads)
+        log(f"[bench] warmup step {i} done")
+    hip.sync()
+    times = []
+    t_start = time.time()
+    for i in range(args.steps):
+        t1 = time.time()
+        res = mp.map_device(d_bases, d_off, n_reads)
+        times.append(time.time() - t1)
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms")
+    hip.sync()
+    elapsed = time.time() - t_start
+    value = n_reads * args.steps / elapsed
+    n_mapped = int(res["mapped"].sum())
+    gbp = float(off[-1]) / 1e9
+
+    # per-stage kernel times (separate pass: timing adds syncs between stages)
+    mp.set_timing(True)
+    res_t = mp.map_device(d_bases, d_off, n_reads)
+    mp.set_timing(False)
+    total_ms, stage_ms = mp.last_timing()
+    assert np.array_equal(res_t["mapped"], res["mapped"])
+    sb = stage_bytes(res, int(off[-1]))
+    dom = max(stage_ms, key=lambda s: stage_ms[s])
+    dom_bytes = sb.get(dom)
+    roof = {"kernel": dom, "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+            "kernel_ms": round(stage_ms[dom], 4)}
+    if dom_bytes:
+        ach = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+        roof.update(achieved=round(ach, 2), frac=round(ach / HBM_PEAK_GBS, 5), algorithmic_bytes=int(dom_bytes))
+    else:
+        roof.update(achieved=None, frac=None, algorithmic_bytes=None)
+    stage_rates = {s: (round(sb[s] / (stage_ms[s] * 1e-3) / 1e9, 2) if s in sb and stage_ms[s] > 0 else None)
+                   for s in stage_ms}
+    log("[bench] stage ms: " + ", ".join(f"{s}={v:.2f}" for s, v in stage_ms.items()))
+
+    # plain form: one read per launch
+    n_plain = min(args.plain_reads, n_reads)
+    plain_reads = [bases[off[i]:off[i + 1]].tobytes() for i in range(n_plain)]
+    tp = time.time()
+    plain = np.concatenate([mp.map_batch([r]) for r in plain_reads]) if n_plain else res[:0]
+    plain_rate = n_plain / (time.time() - tp) if n_plain else None
+    plain_same = bool(np.array_equal(plain[["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]],
+                                     res[:n_plain][["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]]))
+
+    # latency: every read of a step completes with the step
+    lat = np.array(times) * 1e3
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ffi as O  # the CPU oracle: baseline and checker only
+        ncores = len(os.sched_getaffinity(0))
+        tc = time.time()
+        oidx = O.OracleIndex(k, h_frac, segs, None, threads=ncores)
+        log(f"[bench] oracle index built in {time.time() - tc:.1f}s")
+        op = O.params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
+        # bounded sample: grow until ~cpu_seconds of single-thread work
+        n_s, spent, ores = 64, 0.0, None
+        while True:
+            n_s = min(n_s, n_reads)
+            sample = (bases, off[:n_s + 1])
+            ores, spent = oidx.map_reads_timed(op, sample, threads=1)
+            if spent >= args.cpu_seconds / 4 or n_s == n_reads:
+                break
+            n_s = int(n_s * min(8.0, max(2.0, args.cpu_seconds / 4 / max(spent, 1e-3))))
+        cpu1 = n_s / spent
+        ores_all, spent_all = oidx.map_reads_timed(op, sample, threads=ncores)
+        cpun = n_s / spent_all
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import parity_util  # tests/parity_util.py: field-by-field record diff
+        bad = parity_util.diff_results(ores, res[:n_s])
+        bad_all = parity_util.diff_results(ores_all, res[:n_s])
+        parity = {"reads_compared": n_s, "mismatches": len(bad) + len(bad_all), "first": (bad + bad_all)[:3]}
+        cpu = {"value": round(cpu1, 2), "unit": "reads/s", "cores": 1, "kind": "port",
+               "sample": f"first {n_s} reads of the same workload, oracle -@1 ({spent:.1f}s)",
+               "all_cores": {"value": round(cpun, 2), "cores": ncores}}
+        log(f"[bench] CPU oracle: {cpu1:.1f} reads/s at -@1, {cpun:.1f} reads/s at -@{ncores}; parity {parity}")
+
+    out = {
+        "metric": "mapped reads/s, 3.1 Gbp ref r=0.01 (batched vs plain GPU, bit-exact PAF); p99 ms",
+        "value": round(value, 2),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, shmap-rs_amd/csrc/synth.c)",
+        "config": {"workload": args.workload, "reference_gbp": round(float(lens.sum()) / 1e9, 3),
+                   "contigs": int(len(lens)), "reads": n_reads, "read_gbp": round(gbp, 3), "k": k, "r": h_frac,
+                   "t": theta, "d": d, "o": o, "metric": args.metric, "mode": "batched, device-resident"},
+        "gbp_per_s": round(gbp * args.steps / elapsed, 3),
+        "mapped_reads": n_mapped,
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "plain_gpu": {"reads_per_s": round(plain_rate, 2) if plain_rate else None, "reads": n_plain,
+                      "same_records": plain_same},
+        "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
+        "index_build_s": round(t_index, 3),
+        "stage_ms": {s: round(v, 3) for s, v in stage_ms.items()},
+        "stage_gbs": stage_rates,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity_sample": parity,
+    }
+    if cpu:
+        out["gpu_over_cpu_all_cores"] = round(value / cpu["all_cores"]["value"], 1)
+    hip.free(d_bases)
+    hip.free(d_off)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
@@ -46,8 +47,8 @@ class OkResult(C.Structure):
 
 
 RESULT_DTYPE = np.dtype([(n, np.int32) for n in _I32] + [(n, np.int64) for n in _I64] +
-                        [(n, np.int32) for n in _I32B] + [("pad", np.int32)] + [(n, np.float64) for n in _F64],
-                        align=False)
+                        [(n, np.int32) for n in _I32B] + [(n, np.float64) for n in _F64], align=True)
+assert RESULT_DTYPE.itemsize == C.sizeof(OkResult)
 
 _lib = None
 
@@ -89,6 +90,8 @@ def lib():
         L.ok_calc_mapq.restype = C.c_int32
         L.ok_calc_mapq.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                    C.POINTER(C.c_double)]
+        L.ok_unique_pmatches.restype = C.c_int32
+        L.ok_unique_pmatches.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ok_overlap.restype = C.c_double
         L.ok_overlap.argtypes = [C.c_int32] * 6
         _lib = L
@@ -148,6 +151,10 @@ class OracleIndex:
 
     def map_reads(self, prm: OkParams, reads, threads: int = 1) -> np.ndarray:
         """reads: list of bytes, or (bases uint8, offsets int64)."""
+        return self.map_reads_timed(prm, reads, threads)[0]
+
+    def map_reads_timed(self, prm: OkParams, reads, threads: int = 1):
+        """(records, seconds spent inside ok_map_reads)."""
         if isinstance(reads, tuple):
             bases, off = reads
             n = len(off) - 1
@@ -159,14 +166,12 @@ class OracleIndex:
             keep = [np.frombuffer(s, dtype=np.uint8) if isinstance(s, (bytes, bytearray)) else s for s in reads]
             ptrs = (C.c_void_p * max(n, 1))(*[a.ctypes.data if a.size else None for a in keep])
             lens = (C.c_int32 * max(n, 1))(*[len(a) for a in keep])
-        out = (OkResult * max(n, 1))()
-        lib().ok_map_reads(self._p, C.byref(prm), n, ptrs, lens, out, threads)
+        out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        t0 = time.perf_counter()
+        lib().ok_map_reads(self._p, C.byref(prm), n, ptrs, lens, out.ctypes.data, threads)
+        dt = time.perf_counter() - t0
         del keep
-        res = np.zeros(n, dtype=RESULT_DTYPE)
-        for i in range(n):
-            for f, _ in OkResult._fields_:
-                res[i][f] = getattr(out[i], f)
-        return res
+        return out[:n], dt
 
     def format_paf(self, prm: OkParams, qid: str, rec, map_time: float = 0.0) -> str:
         r = OkResult(*[rec[f].item() for f, _ in OkResult._fields_])

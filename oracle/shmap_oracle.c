@@ -1101,6 +1101,27 @@ static int map_read_ws(ws_t *w, const ok_index *idx, const ok_params *prm, const
     return 0;
 }
 
+/* unique_elements_with_info's grouping (src/shmap/seeding.rs:26-68), exposed
+ * for tests: seeds in hash order, each with its occurrence count and pmatches
+ * (decreasing r) concatenated into pm.  Returns the number of seeds. */
+int32_t ok_unique_pmatches(const ok_kmer *p, int32_t n, uint64_t *hash, int32_t *occs, int32_t *pm) {
+    ok_kmer *q = xmalloc(sizeof(ok_kmer) * (size_t)(n > 0 ? n : 1));
+    memcpy(q, p, sizeof(ok_kmer) * (size_t)n);
+    qsort(q, (size_t)n, sizeof(ok_kmer), cmp_kmer_h_rdesc);
+    int32_t u = 0;
+    for (int32_t i = 0, gs = 0; i < n; i++) {
+        if (i + 1 == n || q[i].h != q[i + 1].h) {
+            hash[u] = q[i].h;
+            occs[u] = i - gs + 1;
+            u++;
+            gs = i + 1;
+        }
+        pm[i] = q[i].r;
+    }
+    free(q);
+    return u;
+}
+
 int ok_map_read(const ok_index *idx, const ok_params *p, const uint8_t *seq, int32_t len,
                 ok_result *out) {
     ws_t w;

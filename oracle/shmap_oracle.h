@@ -131,6 +131,7 @@ double ok_hseed(int32_t p, int32_t seeds, int32_t matches);          /* pruning.
 int32_t ok_lis_len(const int32_t *x, int32_t n);                        /* scoring.rs:133-154 */
 int32_t ok_calc_mapq(double j, double j2, int32_t same_strand, int32_t intersection,
                      double theta2, double min_diff, double *j2_out); /* mapping.rs:341-354 */
+int32_t ok_unique_pmatches(const ok_kmer *p, int32_t n, uint64_t *hash, int32_t *occs, int32_t *pm);
 double ok_overlap(int32_t sa, int32_t la, int32_t ra, int32_t sb, int32_t lb, int32_t rb);
 
 #ifdef __cplusplus

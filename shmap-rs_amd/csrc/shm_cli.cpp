@@ -1,0 +1,554 @@
+// shmap -- command-line front end over the gfx950 mapping library.
+//
+// Keeps the reference CLI (src/main.rs:16-70, src/params.rs:14-263): the same
+// flags, defaults and validation messages, PAF on stdout in input order,
+// <-z>.unmapped.paf for unmapped reads and --per-read-stats rows
+// (src/shmap/mod.rs:280-327, 374-650).  Reads are mapped on the GPU in batches;
+// records are written strictly in input order.
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/shmap_hip.h"
+
+namespace {
+
+struct Params {
+    std::string p_file, t_file, params_file, per_read_stats, profile_log;
+    int32_t k = 15;
+    double h_frac = 0.05, theta = 0.9, min_diff = 0.02, max_overlap = 0.5;
+    double rarity_weight = 0, rarity_tiebreak = 0, rarity_alpha = 0;
+    bool have_max_seeds = false, have_max_matches = false;
+    int32_t max_seeds = 0, max_matches = 0;
+    int32_t metric = SHM_CONTAINMENT;
+    int32_t verbose = 0;
+    bool normalize = false, nbp = false, one_sweep = false, abs_pos = false, profile = false;
+    long long threads = 1;
+    unsigned long long per_read_stats_sample = 1;
+    int device = 0;
+    long batch = 16384;
+};
+
+const char *metric_name(int32_t m) {
+    switch (m) {
+    case SHM_JACCARD: return "Jaccard";
+    case SHM_BUCKET_SH: return "bucket_SH";
+    case SHM_BUCKET_LCS: return "bucket_LCS";
+    default: return "Containment";
+    }
+}
+
+// Rust's Display for f64: the shortest decimal that round-trips, never in
+// exponent form.
+std::string rust_f64(double v) {
+    char buf[64];
+    for (int p = 1; p <= 17; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, v);
+        if (strtod(buf, nullptr) == v) break;
+    }
+    // buf holds d.ddde[+-]xx; expand to plain decimal
+    char mant[32];
+    int ex = 0;
+    bool neg = buf[0] == '-';
+    const char *s = buf + (neg ? 1 : 0);
+    int mi = 0;
+    for (; *s && *s != 'e'; s++)
+        if (*s != '.') mant[mi++] = *s;
+    mant[mi] = 0;
+    if (*s == 'e') ex = atoi(s + 1);
+    while (mi > 1 && mant[mi - 1] == '0') mant[--mi] = 0;
+    std::string out = neg ? "-" : "";
+    const int point = ex + 1;  // digits before the decimal point
+    if (point <= 0) {
+        out += "0.";
+        out += std::string(-point, '0');
+        out += mant;
+    } else if (point >= mi) {
+        out += mant;
+        out += std::string(point - mi, '0');
+    } else {
+        out += std::string(mant, point);
+        out += ".";
+        out += mant + point;
+    }
+    return out;
+}
+
+void usage(FILE *f) {
+    fprintf(f,
+            "Map-SHmap: fast and accurate sketch-based read mapper (MI355X)\n\n"
+            "Usage: shmap -p <P_FILE> -s <T_FILE> [OPTIONS]\n\n"
+            "  -p, --pattern <P_FILE>        Reads file (FASTA format)\n"
+            "  -s, --text <T_FILE>           Reference file (FASTA format)\n"
+            "  -k, --ksize <K>               K-mer length [default: 15]\n"
+            "  -r, --hashratio <H_FRAC>      FracMinHash ratio in (0,1] [default: 0.05]\n"
+            "  -S, --max_seeds <MAX_SEEDS>   Maximum seeds in a sketch (accepted, unused)\n"
+            "  -M, --max_matches <M>         Maximum seed matches in a sketch\n"
+            "  -t, --threshold <THETA>       Homology threshold in [0,1] [default: 0.9]\n"
+            "  -d, --min_diff <MIN_DIFF>     Minimum difference between best and second best [default: 0.02]\n"
+            "  -o, --max_overlap <O>         Maximum overlap between best and second best [default: 0.5]\n"
+            "  -m, --metric <METRIC>         Containment, Jaccard, bucket_SH, bucket_LCS [default: Containment]\n"
+            "  -z, --params <PARAMS_FILE>    Output file with parameters (tsv)\n"
+            "  -v, --verbose <VERBOSE>       Verbosity level 0 or 1 [default: 0]\n"
+            "  -n, --normalize               Accepted, unused\n"
+            "  -P, --no_bucket_pruning       Disables bucket pruning\n"
+            "  -B, --one_sweep               One sweep on all matches\n"
+            "  -F, --abs_pos                 Absolute positions instead of k-mer positions\n"
+            "  -@, --threads <THREADS>       Host threads (accepted; mapping runs on the GPU) [default: 1]\n"
+            "  -x, --profile                 Write a timing report (JSON) to --profile-log\n"
+            "      --profile-log <FILE>\n"
+            "      --per-read-stats <FILE>   One TSV row per read\n"
+            "      --per-read-stats-sample <N>  Keep every Nth read [default: 1]\n"
+            "      --device <ID>             GPU device [default: 0]\n"
+            "      --batch <N>               Reads per GPU batch [default: 16384]\n");
+}
+
+struct Record {
+    std::string name;
+    std::vector<uint8_t> seq;
+};
+
+// FASTA/FASTQ records as needletail delivers them (src/io/mod.rs:383-427): the
+// id cut at the first space or tab, line breaks and '\r' removed.
+class FastxReader {
+  public:
+    explicit FastxReader(const std::string &path) : f_(fopen(path.c_str(), "rb")) {}
+    ~FastxReader() {
+        if (f_) fclose(f_);
+    }
+    bool ok() const { return f_ != nullptr; }
+    bool next(Record &r) {
+        std::string line;
+        if (!have_pending_) {
+            do {
+                if (!getline(line)) return false;
+            } while (line.empty() || (line[0] != '>' && line[0] != '@'));
+        } else {
+            line.swap(pending_);
+            have_pending_ = false;
+        }
+        const bool fastq = line[0] == '@';
+        size_t e = 1;
+        while (e < line.size() && line[e] != ' ' && line[e] != '\t') e++;
+        r.name.assign(line, 1, e - 1);
+        r.seq.clear();
+        if (fastq) {
+            if (getline(line)) r.seq.assign(line.begin(), line.end());
+            std::string skip;
+            getline(skip);
+            getline(skip);
+            return true;
+        }
+        while (getline(line)) {
+            if (!line.empty() && line[0] == '>') {
+                pending_.swap(line);
+                have_pending_ = true;
+                break;
+            }
+            r.seq.insert(r.seq.end(), line.begin(), line.end());
+        }
+        return true;
+    }
+
+  private:
+    bool getline(std::string &out) {
+        out.clear();
+        int c;
+        bool any = false;
+        while ((c = fgetc_unlocked(f_)) != EOF) {
+            any = true;
+            if (c == '\n') break;
+            out.push_back((char)c);
+        }
+        if (!any) return false;
+        if (!out.empty() && out.back() == '\r') out.pop_back();
+        return true;
+    }
+    FILE *f_;
+    std::string pending_;
+    bool have_pending_ = false;
+};
+
+double now_s() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+bool parse_i32(const char *s, int32_t &v) {
+    char *e;
+    errno = 0;
+    long long x = strtoll(s, &e, 10);
+    if (errno || *e || e == s || x < INT32_MIN || x > INT32_MAX) return false;
+    v = (int32_t)x;
+    return true;
+}
+bool parse_f64(const char *s, double &v) {
+    char *e;
+    v = strtod(s, &e);
+    return !(*e || e == s);
+}
+
+int bad_value(const char *flag, const char *v) {
+    fprintf(stderr, "error: invalid value '%s' for '%s'\n\nFor more information, try '--help'.\n", v, flag);
+    return 2;
+}
+
+// src/params.rs:191-263 -- Params::validate
+std::string validate(const Params &p) {
+    char b[512];
+    if (p.k <= 0) return snprintf(b, sizeof b, "K-mer length (-k) should be positive. You provided %d.", p.k), b;
+    if (p.h_frac <= 0.0 || p.h_frac > 1.0)
+        return snprintf(b, sizeof b, "Given hash ratio (-r) should be between 0 and 1. You provided %s.",
+                        rust_f64(p.h_frac).c_str()),
+               b;
+    if (p.have_max_seeds && p.max_seeds <= 0)
+        return snprintf(b, sizeof b, "The number of seeds (-S) should be positive. You provided %d.", p.max_seeds), b;
+    if (p.have_max_matches && p.max_matches <= 0)
+        return snprintf(b, sizeof b, "The number of seed matches (-M) should be positive. You provided %d.",
+                        p.max_matches),
+               b;
+    if (p.theta < 0.0 || p.theta > 1.0)
+        return snprintf(b, sizeof b, "The threshold (-t) should be between 0 and 1. You provided %s.",
+                        rust_f64(p.theta).c_str()),
+               b;
+    if (p.min_diff < 0.0)
+        return snprintf(b, sizeof b, "The minimum difference (-d) should be >= 0. You provided %s.",
+                        rust_f64(p.min_diff).c_str()),
+               b;
+    if (p.rarity_weight < 0.0 || p.rarity_weight > 4.0)
+        return snprintf(b, sizeof b, "The rarity weight (--rarity-weight) should be between 0 and 4. You provided %s.",
+                        rust_f64(p.rarity_weight).c_str()),
+               b;
+    if (p.rarity_weight > 0.0 && p.metric != SHM_CONTAINMENT)
+        return snprintf(b, sizeof b,
+                        "--rarity-weight is only defined for -m Containment (got %s). Jaccard's denominator "
+                        "includes reference k-mers absent from the read, which have no frequency to weight by.",
+                        metric_name(p.metric)),
+               b;
+    if (p.max_overlap < 0.0 || p.max_overlap > 1.0)
+        return snprintf(b, sizeof b, "The maximum overlap (-o) should be between 0 and 1. You provided %s.",
+                        rust_f64(p.max_overlap).c_str()),
+               b;
+    if (p.verbose < 0 || p.verbose > 2)
+        return snprintf(b, sizeof b, "--verbose (-v) should be 0, 1, or 2. You provided %d.", p.verbose), b;
+    if (p.threads == 0)
+        return snprintf(b, sizeof b, "The number of threads (-@) should be positive. You provided %lld.", p.threads),
+               b;
+    if (p.per_read_stats_sample == 0)
+        return "--per-read-stats-sample should be >= 1 (1 keeps every read). You provided 0.";
+    // not part of the GPU path
+    if (p.rarity_weight > 0.0 || p.rarity_tiebreak > 0.0 || p.rarity_alpha > 0.0)
+        return "--rarity-weight/--rarity-tiebreak/--rarity-alpha are not supported by the GPU mapper.";
+    if (p.verbose >= 2) return "--verbose 2 (ground-truth analysis) is not supported by the GPU mapper.";
+    return "";
+}
+
+// src/params.rs:266-304 -- the -z parameters TSV (header line, values line, no
+// trailing newline after the values).
+void write_params(const Params &p) {
+    FILE *f = fopen(p.params_file.c_str(), "w");
+    if (!f) return;
+    const std::string opt_s = p.have_max_seeds ? std::to_string(p.max_seeds) : "-1";
+    const std::string opt_m = p.have_max_matches ? std::to_string(p.max_matches) : "-1";
+    const std::vector<std::pair<std::string, std::string>> fields = {
+        {"pFile", p.p_file}, {"tFile", p.t_file}, {"k", std::to_string(p.k)}, {"hFrac", rust_f64(p.h_frac)},
+        {"max_seeds", opt_s}, {"max_matches", opt_m}, {"tThres", rust_f64(p.theta)},
+        {"min_diff", rust_f64(p.min_diff)}, {"max_overlap", rust_f64(p.max_overlap)}, {"paramsFile", p.params_file},
+        {"metric", metric_name(p.metric)}, {"normalize", std::to_string((int)p.normalize)},
+        {"verbose", std::to_string(p.verbose)}, {"no-bucket-pruning", std::to_string((int)p.nbp)},
+        {"one-sweep", std::to_string((int)p.one_sweep)}, {"abs-pos", std::to_string((int)p.abs_pos)},
+        {"threads", std::to_string(p.threads)}};
+    for (auto &kv : fields) fprintf(f, "%s\t", kv.first.c_str());
+    fprintf(f, "\n");
+    for (auto &kv : fields) fprintf(f, "%s\t", kv.second.c_str());
+    fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Params p;
+    bool have_p = false, have_s = false;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        std::string val;
+        bool has_inline = false;
+        if (a.rfind("--", 0) == 0) {
+            size_t eq = a.find('=');
+            if (eq != std::string::npos) {
+                val = a.substr(eq + 1);
+                a = a.substr(0, eq);
+                has_inline = true;
+            }
+        }
+        auto need = [&](const char *flag) -> const char * {
+            if (has_inline) return val.c_str();
+            if (i + 1 >= argc) {
+                fprintf(stderr, "error: a value is required for '%s' but none was supplied\n", flag);
+                exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-h" || a == "--help") {
+            usage(stdout);
+            return 0;
+        } else if (a == "-V" || a == "--version") {
+            printf("shmap 1.4.3 (gfx950)\n");
+            return 0;
+        } else if (a == "-p" || a == "--pattern") {
+            p.p_file = need("--pattern <P_FILE>");
+            have_p = true;
+        } else if (a == "-s" || a == "--text") {
+            p.t_file = need("--text <T_FILE>");
+            have_s = true;
+        } else if (a == "-k" || a == "--ksize") {
+            const char *v = need("--ksize <K>");
+            if (!parse_i32(v, p.k)) return bad_value("--ksize <K>", v);
+        } else if (a == "-r" || a == "--hashratio") {
+            const char *v = need("--hashratio <H_FRAC>");
+            if (!parse_f64(v, p.h_frac)) return bad_value("--hashratio <H_FRAC>", v);
+        } else if (a == "-S" || a == "--max_seeds") {
+            const char *v = need("--max_seeds <MAX_SEEDS>");
+            if (!parse_i32(v, p.max_seeds)) return bad_value("--max_seeds <MAX_SEEDS>", v);
+            p.have_max_seeds = true;
+        } else if (a == "-M" || a == "--max_matches") {
+            const char *v = need("--max_matches <MAX_MATCHES>");
+            if (!parse_i32(v, p.max_matches)) return bad_value("--max_matches <MAX_MATCHES>", v);
+            p.have_max_matches = true;
+        } else if (a == "-t" || a == "--threshold") {
+            const char *v = need("--threshold <THETA>");
+            if (!parse_f64(v, p.theta)) return bad_value("--threshold <THETA>", v);
+        } else if (a == "-d" || a == "--min_diff") {
+            const char *v = need("--min_diff <MIN_DIFF>");
+            if (!parse_f64(v, p.min_diff)) return bad_value("--min_diff <MIN_DIFF>", v);
+        } else if (a == "-o" || a == "--max_overlap") {
+            const char *v = need("--max_overlap <MAX_OVERLAP>");
+            if (!parse_f64(v, p.max_overlap)) return bad_value("--max_overlap <MAX_OVERLAP>", v);
+        } else if (a == "--rarity-weight") {
+            const char *v = need("--rarity-weight <RARITY_WEIGHT>");
+            if (!parse_f64(v, p.rarity_weight)) return bad_value("--rarity-weight <RARITY_WEIGHT>", v);
+        } else if (a == "--rarity-tiebreak") {
+            const char *v = need("--rarity-tiebreak <RARITY_TIEBREAK>");
+            if (!parse_f64(v, p.rarity_tiebreak)) return bad_value("--rarity-tiebreak <RARITY_TIEBREAK>", v);
+        } else if (a == "--rarity-alpha") {
+            const char *v = need("--rarity-alpha <RARITY_ALPHA>");
+            if (!parse_f64(v, p.rarity_alpha)) return bad_value("--rarity-alpha <RARITY_ALPHA>", v);
+        } else if (a == "-m" || a == "--metric") {
+            const char *v = need("--metric <METRIC>");
+            if (!strcmp(v, "Containment")) p.metric = SHM_CONTAINMENT;
+            else if (!strcmp(v, "Jaccard")) p.metric = SHM_JACCARD;
+            else if (!strcmp(v, "bucket_SH")) p.metric = SHM_BUCKET_SH;
+            else if (!strcmp(v, "bucket_LCS")) p.metric = SHM_BUCKET_LCS;
+            else {
+                fprintf(stderr,
+                        "error: invalid value '%s' for '--metric <METRIC>'\n  [possible values: Containment, "
+                        "Jaccard, bucket_SH, bucket_LCS]\n\nFor more information, try '--help'.\n",
+                        v);
+                return 2;
+            }
+        } else if (a == "-z" || a == "--params") {
+            p.params_file = need("--params <PARAMS_FILE>");
+        } else if (a == "-v" || a == "--verbose") {
+            const char *v = need("--verbose <VERBOSE>");
+            if (!parse_i32(v, p.verbose)) return bad_value("--verbose <VERBOSE>", v);
+        } else if (a == "-n" || a == "--normalize") {
+            p.normalize = true;
+        } else if (a == "-P" || a == "--no_bucket_pruning") {
+            p.nbp = true;
+        } else if (a == "-B" || a == "--one_sweep") {
+            p.one_sweep = true;
+        } else if (a == "-F" || a == "--abs_pos") {
+            p.abs_pos = true;
+        } else if (a == "-@" || a == "--threads") {
+            const char *v = need("--threads <THREADS>");
+            char *e;
+            p.threads = strtoll(v, &e, 10);
+            if (*e || e == v || p.threads < 0) return bad_value("--threads <THREADS>", v);
+        } else if (a == "-x" || a == "--profile") {
+            p.profile = true;
+        } else if (a == "--profile-log") {
+            p.profile_log = need("--profile-log <PROFILE_LOG>");
+        } else if (a == "--per-read-stats") {
+            p.per_read_stats = need("--per-read-stats <PER_READ_STATS>");
+        } else if (a == "--per-read-stats-sample") {
+            const char *v = need("--per-read-stats-sample <PER_READ_STATS_SAMPLE>");
+            char *e;
+            p.per_read_stats_sample = strtoull(v, &e, 10);
+            if (*e || e == v) return bad_value("--per-read-stats-sample <PER_READ_STATS_SAMPLE>", v);
+        } else if (a == "--device") {
+            p.device = atoi(need("--device <ID>"));
+        } else if (a == "--batch") {
+            p.batch = atol(need("--batch <N>"));
+            if (p.batch < 1) p.batch = 1;
+        } else {
+            fprintf(stderr, "error: unexpected argument '%s' found\n\nFor more information, try '--help'.\n",
+                    argv[i]);
+            return 2;
+        }
+    }
+    if (!have_p || !have_s) {
+        fprintf(stderr, "error: the following required arguments were not provided:\n%s%s\nUsage: shmap -p <P_FILE> "
+                        "-s <T_FILE>\n\nFor more information, try '--help'.\n",
+                have_p ? "" : "  --pattern <P_FILE>\n", have_s ? "" : "  --text <T_FILE>\n");
+        return 2;
+    }
+    const std::string err = validate(p);
+    if (!err.empty()) {
+        fprintf(stderr, "ERROR: %s\n", err.c_str());
+        return 1;
+    }
+    if (!p.params_file.empty()) {
+        fprintf(stderr, "Writing parameters to %s...\n", p.params_file.c_str());
+        write_params(p);
+    }
+    const double t_start = now_s();
+
+    // ---- index (src/main.rs:45-56)
+    fprintf(stderr, "Indexing %s...\n", p.t_file.c_str());
+    std::vector<Record> ref;
+    {
+        FastxReader rd(p.t_file);
+        if (!rd.ok()) {
+            fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    failed to open \"%s\"\n",
+                    p.t_file.c_str(), p.t_file.c_str());
+            return 1;
+        }
+        Record r;
+        while (rd.next(r)) ref.push_back(r);
+    }
+    shm_sketcher *sk = nullptr;
+    if (shm_sketcher_create(p.k, p.h_frac, &sk)) {
+        fprintf(stderr, "ERROR: %s\n", shm_last_error());
+        return 1;
+    }
+    std::vector<shm_segment> segs(ref.size());
+    for (size_t i = 0; i < ref.size(); i++) segs[i] = {ref[i].name.c_str(), ref[i].seq.data(), (int64_t)ref[i].seq.size()};
+    shm_index *idx = nullptr;
+    const double t_idx = now_s();
+    if (shm_index_build(sk, segs.data(), (int32_t)segs.size(), p.have_max_matches ? p.max_matches : 0, p.device,
+                        &idx)) {
+        fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
+                shm_last_error());
+        return 1;
+    }
+    {
+        int64_t ik, ih, hf, bk, bh;
+        shm_index_stats(idx, &ik, &ih, &hf, &bk, &bh);
+        long long nucl = 0;
+        for (auto &r : ref) nucl += (long long)r.seq.size();
+        fprintf(stderr, " | total nucleotides:     %lld\n | index segments:        %zu\n", nucl, ref.size());
+        fprintf(stderr, " | indexed kmers:         %lld\n | indexed hits:          %lld\n", (long long)ik, (long long)ih);
+        fprintf(stderr, " | | most frequent kmer:      %lld times.\n | | blacklisted kmers:       %lld\n",
+                (long long)hf, (long long)bk);
+        fprintf(stderr, " | indexing time:        %.1fs\n", now_s() - t_idx);
+    }
+    ref.clear();
+    ref.shrink_to_fit();
+
+    // ---- mapping (src/shmap/mod.rs:374-650)
+    shm_map_params mpp{p.k, p.h_frac, p.theta, p.min_diff, p.max_overlap, p.metric, (int32_t)p.nbp,
+                       (int32_t)p.one_sweep, (int32_t)p.abs_pos};
+    shm_mapper *mp = nullptr;
+    if (shm_mapper_create(idx, &mpp, &mp)) {
+        fprintf(stderr, "ERROR: %s\n", shm_last_error());
+        return 1;
+    }
+    fprintf(stderr, "Mapping reads using SHmap...\n");
+    FILE *unmapped = nullptr, *stats = nullptr;
+    if (!p.params_file.empty()) {
+        const std::string fn = p.params_file + ".unmapped.paf";
+        fprintf(stderr, "Unmapped reads to %s\n", fn.c_str());
+        unmapped = fopen(fn.c_str(), "w");
+    }
+    if (!p.per_read_stats.empty()) {
+        fprintf(stderr, "Per-read stats to %s\n", p.per_read_stats.c_str());
+        stats = fopen(p.per_read_stats.c_str(), "w");
+        if (stats)
+            fprintf(stats, "query_id\tread_len\tkmers_sketched\tkmers_seeds\tpossible_matches\texamined_matches\t"
+                           "seed_matches\tseeded_buckets\trefined_buckets\tfinal_buckets\tmatches_in_mapping\tmapped\t"
+                           "mapq\tquery_mapping_s\n");
+    }
+    FastxReader rd(p.p_file);
+    if (!rd.ok()) {
+        fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    failed to open \"%s\"\n",
+                p.p_file.c_str(), p.p_file.c_str());
+        return 1;
+    }
+    static char outbuf[1 << 20];
+    setvbuf(stdout, outbuf, _IOFBF, sizeof outbuf);
+    std::vector<Record> batch;
+    std::vector<shm_read> reads;
+    std::vector<shm_read_result> res;
+    std::string line(8192, '\0');
+    unsigned long long idx_read = 0, n_reads = 0, n_mapped = 0;
+    double map_s = 0;
+    bool more = true;
+    int rc = 0;
+    while (more) {
+        batch.clear();
+        Record r;
+        while ((long)batch.size() < p.batch && (more = rd.next(r))) batch.push_back(std::move(r));
+        if (batch.empty()) break;
+        reads.resize(batch.size());
+        for (size_t i = 0; i < batch.size(); i++) reads[i] = {batch[i].seq.data(), (int32_t)batch[i].seq.size()};
+        res.resize(batch.size());
+        const double t0 = now_s();
+        if (shm_map_batch(mp, reads.data(), (int32_t)reads.size(), res.data())) {
+            fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
+                    shm_last_error());
+            rc = 1;
+            break;
+        }
+        const double dt = now_s() - t0;
+        map_s += dt;
+        const double per_read = dt / batch.size();
+        for (size_t i = 0; i < batch.size(); i++, idx_read++) {
+            const shm_read_result &x = res[i];
+            if (stats && idx_read % p.per_read_stats_sample == 0)
+                fprintf(stats, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n", batch[i].name.c_str(),
+                        x.read_len, x.m, x.S, x.possible_matches, x.total_matches, x.seed_matches, x.seeded_buckets,
+                        x.refined_buckets, x.final_buckets, x.mapped ? x.intersection : 0, x.mapped,
+                        x.mapped ? (x.mapq == 60 ? 60 : 0) : -1, per_read);
+            if (x.mapped) {
+                int n = shm_format_paf(idx, &mpp, batch[i].name.c_str(), &x, per_read, &line[0], line.size());
+                if (n >= (int)line.size()) {
+                    line.resize(n + 1);
+                    n = shm_format_paf(idx, &mpp, batch[i].name.c_str(), &x, per_read, &line[0], line.size());
+                }
+                fwrite(line.data(), 1, n, stdout);
+                fputc('\n', stdout);
+                n_mapped++;
+            } else if (unmapped) {
+                int n = shm_format_unmapped(batch[i].name.c_str(), x.read_len, &line[0], line.size());
+                fwrite(line.data(), 1, n, unmapped);
+                fputc('\n', unmapped);
+            }
+        }
+        n_reads += batch.size();
+    }
+    fflush(stdout);
+    if (unmapped) fclose(unmapped);
+    if (stats) fclose(stats);
+    fprintf(stderr, " | reads:                 %llu\n | mapped reads:          %llu\n | mapping time:         %.2fs\n",
+            n_reads, n_mapped, map_s);
+    if (p.profile) {
+        const std::string path = !p.profile_log.empty() ? p.profile_log
+                                 : !p.params_file.empty() ? p.params_file + ".profile.json"
+                                                          : "shmap.profile.json";
+        if (FILE *f = fopen(path.c_str(), "w")) {
+            fprintf(f, "{\"reads\": %llu, \"mapped_reads\": %llu, \"mapping_s\": %.6f, \"total_s\": %.6f}\n", n_reads,
+                    n_mapped, map_s, now_s() - t_start);
+            fclose(f);
+        }
+    }
+    shm_mapper_free(mp);
+    shm_index_free(idx);
+    shm_sketcher_free(sk);
+    return rc;
+}

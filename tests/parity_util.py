@@ -1,7 +1,5 @@
 """Shared helpers for parity tests: run the CPU oracle and the HIP library on
 the same inputs and diff every field of every read."""
-import math
-
 import numpy as np
 
 import oracle_ffi as O
@@ -15,27 +13,28 @@ MAPPED_FIELDS = ["p_start", "p_end", "strand", "segm_id", "t_l", "t_r", "mapq", 
                  "J", "J2", "sh", "sigmas", "match_inefficiency"]
 
 
-def same_value(a, b) -> bool:
-    a, b = a.item() if hasattr(a, "item") else a, b.item() if hasattr(b, "item") else b
-    if isinstance(a, float) or isinstance(b, float):
-        a, b = float(a), float(b)
-        if math.isnan(a) and math.isnan(b):
-            return True
-        return a == b and math.copysign(1.0, a) == math.copysign(1.0, b)
-    return int(a) == int(b)
+def _eq(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Element-wise equality; floats compared bit for bit, any NaN equal to any NaN."""
+    if a.dtype.kind == "f" or b.dtype.kind == "f":
+        a, b = a.astype(np.float64), b.astype(np.float64)
+        both_nan = np.isnan(a) & np.isnan(b)
+        return both_nan | (a.view(np.int64) == b.view(np.int64))
+    return a.astype(np.int64) == b.astype(np.int64)
 
 
 def diff_results(ores, gres, limit=20):
+    """[(read index, field, oracle value, gpu value)] for every mismatch (first `limit`)."""
+    assert len(ores) == len(gres)
     bad = []
-    for i in range(len(ores)):
-        o, g = ores[i], gres[i]
-        fields = COUNTER_FIELDS + (MAPPED_FIELDS if o["mapped"] and g["mapped"] else [])
-        for f in fields:
-            if not same_value(o[f], g[f]):
-                bad.append((i, f, o[f].item(), g[f].item()))
-        if len(bad) >= limit:
-            break
-    return bad
+    both = (ores["mapped"] == 1) & (gres["mapped"] == 1)
+    for f in COUNTER_FIELDS + MAPPED_FIELDS:
+        ok = _eq(ores[f], gres[f])
+        if f in MAPPED_FIELDS:
+            ok |= ~both
+        for i in np.nonzero(~ok)[0][:limit]:
+            bad.append((int(i), f, ores[f][i].item(), gres[f][i].item()))
+    bad.sort()
+    return bad[:limit]
 
 
 def oracle_map(segments, reads, k, h_frac, max_matches=None, threads=8, **prm):

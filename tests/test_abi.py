@@ -1,0 +1,67 @@
+"""The C-ABI library loads and exports every entry point include/shmap_hip.h
+declares; the CLI keeps the reference's flag validation.  No GPU needed."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "shmap_hip.h")
+LIB = os.path.join(ROOT, "shmap-rs_amd", "lib", "libshmap_hip.so")
+CLI = os.path.join(ROOT, "shmap-rs_amd", "lib", "shmap")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(shm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_mapping_api():
+    names = declared_functions()
+    for required in ("shm_sketcher_create", "shm_sketch", "shm_index_build", "shm_index_count", "shm_index_hits",
+                     "shm_mapper_create", "shm_map_batch", "shm_map_batch_device", "shm_format_paf",
+                     "shm_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_reports_argument_errors_without_a_device():
+    lib = C.CDLL(LIB)
+    lib.shm_last_error.restype = C.c_char_p
+    out = C.c_void_p()
+    assert lib.shm_sketcher_create(0, C.c_double(0.1), C.byref(out)) != 0
+    assert b"K-mer length" in lib.shm_last_error()
+    assert lib.shm_sketcher_create(15, C.c_double(1.5), C.byref(out)) != 0
+    assert b"hash ratio" in lib.shm_last_error()
+
+
+@pytest.mark.parametrize("args,needle", [
+    (["-k", "0"], "K-mer length (-k) should be positive. You provided 0."),
+    (["-r", "0"], "Given hash ratio (-r) should be between 0 and 1. You provided 0."),
+    (["-t", "1.5"], "The threshold (-t) should be between 0 and 1. You provided 1.5."),
+    (["-d", "-1"], "The minimum difference (-d) should be >= 0. You provided -1."),
+    (["-o", "2"], "The maximum overlap (-o) should be between 0 and 1. You provided 2."),
+    (["-M", "0"], "The number of seed matches (-M) should be positive. You provided 0."),
+    (["-S", "-3"], "The number of seeds (-S) should be positive. You provided -3."),
+    (["-@", "0"], "The number of threads (-@) should be positive. You provided 0."),
+    (["-v", "3"], "--verbose (-v) should be 0, 1, or 2. You provided 3."),
+])
+def test_cli_validation_messages(args, needle):
+    """src/params.rs:191-263; src/main.rs:18-21 prints `ERROR: ...` and exits 1."""
+    r = subprocess.run([CLI, "-s", "ref.fa", "-p", "reads.fa", *args], capture_output=True, text=True)
+    assert r.returncode == 1
+    assert r.stderr.strip() == "ERROR: " + needle
+
+
+def test_cli_rejects_an_unknown_metric():
+    r = subprocess.run([CLI, "-s", "ref.fa", "-p", "reads.fa", "-m", "Hamming"], capture_output=True, text=True)
+    assert r.returncode == 2 and "possible values: Containment, Jaccard, bucket_SH, bucket_LCS" in r.stderr
