@@ -187,6 +187,38 @@ __device__ void bitonic_core(K *k, V *v, uint32_t n) {
     }
 }
 
+// Key-only variant (distinct keys), in place; caller synchronises before use.
+template <typename K>
+__device__ __forceinline__ void cswap_k(K *k, uint32_t a, uint32_t b) {
+    const K ka = k[a], kb = k[b];
+    if (kb < ka) {
+        k[a] = kb;
+        k[b] = ka;
+    }
+}
+
+template <typename K>
+__device__ void bitonic_keys(K *k, uint32_t n) {
+    uint32_t N = 1;
+    while (N < n) N <<= 1;
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        const uint32_t half = size >> 1;
+        for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
+            const uint32_t blk = i / half, p = i % half;
+            const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
+            if (b < n) cswap_k(k, a, b);
+        }
+        __syncthreads();
+        for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
+                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                if (b < n) cswap_k(k, a, b);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Sorts global (gk, gv)[0, n) using LDS scratch (sk, sv) of `cap` entries when
 // it fits.  Ends with a barrier; results are visible to the whole workgroup.
 template <typename K, typename V>

@@ -88,23 +88,40 @@ __global__ void __launch_bounds__(WG) k_chunk_counts(SeqSet S, int32_t k, uint64
 }
 
 // One wave per chunk of CHUNK_W windows; the chunk's bases plus the k-1 halo
-// are staged in LDS.  Pass 0 counts kept k-mers per chunk; pass 1 writes them
-// at the scanned chunk offsets, in position order (lanes cover consecutive
-// window runs and write at their exclusive prefix).
+// are staged in LDS with dword loads, and the ntHash seeds with their fixed
+// rotations pre-applied (src/sketch.rs:90-97) sit in LDS tables indexed by the
+// raw byte.  Each lane rolls LANE_W consecutive windows; the initial window is
+// rolled in from a zero hash, which equals the direct definition of
+// src/sketch.rs:180-184.  Pass 0 stores each lane's kept count; pass 1 writes
+// the kept k-mers at the lane's exclusive prefix within the scanned chunk
+// offset, so the output is in position order.
 template <int PASS>
 __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thres,
                                                const uint64_t *chunk_start, uint64_t *chunk_val,
-                                               uint64_t *out_h, int32_t *out_r, uint8_t *out_s) {
-    extern __shared__ uint8_t lds_bases[];
+                                               uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r,
+                                               uint8_t *out_s) {
+    extern __shared__ uint8_t lds[];
+    uint64_t *T_fw = (uint64_t *)lds;   // fw[c]
+    uint64_t *T_rck1 = T_fw + 256;      // rotl(rc[c], k-1)
+    uint64_t *T_fwk = T_fw + 512;       // rotl(fw[c], k)
+    uint64_t *T_rcr1 = T_fw + 768;      // rotr(rc[c], 1)
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    const uint32_t span = (CHUNK_W + k - 1 + 15) & ~15u;
-    uint8_t *buf = lds_bases + (size_t)w * span;
+    const uint32_t span = sketch_span(k);
+    uint8_t *buf = lds + SKETCH_LUT_BYTES + (size_t)w * span;
+    for (int c = threadIdx.x; c < 256; c += WG) {
+        const uint64_t fw = nt_fw((uint8_t)c), rc = nt_rc((uint8_t)c);
+        T_fw[c] = fw;
+        T_rck1[c] = rotl64(rc, (uint32_t)(k - 1));
+        T_fwk[c] = rotl64(fw, (uint32_t)k);
+        T_rcr1[c] = rotr64(rc, 1);
+    }
+    __syncthreads();
     const uint64_t total = chunk_start[S.n];
-    const uint32_t rk = (uint32_t)k, rk1 = (uint32_t)(k - 1);
     for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
         const uint64_t c = cb + w;
         const bool valid = c < total;
         int64_t w0 = 0, w1 = 0;
+        uint32_t shift = 0;
         if (valid) {
             const uint32_t seq = upper_bound_u64(chunk_start, (uint32_t)S.n + 1, c) - 1;
             const int64_t len = S.off[seq + 1] - S.off[seq];
@@ -112,49 +129,70 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             w0 = (int64_t)(c - chunk_start[seq]) * CHUNK_W;
             w1 = w0 + CHUNK_W < nw ? w0 + CHUNK_W : nw;
             const int64_t nb = w1 - w0 + k - 1;
-            const uint8_t *src = S.bases + S.off[seq] + w0;
-            for (int64_t i = lane; i < nb; i += WAVE) buf[i] = src[i];
+            const uintptr_t src = (uintptr_t)(S.bases + S.off[seq] + w0);
+            const uintptr_t a = src & ~(uintptr_t)3;
+            shift = (uint32_t)(src - a);
+            const int64_t ndw = (int64_t)((src + nb - a + 3) >> 2);
+            const uint32_t *s32 = (const uint32_t *)a;
+            uint32_t *b32 = (uint32_t *)buf;
+            const uintptr_t lim = (uintptr_t)(S.bases + S.total);  // never read past the set
+            for (int64_t i = lane; i < ndw; i += WAVE) {
+                const uintptr_t at = a + 4 * (uintptr_t)i;
+                uint32_t v;
+                if (at + 4 <= lim) {
+                    v = s32[i];
+                } else {
+                    v = 0;
+                    for (uint32_t j = 0; j < 4; j++)
+                        if (at + j < lim) v |= (uint32_t)(*(const uint8_t *)(at + j)) << (8 * j);
+                }
+                b32[i + (i >> 5)] = v;  // one dword of skew per 128 bytes
+            }
         }
         __syncthreads();
         if (valid) {
+            // byte q of the chunk sits at q' = q + shift plus one dword of skew per
+            // 128 bytes, so lanes rolling runs 128 windows apart hit different banks
+#define SKB(q) buf[(uint32_t)((q) + shift) + (((uint32_t)((q) + shift) >> 7) << 2)]
             const int64_t nwin = w1 - w0;
             const int64_t lw0 = (int64_t)lane * LANE_W;
             const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
+            uint32_t cnt = 0;
             uint64_t base_out = 0;
-            for (int phase = 0; phase < (PASS == 1 ? 2 : 1); phase++) {
-                uint32_t cnt = 0;
-                if (lw0 < lw1) {
-                    uint64_t hf = 0, hr = 0;
-                    for (int32_t i = 0; i < k; i++) {
-                        const uint8_t ch = buf[lw0 + i];
-                        hf ^= rotl64(nt_fw(ch), (uint32_t)(k - i - 1));
-                        hr ^= rotl64(nt_rc(ch), (uint32_t)i);
-                    }
-                    for (int64_t wi = lw0;; wi++) {
-                        const uint64_t h = hr ^ hf;
-                        if (h <= thres) {
-                            if (PASS == 1 && phase == 1) {
-                                const uint64_t o = base_out + cnt;
-                                out_h[o] = h;
-                                out_r[o] = (int32_t)(w0 + wi + k - 1);
-                                out_s[o] = hf > hr;
-                            }
-                            cnt++;
+            if (PASS == 1) {
+                const uint32_t mine = lane_cnt[c * WAVE + lane];
+                base_out = chunk_val[c] + (wave_incl_scan(mine) - mine);
+            }
+            if (lw0 < lw1) {
+                uint64_t hf = 0, hr = 0;
+                for (int32_t i = 0; i < k; i++) {
+                    const uint8_t ch = SKB(lw0 + i);
+                    hf = rotl64(hf, 1) ^ T_fw[ch];
+                    hr = rotr64(hr, 1) ^ T_rck1[ch];
+                }
+                for (int64_t wi = lw0;; wi++) {
+                    const uint64_t h = hr ^ hf;
+                    if (h <= thres) {
+                        if (PASS == 1) {
+                            const uint64_t o = base_out + cnt;
+                            out_h[o] = h;
+                            out_r[o] = (int32_t)(w0 + wi + k - 1);
+                            out_s[o] = hf > hr;
                         }
-                        if (wi + 1 >= lw1) break;
-                        const uint8_t in = buf[wi + k], out = buf[wi];
-                        hf = rotl64(hf, 1) ^ rotl64(nt_fw(out), rk) ^ nt_fw(in);
-                        hr = rotr64(hr, 1) ^ rotr64(nt_rc(out), 1) ^ rotl64(nt_rc(in), rk1);
+                        cnt++;
                     }
+                    if (wi + 1 >= lw1) break;
+                    const uint8_t in = SKB(wi + k), out = SKB(wi);
+                    hf = rotl64(hf, 1) ^ T_fwk[out] ^ T_fw[in];
+                    hr = rotr64(hr, 1) ^ T_rcr1[out] ^ T_rck1[in];
                 }
-                if (PASS == 0) {
-                    uint32_t s = cnt;
-                    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_xor(s, d);
-                    if (lane == 0) chunk_val[c] = s;
-                } else if (phase == 0) {
-                    const uint32_t inc = wave_incl_scan(cnt);
-                    base_out = chunk_val[c] + (inc - cnt);
-                }
+            }
+#undef SKB
+            if (PASS == 0) {
+                lane_cnt[c * WAVE + lane] = cnt;
+                uint32_t s2 = cnt;
+                for (int d = WAVE / 2; d > 0; d >>= 1) s2 += __shfl_xor(s2, d);
+                if (lane == 0) chunk_val[c] = s2;
             }
         }
         __syncthreads();
@@ -766,6 +804,10 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
             A.bs_max[base + p] = rmax;
             A.bs_alive[base + p] = alive;
             A.bs_sh[base + p] = sh;
+            if (alive) {
+                const uint32_t t = atomicAdd(A.task_count, 1u);
+                A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+            }
         }
         __syncthreads();
     }
@@ -780,145 +822,282 @@ __device__ __forceinline__ int32_t read_seed_of(const MapArgs &A, uint64_t o, in
     return (lo < u && A.sd_h[o + lo] == h) ? lo : -1;
 }
 
+// Inclusive prefix sum of a[0, n) in place, over the workgroup.
+__device__ void wg_scan_inplace(int32_t *a, int64_t n, uint32_t *tmp) {
+    __shared__ int32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t b0 = 0; b0 < n; b0 += WG) {
+        const int64_t i = b0 + threadIdx.x;
+        const int32_t v = i < n ? a[i] : 0;
+        uint32_t tot;
+        const uint32_t ex = wg_excl_scan((uint32_t)v, tmp, &tot);
+        const int32_t c = s_carry;
+        if (i < n) a[i] = c + (int32_t)ex + v;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = c + (int32_t)tot;
+        __syncthreads();
+    }
+}
+
 // find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket that can
-// survive either match_rest pass: the best_fixed_length window sweep
-// (:189-291), the LCS of :114-170, or the bucket_SH record.  The workgroup
-// resolves the read seed of every reference k-mer of the bucket's span; one
-// lane then runs the sequential sweep over LDS.
+// survive either match_rest pass, one workgroup per bucket from the flat task
+// list the prune pass appended.
+//
+// The span's matched reference k-mers come from the read's own seeds: every
+// hit of every seed inside [l0, end) of the segment marks its position with
+// the seed (a position has one hash, so at most one seed).
+//
+// Containment/Jaccard (best_fixed_length, :189-291) run in parallel: window l
+// covers [l, min(l+m, n)); a matched position x of seed s counts in window l
+// exactly when fewer than occs(s) occurrences of s lie in [l, x), i.e. when the
+// occs(s)-th previous occurrence P(x) is before l.  So x adds 1 to every l in
+// [max(P(x)+1, x-m+1, 0), x] and its strand agreement to every l in
+// [max(x-m+1, 0), x]; two difference arrays and scans give intersection(l) and
+// same_strand(l), and the first l of the greatest score wins, as the strict
+// `score > best` of the sequential sweep picks.  With abs_pos the window edge
+// depends on positions, so that mode keeps the sequential sweep; so does LCS
+// (:114-170), whose patience LIS is sequential by nature.
 __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     extern __shared__ uint8_t lds[];
-    int32_t *l_sid = (int32_t *)lds;
-    int32_t *l_diff = (int32_t *)(lds + (size_t)REFINE_CAP * 4);
+    int32_t *l_sid = (int32_t *)lds;           // REFINE_CAP
+    int32_t *l_b = l_sid + REFINE_CAP;         // REFINE_CAP (+1)
+    int32_t *l_c = l_b + REFINE_CAP + 1;       // REFINE_CAP + 1
+    __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_nm;
+    __shared__ double s_bj[NWAVES];
+    __shared__ int32_t s_bl[NWAVES];
     const int ap = A.abs_pos;
-    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+    const uint32_t ntask = *A.task_count;
+    for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
+        const uint2 task = A.tasks[t];
+        const int32_t lr = (int32_t)task.x, p = (int32_t)task.y;
         const int32_t rd = A.r0 + lr;
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
-        const int32_t nb = A.rd_nb[rd];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], hl = A.rd_hl[rd];
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
-        int32_t *diff = (u <= REFINE_CAP) ? l_diff : A.g_diff + o;
-        for (int32_t s = threadIdx.x; s < u; s += WG) diff[s] = (int32_t)A.sd_occs[o + s];
-        __syncthreads();
-        for (int32_t p = 0; p < nb; p++) {
-            if (!A.bs_alive[base + p]) continue;
-            uint32_t seg;
-            int32_t b;
-            decode_bucket(A, A.bs_g[base + p], hl, seg, b);
-            const double sh = A.no_bucket_pruning ? 1.0 : A.bs_sh[base + p];
-            const int64_t ks = I.kstart[seg];
-            const int64_t tlen = I.kstart[seg + 1] - ks;
-            const int64_t bb = (int64_t)b * hl, be = (int64_t)(b + 2) * hl;
-            const int64_t l0 = bb > 0 ? bb : 0;
-            const int64_t end = be < tlen ? be : tlen;
-            const int64_t span = end > l0 ? end - l0 : 0;
-            if (A.metric == METRIC_BUCKET_SH) {
-                if (threadIdx.x == 0) {
-                    const int32_t tl = A.bs_min[base + p], tr = A.bs_max[base + p];
-                    A.sc_j[base + p] = sh;
-                    A.sc_i[base + p] = A.bs_m[base + p];
-                    A.sc_same[base + p] = A.bs_c[base + p];
-                    A.sc_tl[base + p] = tl;
-                    A.sc_tr[base + p] = tr;
-                    A.sc_ssz[base + p] = tr - tl;
-                    A.sc_pend[base + p] = len - 1;
-                    A.sc_seg[base + p] = (int32_t)seg;
-                }
-                continue;
+        uint32_t seg;
+        int32_t b;
+        decode_bucket(A, A.bs_g[base + p], hl, seg, b);
+        const double sh = A.no_bucket_pruning ? 1.0 : A.bs_sh[base + p];
+        if (A.metric == METRIC_BUCKET_SH) {
+            if (threadIdx.x == 0) {
+                const int32_t tl = A.bs_min[base + p], tr = A.bs_max[base + p];
+                A.sc_j[base + p] = sh;
+                A.sc_i[base + p] = A.bs_m[base + p];
+                A.sc_same[base + p] = A.bs_c[base + p];
+                A.sc_tl[base + p] = tl;
+                A.sc_tr[base + p] = tr;
+                A.sc_ssz[base + p] = tr - tl;
+                A.sc_pend[base + p] = len - 1;
+                A.sc_seg[base + p] = (int32_t)seg;
             }
-            int32_t *sid_arr = span <= REFINE_CAP ? l_sid : A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
-            for (int64_t t = l0 + threadIdx.x; t < end; t += WG) {
-                const int32_t s = read_seed_of(A, o, u, I.t_h[ks + t]);
-                int32_t v = -1;
-                if (s >= 0) v = s | ((A.sd_strand[o + s] == I.t_s[ks + t]) ? 0 : 0x40000000);
-                sid_arr[t - l0] = v;
+            continue;
+        }
+        const int64_t ks = I.kstart[seg];
+        const int64_t tlen = I.kstart[seg + 1] - ks;
+        const int64_t bb = (int64_t)b * hl, be = (int64_t)(b + 2) * hl;
+        const int64_t l0 = bb > 0 ? bb : 0;
+        const int64_t end = be < tlen ? be : tlen;
+        const int64_t n = end > l0 ? end - l0 : 0;
+        const bool fits = n <= REFINE_CAP;
+        const bool parallel = fits && n > 0 && !ap && A.metric != METRIC_BUCKET_LCS && u < (1 << 19);
+        int32_t *sid = fits ? l_sid : A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
+        int32_t *keys = fits ? l_b : A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
+        // 1. mark every matched position of the span with its seed
+        for (int64_t x = threadIdx.x; x < n; x += WG) sid[x] = -1;
+        if (parallel)
+            for (int64_t x = threadIdx.x; x <= n; x += WG) l_c[x] = 0;
+        if (threadIdx.x == 0) s_nm = 0;
+        __syncthreads();
+        for (int32_t sd = threadIdx.x; sd < u; sd += WG) {
+            const uint32_t cnt = A.sd_cnt[o + sd];
+            if (cnt == 0) continue;
+            const uint32_t hoff = A.sd_hoff[o + sd];
+            const uint32_t sstr = A.sd_strand[o + sd];
+            for (uint32_t j = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0); j < hoff + cnt; j++) {
+                if (I.hit_seg[j] != seg) break;
+                const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
+                if (x >= n) break;
+                const uint32_t hstr = I.hit_r[j] >> 31;
+                sid[x] = sd | (sstr == hstr ? 0 : 0x40000000);
+                if (parallel) keys[atomicAdd(&s_nm, 1u)] = ((uint32_t)sd << 12) | (uint32_t)x;
+            }
+        }
+        __syncthreads();
+        if (parallel) {
+            // 2. occurrences grouped by seed, in position order
+            const uint32_t nm = s_nm;
+            bitonic_keys((uint32_t *)keys, nm);
+            // 3. intersection difference array
+            for (uint32_t i = threadIdx.x; i < nm; i += WG) {
+                const uint32_t key = (uint32_t)keys[i];
+                const int32_t sd = (int32_t)(key >> 12), x = (int32_t)(key & 0xfff);
+                const uint32_t occ = A.sd_occs[o + sd];
+                int32_t P = -1;
+                if (i >= occ && ((uint32_t)keys[i - occ] >> 12) == (uint32_t)sd) P = (int32_t)((uint32_t)keys[i - occ] & 0xfff);
+                int32_t lo = x - m + 1;
+                if (P + 1 > lo) lo = P + 1;
+                if (lo < 0) lo = 0;
+                atomicAdd(&l_c[lo], 1);
+                atomicAdd(&l_c[x + 1], -1);
+            }
+            __syncthreads();
+            // 4. strand agreement difference array (reusing the key space)
+            for (int64_t x = threadIdx.x; x <= n; x += WG) keys[x] = 0;
+            __syncthreads();
+            for (int64_t x = threadIdx.x; x < n; x += WG) {
+                const int32_t v = sid[x];
+                if (v < 0) continue;
+                const int32_t cd = (v & 0x40000000) ? -1 : 1;
+                int64_t lo = x - m + 1;
+                if (lo < 0) lo = 0;
+                atomicAdd(&keys[lo], cd);
+                atomicAdd(&keys[x + 1], -cd);
+            }
+            __syncthreads();
+            wg_scan_inplace(l_c, n, tmp);
+            wg_scan_inplace(keys, n, tmp);
+            // 5. first window with the greatest score
+            const bool jac = A.metric == METRIC_JACCARD;
+            double bj = -1.0;
+            int32_t bl = 0x7fffffff;
+            for (int64_t l = threadIdx.x; l < n; l += WG) {
+                const int32_t I2 = l_c[l];
+                const int64_t r = l + m < n ? l + m : n;
+                const double score = jac ? (double)I2 / (double)(m + (int32_t)(r - l) - I2) : (double)I2 / (double)m;
+                if (score > bj) {
+                    bj = score;
+                    bl = (int32_t)l;
+                }
+            }
+            for (int d = WAVE / 2; d > 0; d >>= 1) {
+                const double oj = __shfl_xor(bj, d);
+                const int32_t ol = __shfl_xor(bl, d);
+                if (oj > bj || (oj == bj && ol < bl)) {
+                    bj = oj;
+                    bl = ol;
+                }
+            }
+            if ((threadIdx.x & (WAVE - 1)) == 0) {
+                s_bj[threadIdx.x / WAVE] = bj;
+                s_bl[threadIdx.x / WAVE] = bl;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                double J = -1.0;
-                int32_t inter = -1, same = -1, tl = -1, tr = -1, ssz = -1, pend = -1, mseg = -1;
-                if (A.metric == METRIC_BUCKET_LCS) {
-                    // Patience LIS of the pmatches sequence and of its reverse;
-                    // tails hold distinct read positions, so at most m of them.
-                    int32_t *tails = A.g_tails + o;
-                    int32_t best = 0;
-                    for (int dir = 0; dir < 2; dir++) {
-                        int32_t nt = 0;
-                        for (int64_t ii = 0; ii < span; ii++) {
-                            const int64_t t = dir == 0 ? ii : span - 1 - ii;
-                            const int32_t v = sid_arr[t];
-                            if (v < 0) continue;
-                            const int32_t s = v & 0x3fffffff;
-                            const uint32_t first = A.sd_first[o + s], occs = A.sd_occs[o + s];
-                            for (uint32_t x = 0; x < occs; x++) {
-                                const uint32_t at = dir == 0 ? first + occs - 1 - x : first + x;
-                                const int32_t val = A.k_r[o + A.ps_v[o + at]];
-                                int32_t lo2 = -1, hi2 = nt;
-                                while (lo2 + 1 < hi2) {
-                                    const int32_t mid = (lo2 + hi2) / 2;
-                                    if (tails[mid] < val) lo2 = mid; else hi2 = mid;
-                                }
-                                if (hi2 == nt) tails[nt++] = val; else tails[hi2] = val;
-                            }
-                        }
-                        if (nt > best) best = nt;
+                double J = s_bj[0];
+                int32_t L = s_bl[0];
+                for (int w = 1; w < NWAVES; w++)
+                    if (s_bj[w] > J || (s_bj[w] == J && s_bl[w] < L)) {
+                        J = s_bj[w];
+                        L = s_bl[w];
                     }
-                    J = (double)best / (double)m;
-                    inter = A.bs_m[base + p];
-                    same = A.bs_c[base + p];
-                    tl = A.bs_min[base + p];
-                    tr = A.bs_max[base + p];
-                    ssz = tr - tl;
-                    pend = len - 1;
-                    mseg = (int32_t)seg;
-                } else {
-                    const bool jac = A.metric == METRIC_JACCARD;
-                    const int32_t p_sz = len - A.k;
-                    int64_t l = l0, r = l0;
-                    int32_t I2 = 0, ss = 0;
-                    while (l < end) {
-                        const int32_t tlr = I.t_r[ks + l];
-                        while (r < end) {
-                            const bool in_w = ap ? I.t_r[ks + r] < tlr + m : r < l + m;
-                            if (!in_w) break;
-                            const int32_t v = sid_arr[r - l0];
-                            if (v >= 0) {
-                                ss += (v & 0x40000000) ? -1 : 1;
-                                if (--diff[v & 0x3fffffff] >= 0) I2++;
-                            }
-                            r++;
-                        }
-                        const int32_t s_kmers = (int32_t)(r - l);
-                        const double score = jac ? (double)I2 / (double)(m + s_kmers - I2)
-                                                 : (double)I2 / (double)m;
-                        if (l < r && score > J) {
-                            J = score;
-                            inter = I2;
-                            same = ss;
-                            tl = I.t_r[ks + l];
-                            tr = I.t_r[ks + r - 1];
-                            ssz = tr - tl + 1;
-                            pend = p_sz - 1;
-                            mseg = (int32_t)seg;
-                        }
-                        const int32_t v = sid_arr[l - l0];
-                        if (v >= 0) {
-                            ss -= (v & 0x40000000) ? -1 : 1;
-                            if (++diff[v & 0x3fffffff] >= 1) I2--;
-                        }
-                        l++;
-                    }
-                }
+                const int64_t r = L + m < n ? L + m : n;
+                const int32_t tl = I.t_r[ks + l0 + L], tr = I.t_r[ks + l0 + r - 1];
                 A.sc_j[base + p] = J;
-                A.sc_i[base + p] = inter;
-                A.sc_same[base + p] = same;
+                A.sc_i[base + p] = l_c[L];
+                A.sc_same[base + p] = keys[L];
                 A.sc_tl[base + p] = tl;
                 A.sc_tr[base + p] = tr;
-                A.sc_ssz[base + p] = ssz;
-                A.sc_pend[base + p] = pend;
-                A.sc_seg[base + p] = mseg;
+                A.sc_ssz[base + p] = tr - tl + 1;
+                A.sc_pend[base + p] = len - A.k - 1;
+                A.sc_seg[base + p] = (int32_t)seg;
             }
             __syncthreads();
+            continue;
+        }
+        // sequential forms: LCS, abs_pos windows, or spans too large for LDS
+        int32_t *tr_arr = keys;
+        int32_t *diff = u <= REFINE_CAP ? l_c : A.g_diff + (uint64_t)blockIdx.x * A.diff_stride;
+        for (int64_t x = threadIdx.x; x < n; x += WG) tr_arr[x] = I.t_r[ks + l0 + x];
+        for (int32_t x = threadIdx.x; x < u; x += WG) diff[x] = (int32_t)A.sd_occs[o + x];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double J = -1.0;
+            int32_t inter = -1, same = -1, tl = -1, tr = -1, ssz = -1, pend = -1, mseg = -1;
+            if (A.metric == METRIC_BUCKET_LCS) {
+                // Patience LIS of the pmatches sequence and of its reverse; the
+                // tails hold distinct read positions, so at most m of them.
+                int32_t *tails = A.g_tails + (uint64_t)blockIdx.x * A.diff_stride;
+                int32_t best = 0;
+                for (int dir = 0; dir < 2; dir++) {
+                    int32_t nt = 0;
+                    for (int64_t ii = 0; ii < n; ii++) {
+                        const int64_t x = dir == 0 ? ii : n - 1 - ii;
+                        const int32_t v = sid[x];
+                        if (v < 0) continue;
+                        const int32_t sd = v & 0x3fffffff;
+                        const uint32_t first = A.sd_first[o + sd], occs = A.sd_occs[o + sd];
+                        for (uint32_t q = 0; q < occs; q++) {
+                            const uint32_t at = dir == 0 ? first + occs - 1 - q : first + q;
+                            const int32_t val = A.k_r[o + A.ps_v[o + at]];
+                            int32_t lo2 = -1, hi2 = nt;
+                            while (lo2 + 1 < hi2) {
+                                const int32_t mid = (lo2 + hi2) / 2;
+                                if (tails[mid] < val) lo2 = mid; else hi2 = mid;
+                            }
+                            if (hi2 == nt) tails[nt++] = val; else tails[hi2] = val;
+                        }
+                    }
+                    if (nt > best) best = nt;
+                }
+                J = (double)best / (double)m;
+                inter = A.bs_m[base + p];
+                same = A.bs_c[base + p];
+                tl = A.bs_min[base + p];
+                tr = A.bs_max[base + p];
+                ssz = tr - tl;
+                pend = len - 1;
+                mseg = (int32_t)seg;
+            } else {
+                const bool jac = A.metric == METRIC_JACCARD;
+                int64_t l = 0, r = 0, bl = -1, br = -1;
+                int32_t I2 = 0, ss = 0;
+                while (l < n) {
+                    const int32_t tlr = tr_arr[l];
+                    while (r < n) {
+                        const bool in_w = ap ? tr_arr[r] < tlr + m : r < l + m;
+                        if (!in_w) break;
+                        const int32_t v = sid[r];
+                        if (v >= 0) {
+                            ss += (v & 0x40000000) ? -1 : 1;
+                            if (--diff[v & 0x3fffffff] >= 0) I2++;
+                        }
+                        r++;
+                    }
+                    const int32_t s_kmers = (int32_t)(r - l);
+                    const double score = jac ? (double)I2 / (double)(m + s_kmers - I2) : (double)I2 / (double)m;
+                    if (l < r && score > J) {
+                        J = score;
+                        inter = I2;
+                        same = ss;
+                        bl = l;
+                        br = r - 1;
+                    }
+                    const int32_t v = sid[l];
+                    if (v >= 0) {
+                        ss -= (v & 0x40000000) ? -1 : 1;
+                        if (++diff[v & 0x3fffffff] >= 1) I2--;
+                    }
+                    l++;
+                }
+                if (bl >= 0) {
+                    tl = tr_arr[bl];
+                    tr = tr_arr[br];
+                    ssz = tr - tl + 1;
+                    pend = len - A.k - 1;
+                    mseg = (int32_t)seg;
+                }
+            }
+            A.sc_j[base + p] = J;
+            A.sc_i[base + p] = inter;
+            A.sc_same[base + p] = same;
+            A.sc_tl[base + p] = tl;
+            A.sc_tr[base + p] = tr;
+            A.sc_ssz[base + p] = ssz;
+            A.sc_pend[base + p] = pend;
+            A.sc_seg[base + p] = mseg;
         }
         __syncthreads();
     }
@@ -1064,20 +1243,20 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
     k_chunk_counts<<<flat_grid((uint64_t)S.n, 1024), WG, 0, st>>>(S, k, nch);
 }
 
-size_t sketch_lds(int32_t k) { return (size_t)NWAVES * ((CHUNK_W + k - 1 + 15) & ~15u); }
+size_t sketch_lds(int32_t k) { return SKETCH_LUT_BYTES + (size_t)NWAVES * sketch_span(k); }
 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   uint64_t *chunk_val, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
+                   uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
                    uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
     uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
     if (g < 1) g = 1;
     if (g > max_grid) g = max_grid;
     if (pass == 0)
-        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, out_h,
-                                                             out_r, out_s);
+        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, lane_cnt,
+                                                             out_h, out_r, out_s);
     else
-        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, out_h,
-                                                             out_r, out_s);
+        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, lane_cnt,
+                                                             out_h, out_r, out_s);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
@@ -1133,7 +1312,7 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 }
 
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
-static const size_t REFINE_LDS = (size_t)REFINE_CAP * 8;
+static const size_t REFINE_LDS = ((size_t)REFINE_CAP * 3 + 2) * 4;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
 void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out,
@@ -1150,7 +1329,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_COUNT_BUCKETS: k_count_buckets<<<gg, WG, 0, st>>>(A); break;
     case ST_EXTRACT: k_extract<<<gg, WG, SORT_LDS, st>>>(A); break;
     case ST_PRUNE: k_prune<<<gg, WG, 0, st>>>(A, I); break;
-    case ST_REFINE: k_refine<<<gg, WG, REFINE_LDS, st>>>(A, I); break;
+    case ST_REFINE: k_refine<<<max_grid, WG, REFINE_LDS, st>>>(A, I); break;
     case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;
     default: break;
     }

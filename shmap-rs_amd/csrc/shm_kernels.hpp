@@ -9,22 +9,32 @@
 
 namespace shm {
 
-constexpr int CHUNK_W = 4096;             // windows per wave-chunk when sketching
-constexpr int LANE_W = CHUNK_W / 64;      // windows per lane
+constexpr int LANE_W = 128;               // consecutive windows rolled by one lane
+constexpr int CHUNK_W = 64 * LANE_W;      // windows per wave-chunk when sketching
+constexpr uint32_t SKETCH_LUT_BYTES = 4 * 256 * 8;  // pre-rotated ntHash tables in LDS
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
-constexpr int32_t REFINE_CAP = 6144;      // int32 entries per LDS array in refine
+constexpr int32_t REFINE_CAP = 4096;      // int32 entries per LDS array in refine
 constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
+
+// LDS bytes staged per wave: the chunk's CHUNK_W + k - 1 bases plus up to 3
+// bytes of dword-alignment slack on each side, plus one dword of bank skew per
+// 128 bytes, rounded up to 16.
+__host__ __device__ inline uint32_t sketch_span(int32_t k) {
+    const uint32_t raw = (uint32_t)(CHUNK_W + k + 8);
+    return (raw + ((raw >> 7) + 1) * 4 + 15u) & ~15u;
+}
 
 // A set of sequences: sequence i is bases[off[i], off[i+1]).
 struct SeqSet {
     const uint8_t *bases;
     const int64_t *off;
     int32_t n;
+    int64_t total;  // bytes readable at bases (off[n])
 };
 
 struct ShardGeom {
@@ -77,10 +87,14 @@ struct MapArgs {
     double *bs_sh;
     double *sc_j;
     int32_t *sc_i, *sc_same, *sc_tl, *sc_tr, *sc_ssz, *sc_pend, *sc_seg;
-    // scratch
-    int32_t *g_sid;        // per workgroup, sid_stride entries
+    // refine tasks: (group read, sorted bucket) of every bucket that can survive
+    uint2 *tasks;
+    uint32_t *task_count;
+    // per-workgroup scratch for spans/reads that overflow LDS
+    int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;
-    int32_t *g_diff, *g_tails; // at koff
+    int32_t *g_diff, *g_tails; // diff_stride entries each
+    uint64_t diff_stride;
 };
 
 }  // namespace shm
@@ -112,8 +126,8 @@ void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint6
 void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
 size_t sketch_lds(int32_t k);
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   uint64_t *chunk_val, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint64_t n_chunks,
-                   uint32_t max_grid, hipStream_t st);
+                   uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
+                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
                      hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);

@@ -133,6 +133,7 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
     DevBuf<uint64_t> chunk_start, chunk_cnt, chunk_koff, koff, part;
+    DevBuf<uint32_t> lane_cnt;
     std::vector<uint64_t> h_koff;
 };
 
@@ -153,12 +154,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.chunk_koff.ensure(C + 1));
     RCHK(so.koff.ensure(n + 1));
     RCHK(so.part.ensure(SCAN_PARTS + 1));
+    RCHK(so.lane_cnt.ensure(C * WAVE + 1));
     HIPCHK(hipMemcpyAsync(so.chunk_start.p, cs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
-    SeqSet S{d_bases, d_off, n};
+    SeqSet S{d_bases, d_off, n, h_off[n]};
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
-        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_cnt.p, nullptr, nullptr, nullptr, C,
-                      max_grid, st);
+        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_cnt.p, so.lane_cnt.p, nullptr, nullptr,
+                      nullptr, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
     launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, st);
@@ -171,8 +173,8 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(out_s.ensure(M + 1));
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
-        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_koff.p, out_h.p, out_r.p, out_s.p, C,
-                      max_grid, st);
+        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_koff.p, so.lane_cnt.p, out_h.p, out_r.p,
+                      out_s.p, C, max_grid, st);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
@@ -185,7 +187,7 @@ extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t le
     HIPCHK(hipStreamCreate(&st));
     DevBuf<uint8_t> bases;
     DevBuf<int64_t> off;
-    RCHK(bases.ensure(len + 1));
+    RCHK(bases.ensure(len + 64));
     RCHK(off.ensure(2));
     std::vector<int64_t> h_off = {0, len};
     if (len > 0) HIPCHK(hipMemcpyAsync(bases.p, seq, len, hipMemcpyHostToDevice, st));
@@ -512,7 +514,9 @@ struct shm_mapper {
     DevBuf<int32_t> bk_i32;  // tmp_m,c,min,max + bs_m,c,min,max + sc_* (7)
     DevBuf<uint8_t> bs_alive;
     DevBuf<double> bs_sh, sc_j;
-    DevBuf<int32_t> g_sid, g_diff, g_tails;
+    DevBuf<int32_t> g_sid, g_tr, g_diff, g_tails;
+    DevBuf<uint2> tasks;
+    DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
 };
 
@@ -660,8 +664,6 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->sd_hoff.ensure(M + 1));
     RCHK(mp->sd_ord.ensure(M + 1));
     RCHK(mp->sd_epre.ensure(M + 1));
-    RCHK(mp->g_diff.ensure(M + 1));
-    RCHK(mp->g_tails.ensure(M + 1));
     RCHK(mp->rd_i32.ensure((size_t)R * 10 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
     RCHK(mp->d_out.ensure(R));
@@ -709,8 +711,6 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_nb = ri + 9 * (size_t)R;
     A.rd_E = mp->rd_u64.p;
     A.rd_cap = mp->rd_u64.p + R;
-    A.g_diff = mp->g_diff.p;
-    A.g_tails = mp->g_tails.p;
     A.r0 = 0;
     A.RG = R;
 
@@ -738,10 +738,12 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         std::vector<uint64_t> hb(RG + 1), tb(RG + 1);
         hb[0] = tb[0] = 0;
         int32_t max_hl = 0;
+        uint64_t max_m = 0;
         for (int32_t i = 0; i < RG; i++) {
             hb[i + 1] = hb[i] + E[r0 + i];
             tb[i + 1] = tb[i] + cap[r0 + i];
             max_hl = std::max(max_hl, hl[r0 + i]);
+            max_m = std::max(max_m, mp->so.h_koff[r0 + i + 1] - mp->so.h_koff[r0 + i]);
         }
         RCHK(mp->grp_base.ensure(3 * (size_t)(RG + 1)));
         HIPCHK(hipMemcpyAsync(mp->grp_base.p, hb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
@@ -788,8 +790,18 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->bs_alive.ensure(B + 1));
         RCHK(mp->bs_sh.ensure(B + 1));
         RCHK(mp->sc_j.ensure(B + 1));
+        // per-workgroup scratch, used only by spans / reads too large for LDS
         const uint64_t stride = 2ull * (uint64_t)max_hl + 2;
-        RCHK(mp->g_sid.ensure(stride * mp->max_grid + 1));
+        const uint64_t dstride = max_m + 1;
+        const bool big_span = 2ull * (uint64_t)max_hl > (uint64_t)REFINE_CAP;
+        const bool big_read = max_m > (uint64_t)REFINE_CAP;
+        RCHK(mp->g_sid.ensure(big_span ? stride * mp->max_grid + 1 : 1));
+        RCHK(mp->g_tr.ensure(big_span ? stride * mp->max_grid + 1 : 1));
+        RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
+        RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
+        RCHK(mp->tasks.ensure(B + 1));
+        RCHK(mp->task_count.ensure(1));
+        HIPCHK(hipMemsetAsync(mp->task_count.p, 0, 4, st));
         A.bk_key = mp->bk_key.p;
         A.bk_val = mp->bk_val.p;
         A.tmp_g = mp->tmp_g.p;
@@ -815,7 +827,13 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.bs_sh = mp->bs_sh.p;
         A.sc_j = mp->sc_j.p;
         A.g_sid = mp->g_sid.p;
+        A.g_tr = mp->g_tr.p;
         A.sid_stride = stride;
+        A.g_diff = mp->g_diff.p;
+        A.g_tails = mp->g_tails.p;
+        A.diff_stride = dstride;
+        A.tasks = mp->tasks.p;
+        A.task_count = mp->task_count.p;
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
             TIME_BEGIN(s);
             launch_stage(s, A, I, mp->d_out.p, 0, mp->max_grid, st);
