@@ -150,6 +150,8 @@ def main():
     ap.add_argument("--plain-reads", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
@@ -173,6 +175,16 @@ def main():
     t_index = time.time() - t0
     st = idx.stats()
     log(f"[bench] GPU index: {st['indexed_kmers']} kmers, {st['indexed_hits']} hits in {t_index:.2f}s")
+
+    if args.calib_copy_mib:
+        nbytes = args.calib_copy_mib << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        assert hip.L.hipMalloc(C.byref(src), nbytes) == 0 and hip.L.hipMalloc(C.byref(dst), nbytes) == 0
+        assert hip.L.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+        hip.sync()
+        hip.free(src.value)
+        hip.free(dst.value)
+        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
 
     prm = G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
     mp = G.Mapper(idx, prm)

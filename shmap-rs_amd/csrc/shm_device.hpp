@@ -45,6 +45,20 @@ __device__ __forceinline__ uint64_t rotr64(uint64_t x, uint32_t n) {
     return n ? (x >> n) | (x << (64u - n)) : x;
 }
 
+// Rotations by one bit as two funnel shifts (v_alignbit_b32) on the halves.
+__device__ __forceinline__ uint64_t rotl1(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 31);  // hi << 1 | lo >> 31
+    const uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 31);  // lo << 1 | hi >> 31
+    return ((uint64_t)nhi << 32) | nlo;
+}
+__device__ __forceinline__ uint64_t rotr1(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t nhi = __builtin_amdgcn_alignbit(lo, hi, 1);  // hi >> 1 | lo << 31
+    const uint32_t nlo = __builtin_amdgcn_alignbit(hi, lo, 1);  // lo >> 1 | hi << 31
+    return ((uint64_t)nhi << 32) | nlo;
+}
+
 // ntHash seed of a base, forward strand; 0 for anything but ACGTacgt.
 __device__ __forceinline__ uint64_t nt_fw(uint8_t c) {
     const uint8_t l = c | 0x20;
@@ -89,6 +103,13 @@ __device__ __forceinline__ uint32_t index_lookup(const DevIndex &I, uint64_t h, 
 }
 
 // ---- wave / workgroup scans --------------------------------------------------
+
+// Orders the wave's LDS accesses around a point every lane reaches.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -215,6 +236,30 @@ __device__ void bitonic_keys(K *k, uint32_t n) {
                 if (b < n) cswap_k(k, a, b);
             }
             __syncthreads();
+        }
+    }
+}
+
+// One wave sorts n (key, val) pairs in LDS, ascending by (key, val).
+template <typename K, typename V>
+__device__ void wave_bitonic(K *k, V *v, uint32_t n) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint32_t N = 1;
+    while (N < n) N <<= 1;
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        const uint32_t half = size >> 1;
+        for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
+            const uint32_t blk = i / half, p = i % half;
+            const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
+            if (b < n) cswap(k, v, a, b);
+        }
+        wave_sync();
+        for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
+                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                if (b < n) cswap(k, v, a, b);
+            }
+            wave_sync();
         }
     }
 }

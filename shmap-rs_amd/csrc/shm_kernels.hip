@@ -87,43 +87,68 @@ __global__ void __launch_bounds__(WG) k_chunk_counts(SeqSet S, int32_t k, uint64
     }
 }
 
-// One wave per chunk of CHUNK_W windows; the chunk's bases plus the k-1 halo
-// are staged in LDS with dword loads, and the ntHash seeds with their fixed
-// rotations pre-applied (src/sketch.rs:90-97) sit in LDS tables indexed by the
-// raw byte.  Each lane rolls LANE_W consecutive windows; the initial window is
-// rolled in from a zero hash, which equals the direct definition of
+// Sequence of every chunk, looked up once per chunk in parallel so the sketch
+// waves start without a dependent search.
+__global__ void __launch_bounds__(WG) k_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq) {
+    const uint64_t total = chunk_start[S.n];
+    for (uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x; c < total; c += (uint64_t)gridDim.x * WG)
+        chunk_seq[c] = upper_bound_u64(chunk_start, (uint32_t)S.n + 1, c) - 1;
+}
+
+// Codes of four packed bytes: ((c >> 1) ^ (c >> 2)) & 3 sends a/c/g/t in
+// either case to 0/1/2/3; a byte that is not one of those letters (whose ntHash
+// seed is 0, src/sketch.rs:73-88) gets bit 2 set, and table entries 4-7 are 0.
+// Validity is checked by rebuilding each letter from its code with a byte
+// permute and comparing.
+__device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
+    const uint32_t x = v | 0x20202020u;                               // lower case
+    const uint32_t code = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
+    const uint32_t back = __builtin_amdgcn_perm(0x74676361u, 0x74676361u, code); // "acgt"[code]
+    uint32_t bad = x ^ back;                                          // nonzero byte = not ACGT
+    bad |= bad >> 4;
+    bad |= bad >> 2;
+    bad |= bad >> 1;
+    return code | ((bad & 0x01010101u) << 2);
+}
+
+// One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
+// are staged in LDS as one code byte per base (one dword of bank skew per 128
+// bytes, so lanes rolling runs 128 windows apart hit different banks).  The
+// ntHash seeds with the fixed rotations of the rolling update pre-applied
+// (src/sketch.rs:90-97) sit in two 8-entry LDS tables (codes 4-7 are the zero
+// seed of non-ACGT bytes): the incoming base's
+// (fw, rotl(rc, k-1)) and the outgoing base's (rotl(fw, k), rotr(rc, 1)), one
+// 16-byte load each.  Each lane rolls LANE_W consecutive windows; its first
+// window is rolled in from a zero hash, which equals the direct definition of
 // src/sketch.rs:180-184.  Pass 0 stores each lane's kept count; pass 1 writes
 // the kept k-mers at the lane's exclusive prefix within the scanned chunk
 // offset, so the output is in position order.
 template <int PASS>
 __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thres,
-                                               const uint64_t *chunk_start, uint64_t *chunk_val,
-                                               uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r,
-                                               uint8_t *out_s) {
+                                               const uint64_t *chunk_start, const uint32_t *chunk_seq,
+                                               uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
+                                               int32_t *out_r, uint8_t *out_s) {
     extern __shared__ uint8_t lds[];
-    uint64_t *T_fw = (uint64_t *)lds;   // fw[c]
-    uint64_t *T_rck1 = T_fw + 256;      // rotl(rc[c], k-1)
-    uint64_t *T_fwk = T_fw + 512;       // rotl(fw[c], k)
-    uint64_t *T_rcr1 = T_fw + 768;      // rotr(rc[c], 1)
+    ulonglong2 *T_in = (ulonglong2 *)lds;  // code -> (fw, rotl(rc, k-1))
+    ulonglong2 *T_out = T_in + 8;          // code -> (rotl(fw, k), rotr(rc, 1))
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     const uint32_t span = sketch_span(k);
     uint8_t *buf = lds + SKETCH_LUT_BYTES + (size_t)w * span;
-    for (int c = threadIdx.x; c < 256; c += WG) {
-        const uint64_t fw = nt_fw((uint8_t)c), rc = nt_rc((uint8_t)c);
-        T_fw[c] = fw;
-        T_rck1[c] = rotl64(rc, (uint32_t)(k - 1));
-        T_fwk[c] = rotl64(fw, (uint32_t)k);
-        T_rcr1[c] = rotr64(rc, 1);
+    if (threadIdx.x < 8) {
+        const uint8_t c = "ACGTNNNN"[threadIdx.x];  // code order of base_codes4
+        const uint64_t fw = nt_fw(c), rc = nt_rc(c);
+        T_in[threadIdx.x] = make_ulonglong2(fw, rotl64(rc, (uint32_t)(k - 1)));
+        T_out[threadIdx.x] = make_ulonglong2(rotl64(fw, (uint32_t)k), rotr64(rc, 1));
     }
     __syncthreads();
     const uint64_t total = chunk_start[S.n];
+#define SKB(q) buf[(uint32_t)(q) + (((uint32_t)(q) >> 7) << 2)]
     for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
         const uint64_t c = cb + w;
         const bool valid = c < total;
         int64_t w0 = 0, w1 = 0;
-        uint32_t shift = 0;
         if (valid) {
-            const uint32_t seq = upper_bound_u64(chunk_start, (uint32_t)S.n + 1, c) - 1;
+            const uint32_t seq = chunk_seq[c];
             const int64_t len = S.off[seq + 1] - S.off[seq];
             const int64_t nw = len - k + 1;
             w0 = (int64_t)(c - chunk_start[seq]) * CHUNK_W;
@@ -131,29 +156,31 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             const int64_t nb = w1 - w0 + k - 1;
             const uintptr_t src = (uintptr_t)(S.bases + S.off[seq] + w0);
             const uintptr_t a = src & ~(uintptr_t)3;
-            shift = (uint32_t)(src - a);
-            const int64_t ndw = (int64_t)((src + nb - a + 3) >> 2);
+            const uint32_t sh = (uint32_t)(src - a) * 8;
+            const uintptr_t lim = (uintptr_t)(S.bases + S.total);  // never read past the set
             const uint32_t *s32 = (const uint32_t *)a;
             uint32_t *b32 = (uint32_t *)buf;
-            const uintptr_t lim = (uintptr_t)(S.bases + S.total);  // never read past the set
-            for (int64_t i = lane; i < ndw; i += WAVE) {
-                const uintptr_t at = a + 4 * (uintptr_t)i;
-                uint32_t v;
-                if (at + 4 <= lim) {
-                    v = s32[i];
-                } else {
-                    v = 0;
+            const int64_t ng = (nb + 3) >> 2;  // code dwords
+            if (a + 4 * (uintptr_t)(ng + 1) <= lim) {
+                // every aligned dword this chunk needs lies inside the set
+#pragma unroll 4
+                for (int64_t g = lane; g < ng; g += WAVE) {
+                    const uint32_t d0 = s32[g], d1 = s32[g + 1];
+                    const uint32_t v = sh ? (uint32_t)((((uint64_t)d1 << 32) | d0) >> sh) : d0;
+                    b32[g + (g >> 5)] = base_codes4(v);
+                }
+            } else {
+                for (int64_t g = lane; g < ng; g += WAVE) {
+                    const uintptr_t at = src + 4 * (uintptr_t)g;  // first byte of this group
+                    uint32_t v = 0;
                     for (uint32_t j = 0; j < 4; j++)
                         if (at + j < lim) v |= (uint32_t)(*(const uint8_t *)(at + j)) << (8 * j);
+                    b32[g + (g >> 5)] = base_codes4(v);
                 }
-                b32[i + (i >> 5)] = v;  // one dword of skew per 128 bytes
             }
         }
         __syncthreads();
         if (valid) {
-            // byte q of the chunk sits at q' = q + shift plus one dword of skew per
-            // 128 bytes, so lanes rolling runs 128 windows apart hit different banks
-#define SKB(q) buf[(uint32_t)((q) + shift) + (((uint32_t)((q) + shift) >> 7) << 2)]
             const int64_t nwin = w1 - w0;
             const int64_t lw0 = (int64_t)lane * LANE_W;
             const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
@@ -166,28 +193,72 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             if (lw0 < lw1) {
                 uint64_t hf = 0, hr = 0;
                 for (int32_t i = 0; i < k; i++) {
-                    const uint8_t ch = SKB(lw0 + i);
-                    hf = rotl64(hf, 1) ^ T_fw[ch];
-                    hr = rotr64(hr, 1) ^ T_rck1[ch];
+                    const ulonglong2 ti = T_in[SKB(lw0 + i)];
+                    hf = rotl1(hf) ^ ti.x;
+                    hr = rotr1(hr) ^ ti.y;
                 }
-                for (int64_t wi = lw0;; wi++) {
+                auto put = [&](int64_t wi, uint64_t h, bool strand) {
+                    const uint64_t o = base_out + cnt;
+                    out_h[o] = h;
+                    out_r[o] = (int32_t)(w0 + wi + k - 1);
+                    out_s[o] = strand;
+                };
+                {
                     const uint64_t h = hr ^ hf;
                     if (h <= thres) {
-                        if (PASS == 1) {
-                            const uint64_t o = base_out + cnt;
-                            out_h[o] = h;
-                            out_r[o] = (int32_t)(w0 + wi + k - 1);
-                            out_s[o] = hf > hr;
-                        }
+                        if (PASS == 1) put(lw0, h, hf > hr);
                         cnt++;
                     }
-                    if (wi + 1 >= lw1) break;
-                    const uint8_t in = SKB(wi + k), out = SKB(wi);
-                    hf = rotl64(hf, 1) ^ T_fwk[out] ^ T_fw[in];
-                    hr = rotr64(hr, 1) ^ T_rcr1[out] ^ T_rck1[in];
+                }
+                // windows lw0+1 .. lw1-1; the step into window lw0+j+1 rolls out
+                // base lw0+j and in base lw0+j+k.  Out positions are dword aligned
+                // (lw0 is a multiple of LANE_W), so codes are read four at a time;
+                // the in stream is re-aligned by a funnel shift of two dwords.
+                // Kept windows are rare, so the four steps only record a keep mask
+                // and the writes (pass 1) happen after them.
+                const int64_t nsteps = lw1 - lw0 - 1;
+                const uint32_t ish = (uint32_t)(k & 3) * 8;
+                int64_t j = 0;
+                for (; j + 4 <= nsteps; j += 4) {
+                    const uint32_t pq = (uint32_t)(lw0 + j);
+                    const uint32_t od = *(const uint32_t *)&SKB(pq);
+                    const uint32_t iq = (uint32_t)(lw0 + j + k) & ~3u;
+                    const uint32_t i0 = *(const uint32_t *)&SKB(iq);
+                    const uint32_t i1 = *(const uint32_t *)&SKB(iq + 4);
+                    const uint32_t id = ish ? (uint32_t)((((uint64_t)i1 << 32) | i0) >> ish) : i0;
+                    uint64_t hs[4];
+                    uint32_t keep = 0, str = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const ulonglong2 to = T_out[(od >> (8 * t)) & 0xff], ti = T_in[(id >> (8 * t)) & 0xff];
+                        hf = rotl1(hf) ^ to.x ^ ti.x;
+                        hr = rotr1(hr) ^ to.y ^ ti.y;
+                        hs[t] = hr ^ hf;
+                        keep |= (uint32_t)(hs[t] <= thres) << t;
+                        str |= (uint32_t)(hf > hr) << t;
+                    }
+                    if (PASS == 1 && keep) {
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            if (keep & (1u << t)) {
+                                put(lw0 + j + t + 1, hs[t], (str >> t) & 1u);
+                                cnt++;
+                            }
+                    } else {
+                        cnt += __builtin_popcount(keep);
+                    }
+                }
+                for (; j < nsteps; j++) {
+                    const ulonglong2 to = T_out[SKB(lw0 + j)], ti = T_in[SKB(lw0 + j + k)];
+                    hf = rotl1(hf) ^ to.x ^ ti.x;
+                    hr = rotr1(hr) ^ to.y ^ ti.y;
+                    const uint64_t h = hr ^ hf;
+                    if (h <= thres) {
+                        if (PASS == 1) put(lw0 + j + 1, h, hf > hr);
+                        cnt++;
+                    }
                 }
             }
-#undef SKB
             if (PASS == 0) {
                 lane_cnt[c * WAVE + lane] = cnt;
                 uint32_t s2 = cnt;
@@ -197,6 +268,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
         }
         __syncthreads();
     }
+#undef SKB
 }
 
 // Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]].
@@ -405,6 +477,7 @@ __global__ void __launch_bounds__(WG) k_sort_kmers(MapArgs A) {
     for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
         const uint64_t o = A.koff[rd];
         const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        if (m <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
         for (uint32_t i = threadIdx.x; i < m; i += WG) {
             A.ps_h[o + i] = A.k_h[o + i];
             A.ps_v[o + i] = i;
@@ -423,6 +496,7 @@ __global__ void __launch_bounds__(WG) k_group_seeds(MapArgs A, DevIndex I) {
     for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
         const uint64_t o = A.koff[rd];
         const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        if (m <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
         if (threadIdx.x == 0) {
             s_u = 0;
             s_pm = 0;
@@ -467,12 +541,130 @@ __global__ void __launch_bounds__(WG) k_group_seeds(MapArgs A, DevIndex I) {
     }
 }
 
+// The seeding stages for one read per wave (reads with at most WAVE_SEED_CAP
+// sketched k-mers; larger reads take the workgroup kernels below):
+//   unique_elements_with_info (src/shmap/seeding.rs:26-77): sort by (hash,
+//     position), one seed per hash run keeping the smallest-r strand, index
+//     probe, stable rarest-first order as a sort by (hits_in_t, seed_num);
+//   the seed budget S, halflen and the consumed prefix of match_seeds
+//     (src/shmap/mod.rs:726-742, src/shmap/seeding.rs:101-106, 207-209).
+// Everything stays in the wave's LDS slice; seeds are written out once.
+__global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    uint8_t *my = lds + (size_t)w * WAVE_SEED_LDS;
+    uint64_t *kh = (uint64_t *)my;                                  // hashes
+    uint32_t *kv = (uint32_t *)(my + WAVE_SEED_CAP * 8);            // positions
+    uint64_t *sk = (uint64_t *)(my + WAVE_SEED_CAP * 12);           // (hits, seed) keys
+    uint32_t *sv = (uint32_t *)(my + WAVE_SEED_CAP * 20);           // seed ids
+    uint32_t *so = (uint32_t *)(my + WAVE_SEED_CAP * 24);           // occs by seed id
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const double theta2 = A.theta - A.min_diff;
+    for (int32_t rd = blockIdx.x * NWAVES + w; rd < A.R; rd += gridDim.x * NWAVES) {
+        const uint64_t o = A.koff[rd];
+        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        if (m > WAVE_SEED_CAP) continue;
+        for (uint32_t i = lane; i < m; i += WAVE) {
+            kh[i] = A.k_h[o + i];
+            kv[i] = i;
+        }
+        wave_sync();
+        wave_bitonic(kh, kv, m);
+        // runs of equal hash -> seeds in hash order
+        uint32_t u = 0, pm = 0;
+        for (uint32_t b0 = 0; b0 < m; b0 += WAVE) {
+            const uint32_t i = b0 + lane;
+            const bool head = i < m && (i == 0 || kh[i] != kh[i - 1]);
+            const uint64_t bal = __ballot(head);
+            if (i < m) A.ps_v[o + i] = kv[i];
+            if (head) {
+                const uint32_t sid = u + __popcll(bal & lt);
+                const uint64_t h = kh[i];
+                uint32_t e = i + 1;
+                while (e < m && kh[e] == h) e++;
+                uint32_t hoff;
+                const uint32_t cnt = index_lookup(I, h, hoff);
+                A.sd_h[o + sid] = h;
+                A.sd_first[o + sid] = i;
+                A.sd_occs[o + sid] = e - i;
+                A.sd_strand[o + sid] = A.k_s[o + kv[i]];
+                A.sd_cnt[o + sid] = cnt;
+                A.sd_hoff[o + sid] = hoff;
+                sk[sid] = ((uint64_t)cnt << 32) | sid;
+                sv[sid] = sid;
+                so[sid] = e - i;
+                pm += cnt;
+            }
+            u += __popcll(bal);
+        }
+        for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
+        wave_sync();
+        wave_bitonic(sk, sv, u);
+        // budget: consume seeds while fewer than S occurrences are consumed
+        const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
+        const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
+        const int32_t hl = A.abs_pos ? len : (int32_t)m;
+        const bool ok = hl >= MIN_HALFLEN;
+        uint32_t occ_carry = 0, cnt_carry = 0, c = 0, seeds = 0, mx = 0;
+        for (uint32_t b0 = 0; ok && b0 < u; b0 += WAVE) {
+            const uint32_t q = b0 + lane;
+            uint32_t occ = 0, cnt = 0, sid = 0;
+            if (q < u) {
+                sid = sv[q];
+                occ = so[sid];
+                cnt = (uint32_t)(sk[q] >> 32);
+                A.sd_ord[o + q] = sid;
+            }
+            const uint32_t oinc = wave_incl_scan(occ);
+            const bool consumed = q < u && (int64_t)(occ_carry + oinc - occ) < (int64_t)S;
+            const uint32_t cv = consumed ? cnt : 0u;
+            const uint32_t cinc = wave_incl_scan(cv);
+            if (consumed) A.sd_epre[o + q] = cnt_carry + cinc - cv;
+            const uint64_t cb = __ballot(consumed);
+            c += __popcll(cb);
+            uint32_t so_ = consumed ? occ : 0u, mm = consumed ? cnt : 0u;
+            for (int d = WAVE / 2; d > 0; d >>= 1) {
+                so_ += __shfl_xor(so_, d);
+                const uint32_t om = __shfl_xor(mm, d);
+                mm = om > mm ? om : mm;
+            }
+            seeds += so_;
+            mx = mm > mx ? mm : mx;
+            occ_carry += __shfl(oinc, WAVE - 1);
+            cnt_carry += __shfl(cinc, WAVE - 1);
+        }
+        if (!ok)
+            for (uint32_t q = lane; q < u; q += WAVE) A.sd_ord[o + q] = sv[q];
+        if (lane == 0) {
+            const uint32_t E = cnt_carry;
+            A.rd_u[rd] = (int32_t)u;
+            A.rd_possible[rd] = (int32_t)pm;  // an i32 sum (src/shmap/mod.rs:689)
+            A.rd_S[rd] = S;
+            A.rd_hl[rd] = hl;
+            A.rd_ok[rd] = ok ? 1 : 0;
+            A.rd_c[rd] = (int32_t)c;
+            A.rd_bseeds[rd] = (int32_t)seeds;
+            A.rd_max_seed[rd] = (int32_t)mx;
+            A.rd_seed_matches[rd] = (int32_t)E;
+            A.rd_E[rd] = E;
+            uint64_t cap = 0;
+            if (E > 0) {
+                cap = 64;
+                while (cap < 2ull * E) cap <<= 1;
+            }
+            A.rd_cap[rd] = cap;
+        }
+        wave_sync();
+    }
+}
+
 __global__ void __launch_bounds__(WG) k_sort_seeds(MapArgs A) {
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
     for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
         const uint64_t o = A.koff[rd];
+        if (A.koff[rd + 1] - o <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
         const uint32_t u = (uint32_t)A.rd_u[rd];
         wg_sort_pairs(A.sd_key + o, A.sd_ord + o, u, sk, sv, SORT_CAP);
     }
@@ -488,6 +680,7 @@ __global__ void __launch_bounds__(WG) k_budget(MapArgs A) {
     for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
         const uint64_t o = A.koff[rd];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        if (m <= (int32_t)WAVE_SEED_CAP) continue;  // handled by k_seed_wave
         const int32_t u = A.rd_u[rd];
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const double theta2 = A.theta - A.min_diff;
@@ -1245,18 +1438,23 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
 
 size_t sketch_lds(int32_t k) { return SKETCH_LUT_BYTES + (size_t)NWAVES * sketch_span(k); }
 
+void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
+                      hipStream_t st) {
+    k_chunk_seq<<<flat_grid(n_chunks, 8192), WG, 0, st>>>(S, chunk_start, chunk_seq);
+}
+
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
+                   int32_t *out_r, uint8_t *out_s, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
     uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
     if (g < 1) g = 1;
     if (g > max_grid) g = max_grid;
     if (pass == 0)
-        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, lane_cnt,
-                                                             out_h, out_r, out_s);
+        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
+                                                             lane_cnt, out_h, out_r, out_s);
     else
-        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_val, lane_cnt,
-                                                             out_h, out_r, out_s);
+        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
+                                                             lane_cnt, out_h, out_r, out_s);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
@@ -1320,6 +1518,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     const uint32_t gb = read_grid(A.R, max_grid);   // batch-wide per-read kernels
     const uint32_t gg = read_grid(A.RG, max_grid);  // group per-read kernels
     switch (stage) {
+    case ST_SEED_WAVE: k_seed_wave<<<read_grid((A.R + NWAVES - 1) / NWAVES, max_grid), WG, (size_t)NWAVES * WAVE_SEED_LDS, st>>>(A, I); break;
     case ST_SORT_KMERS: k_sort_kmers<<<gb, WG, SORT_LDS, st>>>(A); break;
     case ST_GROUP_SEEDS: k_group_seeds<<<gb, WG, 0, st>>>(A, I); break;
     case ST_SORT_SEEDS: k_sort_seeds<<<gb, WG, SORT_LDS, st>>>(A); break;

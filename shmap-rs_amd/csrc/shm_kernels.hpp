@@ -11,13 +11,15 @@ namespace shm {
 
 constexpr int LANE_W = 128;               // consecutive windows rolled by one lane
 constexpr int CHUNK_W = 64 * LANE_W;      // windows per wave-chunk when sketching
-constexpr uint32_t SKETCH_LUT_BYTES = 4 * 256 * 8;  // pre-rotated ntHash tables in LDS
+constexpr uint32_t SKETCH_LUT_BYTES = 256;  // two 8-entry tables of pre-rotated ntHash seed pairs
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
 constexpr int32_t REFINE_CAP = 4096;      // int32 entries per LDS array in refine
 constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
+constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
+constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 28;  // per-wave LDS of k_seed_wave
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
@@ -106,6 +108,7 @@ struct DevIndex;
 enum Stage {
     ST_SKETCH_COUNT = 0,
     ST_SKETCH_WRITE,
+    ST_SEED_WAVE,
     ST_SORT_KMERS,
     ST_GROUP_SEEDS,
     ST_SORT_SEEDS,
@@ -125,9 +128,11 @@ void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint6
                  hipStream_t st);
 void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
 size_t sketch_lds(int32_t k);
+void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
+                      hipStream_t st);
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
+                   int32_t *out_r, uint8_t *out_s, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
                      hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);

@@ -133,7 +133,7 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
     DevBuf<uint64_t> chunk_start, chunk_cnt, chunk_koff, koff, part;
-    DevBuf<uint32_t> lane_cnt;
+    DevBuf<uint32_t> lane_cnt, chunk_seq;
     std::vector<uint64_t> h_koff;
 };
 
@@ -155,12 +155,14 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.koff.ensure(n + 1));
     RCHK(so.part.ensure(SCAN_PARTS + 1));
     RCHK(so.lane_cnt.ensure(C * WAVE + 1));
+    RCHK(so.chunk_seq.ensure(C + 1));
     HIPCHK(hipMemcpyAsync(so.chunk_start.p, cs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
     SeqSet S{d_bases, d_off, n, h_off[n]};
+    if (C > 0) launch_chunk_seq(S, so.chunk_start.p, so.chunk_seq.p, C, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
-        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_cnt.p, so.lane_cnt.p, nullptr, nullptr,
-                      nullptr, C, max_grid, st);
+        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_cnt.p, so.lane_cnt.p,
+                      nullptr, nullptr, nullptr, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
     launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, st);
@@ -173,8 +175,8 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(out_s.ensure(M + 1));
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
-        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_koff.p, so.lane_cnt.p, out_h.p, out_r.p,
-                      out_s.p, C, max_grid, st);
+        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_koff.p, so.lane_cnt.p,
+                      out_h.p, out_r.p, out_s.p, C, max_grid, st);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
@@ -479,7 +481,7 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 // mapper
 // ---------------------------------------------------------------------------
 
-static const char *kStageNames[ST_COUNT] = {"sketch_count", "sketch_write", "sort_kmers", "group_seeds",
+static const char *kStageNames[ST_COUNT] = {"sketch_count", "sketch_write", "seed_wave", "sort_kmers", "group_seeds",
                                             "sort_seeds",   "budget",       "tab_init",   "contrib",
                                             "count_buckets", "extract",     "prune",      "refine",
                                             "final"};
@@ -714,7 +716,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.r0 = 0;
     A.RG = R;
 
-    for (Stage s : {ST_SORT_KMERS, ST_GROUP_SEEDS, ST_SORT_SEEDS, ST_BUDGET}) {
+    bool any_large = false;  // a read whose sketch exceeds one wave's LDS slice
+    for (int32_t i = 0; i < R && !any_large; i++) any_large = mp->so.h_koff[i + 1] - mp->so.h_koff[i] > WAVE_SEED_CAP;
+    for (Stage s : {ST_SEED_WAVE, ST_SORT_KMERS, ST_GROUP_SEEDS, ST_SORT_SEEDS, ST_BUDGET}) {
+        if (s != ST_SEED_WAVE && !any_large) continue;
         TIME_BEGIN(s);
         launch_stage(s, A, I, nullptr, 0, mp->max_grid, st);
         TIME_END(s);
