@@ -121,7 +121,7 @@ class Hip:
 
 
 # ---------------------------------------------------------------------------
-# roofline: algorithmic bytes of each stage (documented in DESIGN.md)
+# roofline: algorithmic bytes of each stage
 # ---------------------------------------------------------------------------
 
 def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
@@ -129,8 +129,9 @@ def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
     u = res["u"].astype(np.int64).sum()
     E = res["seed_matches"].astype(np.int64).clip(min=0).sum()
     return {
-        "sketch_count": bases_total,                       # every base read once
-        "sketch_write": bases_total + 13 * m,              # bases + (h, r, strand) per kept k-mer
+        "sketch": bases_total + 13 * m,                    # every base read once, (h, r, strand) per kept k-mer
+        "sketch_compact": 2 * 13 * m,                      # staged k-mers moved to their final positions
+        "seed_wave": 13 * m + 16 * u + 37 * u,             # sketch in, one table slot per probe, seed rows out
         "sort_kmers": 8 * m + 2 * 12 * m,                  # hashes in, (hash, index) pairs out
         "group_seeds": 12 * m + 16 * u + 37 * u,           # sorted pairs, one table slot per probe, seed rows
         "sort_seeds": 2 * 12 * u,
@@ -213,6 +214,8 @@ def main():
     res_t = mp.map_device(d_bases, d_off, n_reads)
     mp.set_timing(False)
     total_ms, stage_ms = mp.last_timing()
+    counts = mp.last_counts()
+    log(f"[bench] work counts: {counts}")
     assert np.array_equal(res_t["mapped"], res["mapped"])
     sb = stage_bytes(res, int(off[-1]))
     dom = max(stage_ms, key=lambda s: stage_ms[s])
@@ -297,6 +300,7 @@ def main():
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
         "index_build_s": round(t_index, 3),
         "stage_ms": {s: round(v, 3) for s, v in stage_ms.items()},
+        "work_counts": counts,
         "stage_gbs": stage_rates,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -169,6 +169,13 @@ int shm_format_unmapped(const char *query_id, int32_t read_len, char *buf, size_
 int shm_mapper_last_timing(const shm_mapper *mp, double *total_ms, double *stage_ms,
                            int32_t n_stages, int32_t *n_written);
 const char *shm_mapper_stage_name(int32_t stage);
+/* Work counts of the last batch on this mapper: [0] k-mers sketched, [1]
+ * distinct k-mers (seeds), [2] buckets counted, [3] buckets refined (those that
+ * survive the full-extension prune and that a match_rest pass can reach), [4]
+ * read groups (table-budget batches), [5] first-pass refine tasks, [6] buckets
+ * only a second pass can reach (refined for reads mapped by the first).
+ * Writes up to n values and their number to *n_written. */
+int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */
 int shm_mapper_set_timing(shm_mapper *mp, int32_t on);

@@ -8,6 +8,7 @@
 // reads, each read's intermediate data living in flat arrays at per-read
 // offsets.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "shm_device.hpp"
@@ -116,18 +117,23 @@ __device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
 // bytes, so lanes rolling runs 128 windows apart hit different banks).  The
 // ntHash seeds with the fixed rotations of the rolling update pre-applied
 // (src/sketch.rs:90-97) sit in two 8-entry LDS tables (codes 4-7 are the zero
-// seed of non-ACGT bytes): the incoming base's
-// (fw, rotl(rc, k-1)) and the outgoing base's (rotl(fw, k), rotr(rc, 1)), one
-// 16-byte load each.  Each lane rolls LANE_W consecutive windows; its first
-// window is rolled in from a zero hash, which equals the direct definition of
-// src/sketch.rs:180-184.  Pass 0 stores each lane's kept count; pass 1 writes
-// the kept k-mers at the lane's exclusive prefix within the scanned chunk
-// offset, so the output is in position order.
+// seed of non-ACGT bytes): the incoming base's (fw, rotl(rc, k-1)) and the
+// outgoing base's (rotl(fw, k), rotr(rc, 1)), one 16-byte load each.  Each lane
+// rolls LANE_W consecutive windows; its first window is rolled in from a zero
+// hash, which equals the direct definition of src/sketch.rs:180-184.
+//
+// PASS 0 visits every chunk and writes each lane's kept k-mers, in position
+// order, to lane-strided staging slots (slot i of lane l of chunk c at
+// (c*lcap + i)*WAVE + l, for i < lcap); it stores every lane's full kept count
+// and the chunk's total, and lists the chunks in which a lane kept more than
+// lcap.  PASS 1 re-rolls only the listed chunks and writes straight to the final
+// positions: the chunk's scanned offset plus the lane's exclusive prefix.
 template <int PASS>
 __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thres,
                                                const uint64_t *chunk_start, const uint32_t *chunk_seq,
                                                uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                                               int32_t *out_r, uint8_t *out_s) {
+                                               int32_t *out_r, uint8_t *out_s, uint32_t lcap,
+                                               uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
     ulonglong2 *T_in = (ulonglong2 *)lds;  // code -> (fw, rotl(rc, k-1))
     ulonglong2 *T_out = T_in + 8;          // code -> (rotl(fw, k), rotr(rc, 1))
@@ -141,11 +147,11 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
         T_out[threadIdx.x] = make_ulonglong2(rotl64(fw, (uint32_t)k), rotr64(rc, 1));
     }
     __syncthreads();
-    const uint64_t total = chunk_start[S.n];
+    const uint64_t total = PASS == 0 ? chunk_start[S.n] : (uint64_t)*ovf_count;
 #define SKB(q) buf[(uint32_t)(q) + (((uint32_t)(q) >> 7) << 2)]
     for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
-        const uint64_t c = cb + w;
-        const bool valid = c < total;
+        const bool valid = cb + w < total;
+        const uint64_t c = !valid ? 0 : PASS == 0 ? cb + w : (uint64_t)ovf_list[cb + w];
         int64_t w0 = 0, w1 = 0;
         if (valid) {
             const uint32_t seq = chunk_seq[c];
@@ -198,7 +204,13 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                     hr = rotr1(hr) ^ ti.y;
                 }
                 auto put = [&](int64_t wi, uint64_t h, bool strand) {
-                    const uint64_t o = base_out + cnt;
+                    uint64_t o;
+                    if (PASS == 1) {
+                        o = base_out + cnt;
+                    } else {
+                        if (cnt >= lcap) return;
+                        o = ((uint64_t)c * lcap + cnt) * WAVE + lane;
+                    }
                     out_h[o] = h;
                     out_r[o] = (int32_t)(w0 + wi + k - 1);
                     out_s[o] = strand;
@@ -206,7 +218,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                 {
                     const uint64_t h = hr ^ hf;
                     if (h <= thres) {
-                        if (PASS == 1) put(lw0, h, hf > hr);
+                        put(lw0, h, hf > hr);
                         cnt++;
                     }
                 }
@@ -237,15 +249,13 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                         keep |= (uint32_t)(hs[t] <= thres) << t;
                         str |= (uint32_t)(hf > hr) << t;
                     }
-                    if (PASS == 1 && keep) {
+                    if (keep) {
 #pragma unroll
                         for (int t = 0; t < 4; t++)
                             if (keep & (1u << t)) {
                                 put(lw0 + j + t + 1, hs[t], (str >> t) & 1u);
                                 cnt++;
                             }
-                    } else {
-                        cnt += __builtin_popcount(keep);
                     }
                 }
                 for (; j < nsteps; j++) {
@@ -254,7 +264,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                     hr = rotr1(hr) ^ to.y ^ ti.y;
                     const uint64_t h = hr ^ hf;
                     if (h <= thres) {
-                        if (PASS == 1) put(lw0 + j + 1, h, hf > hr);
+                        put(lw0 + j + 1, h, hf > hr);
                         cnt++;
                     }
                 }
@@ -264,11 +274,40 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                 uint32_t s2 = cnt;
                 for (int d = WAVE / 2; d > 0; d >>= 1) s2 += __shfl_xor(s2, d);
                 if (lane == 0) chunk_val[c] = s2;
+                if (__ballot(cnt > lcap) != 0 && lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)c;
             }
         }
         __syncthreads();
     }
 #undef SKB
+}
+
+// Moves the staged k-mers of every chunk without a lane overflow to their
+// final positions (the chunk's scanned offset plus the lane's exclusive
+// prefix); the listed overflow chunks are written by k_sketch<1>.
+__global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
+                                                       const uint64_t *chunk_koff, uint32_t lcap,
+                                                       const uint64_t *st_h, const int32_t *st_r,
+                                                       const uint8_t *st_s, uint64_t *out_h, int32_t *out_r,
+                                                       uint8_t *out_s) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    for (uint64_t c = (uint64_t)blockIdx.x * NWAVES + w; c < n_chunks; c += (uint64_t)gridDim.x * NWAVES) {
+        const uint32_t cnt = lane_cnt[c * WAVE + lane];
+        if (__ballot(cnt > lcap) != 0) continue;
+        const uint32_t pre = wave_incl_scan(cnt) - cnt;
+        uint32_t mx = cnt;
+        for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d));
+        const uint64_t ob = chunk_koff[c] + pre;
+        const uint64_t sb = (uint64_t)c * lcap * WAVE + lane;
+        for (uint32_t i = 0; i < mx; i++) {
+            if (i < cnt) {
+                const uint64_t si = sb + (uint64_t)i * WAVE;
+                out_h[ob + i] = st_h[si];
+                out_r[ob + i] = st_r[si];
+                out_s[ob + i] = st_s[si];
+            }
+        }
+    }
 }
 
 // Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]].
@@ -953,6 +992,7 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
         const int32_t seeds0 = A.rd_bseeds[rd];
+        if (threadIdx.x == 0) A.rd_flag[rd] = INT32_MAX;
         for (int32_t p = threadIdx.x; p < nb; p += WG) {
             int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
             int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
@@ -998,8 +1038,13 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
             A.bs_alive[base + p] = alive;
             A.bs_sh[base + p] = sh;
             if (alive) {
-                const uint32_t t = atomicAdd(A.task_count, 1u);
-                A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+                if (A.no_bucket_pruning || !(sh < A.theta)) {
+                    const uint32_t t = atomicAdd(&A.task_count[0], 1u);
+                    A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+                } else {
+                    const uint32_t t = atomicAdd(&A.task_count[1], 1u);
+                    A.tasks[A.task_cap - 1 - t] = make_uint2((uint32_t)lr, (uint32_t)p);
+                }
             }
         }
         __syncthreads();
@@ -1057,15 +1102,28 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     int32_t *l_b = l_sid + REFINE_CAP;         // REFINE_CAP (+1)
     int32_t *l_c = l_b + REFINE_CAP + 1;       // REFINE_CAP + 1
     __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_pre[WG], s_lo[WG];
+    __shared__ uint8_t s_str[WG];
     __shared__ uint32_t s_nm;
     __shared__ double s_bj[NWAVES];
     __shared__ int32_t s_bl[NWAVES];
     const int ap = A.abs_pos;
-    const uint32_t ntask = *A.task_count;
+    const uint32_t ntask = A.task_count[A.refine_pass];
     for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
-        const uint2 task = A.tasks[t];
+        const uint2 task = A.tasks[A.refine_pass == 0 ? t : A.task_cap - 1 - t];
         const int32_t lr = (int32_t)task.x, p = (int32_t)task.y;
         const int32_t rd = A.r0 + lr;
+        if (A.refine_pass == 1) {
+            // The second pass runs only for reads whose first pass mapped, with
+            // threshold best*(1-min_diff); best is at least the score of the
+            // first sorted bucket that beat theta (the first pass takes it, and
+            // its threshold only rises), so a bound below that product is never
+            // reached.
+            const int32_t first = A.rd_flag[rd];
+            if (first == INT32_MAX) continue;
+            if (A.bs_sh[A.bk_base[lr] + p] < A.sc_j[A.bk_base[lr] + first] * (1.0 - A.min_diff)) continue;
+            if (threadIdx.x == 0) atomicAdd(&A.task_count[2], 1u);
+        }
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], hl = A.rd_hl[rd];
@@ -1077,6 +1135,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         if (A.metric == METRIC_BUCKET_SH) {
             if (threadIdx.x == 0) {
                 const int32_t tl = A.bs_min[base + p], tr = A.bs_max[base + p];
+                if (sh > A.theta) atomicMin(&A.rd_flag[rd], p);
                 A.sc_j[base + p] = sh;
                 A.sc_i[base + p] = A.bs_m[base + p];
                 A.sc_same[base + p] = A.bs_c[base + p];
@@ -1098,27 +1157,50 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         const bool parallel = fits && n > 0 && !ap && A.metric != METRIC_BUCKET_LCS && u < (1 << 19);
         int32_t *sid = fits ? l_sid : A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
         int32_t *keys = fits ? l_b : A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
-        // 1. mark every matched position of the span with its seed
+        // 1. mark every matched position of the span with its seed.  WG seeds
+        //    at a time: each thread finds its seed's hit range inside the span,
+        //    a scan lays the ranges end to end, and the whole workgroup marks
+        //    the hits, so a repeat k-mer with many hits in the span is not left
+        //    to one thread.
         for (int64_t x = threadIdx.x; x < n; x += WG) sid[x] = -1;
         if (parallel)
             for (int64_t x = threadIdx.x; x <= n; x += WG) l_c[x] = 0;
         if (threadIdx.x == 0) s_nm = 0;
         __syncthreads();
-        for (int32_t sd = threadIdx.x; sd < u; sd += WG) {
-            const uint32_t cnt = A.sd_cnt[o + sd];
-            if (cnt == 0) continue;
-            const uint32_t hoff = A.sd_hoff[o + sd];
-            const uint32_t sstr = A.sd_strand[o + sd];
-            for (uint32_t j = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0); j < hoff + cnt; j++) {
-                if (I.hit_seg[j] != seg) break;
-                const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
-                if (x >= n) break;
-                const uint32_t hstr = I.hit_r[j] >> 31;
-                sid[x] = sd | (sstr == hstr ? 0 : 0x40000000);
-                if (parallel) keys[atomicAdd(&s_nm, 1u)] = ((uint32_t)sd << 12) | (uint32_t)x;
+        for (int32_t s0 = 0; s0 < u; s0 += WG) {
+            const int32_t sd = s0 + (int32_t)threadIdx.x;
+            uint32_t lo = 0, nin = 0, sstr = 0;
+            if (sd < u) {
+                const uint32_t cnt = A.sd_cnt[o + sd];
+                if (cnt) {
+                    const uint32_t hoff = A.sd_hoff[o + sd];
+                    lo = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0);
+                    nin = hits_lower_bound(I, lo, hoff + cnt, seg, l0 + n, 0) - lo;
+                    sstr = A.sd_strand[o + sd];
+                }
             }
+            uint32_t tot;
+            const uint32_t ex = wg_excl_scan(nin, tmp, &tot);  // ends with a barrier
+            s_pre[threadIdx.x] = ex;
+            s_lo[threadIdx.x] = lo;
+            s_str[threadIdx.x] = (uint8_t)sstr;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < tot; i += WG) {
+                // the last seed slot whose range starts at or before i
+                uint32_t a = 0, b = WG;
+                while (b - a > 1) {
+                    const uint32_t mid = (a + b) >> 1;
+                    if (s_pre[mid] <= i) a = mid; else b = mid;
+                }
+                const uint32_t j = s_lo[a] + (i - s_pre[a]);
+                const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
+                const uint32_t hstr = I.hit_r[j] >> 31;
+                const int32_t sdj = s0 + (int32_t)a;
+                sid[x] = sdj | (s_str[a] == hstr ? 0 : 0x40000000);
+                if (parallel) keys[atomicAdd(&s_nm, 1u)] = ((uint32_t)sdj << 12) | (uint32_t)x;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         if (parallel) {
             // 2. occurrences grouped by seed, in position order
             const uint32_t nm = s_nm;
@@ -1188,6 +1270,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                     }
                 const int64_t r = L + m < n ? L + m : n;
                 const int32_t tl = I.t_r[ks + l0 + L], tr = I.t_r[ks + l0 + r - 1];
+                if (J > A.theta) atomicMin(&A.rd_flag[rd], p);
                 A.sc_j[base + p] = J;
                 A.sc_i[base + p] = l_c[L];
                 A.sc_same[base + p] = keys[L];
@@ -1283,6 +1366,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                     mseg = (int32_t)seg;
                 }
             }
+            if (J > A.theta) atomicMin(&A.rd_flag[rd], p);
             A.sc_j[base + p] = J;
             A.sc_i[base + p] = inter;
             A.sc_same[base + p] = same;
@@ -1438,6 +1522,13 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
 
 size_t sketch_lds(int32_t k) { return SKETCH_LUT_BYTES + (size_t)NWAVES * sketch_span(k); }
 
+uint32_t sketch_lane_cap(uint64_t thres) {
+    const double frac = ((double)thres + 1.0) / 18446744073709551616.0;  // keep probability
+    const double lam = frac * LANE_W;
+    const double cap = ceil(lam + 5.0 * sqrt(lam) + 4.0);
+    return cap >= (double)LANE_W ? (uint32_t)LANE_W : (uint32_t)cap;
+}
+
 void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
                       hipStream_t st) {
     k_chunk_seq<<<flat_grid(n_chunks, 8192), WG, 0, st>>>(S, chunk_start, chunk_seq);
@@ -1445,16 +1536,29 @@ void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq
 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
                    const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                   int32_t *out_r, uint8_t *out_s, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+                   int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list, uint32_t *ovf_count,
+                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
     uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
     if (g < 1) g = 1;
     if (g > max_grid) g = max_grid;
     if (pass == 0)
         k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
-                                                             lane_cnt, out_h, out_r, out_s);
+                                                             lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+                                                             ovf_count);
     else
         k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
-                                                             lane_cnt, out_h, out_r, out_s);
+                                                             lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+                                                             ovf_count);
+}
+
+void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
+                           uint32_t lcap, const uint64_t *st_h, const int32_t *st_r, const uint8_t *st_s,
+                           uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t max_grid, hipStream_t st) {
+    uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    k_sketch_compact<<<(uint32_t)g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, st_h, st_r, st_s, out_h,
+                                                 out_r, out_s);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
@@ -1528,7 +1632,15 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_COUNT_BUCKETS: k_count_buckets<<<gg, WG, 0, st>>>(A); break;
     case ST_EXTRACT: k_extract<<<gg, WG, SORT_LDS, st>>>(A); break;
     case ST_PRUNE: k_prune<<<gg, WG, 0, st>>>(A, I); break;
-    case ST_REFINE: k_refine<<<max_grid, WG, REFINE_LDS, st>>>(A, I); break;
+    case ST_REFINE: {
+        // first-pass candidates, then the second-pass-only band of flagged reads
+        MapArgs P = A;
+        P.refine_pass = 0;
+        k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+        P.refine_pass = 1;
+        k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+        break;
+    }
     case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;
     default: break;
     }

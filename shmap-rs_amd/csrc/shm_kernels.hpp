@@ -90,8 +90,17 @@ struct MapArgs {
     double *sc_j;
     int32_t *sc_i, *sc_same, *sc_tl, *sc_tr, *sc_ssz, *sc_pend, *sc_seg;
     // refine tasks: (group read, sorted bucket) of every bucket that can survive
+    // a match_rest pass.  Buckets whose bound reaches theta fill tasks[0, ..)
+    // (task_count[0]); those only the second pass can reach (bound in
+    // [theta*(1-min_diff), theta)) fill tasks from the end, tasks[cap-1-i]
+    // (task_count[1]), and are refined only for reads with a first-pass score
+    // above theta: rd_flag holds the first sorted bucket with such a score
+    // (INT32_MAX if none).  task_count[2] counts those refined.
     uint2 *tasks;
     uint32_t *task_count;
+    uint32_t task_cap;
+    int32_t refine_pass;
+    int32_t *rd_flag;
     // per-workgroup scratch for spans/reads that overflow LDS
     int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;
@@ -106,8 +115,8 @@ namespace shm {
 struct DevIndex;
 
 enum Stage {
-    ST_SKETCH_COUNT = 0,
-    ST_SKETCH_WRITE,
+    ST_SKETCH = 0,
+    ST_SKETCH_COMPACT,
     ST_SEED_WAVE,
     ST_SORT_KMERS,
     ST_GROUP_SEEDS,
@@ -130,9 +139,19 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
 size_t sketch_lds(int32_t k);
 void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
                       hipStream_t st);
+// pass 0: roll every chunk into lane-strided staging (lcap slots per lane),
+// listing chunks with a lane over lcap; pass 1: re-roll the listed chunks
+// straight to their final positions (chunk_val = scanned chunk offsets).
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
                    const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                   int32_t *out_r, uint8_t *out_s, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+                   int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list, uint32_t *ovf_count,
+                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
+                           uint32_t lcap, const uint64_t *st_h, const int32_t *st_r, const uint8_t *st_s,
+                           uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t max_grid, hipStream_t st);
+// Staging slots per lane for a keep threshold: the expected kept count of
+// LANE_W windows plus five standard deviations and a margin, at most LANE_W.
+uint32_t sketch_lane_cap(uint64_t thres);
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
                      hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);

@@ -133,8 +133,12 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
     DevBuf<uint64_t> chunk_start, chunk_cnt, chunk_koff, koff, part;
-    DevBuf<uint32_t> lane_cnt, chunk_seq;
+    DevBuf<uint32_t> lane_cnt, chunk_seq, ovf;  // ovf: [0] overflow count, then the chunk list
+    DevBuf<uint64_t> st_h;                      // lane-strided staging of the rolling pass
+    DevBuf<int32_t> st_r;
+    DevBuf<uint8_t> st_s;
     std::vector<uint64_t> h_koff;
+    uint32_t n_overflow = 0;
 };
 
 static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off,
@@ -159,15 +163,24 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     HIPCHK(hipMemcpyAsync(so.chunk_start.p, cs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
     SeqSet S{d_bases, d_off, n, h_off[n]};
     if (C > 0) launch_chunk_seq(S, so.chunk_start.p, so.chunk_seq.p, C, st);
+    const uint32_t lcap = sketch_lane_cap(sk->thres);
+    const uint64_t slots = C * WAVE * (uint64_t)lcap;
+    RCHK(so.st_h.ensure(slots + 1));
+    RCHK(so.st_r.ensure(slots + 1));
+    RCHK(so.st_s.ensure(slots + 1));
+    RCHK(so.ovf.ensure(C + 2));
+    uint32_t *ovf_count = so.ovf.p, *ovf_list = so.ovf.p + 1;
+    HIPCHK(hipMemsetAsync(ovf_count, 0, 4, st));
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
         launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_cnt.p, so.lane_cnt.p,
-                      nullptr, nullptr, nullptr, C, max_grid, st);
+                      so.st_h.p, so.st_r.p, so.st_s.p, lcap, ovf_list, ovf_count, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
     launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, st);
     so.h_koff.resize(n + 1);
     HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&so.n_overflow, ovf_count, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const uint64_t M = so.h_koff[n];
     RCHK(out_h.ensure(M + 1));
@@ -175,8 +188,11 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(out_s.ensure(M + 1));
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
+        launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.st_h.p, so.st_r.p, so.st_s.p, out_h.p,
+                              out_r.p, out_s.p, max_grid, st);
+    if (so.n_overflow > 0)
         launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_koff.p, so.lane_cnt.p,
-                      out_h.p, out_r.p, out_s.p, C, max_grid, st);
+                      out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
@@ -481,10 +497,12 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 // mapper
 // ---------------------------------------------------------------------------
 
-static const char *kStageNames[ST_COUNT] = {"sketch_count", "sketch_write", "seed_wave", "sort_kmers", "group_seeds",
+static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "sort_kmers", "group_seeds",
                                             "sort_seeds",   "budget",       "tab_init",   "contrib",
                                             "count_buckets", "extract",     "prune",      "refine",
                                             "final"};
+
+constexpr int SHM_N_COUNTS = 7;  // see shm_mapper_last_counts
 
 struct shm_mapper {
     const shm_index *idx = nullptr;
@@ -497,6 +515,7 @@ struct shm_mapper {
     hipEvent_t ev[ST_COUNT][2];
     double stage_ms[ST_COUNT] = {};
     double total_ms = 0;
+    int64_t counts[SHM_N_COUNTS] = {};
     // batch buffers
     DevBuf<uint8_t> bases;
     DevBuf<int64_t> roff;
@@ -571,6 +590,14 @@ extern "C" const char *shm_mapper_stage_name(int32_t s) {
     return (s >= 0 && s < ST_COUNT) ? kStageNames[s] : "";
 }
 
+extern "C" int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written) {
+    if (!mp || (n > 0 && !counts)) return fail(SHM_ERR_ARG, "invalid argument");
+    int32_t w = 0;
+    for (; w < n && w < SHM_N_COUNTS; w++) counts[w] = mp->counts[w];
+    if (n_written) *n_written = w;
+    return SHM_OK;
+}
+
 extern "C" int shm_mapper_last_timing(const shm_mapper *mp, double *total_ms, double *stage_ms, int32_t n,
                                       int32_t *n_written) {
     if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
@@ -640,6 +667,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     HIPCHK(hipEventCreate(&t0));
     HIPCHK(hipEventCreate(&t1));
     HIPCHK(hipEventRecord(t0, st));
+    for (int i = 0; i < SHM_N_COUNTS; i++) mp->counts[i] = 0;
     if (R == 0) {
         mp->total_ms = 0;
         (void)hipEventDestroy(t0);
@@ -649,10 +677,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
 
     // sketch
     RCHK(sketch_device(&mp->sk, d_seq, d_off, h_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
-                       mp->timing ? mp->ev[ST_SKETCH_COUNT] : nullptr, mp->timing ? mp->ev[ST_SKETCH_WRITE] : nullptr));
+                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr));
     if (mp->timing) {
-        pending.push_back(ST_SKETCH_COUNT);
-        pending.push_back(ST_SKETCH_WRITE);
+        pending.push_back(ST_SKETCH);
+        pending.push_back(ST_SKETCH_COMPACT);
     }
     const uint64_t M = mp->so.h_koff[R];
     RCHK(mp->ps_h.ensure(M + 1));
@@ -666,7 +694,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->sd_hoff.ensure(M + 1));
     RCHK(mp->sd_ord.ensure(M + 1));
     RCHK(mp->sd_epre.ensure(M + 1));
-    RCHK(mp->rd_i32.ensure((size_t)R * 10 + 1));
+    RCHK(mp->rd_i32.ensure((size_t)R * 11 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
     RCHK(mp->d_out.ensure(R));
 
@@ -711,6 +739,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_max_seed = ri + 7 * (size_t)R;
     A.rd_seed_matches = ri + 8 * (size_t)R;
     A.rd_nb = ri + 9 * (size_t)R;
+    A.rd_flag = ri + 10 * (size_t)R;
     A.rd_E = mp->rd_u64.p;
     A.rd_cap = mp->rd_u64.p + R;
     A.r0 = 0;
@@ -734,6 +763,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(flush_timing(mp, pending));
 
     // bucketing, in groups of reads whose tables fit the budget
+    RCHK(mp->task_count.ensure(3 * (size_t)R + 3));  // three refine-task counters per group
+    int32_t ngroups = 0;
+    int64_t nbuckets = 0;
     int32_t r0 = 0;
     while (r0 < R) {
         int32_t r1 = r0;
@@ -805,8 +837,8 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->tasks.ensure(B + 1));
-        RCHK(mp->task_count.ensure(1));
-        HIPCHK(hipMemsetAsync(mp->task_count.p, 0, 4, st));
+        uint32_t *task_count = mp->task_count.p + 3 * (size_t)ngroups;
+        HIPCHK(hipMemsetAsync(task_count, 0, 12, st));
         A.bk_key = mp->bk_key.p;
         A.bk_val = mp->bk_val.p;
         A.tmp_g = mp->tmp_g.p;
@@ -838,15 +870,20 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.g_tails = mp->g_tails.p;
         A.diff_stride = dstride;
         A.tasks = mp->tasks.p;
-        A.task_count = mp->task_count.p;
+        A.task_count = task_count;
+        A.task_cap = (uint32_t)(B + 1);
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
             TIME_BEGIN(s);
             launch_stage(s, A, I, mp->d_out.p, 0, mp->max_grid, st);
             TIME_END(s);
         }
         HIPCHK(hipGetLastError());
+        nbuckets += (int64_t)B;
+        ngroups++;
         r0 = r1;
     }
+    std::vector<uint32_t> ntask(3 * (size_t)ngroups);
+    HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(t1, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -857,6 +894,20 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     for (int32_t i = 0; i < R; i++) finalize_result(mp->prm, out[i]);
+    int64_t first = 0, deferred = 0, deferred_run = 0, seeds = 0;
+    for (int32_t g = 0; g < ngroups; g++) {
+        first += ntask[3 * g];
+        deferred += ntask[3 * g + 1];
+        deferred_run += ntask[3 * g + 2];
+    }
+    for (int32_t i = 0; i < R; i++) seeds += out[i].u;
+    mp->counts[0] = (int64_t)M;
+    mp->counts[1] = seeds;
+    mp->counts[2] = nbuckets;
+    mp->counts[3] = first + deferred_run;
+    mp->counts[4] = ngroups;
+    mp->counts[5] = first;
+    mp->counts[6] = deferred;
     return SHM_OK;
 }
 

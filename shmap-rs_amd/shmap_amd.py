@@ -106,6 +106,7 @@ def lib():
         L.shm_mapper_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                              C.c_int32, C.POINTER(C.c_int32)]
         L.shm_mapper_stage_name.argtypes = [C.c_int32]
+        L.shm_mapper_last_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -265,6 +266,16 @@ class Mapper:
         nw = C.c_int32()
         _check(lib().shm_mapper_last_timing(self._p, C.byref(tot), arr, 32, C.byref(nw)))
         return tot.value, {lib().shm_mapper_stage_name(i).decode(): arr[i] for i in range(nw.value)}
+
+    def last_counts(self) -> dict:
+        """Work counts of the last batch (k-mers, seeds, buckets, refined buckets, groups)."""
+        arr = (C.c_int64 * 8)()
+        nw = C.c_int32()
+        _check(lib().shm_mapper_last_counts(self._p, arr, 8, C.byref(nw)))
+        return {COUNT_NAMES[i]: int(arr[i]) for i in range(min(nw.value, len(COUNT_NAMES)))}
+
+
+COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "refine_first", "second_pass_band")
 
 
 def format_paf(index: SketchIndex, params: MapParams, qid: str, rec, map_time: float = 0.0) -> str:

@@ -41,6 +41,32 @@ def test_sketch_matches_oracle(k, h_frac):
             assert np.array_equal(g["strand"], o["strand"])
 
 
+@pytest.mark.parametrize("k,h_frac", [(25, 0.01), (31, 0.1), (15, 0.5)])
+def test_sketch_lane_overflow_matches_oracle(k, h_frac):
+    """Runs of N: a window of k N's hashes to 0 and is always kept, so a run
+    longer than k + 127 keeps every window of a lane, far above the expected
+    keep rate.  The rolling pass's per-lane staging overflows there and those
+    chunks are written by the re-roll pass."""
+    flank = _rng_seq(11, 20_000)
+    s = (flank + b"N" * 5000 + flank[:9_999] + b"n" * 300 + flank[:1234] + b"N" * (k - 1) + flank[:77]
+         + b"N" * k + flank[:500] + b"N" * (k + 128) + flank[:3000])
+    o = O.sketch(k, h_frac, s, 0)
+    lanes = np.bincount((o["r"] - k + 1) // 128)
+    assert lanes.max() == 128  # at least one lane keeps all of its windows
+    g = G.FracMinHash(k, h_frac).sketch_slice_into(s, 0)
+    assert len(g) == len(o)
+    assert np.array_equal(g["r"], o["r"]) and np.array_equal(g["h"], o["h"])
+    assert np.array_equal(g["strand"], o["strand"])
+    # reads with such runs through the mapper
+    ref = [("chr1", flank + b"N" * 400 + flank[::-1])]
+    reads = [s[1000:26_000], b"N" * 9000, s[15_000:32_000], flank[5000:15000]]
+    prm = dict(theta=0.4, min_diff=0.075, max_overlap=0.3)
+    _, _, ores = P.oracle_map(ref, reads, k, h_frac, **prm)
+    _, _, _, gres = P.gpu_map(ref, reads, k, h_frac, **prm)
+    bad = P.diff_results(ores, gres)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("max_matches", [None, 1, 3])
 def test_index_counts_and_hits_match_oracle(max_matches):
     unit = _rng_seq(9, 171)
