@@ -162,6 +162,13 @@ int shm_format_paf(const shm_index *idx, const shm_map_params *params, const cha
                    const shm_read_result *r, double map_time, char *buf, size_t cap);
 int shm_format_unmapped(const char *query_id, int32_t read_len, char *buf, size_t cap);
 
+/* Page-locked host memory for read and result buffers: copies between it and
+ * the device run at full link speed (shm_map_batch_device writes its results
+ * straight into `out`).  No reference counterpart: host allocation is the
+ * caller's in the Rust crate. */
+int shm_host_alloc(size_t bytes, void **out);
+void shm_host_free(void *p);
+
 /* Device kernel timing of the last batch on this mapper, in milliseconds:
  * total GPU time and the time of each stage (HIP events on the mapper's own
  * stream).  stage_ms receives up to n_stages values; names via

@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             const uintptr_t src = (uintptr_t)(S.bases + S.off[seq] + w0);
             const uintptr_t a = src & ~(uintptr_t)3;
             const uint32_t sh = (uint32_t)(src - a) * 8;
-            const uintptr_t lim = (uintptr_t)(S.bases + S.total);  // never read past the set
+            const uintptr_t lim = (uintptr_t)(S.bases + S.off[S.n]);  // never read past the set
             const uint32_t *s32 = (const uint32_t *)a;
             uint32_t *b32 = (uint32_t *)buf;
             const int64_t ng = (nb + 3) >> 2;  // code dwords
@@ -310,11 +310,38 @@ __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const 
     }
 }
 
-// Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]].
+// Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]],
+// and the largest per-sequence sketch into *max_m (zeroed by the caller).
 __global__ void __launch_bounds__(WG) k_seq_koff(int32_t n, const uint64_t *chunk_start,
-                                                 const uint64_t *chunk_koff, uint64_t *koff) {
-    for (int32_t i = blockIdx.x * WG + threadIdx.x; i <= n; i += gridDim.x * WG)
-        koff[i] = chunk_koff[chunk_start[i]];
+                                                 const uint64_t *chunk_koff, uint64_t *koff,
+                                                 unsigned long long *max_m) {
+    for (int32_t i0 = blockIdx.x * WG; i0 <= n; i0 += gridDim.x * WG) {
+        const int32_t i = i0 + (int32_t)threadIdx.x;
+        uint64_t m = 0;
+        if (i <= n) {
+            const uint64_t a = chunk_koff[chunk_start[i]];
+            koff[i] = a;
+            if (i < n) m = chunk_koff[chunk_start[i + 1]] - a;
+        }
+        for (int d = WAVE / 2; d > 0; d >>= 1) {
+            const uint64_t o = __shfl_xor(m, d);
+            m = o > m ? o : m;
+        }
+        if ((threadIdx.x & (WAVE - 1)) == 0 && m) atomicMax(max_m, (unsigned long long)m);
+    }
+}
+
+// Largest value of a[0, n) into *out (zeroed by the caller; values >= 0).
+__global__ void __launch_bounds__(WG) k_max_i32(const int32_t *a, int32_t n, unsigned long long *out) {
+    for (int32_t i0 = blockIdx.x * WG; i0 < n; i0 += gridDim.x * WG) {
+        const int32_t i = i0 + (int32_t)threadIdx.x;
+        int32_t v = i < n ? a[i] : 0;
+        for (int d = WAVE / 2; d > 0; d >>= 1) {
+            const int32_t o = __shfl_xor(v, d);
+            v = o > v ? o : v;
+        }
+        if ((threadIdx.x & (WAVE - 1)) == 0 && v > 0) atomicMax(out, (unsigned long long)v);
+    }
 }
 
 // ============================================================================
@@ -913,7 +940,10 @@ __global__ void __launch_bounds__(WG) k_count_buckets(MapArgs A) {
         for (int d = WAVE / 2; d > 0; d >>= 1) n += __shfl_xor(n, d);
         if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_n, n);
         __syncthreads();
-        if (threadIdx.x == 0) A.rd_nb[rd] = (int32_t)s_n;
+        if (threadIdx.x == 0) {
+            A.rd_nb[rd] = (int32_t)s_n;
+            A.grp_nb[lr] = s_n;
+        }
         __syncthreads();
     }
 }
@@ -1562,8 +1592,13 @@ void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const ui
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
-                     hipStream_t st) {
-    k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, chunk_start, chunk_koff, koff);
+                     uint64_t *max_m, hipStream_t st) {
+    k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, chunk_start, chunk_koff, koff,
+                                                                (unsigned long long *)max_m);
+}
+
+void launch_max_i32(const int32_t *a, int32_t n, uint64_t *out, hipStream_t st) {
+    k_max_i32<<<flat_grid((uint64_t)n, 1024), WG, 0, st>>>(a, n, (unsigned long long *)out);
 }
 
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st) {

@@ -31,12 +31,12 @@ __host__ __device__ inline uint32_t sketch_span(int32_t k) {
     return (raw + ((raw >> 7) + 1) * 4 + 15u) & ~15u;
 }
 
-// A set of sequences: sequence i is bases[off[i], off[i+1]).
+// A set of sequences: sequence i is bases[off[i], off[i+1]); bytes up to
+// bases + off[n] are readable.
 struct SeqSet {
     const uint8_t *bases;
     const int64_t *off;
     int32_t n;
-    int64_t total;  // bytes readable at bases (off[n])
 };
 
 struct ShardGeom {
@@ -75,6 +75,7 @@ struct MapArgs {
     uint64_t *rd_E, *rd_cap;
     // per group read (local index), RG+1 entries
     const uint64_t *hit_base, *tab_base, *bk_base;
+    uint64_t *grp_nb;  // each group read's bucket count, scanned into bk_base
     // per-read bucket tables (group)
     uint32_t *tb_key;
     uint64_t *tb_mc;
@@ -153,7 +154,8 @@ void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const ui
 // LANE_W windows plus five standard deviations and a margin, at most LANE_W.
 uint32_t sketch_lane_cap(uint64_t thres);
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
-                     hipStream_t st);
+                     uint64_t *max_m, hipStream_t st);
+void launch_max_i32(const int32_t *a, int32_t n, uint64_t *out, hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);
 void launch_radix_pass(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t n,
                        uint32_t shift, uint64_t *hist, uint64_t *offs, uint64_t *part, hipStream_t st);

@@ -84,6 +84,51 @@ struct DevBuf {
     }
 };
 
+// Page-locked host memory: device copies to and from it run at full link speed
+// without the runtime's staging.
+template <typename T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= cap && p) return SHM_OK;
+        release();
+        size_t c = n < 16 ? 16 : n + n / 4;
+        hipError_t e = hipHostMalloc((void **)&p, c * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SHM_ERR_NOMEM, "hipHostMalloc of %zu bytes failed: %s", c * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        cap = c;
+        return SHM_OK;
+    }
+};
+
+extern "C" int shm_host_alloc(size_t bytes, void **out) {
+    if (!out) return fail(SHM_ERR_ARG, "null output");
+    *out = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        (void)hipGetLastError();
+        return fail(SHM_ERR_NOMEM, "hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SHM_OK;
+}
+
+extern "C" void shm_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
 // FracMinHash threshold (src/sketch.rs:126-132): (h_frac * u64::MAX as f64) as
 // u64 below 1, where u64::MAX as f64 is 2^64; u64::MAX otherwise.
 static uint64_t hash_threshold(double h_frac) {
@@ -132,36 +177,37 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // Sketches a set of sequences already on the device: chunk table on the host,
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
-    DevBuf<uint64_t> chunk_start, chunk_cnt, chunk_koff, koff, part;
+    DevBuf<uint64_t> nch, chunk_start, chunk_cnt, chunk_koff, koff, part, stat;  // stat: [0] largest sketch
     DevBuf<uint32_t> lane_cnt, chunk_seq, ovf;  // ovf: [0] overflow count, then the chunk list
     DevBuf<uint64_t> st_h;                      // lane-strided staging of the rolling pass
     DevBuf<int32_t> st_r;
     DevBuf<uint8_t> st_s;
-    std::vector<uint64_t> h_koff;
+    std::vector<uint64_t> h_koff;  // per-sequence offsets, when asked for
+    PinnedBuf<uint64_t> h_sum;     // total, largest sketch, overflow chunks
+    uint64_t M = 0, max_m = 0;
     uint32_t n_overflow = 0;
 };
 
-static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off,
-                         const std::vector<int64_t> &h_off, int32_t n, SketchOut &so, DevBuf<uint64_t> &out_h,
-                         DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s, uint32_t max_grid, hipStream_t st,
-                         hipEvent_t *ev_count, hipEvent_t *ev_write) {
-    std::vector<uint64_t> cs(n + 1);
-    cs[0] = 0;
-    for (int32_t i = 0; i < n; i++) {
-        const int64_t len = h_off[i + 1] - h_off[i];
-        const int64_t nw = len >= sk->k ? len - sk->k + 1 : 0;
-        cs[i + 1] = cs[i] + (uint64_t)((nw + CHUNK_W - 1) / CHUNK_W);
-    }
-    const uint64_t C = cs[n];
+static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off, int32_t n,
+                         SketchOut &so, DevBuf<uint64_t> &out_h, DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s,
+                         uint32_t max_grid, hipStream_t st, hipEvent_t *ev_count, hipEvent_t *ev_write,
+                         bool full_koff) {
+    // chunk table: chunks per sequence, scanned on the device
+    SeqSet S{d_bases, d_off, n};
     RCHK(so.chunk_start.ensure(n + 1));
-    RCHK(so.chunk_cnt.ensure(C + 1));
-    RCHK(so.chunk_koff.ensure(C + 1));
+    RCHK(so.nch.ensure(n + 1));
     RCHK(so.koff.ensure(n + 1));
     RCHK(so.part.ensure(SCAN_PARTS + 1));
+    RCHK(so.h_sum.ensure(3));
+    launch_chunk_counts(S, sk->k, so.nch.p, st);
+    launch_scan(so.nch.p, so.chunk_start.p, nullptr, n, so.part.p, st);
+    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.chunk_start.p + n, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint64_t C = so.h_sum.p[0];
+    RCHK(so.chunk_cnt.ensure(C + 1));
+    RCHK(so.chunk_koff.ensure(C + 1));
     RCHK(so.lane_cnt.ensure(C * WAVE + 1));
     RCHK(so.chunk_seq.ensure(C + 1));
-    HIPCHK(hipMemcpyAsync(so.chunk_start.p, cs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
-    SeqSet S{d_bases, d_off, n, h_off[n]};
     if (C > 0) launch_chunk_seq(S, so.chunk_start.p, so.chunk_seq.p, C, st);
     const uint32_t lcap = sketch_lane_cap(sk->thres);
     const uint64_t slots = C * WAVE * (uint64_t)lcap;
@@ -177,12 +223,23 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
                       so.st_h.p, so.st_r.p, so.st_s.p, lcap, ovf_list, ovf_count, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
-    launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, st);
-    so.h_koff.resize(n + 1);
-    HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&so.n_overflow, ovf_count, 4, hipMemcpyDeviceToHost, st));
+    RCHK(so.stat.ensure(1));
+    HIPCHK(hipMemsetAsync(so.stat.p, 0, 8, st));
+    launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, so.stat.p, st);
+    if (full_koff) {
+        so.h_koff.resize(n + 1);
+        HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    }
+    RCHK(so.h_sum.ensure(3));
+    so.h_sum.p[2] = 0;
+    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.koff.p + n, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(so.h_sum.p + 1, so.stat.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(so.h_sum.p + 2, ovf_count, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const uint64_t M = so.h_koff[n];
+    so.M = so.h_sum.p[0];
+    so.max_m = so.h_sum.p[1];
+    so.n_overflow = (uint32_t)so.h_sum.p[2];
+    const uint64_t M = so.M;
     RCHK(out_h.ensure(M + 1));
     RCHK(out_r.ensure(M + 1));
     RCHK(out_s.ensure(M + 1));
@@ -216,8 +273,9 @@ extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t le
     DevBuf<uint8_t> ks;
     // (s.len() as RPos) < k yields nothing (src/sketch.rs:170-173)
     const bool empty = (int32_t)len < sk->k;
-    if (!empty) RCHK(sketch_device(sk, bases.p, off.p, h_off, 1, so, kh, kr, ks, 1024, st, nullptr, nullptr));
-    const int64_t M = empty ? 0 : (int64_t)so.h_koff[1];
+    if (!empty)
+        RCHK(sketch_device(sk, bases.p, off.p, 1, so, kh, kr, ks, 1024, st, nullptr, nullptr, false));
+    const int64_t M = empty ? 0 : (int64_t)so.M;
     *n = M;
     if (M > cap) {
         (void)hipStreamDestroy(st);
@@ -327,8 +385,8 @@ extern "C" int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, 
 
     // 1. sketch every segment (segments shorter than k contribute nothing)
     SketchOut so;
-    if ((rc = sketch_device(sk, bases.p, d_off.p, h_off, n_segments, so, idx->t_h, idx->t_r, idx->t_s, max_grid, st,
-                            nullptr, nullptr)) != SHM_OK)
+    if ((rc = sketch_device(sk, bases.p, d_off.p, n_segments, so, idx->t_h, idx->t_r, idx->t_s, max_grid, st,
+                            nullptr, nullptr, true)) != SHM_OK)
         return cleanup(rc);
     const uint64_t N = so.h_koff[n_segments];
     idx->n_kmers = (int64_t)N;
@@ -519,6 +577,8 @@ struct shm_mapper {
     // batch buffers
     DevBuf<uint8_t> bases;
     DevBuf<int64_t> roff;
+    PinnedBuf<uint8_t> h_bases;  // upload staging of shm_map_batch
+    PinnedBuf<int64_t> h_roff;
     SketchOut so;
     DevBuf<uint64_t> k_h, ps_h, sd_h, sd_key;
     DevBuf<int32_t> k_r;
@@ -526,7 +586,9 @@ struct shm_mapper {
     DevBuf<uint32_t> ps_v, sd_first, sd_occs, sd_cnt, sd_hoff, sd_ord, sd_epre;
     DevBuf<int32_t> rd_i32;  // 10 per-read int arrays
     DevBuf<uint64_t> rd_u64; // 2 per-read u64 arrays
-    DevBuf<uint64_t> grp_base; // 3 x (RG+1)
+    DevBuf<uint64_t> grp_base; // 3 x (RG+1): hit, table and bucket offsets of a group's reads
+    DevBuf<uint64_t> grp_nb, part, stat;
+    PinnedBuf<uint64_t> h_sum;  // totals read back between stages
     DevBuf<uint32_t> tb_key;
     DevBuf<uint64_t> tb_mc;
     DevBuf<int32_t> tb_min, tb_max;
@@ -654,10 +716,8 @@ static int flush_timing(shm_mapper *mp, std::vector<int> &pending) {
     return SHM_OK;
 }
 
-// Maps R reads whose bases are on the device.  h_off are the host copies of
-// the read offsets.
-static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, const std::vector<int64_t> &h_off,
-                      int32_t R, shm_read_result *out) {
+// Maps R reads whose bases and offsets are on the device.
+static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R, shm_read_result *out) {
     hipStream_t st = mp->st;
     const shm_index *ix = mp->idx;
     const DevIndex I = ix->view();
@@ -676,13 +736,14 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     }
 
     // sketch
-    RCHK(sketch_device(&mp->sk, d_seq, d_off, h_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
-                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr));
+    RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
+                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
+                       false));
     if (mp->timing) {
         pending.push_back(ST_SKETCH);
         pending.push_back(ST_SKETCH_COMPACT);
     }
-    const uint64_t M = mp->so.h_koff[R];
+    const uint64_t M = mp->so.M;
     RCHK(mp->ps_h.ensure(M + 1));
     RCHK(mp->ps_v.ensure(M + 1));
     RCHK(mp->sd_h.ensure(M + 1));
@@ -745,8 +806,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.r0 = 0;
     A.RG = R;
 
-    bool any_large = false;  // a read whose sketch exceeds one wave's LDS slice
-    for (int32_t i = 0; i < R && !any_large; i++) any_large = mp->so.h_koff[i + 1] - mp->so.h_koff[i] > WAVE_SEED_CAP;
+    const bool any_large = mp->so.max_m > WAVE_SEED_CAP;  // a read whose sketch exceeds one wave's LDS slice
     for (Stage s : {ST_SEED_WAVE, ST_SORT_KMERS, ST_GROUP_SEEDS, ST_SORT_SEEDS, ST_BUDGET}) {
         if (s != ST_SEED_WAVE && !any_large) continue;
         TIME_BEGIN(s);
@@ -754,13 +814,34 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         TIME_END(s);
     }
     HIPCHK(hipGetLastError());
-    std::vector<uint64_t> E(R), cap(R);
-    std::vector<int32_t> hl(R);
-    HIPCHK(hipMemcpyAsync(E.data(), A.rd_E, R * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cap.data(), A.rd_cap, R * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(hl.data(), A.rd_hl, R * 4, hipMemcpyDeviceToHost, st));
+    // hit and table offsets of every read by device scans; only the totals and
+    // the largest halflen come back
+    RCHK(mp->grp_base.ensure(3 * (size_t)(R + 1)));
+    RCHK(mp->grp_nb.ensure((size_t)R + 1));
+    RCHK(mp->part.ensure(SCAN_PARTS + 1));
+    RCHK(mp->stat.ensure(1));
+    RCHK(mp->h_sum.ensure(4));
+    HIPCHK(hipMemsetAsync(mp->stat.p, 0, 8, st));
+    launch_max_i32(A.rd_hl, R, mp->stat.p, st);
+    launch_scan(A.rd_E, mp->grp_base.p, nullptr, R, mp->part.p, st);
+    launch_scan(A.rd_cap, mp->grp_base.p + (R + 1), nullptr, R, mp->part.p, st);
+    HIPCHK(hipMemcpyAsync(mp->h_sum.p, mp->grp_base.p + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mp->h_sum.p + 1, mp->grp_base.p + (R + 1) + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mp->h_sum.p + 2, mp->stat.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
+    const uint64_t E_all = mp->h_sum.p[0], T_all = mp->h_sum.p[1];
+    const int32_t hl_all = (int32_t)mp->h_sum.p[2];
+    // Tables of all reads over the budget: split into groups on the host.
+    const bool one_group = T_all <= mp->tab_budget;
+    std::vector<uint64_t> E, cap;
+    if (!one_group) {
+        E.resize(R);
+        cap.resize(R);
+        HIPCHK(hipMemcpyAsync(E.data(), A.rd_E, R * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(cap.data(), A.rd_cap, R * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
 
     // bucketing, in groups of reads whose tables fit the budget
     RCHK(mp->task_count.ensure(3 * (size_t)R + 3));  // three refine-task counters per group
@@ -768,24 +849,29 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     int64_t nbuckets = 0;
     int32_t r0 = 0;
     while (r0 < R) {
-        int32_t r1 = r0;
-        uint64_t tsum = 0;
-        while (r1 < R && (r1 == r0 || tsum + cap[r1] <= mp->tab_budget)) tsum += cap[r1++];
-        const int32_t RG = r1 - r0;
-        std::vector<uint64_t> hb(RG + 1), tb(RG + 1);
-        hb[0] = tb[0] = 0;
-        int32_t max_hl = 0;
-        uint64_t max_m = 0;
-        for (int32_t i = 0; i < RG; i++) {
-            hb[i + 1] = hb[i] + E[r0 + i];
-            tb[i + 1] = tb[i] + cap[r0 + i];
-            max_hl = std::max(max_hl, hl[r0 + i]);
-            max_m = std::max(max_m, mp->so.h_koff[r0 + i + 1] - mp->so.h_koff[r0 + i]);
+        int32_t r1 = R;
+        uint64_t T = T_all, E_grp = E_all;
+        // scratch sizing bounds: the batch's largest halflen and sketch
+        const int32_t max_hl = hl_all;
+        const uint64_t max_m = mp->so.max_m;
+        if (!one_group) {
+            r1 = r0;
+            uint64_t tsum = 0;
+            while (r1 < R && (r1 == r0 || tsum + cap[r1] <= mp->tab_budget)) tsum += cap[r1++];
+            const int32_t RG = r1 - r0;
+            std::vector<uint64_t> hb(RG + 1), tb(RG + 1);
+            hb[0] = tb[0] = 0;
+            for (int32_t i = 0; i < RG; i++) {
+                hb[i + 1] = hb[i] + E[r0 + i];
+                tb[i + 1] = tb[i] + cap[r0 + i];
+            }
+            HIPCHK(hipMemcpyAsync(mp->grp_base.p, hb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(mp->grp_base.p + (RG + 1), tb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
+            T = tb[RG];
+            E_grp = hb[RG];
+            HIPCHK(hipStreamSynchronize(st));  // hb and tb leave scope
         }
-        RCHK(mp->grp_base.ensure(3 * (size_t)(RG + 1)));
-        HIPCHK(hipMemcpyAsync(mp->grp_base.p, hb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(mp->grp_base.p + (RG + 1), tb.data(), (RG + 1) * 8, hipMemcpyHostToDevice, st));
-        const uint64_t T = tb[RG];
+        const int32_t RG = r1 - r0;
         RCHK(mp->tb_key.ensure(T + 1));
         RCHK(mp->tb_mc.ensure(T + 1));
         RCHK(mp->tb_min.ensure(T + 1));
@@ -795,6 +881,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.hit_base = mp->grp_base.p;
         A.tab_base = mp->grp_base.p + (RG + 1);
         A.bk_base = mp->grp_base.p + 2 * (size_t)(RG + 1);
+        A.grp_nb = mp->grp_nb.p;
         A.tb_key = mp->tb_key.p;
         A.tb_mc = mp->tb_mc.p;
         A.tb_min = mp->tb_min.p;
@@ -803,22 +890,18 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         launch_stage(ST_TAB_INIT, A, I, nullptr, T, mp->max_grid, st);
         TIME_END(ST_TAB_INIT);
         TIME_BEGIN(ST_CONTRIB);
-        launch_stage(ST_CONTRIB, A, I, nullptr, hb[RG], mp->max_grid, st);
+        launch_stage(ST_CONTRIB, A, I, nullptr, E_grp, mp->max_grid, st);
         TIME_END(ST_CONTRIB);
         TIME_BEGIN(ST_COUNT_BUCKETS);
         launch_stage(ST_COUNT_BUCKETS, A, I, nullptr, 0, mp->max_grid, st);
         TIME_END(ST_COUNT_BUCKETS);
+        launch_scan(mp->grp_nb.p, mp->grp_base.p + 2 * (size_t)(RG + 1), nullptr, RG, mp->part.p, st);
         HIPCHK(hipGetLastError());
-        std::vector<int32_t> nb(RG);
-        HIPCHK(hipMemcpyAsync(nb.data(), A.rd_nb + r0, RG * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(mp->h_sum.p + 3, mp->grp_base.p + 2 * (size_t)(RG + 1) + RG, 8, hipMemcpyDeviceToHost,
+                              st));
         HIPCHK(hipStreamSynchronize(st));
         RCHK(flush_timing(mp, pending));
-        std::vector<uint64_t> bb(RG + 1);
-        bb[0] = 0;
-        for (int32_t i = 0; i < RG; i++) bb[i + 1] = bb[i] + (uint64_t)nb[i];
-        const uint64_t B = bb[RG];
-        HIPCHK(hipMemcpyAsync(mp->grp_base.p + 2 * (size_t)(RG + 1), bb.data(), (RG + 1) * 8, hipMemcpyHostToDevice,
-                              st));
+        const uint64_t B = mp->h_sum.p[3];
         RCHK(mp->bk_key.ensure(B + 1));
         RCHK(mp->bk_val.ensure(B + 1));
         RCHK(mp->tmp_g.ensure(B + 1));
@@ -921,24 +1004,24 @@ extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, s
     }
     RCHK(mp->bases.ensure(off[n] + 64));
     RCHK(mp->roff.ensure(n + 1));
-    // stage through one pinned-free host copy: reads are separate allocations
-    std::vector<uint8_t> h(off[n] > 0 ? off[n] : 1);
+    // gather the reads into one page-locked staging copy, then one upload (the
+    // previous call ended with a stream synchronisation, so no copy still reads
+    // the staging buffers)
+    RCHK(mp->h_bases.ensure(off[n] + 1));
+    RCHK(mp->h_roff.ensure(n + 1));
     for (int32_t i = 0; i < n; i++)
-        if (reads[i].len) memcpy(h.data() + off[i], reads[i].seq, reads[i].len);
-    HIPCHK(hipMemcpyAsync(mp->bases.p, h.data(), off[n], hipMemcpyHostToDevice, mp->st));
-    HIPCHK(hipMemcpyAsync(mp->roff.p, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
-    HIPCHK(hipStreamSynchronize(mp->st));
-    return map_device(mp, mp->bases.p, mp->roff.p, off, n, out);
+        if (reads[i].len) memcpy(mp->h_bases.p + off[i], reads[i].seq, reads[i].len);
+    memcpy(mp->h_roff.p, off.data(), (n + 1) * 8);
+    HIPCHK(hipMemcpyAsync(mp->bases.p, mp->h_bases.p, off[n], hipMemcpyHostToDevice, mp->st));
+    HIPCHK(hipMemcpyAsync(mp->roff.p, mp->h_roff.p, (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
+    return map_device(mp, mp->bases.p, mp->roff.p, n, out);
 }
 
 extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,
                                     shm_read_result *out) {
     if (!mp || n < 0 || (n > 0 && (!d_seq || !d_off || !out))) return fail(SHM_ERR_ARG, "invalid argument");
     HIPCHK(hipSetDevice(mp->idx->device));
-    std::vector<int64_t> off(n + 1);
-    HIPCHK(hipMemcpyAsync(off.data(), d_off, (n + 1) * 8, hipMemcpyDeviceToHost, mp->st));
-    HIPCHK(hipStreamSynchronize(mp->st));
-    return map_device(mp, d_seq, d_off, off, n, out);
+    return map_device(mp, d_seq, d_off, n, out);
 }
 
 // ---------------------------------------------------------------------------

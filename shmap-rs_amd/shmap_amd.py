@@ -107,6 +107,8 @@ def lib():
                                              C.c_int32, C.POINTER(C.c_int32)]
         L.shm_mapper_stage_name.argtypes = [C.c_int32]
         L.shm_mapper_last_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]
+        L.shm_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.shm_host_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -254,9 +256,14 @@ class Mapper:
         del keep
         return out[:n]
 
-    def map_device(self, d_bases_ptr: int, d_off_ptr: int, n: int) -> np.ndarray:
-        """Maps n reads whose bases and offsets are already in device memory."""
-        out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+    def map_device(self, d_bases_ptr: int, d_off_ptr: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
+        """Maps n reads whose bases and offsets are already in device memory.
+        `out` (RESULT_DTYPE, at least n records; page-locked from host_array for
+        full copy speed) receives the results; a new array otherwise."""
+        if out is None:
+            out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        elif out.dtype != RESULT_DTYPE or len(out) < n or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous RESULT_DTYPE array of at least n records")
         _check(lib().shm_map_batch_device(self._p, d_bases_ptr, d_off_ptr, n, out.ctypes.data))
         return out[:n]
 
@@ -273,6 +280,28 @@ class Mapper:
         nw = C.c_int32()
         _check(lib().shm_mapper_last_counts(self._p, arr, 8, C.byref(nw)))
         return {COUNT_NAMES[i]: int(arr[i]) for i in range(min(nw.value, len(COUNT_NAMES)))}
+
+
+class _HostBlock:
+    """Owner of one page-locked allocation; freed when the last view goes."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        _check(lib().shm_host_alloc(max(nbytes, 1), C.byref(self.ptr)))
+
+    def __del__(self):
+        if self.ptr:
+            lib().shm_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def host_array(n: int, dtype) -> np.ndarray:
+    """A numpy array of n items in page-locked host memory (shm_host_alloc)."""
+    dt = np.dtype(dtype)
+    blk = _HostBlock(n * dt.itemsize)
+    raw = (C.c_uint8 * max(n * dt.itemsize, 1)).from_address(blk.ptr.value)
+    raw._owner = blk  # keep the block alive as long as any view of it
+    return np.frombuffer(raw, dtype=dt, count=n)
 
 
 COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "refine_first", "second_pass_band")
