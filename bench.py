@@ -42,6 +42,12 @@ def _load(name, path):
 G = _load("shmap_amd", os.path.join(ROOT, "shmap-rs_amd", "shmap_amd.py"))
 
 HBM_PEAK_GBS = 8000.0
+PROBE_COPY_BYTES = 1 << 30  # calibration copy: reads and writes exactly this many bytes
+
+# main kernel of each timed stage (for the counter passes)
+STAGE_KERNEL = {"sketch": "k_sketch<0>(", "sketch_compact": "k_sketch_compact(", "seed_wave": "k_seed_wave(",
+                "contrib": "k_contrib(", "count_buckets": "k_count_buckets(", "extract": "k_extract(",
+                "prune": "k_prune(", "refine": "k_refine(", "final": "k_final("}
 
 
 def log(*a):
@@ -140,6 +146,58 @@ def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
     }
 
 
+def _pmc_pass(counter: str, args, out_dir: str) -> dict:
+    """One rocprofv3 --pmc pass over a probe run; {kernel name: [values per dispatch]} in KiB."""
+    import csv
+    import subprocess
+    cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "--output-format", "csv",
+           "-d", out_dir, "-o", counter.lower(), "--", sys.executable, os.path.abspath(__file__), "--pmc-probe",
+           "--workload", args.workload, "--metric", args.metric]
+    if args.reads:
+        cmd += ["--reads", str(args.reads)]
+    log(f"[bench] counter pass {counter} ...")
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, cwd="/tmp",
+                       env=dict(os.environ, TMPDIR="/tmp"))
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}: {r.stderr[-400:]}")
+    found = []
+    for dp, _, fs in os.walk(out_dir):
+        found += [os.path.join(dp, f) for f in fs if f.startswith(counter.lower()) and
+                  f.endswith("counter_collection.csv")]
+    if not found:
+        raise RuntimeError(f"no counter_collection.csv for {counter}")
+    vals = {}
+    for f in found:
+        for row in csv.DictReader(open(f)):
+            if row.get("Counter_Name") != counter:
+                continue
+            vals.setdefault(row["Kernel_Name"], []).append((int(row["Dispatch_Id"]), float(row["Counter_Value"])))
+    log(f"[bench] counter pass {counter} done")
+    return {k: [v for _, v in sorted(x)] for k, x in vals.items()}
+
+
+def measure_traffic(args, stage: str) -> dict:
+    """HBM bytes of the stage's kernel in one mapping step, from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
+    for a copy of PROBE_COPY_BYTES."""
+    import tempfile
+    want = STAGE_KERNEL[stage]
+    res = {"kernel": want.rstrip("("), "counters": "FETCH_SIZE, WRITE_SIZE"}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for counter, key in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+            vals = _pmc_pass(counter, args, os.path.join(td, counter.lower()))
+            copy = [v for k, x in vals.items() if "k_probe_copy(" in k for v in x]
+            kern = [x for k, x in vals.items() if want in k]
+            if not copy or not kern:
+                raise RuntimeError(f"{counter}: dispatches not found")
+            scale = PROBE_COPY_BYTES / (copy[-1] * 1024.0)
+            res[key + "_kib"] = kern[0][-1]  # the mapping step's dispatch (the index build's comes first)
+            res[key + "_scale"] = round(scale, 4)
+            res[key + "_bytes"] = int(kern[0][-1] * 1024.0 * scale)
+    res["bytes"] = res["fetch_bytes"] + res["write_bytes"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,8 +209,9 @@ def main():
     ap.add_argument("--plain-reads", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="also run one device-to-device copy of this size (reference for PMC byte counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM traffic")
+    ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
@@ -177,15 +236,18 @@ def main():
     st = idx.stats()
     log(f"[bench] GPU index: {st['indexed_kmers']} kmers, {st['indexed_hits']} hits in {t_index:.2f}s")
 
-    if args.calib_copy_mib:
-        nbytes = args.calib_copy_mib << 20
-        src, dst = C.c_void_p(), C.c_void_p()
-        assert hip.L.hipMalloc(C.byref(src), nbytes) == 0 and hip.L.hipMalloc(C.byref(dst), nbytes) == 0
-        assert hip.L.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    if args.pmc_probe:
+        # one calibration copy, then one mapping step: run under rocprofv3 --pmc
+        probe = C.CDLL(os.path.join(ROOT, "shmap-rs_amd", "lib", "libshm_probe.so"))
+        probe.shm_probe_copy.argtypes = [C.c_uint64]
+        assert probe.shm_probe_copy(PROBE_COPY_BYTES) == 0
+        prm = G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
+        mp = G.Mapper(idx, prm)
+        d_bases, d_off = hip.upload(bases), hip.upload(off)
+        mp.map_device(d_bases, d_off, n_reads)
         hip.sync()
-        hip.free(src.value)
-        hip.free(dst.value)
-        log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
+        log("[bench] probe step done")
+        return
 
     prm = G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
     mp = G.Mapper(idx, prm)
@@ -228,6 +290,14 @@ def main():
         roof.update(achieved=round(ach, 2), frac=round(ach / HBM_PEAK_GBS, 5), algorithmic_bytes=int(dom_bytes))
     else:
         roof.update(achieved=None, frac=None, algorithmic_bytes=None)
+    if not args.no_traffic and dom in STAGE_KERNEL:
+        try:
+            tr = measure_traffic(args, dom)
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_detail"] = tr
+        except Exception as e:  # the counters are a report, not the measurement
+            roof["traffic_error"] = str(e)[:300]
+            log(f"[bench] traffic: {e}")
     stage_rates = {s: (round(sb[s] / (stage_ms[s] * 1e-3) / 1e9, 2) if s in sb and stage_ms[s] > 0 else None)
                    for s in stage_ms}
     log("[bench] stage ms: " + ", ".join(f"{s}={v:.2f}" for s, v in stage_ms.items()))
