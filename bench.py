@@ -311,6 +311,33 @@ def main():
     plain_same = bool(np.array_equal(plain[["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]],
                                      res[:n_plain][["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]]))
 
+    # PCIe-inclusive: reads start in (page-locked) host memory, uploaded inside the step
+    h_bases = G.host_array(len(bases), np.uint8)
+    h_bases[:] = bases
+    hip.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.sync()
+    tq = time.time()
+    for _ in range(2):
+        assert hip.L.hipMemcpy(C.c_void_p(d_bases), C.c_void_p(h_bases.ctypes.data), len(bases), 1) == 0
+        assert hip.L.hipMemcpy(C.c_void_p(d_off), C.c_void_p(off.ctypes.data), off.nbytes, 1) == 0
+        mp.map_device(d_bases, d_off, n_reads, out=out)
+    pcie_ms = (time.time() - tq) / 2 * 1e3
+    del h_bases
+    pcie = {"reads_per_s": round(n_reads / pcie_ms * 1e3, 2), "ms_per_step": round(pcie_ms, 3),
+            "upload_gb": round(len(bases) / 1e9, 3)}
+    log(f"[bench] PCIe-inclusive: {pcie}")
+
+    # whole-path algorithmic bytes per step: bases, one 16 B probe per distinct
+    # k-mer, 8 B per examined hit, 2m reference k-mers of 12 B per refined
+    # bucket, 16 B per sketched k-mer (the prune pass's hit reads are not counted)
+    Lr = (off[1:] - off[:-1]).astype(np.int64)
+    pb = int((Lr + 16 * res["u"].astype(np.int64) + 8 * res["seed_matches"].astype(np.int64).clip(min=0)
+              + 24 * res["refined_buckets"].astype(np.int64) * res["m"].astype(np.int64)
+              + 16 * res["m"].astype(np.int64)).sum())
+    path_gbs = pb * args.steps / elapsed / 1e9
+    path_roof = {"bytes_per_step": pb, "achieved": round(path_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(path_gbs / HBM_PEAK_GBS, 5)}
+
     # latency: every read of a step completes with the step
     lat = np.array(times) * 1e3
 
@@ -369,6 +396,8 @@ def main():
         "plain_gpu": {"reads_per_s": round(plain_rate, 2) if plain_rate else None, "reads": n_plain,
                       "same_records": plain_same},
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
+        "pcie_inclusive": pcie,
+        "path_roofline": path_roof,
         "index_build_s": round(t_index, 3),
         "stage_ms": {s: round(v, 3) for s, v in stage_ms.items()},
         "work_counts": counts,
