@@ -198,6 +198,66 @@ def measure_traffic(args, stage: str) -> dict:
     return res
 
 
+def run_serving(idx, prm, lens, ref, args) -> dict:
+    """Config 5: Poisson arrivals of reads (lognormal 1-100 kb, mean 15 kb, an
+    even mix of 0.1% and 7% error profiles) into the serving loop over the
+    resident index; per-read latency from scheduled arrival to result."""
+    import threading
+    n = args.serve_reads
+    bases, off, _ = G.synth_reads(5, ref, lens, n, (15_000, 10_000), lognormal=True, clip=(1_000, 100_000),
+                                  errors=((0.001, 0.6, 0.2, 0.2), (0.07, 0.4, 0.3, 0.3)), both_strands=True)
+    hb = G.host_array(len(bases), np.uint8)  # page-locked: a batch of consecutive arrivals is one upload
+    hb[:] = bases
+    rng = np.random.default_rng(11)
+    arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    lg = C.CDLL(os.path.join(ROOT, "shmap-rs_amd", "lib", "libshm_probe.so"))
+    lg.shm_loadgen_replay.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
+    srv = G.Server(idx, prm, workers=args.serve_workers, max_batch_reads=args.serve_batch,
+                   max_batch_bases=1 << 31)
+    runs = []
+    for rep in range(2):  # the first run warms every worker's device buffers
+        t_begin = srv.now() + 0.005
+        rc = []
+        th = threading.Thread(target=lambda: rc.append(lg.shm_loadgen_replay(
+            srv._p, hb.ctypes.data, off.ctypes.data, arrival.ctypes.data, n, t_begin)))
+        th.start()
+        got, have = [], 0
+        while have < n:
+            g = srv.poll(cap=65536, min_results=1, timeout_ms=60_000)
+            if not len(g) and not th.is_alive() and srv.counts() == (0, 0, 0):
+                break
+            got.append(g)
+            have += len(g)
+        th.join()
+        if rc and rc[0] != 0 or have < n:
+            raise RuntimeError(f"serving run failed: rc={rc} got={have}/{n} {srv.error()}")
+        s = np.concatenate(got)
+        s = s[np.argsort(s["tag"])]
+        runs.append((t_begin, s))
+        log(f"[bench] serving run {rep} done: {n} reads")
+    t_begin, s = runs[-1]
+    srv.close()
+    lat = (s["done_s"] - (t_begin + arrival)) * 1e3
+    span = float(s["done_s"].max() - t_begin)
+    # the same reads in one device-resident batch: results must not depend on batching
+    mp = G.Mapper(idx, prm)
+    whole = mp.map_batch((hb, off))
+    same = all(np.array_equal(whole[f].view(np.int64) if whole.dtype[f] == np.float64 else whole[f],
+                              s["r"][f].view(np.int64) if whole.dtype[f] == np.float64 else s["r"][f])
+               for f in whole.dtype.names)
+    keys = s["worker"].astype(np.float64) * 1e6 + s["start_s"]
+    _, first = np.unique(keys, return_index=True)
+    bt = (s["done_s"][first] - s["start_s"][first]) * 1e3
+    log(f"[bench] serving batches: {len(first)}, batch ms p50 {np.percentile(bt, 50):.2f} "
+        f"max {bt.max():.2f}, queue wait p50 {np.percentile((s['start_s'] - t_begin - arrival) * 1e3, 50):.2f} ms")
+    return {"offered_reads_per_s": args.serve_rate, "reads": n, "reads_per_s": round(n / span, 1),
+            "gbp_per_s": round(float(off[-1]) / span / 1e9, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
+            "mean_batch_reads": round(float(s["batch_reads"].mean()), 1), "workers": args.serve_workers,
+            "mapped_reads": int(s["r"]["mapped"].sum()), "same_as_one_batch": same,
+            "workload": "lognormal 1-100 kb (mean 15 kb), 50/50 0.1% and 7% errors, Poisson arrivals"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +269,11 @@ def main():
     ap.add_argument("--plain-reads", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--serve-reads", type=int, default=200_000)
+    ap.add_argument("--serve-rate", type=float, default=1_000_000.0, help="offered reads/s (Poisson arrivals)")
+    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-batch", type=int, default=16384, help="admission budget: reads per batch")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM traffic")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
@@ -338,6 +403,11 @@ def main():
     path_roof = {"bytes_per_step": pb, "achieved": round(path_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(path_gbs / HBM_PEAK_GBS, 5)}
 
+    serving = None
+    if not args.no_serve:
+        serving = run_serving(idx, prm, lens, ref, args)
+        log(f"[bench] serving: {serving}")
+
     # latency: every read of a step completes with the step
     lat = np.array(times) * 1e3
 
@@ -397,6 +467,7 @@ def main():
                       "same_records": plain_same},
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
         "pcie_inclusive": pcie,
+        "serving": serving,
         "path_roofline": path_roof,
         "index_build_s": round(t_index, 3),
         "stage_ms": {s: round(v, 3) for s, v in stage_ms.items()},

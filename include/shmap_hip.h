@@ -177,15 +177,66 @@ int shm_mapper_last_timing(const shm_mapper *mp, double *total_ms, double *stage
                            int32_t n_stages, int32_t *n_written);
 const char *shm_mapper_stage_name(int32_t stage);
 /* Work counts of the last batch on this mapper: [0] k-mers sketched, [1]
- * distinct k-mers (seeds), [2] buckets counted, [3] buckets refined (those that
- * survive the full-extension prune and that a match_rest pass can reach), [4]
- * read groups (table-budget batches), [5] first-pass refine tasks, [6] buckets
- * only a second pass can reach (refined for reads mapped by the first).
- * Writes up to n values and their number to *n_written. */
+ * distinct k-mers (seeds), [2] buckets counted, [3] buckets refined (a superset
+ * of what the match_rest passes read), [4] read groups (table-budget batches),
+ * [5] first-pass candidates (bound at least theta after the full-extension
+ * prune), [6] buckets only a second pass can reach.  Writes up to n values and
+ * their number to *n_written. */
 int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */
 int shm_mapper_set_timing(shm_mapper *mp, int32_t on);
+
+/* ---- Serving loop: continuous batching over one resident index ----------
+ *
+ * No reference counterpart: shmap-rs maps one read file per run
+ * (src/shmap/mod.rs:374-650).  The server keeps the index on the device and
+ * maps reads as callers submit them.  `workers` host threads each own one
+ * mapper (one HIP stream); a free worker takes every queued read up to the
+ * admission budget (max_batch_reads reads, max_batch_bases bases, which bound
+ * the batch's device memory) and maps them as one batch, so reads that arrive
+ * while other batches are on the GPU are admitted as soon as a slot frees, and
+ * the batches of different workers overlap on the device.  A read's result
+ * does not depend on the batch it lands in. */
+typedef struct shm_server shm_server;
+
+typedef struct {
+    int32_t workers;         /* concurrent batches (mappers/streams); 0 = 2 */
+    int32_t max_batch_reads; /* admission budget per batch; 0 = 65536 */
+    int64_t max_batch_bases; /* admission budget per batch; 0 = 2^31 */
+} shm_server_config;
+
+typedef struct {
+    uint64_t tag;          /* the caller's id from shm_server_submit */
+    double submit_s;       /* seconds since shm_server_create: submitted, */
+    double start_s;        /* its batch started, */
+    double done_s;         /* its result was ready */
+    int32_t batch_reads;   /* reads in the batch it was mapped with */
+    int32_t worker;
+    shm_read_result r;     /* r.status != 0: the batch failed (see shm_server_error) */
+} shm_served;
+
+int shm_server_create(const shm_index *idx, const shm_map_params *params, const shm_server_config *cfg,
+                      shm_server **out);
+/* Queues reads (the bases are copied); thread-safe. */
+int shm_server_submit(shm_server *s, uint64_t tag, const uint8_t *seq, int32_t len);
+int shm_server_submit_many(shm_server *s, const uint64_t *tags, const shm_read *reads, int32_t n);
+/* Queues reads without copying: the caller keeps each read's bytes alive and
+ * unchanged until its result has been polled (or the server freed).  Reads
+ * laid out back to back in one buffer reach the device in one copy per batch
+ * (at full link speed when the buffer comes from shm_host_alloc). */
+int shm_server_submit_borrowed(shm_server *s, const uint64_t *tags, const shm_read *reads, int32_t n);
+/* Takes up to cap finished results, waiting up to timeout_ms (-1: no limit)
+ * for at least min_results of them (fewer if nothing more is queued or running). */
+int shm_server_poll(shm_server *s, shm_served *out, int32_t cap, int32_t min_results, int32_t timeout_ms,
+                    int32_t *n);
+/* Reads queued, being mapped, and finished but not yet polled. */
+int shm_server_counts(shm_server *s, int64_t *queued, int64_t *running, int64_t *finished);
+double shm_server_now(const shm_server *s);
+/* The message of the first failed batch ("" if none). */
+const char *shm_server_error(shm_server *s);
+/* Waits for running batches, drops queued reads, frees everything. */
+void shm_server_free(shm_server *s);
 
 #ifdef __cplusplus
 }

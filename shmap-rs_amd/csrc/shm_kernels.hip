@@ -1014,6 +1014,7 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
 // survival for any threshold; extension stops once sh falls below
 // theta*(1-min_diff), under every threshold either match_rest pass can use.
 __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
+    __shared__ int32_t s_first;
     const int ap = A.abs_pos;
     for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
         const int32_t rd = A.r0 + lr;
@@ -1022,7 +1023,11 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
         const int32_t seeds0 = A.rd_bseeds[rd];
-        if (threadIdx.x == 0) A.rd_flag[rd] = INT32_MAX;
+        if (threadIdx.x == 0) {
+            A.rd_flag[rd] = INT32_MAX;
+            s_first = INT32_MAX;
+        }
+        __syncthreads();
         for (int32_t p = threadIdx.x; p < nb; p += WG) {
             int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
             int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
@@ -1071,12 +1076,15 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 if (A.no_bucket_pruning || !(sh < A.theta)) {
                     const uint32_t t = atomicAdd(&A.task_count[0], 1u);
                     A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+                    atomicMin(&s_first, p);
                 } else {
                     const uint32_t t = atomicAdd(&A.task_count[1], 1u);
                     A.tasks[A.task_cap - 1 - t] = make_uint2((uint32_t)lr, (uint32_t)p);
                 }
             }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) A.rd_first[rd] = s_first;
         __syncthreads();
     }
 }
@@ -1105,6 +1113,62 @@ __device__ void wg_scan_inplace(int32_t *a, int64_t n, uint32_t *tmp) {
         __syncthreads();
         if (threadIdx.x == 0) s_carry = c + (int32_t)tot;
         __syncthreads();
+    }
+}
+
+// Selects one refine round's buckets into A.sel (count task_count[2 + round]).
+// A superset of what the two match_rest passes read is refined, in rounds:
+//   0  each read's first first-pass candidate in sorted order -- the first pass
+//      always refines it (its threshold is still theta there);
+//   1  the other first-pass candidates whose bound reaches
+//      L1 = max(theta, J(first)): after the first candidate the first pass's
+//      threshold is at least L1, so lower bounds are never refined;
+//   2  for reads the first pass maps, what the second pass can reach and rounds
+//      0-1 did not refine: bounds of at least
+//      L2 = J(first bucket above theta) * (1 - min_diff), which is at most the
+//      second pass's starting threshold best * (1 - min_diff).
+__global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
+    const int32_t round = A.refine_pass;
+    const uint32_t c0 = A.task_count[0], c1 = A.task_count[1];
+    const uint64_t n = round == 0 ? (uint64_t)A.RG : round == 1 ? c0 : (uint64_t)c0 + c1;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    for (uint64_t b0 = (uint64_t)blockIdx.x * WG; b0 < n; b0 += (uint64_t)gridDim.x * WG) {
+        const uint64_t t = b0 + threadIdx.x;
+        bool keep = false;
+        uint2 task = make_uint2(0, 0);
+        if (t < n) {
+            if (round == 0) {
+                const int32_t p = A.rd_first[A.r0 + (int32_t)t];
+                keep = p != INT32_MAX;
+                task = make_uint2((uint32_t)t, (uint32_t)p);
+            } else {
+                task = A.tasks[t < c0 ? t : A.task_cap - 1 - (t - c0)];
+                const int32_t lr = (int32_t)task.x, p = (int32_t)task.y;
+                const int32_t rd = A.r0 + lr;
+                const uint64_t bx = A.bk_base[lr];
+                const int32_t first = A.rd_first[rd];
+                const double shp = A.bs_sh[bx + p];
+                bool in_r1 = false;  // refined in round 1 (first-pass candidates only)
+                if (t < c0 && p != first) {
+                    const double jf = A.sc_j[bx + first];
+                    in_r1 = !(shp < (jf > A.theta ? jf : A.theta));
+                }
+                if (round == 1) {
+                    keep = in_r1;
+                } else {
+                    const int32_t succ = A.rd_flag[rd];
+                    keep = succ != INT32_MAX && p != first && !in_r1 &&
+                           !(shp < A.sc_j[bx + succ] * (1.0 - A.min_diff));
+                }
+            }
+        }
+        const uint64_t bal = __ballot(keep);
+        if (bal) {
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&A.task_count[2 + round], (uint32_t)__popcll(bal));
+            at = __shfl(at, 0);
+            if (keep) A.sel[at + __popcll(bal & ((1ull << lane) - 1ull))] = task;
+        }
     }
 }
 
@@ -1138,22 +1202,21 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     __shared__ double s_bj[NWAVES];
     __shared__ int32_t s_bl[NWAVES];
     const int ap = A.abs_pos;
-    const uint32_t ntask = A.task_count[A.refine_pass];
+    // Rounds (see the task-list comment of MapArgs):
+    //   0  each read's first first-pass candidate in sorted order -- the first
+    //      pass always refines it (its threshold is still theta there);
+    //   1  the other first-pass candidates whose bound reaches
+    //      L1 = max(theta, J(first)): after the first candidate the first
+    //      pass's threshold is at least L1, so lower bounds are never refined;
+    //   2  for reads the first pass maps, what the second pass can reach and
+    //      rounds 0-1 did not refine: bounds of at least
+    //      L2 = J(first bucket above theta) * (1 - min_diff), which is at most
+    //      the second pass's starting threshold best * (1 - min_diff).
+    const uint32_t ntask = A.task_count[2 + A.refine_pass];
     for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
-        const uint2 task = A.tasks[A.refine_pass == 0 ? t : A.task_cap - 1 - t];
+        const uint2 task = A.sel[t];
         const int32_t lr = (int32_t)task.x, p = (int32_t)task.y;
         const int32_t rd = A.r0 + lr;
-        if (A.refine_pass == 1) {
-            // The second pass runs only for reads whose first pass mapped, with
-            // threshold best*(1-min_diff); best is at least the score of the
-            // first sorted bucket that beat theta (the first pass takes it, and
-            // its threshold only rises), so a bound below that product is never
-            // reached.
-            const int32_t first = A.rd_flag[rd];
-            if (first == INT32_MAX) continue;
-            if (A.bs_sh[A.bk_base[lr] + p] < A.sc_j[A.bk_base[lr] + first] * (1.0 - A.min_diff)) continue;
-            if (threadIdx.x == 0) atomicAdd(&A.task_count[2], 1u);
-        }
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], hl = A.rd_hl[rd];
@@ -1668,12 +1731,13 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_EXTRACT: k_extract<<<gg, WG, SORT_LDS, st>>>(A); break;
     case ST_PRUNE: k_prune<<<gg, WG, 0, st>>>(A, I); break;
     case ST_REFINE: {
-        // first-pass candidates, then the second-pass-only band of flagged reads
+        // three rounds: first candidates, other first-pass candidates, second pass
         MapArgs P = A;
-        P.refine_pass = 0;
-        k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
-        P.refine_pass = 1;
-        k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+        for (int32_t round = 0; round < 3; round++) {
+            P.refine_pass = round;
+            k_refine_select<<<max_grid, WG, 0, st>>>(P);
+            k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+        }
         break;
     }
     case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;

@@ -91,17 +91,20 @@ struct MapArgs {
     double *sc_j;
     int32_t *sc_i, *sc_same, *sc_tl, *sc_tr, *sc_ssz, *sc_pend, *sc_seg;
     // refine tasks: (group read, sorted bucket) of every bucket that can survive
-    // a match_rest pass.  Buckets whose bound reaches theta fill tasks[0, ..)
-    // (task_count[0]); those only the second pass can reach (bound in
-    // [theta*(1-min_diff), theta)) fill tasks from the end, tasks[cap-1-i]
-    // (task_count[1]), and are refined only for reads with a first-pass score
-    // above theta: rd_flag holds the first sorted bucket with such a score
-    // (INT32_MAX if none).  task_count[2] counts those refined.
-    uint2 *tasks;
+    // a match_rest pass.  Buckets whose bound reaches theta (first-pass
+    // candidates) fill tasks[0, ..) (task_count[0]); those only the second pass
+    // can reach (bound in [theta*(1-min_diff), theta)) fill tasks from the end,
+    // tasks[cap-1-i] (task_count[1]).  rd_first holds each read's first
+    // first-pass candidate in sorted order, rd_flag the first sorted bucket
+    // scoring above theta (INT32_MAX if none).  Refinement runs in three rounds
+    // (refine_pass 0-2) that refine a superset of what the two match_rest
+    // passes read; each round's buckets are selected into sel, counted in
+    // task_count[2 + round].
+    uint2 *tasks, *sel;
     uint32_t *task_count;
     uint32_t task_cap;
     int32_t refine_pass;
-    int32_t *rd_flag;
+    int32_t *rd_flag, *rd_first;
     // per-workgroup scratch for spans/reads that overflow LDS
     int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;

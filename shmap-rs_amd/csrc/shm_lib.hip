@@ -598,7 +598,7 @@ struct shm_mapper {
     DevBuf<uint8_t> bs_alive;
     DevBuf<double> bs_sh, sc_j;
     DevBuf<int32_t> g_sid, g_tr, g_diff, g_tails;
-    DevBuf<uint2> tasks;
+    DevBuf<uint2> tasks, sel;
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
 };
@@ -755,7 +755,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->sd_hoff.ensure(M + 1));
     RCHK(mp->sd_ord.ensure(M + 1));
     RCHK(mp->sd_epre.ensure(M + 1));
-    RCHK(mp->rd_i32.ensure((size_t)R * 11 + 1));
+    RCHK(mp->rd_i32.ensure((size_t)R * 12 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
     RCHK(mp->d_out.ensure(R));
 
@@ -801,6 +801,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_seed_matches = ri + 8 * (size_t)R;
     A.rd_nb = ri + 9 * (size_t)R;
     A.rd_flag = ri + 10 * (size_t)R;
+    A.rd_first = ri + 11 * (size_t)R;
     A.rd_E = mp->rd_u64.p;
     A.rd_cap = mp->rd_u64.p + R;
     A.r0 = 0;
@@ -844,7 +845,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     }
 
     // bucketing, in groups of reads whose tables fit the budget
-    RCHK(mp->task_count.ensure(3 * (size_t)R + 3));  // three refine-task counters per group
+    RCHK(mp->task_count.ensure(5 * (size_t)R + 5));  // five refine-task counters per group
     int32_t ngroups = 0;
     int64_t nbuckets = 0;
     int32_t r0 = 0;
@@ -920,8 +921,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->tasks.ensure(B + 1));
-        uint32_t *task_count = mp->task_count.p + 3 * (size_t)ngroups;
-        HIPCHK(hipMemsetAsync(task_count, 0, 12, st));
+        RCHK(mp->sel.ensure(B + (uint64_t)RG + 1));
+        uint32_t *task_count = mp->task_count.p + 5 * (size_t)ngroups;
+        HIPCHK(hipMemsetAsync(task_count, 0, 20, st));
         A.bk_key = mp->bk_key.p;
         A.bk_val = mp->bk_val.p;
         A.tmp_g = mp->tmp_g.p;
@@ -953,6 +955,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.g_tails = mp->g_tails.p;
         A.diff_stride = dstride;
         A.tasks = mp->tasks.p;
+        A.sel = mp->sel.p;
         A.task_count = task_count;
         A.task_cap = (uint32_t)(B + 1);
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
@@ -965,7 +968,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         ngroups++;
         r0 = r1;
     }
-    std::vector<uint32_t> ntask(3 * (size_t)ngroups);
+    std::vector<uint32_t> ntask(5 * (size_t)ngroups);
     HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(t1, st));
@@ -977,19 +980,19 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     for (int32_t i = 0; i < R; i++) finalize_result(mp->prm, out[i]);
-    int64_t first = 0, deferred = 0, deferred_run = 0, seeds = 0;
+    int64_t cand = 0, deferred = 0, refined = 0, seeds = 0;
     for (int32_t g = 0; g < ngroups; g++) {
-        first += ntask[3 * g];
-        deferred += ntask[3 * g + 1];
-        deferred_run += ntask[3 * g + 2];
+        cand += ntask[5 * g];
+        deferred += ntask[5 * g + 1];
+        refined += ntask[5 * g + 2] + ntask[5 * g + 3] + ntask[5 * g + 4];
     }
     for (int32_t i = 0; i < R; i++) seeds += out[i].u;
     mp->counts[0] = (int64_t)M;
     mp->counts[1] = seeds;
     mp->counts[2] = nbuckets;
-    mp->counts[3] = first + deferred_run;
+    mp->counts[3] = refined;
     mp->counts[4] = ngroups;
-    mp->counts[5] = first;
+    mp->counts[5] = cand;
     mp->counts[6] = deferred;
     return SHM_OK;
 }
@@ -1004,15 +1007,22 @@ extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, s
     }
     RCHK(mp->bases.ensure(off[n] + 64));
     RCHK(mp->roff.ensure(n + 1));
-    // gather the reads into one page-locked staging copy, then one upload (the
-    // previous call ended with a stream synchronisation, so no copy still reads
-    // the staging buffers)
-    RCHK(mp->h_bases.ensure(off[n] + 1));
+    // Reads back to back in one caller buffer go up in one copy; otherwise they
+    // are gathered into one page-locked staging copy first.  (The previous call
+    // ended with a stream synchronisation, so no copy still reads the staging
+    // buffers.)
+    bool contiguous = n > 0;
+    for (int32_t i = 1; i < n && contiguous; i++) contiguous = reads[i].seq == reads[i - 1].seq + reads[i - 1].len;
     RCHK(mp->h_roff.ensure(n + 1));
-    for (int32_t i = 0; i < n; i++)
-        if (reads[i].len) memcpy(mp->h_bases.p + off[i], reads[i].seq, reads[i].len);
     memcpy(mp->h_roff.p, off.data(), (n + 1) * 8);
-    HIPCHK(hipMemcpyAsync(mp->bases.p, mp->h_bases.p, off[n], hipMemcpyHostToDevice, mp->st));
+    if (contiguous && off[n] > 0) {
+        HIPCHK(hipMemcpyAsync(mp->bases.p, reads[0].seq, off[n], hipMemcpyHostToDevice, mp->st));
+    } else if (off[n] > 0) {
+        RCHK(mp->h_bases.ensure(off[n] + 1));
+        for (int32_t i = 0; i < n; i++)
+            if (reads[i].len) memcpy(mp->h_bases.p + off[i], reads[i].seq, reads[i].len);
+        HIPCHK(hipMemcpyAsync(mp->bases.p, mp->h_bases.p, off[n], hipMemcpyHostToDevice, mp->st));
+    }
     HIPCHK(hipMemcpyAsync(mp->roff.p, mp->h_roff.p, (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
     return map_device(mp, mp->bases.p, mp->roff.p, n, out);
 }

@@ -66,6 +66,21 @@ RESULT_DTYPE = np.dtype([(n, np.int32) for n in _RESULT_INT_FIELDS] + [(n, np.fl
 assert RESULT_DTYPE.itemsize == C.sizeof(ReadResult)
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("max_batch_reads", C.c_int32), ("max_batch_bases", C.c_int64)]
+
+
+class Served(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
+                ("batch_reads", C.c_int32), ("worker", C.c_int32), ("r", ReadResult)]
+
+
+SERVED_DTYPE = np.dtype([("tag", np.uint64), ("submit_s", np.float64), ("start_s", np.float64),
+                         ("done_s", np.float64), ("batch_reads", np.int32), ("worker", np.int32),
+                         ("r", RESULT_DTYPE)], align=True)
+assert SERVED_DTYPE.itemsize == C.sizeof(Served)
+
+
 class ShmError(RuntimeError):
     pass
 
@@ -108,6 +123,18 @@ def lib():
         L.shm_mapper_stage_name.argtypes = [C.c_int32]
         L.shm_mapper_last_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]
         L.shm_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.shm_server_create.argtypes = [C.c_void_p, C.POINTER(MapParams), C.POINTER(ServerConfig),
+                                        C.POINTER(C.c_void_p)]
+        L.shm_server_submit.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]
+        L.shm_server_submit_many.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.shm_server_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32)]
+        L.shm_server_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 3
+        L.shm_server_now.argtypes = [C.c_void_p]
+        L.shm_server_now.restype = C.c_double
+        L.shm_server_error.argtypes = [C.c_void_p]
+        L.shm_server_error.restype = C.c_char_p
+        L.shm_server_free.argtypes = [C.c_void_p]
         L.shm_host_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -282,6 +309,56 @@ class Mapper:
         return {COUNT_NAMES[i]: int(arr[i]) for i in range(min(nw.value, len(COUNT_NAMES)))}
 
 
+class Server:
+    """Serving loop over a resident index (shm_server_*): reads are submitted as
+    they arrive and mapped in batches by `workers` concurrent mappers."""
+
+    def __init__(self, index: SketchIndex, params: MapParams, workers: int = 2, max_batch_reads: int = 0,
+                 max_batch_bases: int = 0):
+        self._index = index
+        self._params = params
+        cfg = ServerConfig(workers, max_batch_reads, max_batch_bases)
+        self._p = C.c_void_p()
+        _check(lib().shm_server_create(index._p, C.byref(params), C.byref(cfg), C.byref(self._p)))
+
+    def now(self) -> float:
+        return lib().shm_server_now(self._p)
+
+    def submit(self, tag: int, seq: bytes) -> None:
+        _check(lib().shm_server_submit(self._p, tag, seq, len(seq)))
+
+    def submit_many(self, tags, seqs) -> None:
+        tags = np.ascontiguousarray(tags, dtype=np.uint64)
+        keep = [bytes(x) for x in seqs]
+        arr = (Read * max(len(keep), 1))()
+        for i, x in enumerate(keep):
+            arr[i].seq = C.cast(C.c_char_p(x), C.c_void_p)
+            arr[i].len = len(x)
+        _check(lib().shm_server_submit_many(self._p, tags.ctypes.data, arr, len(keep)))
+
+    def poll(self, cap: int = 65536, min_results: int = 1, timeout_ms: int = -1) -> np.ndarray:
+        out = np.zeros(max(cap, 1), dtype=SERVED_DTYPE)
+        n = C.c_int32()
+        _check(lib().shm_server_poll(self._p, out.ctypes.data, cap, min_results, timeout_ms, C.byref(n)))
+        return out[:n.value]
+
+    def counts(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().shm_server_counts(self._p, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def error(self) -> str:
+        return lib().shm_server_error(self._p).decode()
+
+    def close(self) -> None:
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.shm_server_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+
 class _HostBlock:
     """Owner of one page-locked allocation; freed when the last view goes."""
 
@@ -304,7 +381,7 @@ def host_array(n: int, dtype) -> np.ndarray:
     return np.frombuffer(raw, dtype=dt, count=n)
 
 
-COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "refine_first", "second_pass_band")
+COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band")
 
 
 def format_paf(index: SketchIndex, params: MapParams, qid: str, rec, map_time: float = 0.0) -> str:
