@@ -46,6 +46,7 @@ PROBE_COPY_BYTES = 1 << 30  # calibration copy: reads and writes exactly this ma
 
 # main kernel of each timed stage (for the counter passes)
 STAGE_KERNEL = {"sketch": "k_sketch<0>(", "sketch_compact": "k_sketch_compact(", "seed_wave": "k_seed_wave(",
+                "seed_wg": "k_seed_wg(",
                 "contrib": "k_contrib(", "count_buckets": "k_count_buckets(", "extract": "k_extract(",
                 "prune": "k_prune(", "refine": "k_refine(", "final": "k_final("}
 
@@ -138,10 +139,6 @@ def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
         "sketch": bases_total + 13 * m,                    # every base read once, (h, r, strand) per kept k-mer
         "sketch_compact": 2 * 13 * m,                      # staged k-mers moved to their final positions
         "seed_wave": 13 * m + 16 * u + 37 * u,             # sketch in, one table slot per probe, seed rows out
-        "sort_kmers": 8 * m + 2 * 12 * m,                  # hashes in, (hash, index) pairs out
-        "group_seeds": 12 * m + 16 * u + 37 * u,           # sorted pairs, one table slot per probe, seed rows
-        "sort_seeds": 2 * 12 * u,
-        "budget": 12 * u,
         "contrib": 12 * E,                                 # one packed hit record per examined hit
     }
 

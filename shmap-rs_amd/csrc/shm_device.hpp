@@ -264,6 +264,30 @@ __device__ void wave_bitonic(K *k, V *v, uint32_t n) {
     }
 }
 
+// One wave sorts n distinct keys in LDS, ascending.
+template <typename K>
+__device__ void wave_bitonic_keys(K *k, uint32_t n) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint32_t N = 1;
+    while (N < n) N <<= 1;
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        const uint32_t half = size >> 1;
+        for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
+            const uint32_t blk = i / half, p = i % half;
+            const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
+            if (b < n) cswap_k(k, a, b);
+        }
+        wave_sync();
+        for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
+                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                if (b < n) cswap_k(k, a, b);
+            }
+            wave_sync();
+        }
+    }
+}
+
 // Sorts global (gk, gv)[0, n) using LDS scratch (sk, sv) of `cap` entries when
 // it fits.  Ends with a barrier; results are visible to the whole workgroup.
 template <typename K, typename V>

@@ -532,42 +532,179 @@ __global__ void __launch_bounds__(WG) k_insert(const uint64_t *uk_h, const uint6
 // whose tables fit the device budget.
 // ============================================================================
 
-// Sort one read's k-mers by (hash, position): the order of
-// unique_elements_with_info (src/shmap/seeding.rs:28-34) with every run kept
-// in increasing r, so its first element is the smallest-r occurrence whose
-// strand the seed keeps and pmatches is the run read backwards.
-__global__ void __launch_bounds__(WG) k_sort_kmers(MapArgs A) {
+// The seeding stages for one read per wave (reads with at most WAVE_SEED_CAP
+// sketched k-mers; larger reads are listed for k_seed_wg):
+//   unique_elements_with_info (src/shmap/seeding.rs:26-77): sort by (hash,
+//     position), one seed per hash run keeping the smallest-r strand, index
+//     probe, stable rarest-first order as a sort by (hits_in_t, seed_num);
+//   the seed budget S, halflen and the consumed prefix of match_seeds
+//     (src/shmap/mod.rs:726-742, src/shmap/seeding.rs:101-106, 207-209).
+// Everything stays in the wave's LDS slice; seeds are written out once.
+__global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    // Per-wave LDS, 14 B per k-mer slot:
+    //   kh  u64: hashes for the (hash, position) sort and the run scan; then the
+    //            (hits << 32 | seed id) keys of the rarest-first sort
+    //   kv  u32: positions; then, at index seed id, each seed's hit count
+    //   so  u16: occurrences of each seed id
+    uint8_t *my = lds + (size_t)w * WAVE_SEED_LDS;
+    uint64_t *kh = (uint64_t *)my;
+    uint32_t *kv = (uint32_t *)(my + WAVE_SEED_CAP * 8);
+    uint16_t *so = (uint16_t *)(my + WAVE_SEED_CAP * 12);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const double theta2 = A.theta - A.min_diff;
+    for (int32_t rd = blockIdx.x * NWAVES + w; rd < A.R; rd += gridDim.x * NWAVES) {
+        const uint64_t o = A.koff[rd];
+        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
+        if (m > WAVE_SEED_CAP) {  // a workgroup's job (k_seed_wg)
+            if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)rd;
+            continue;
+        }
+        for (uint32_t i = lane; i < m; i += WAVE) {
+            kh[i] = A.k_h[o + i];
+            kv[i] = i;
+        }
+        wave_sync();
+        wave_bitonic(kh, kv, m);
+        // runs of equal hash -> seeds in hash order.  A seed id never exceeds
+        // the position of its run head, so kv[sid] can take the seed's hit
+        // count once this chunk has read its own positions (kv of later chunks
+        // is untouched).
+        uint32_t u = 0, pm = 0;
+        for (uint32_t b0 = 0; b0 < m; b0 += WAVE) {
+            const uint32_t i = b0 + lane;
+            const uint32_t pos = i < m ? kv[i] : 0u;
+            const bool head = i < m && (i == 0 || kh[i] != kh[i - 1]);
+            const uint64_t bal = __ballot(head);
+            if (i < m) A.ps_v[o + i] = pos;
+            uint32_t sid = 0, cnt = 0, e = 0;
+            if (head) {
+                sid = u + __popcll(bal & lt);
+                const uint64_t h = kh[i];
+                e = i + 1;
+                while (e < m && kh[e] == h) e++;
+                uint32_t hoff;
+                cnt = index_lookup(I, h, hoff);
+                A.sd_h[o + sid] = h;
+                A.sd_first[o + sid] = i;
+                A.sd_occs[o + sid] = e - i;
+                A.sd_strand[o + sid] = A.k_s[o + pos];
+                A.sd_cnt[o + sid] = cnt;
+                A.sd_hoff[o + sid] = hoff;
+                pm += cnt;
+            }
+            wave_sync();  // every lane has read kv[b0, b0+64) and kh
+            if (head) {
+                kv[sid] = cnt;
+                so[sid] = (uint16_t)(e - i);
+            }
+            u += __popcll(bal);
+        }
+        for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
+        wave_sync();
+        for (uint32_t q = lane; q < u; q += WAVE) kh[q] = ((uint64_t)kv[q] << 32) | q;
+        wave_sync();
+        wave_bitonic_keys(kh, u);
+        // budget: consume seeds while fewer than S occurrences are consumed
+        const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
+        const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
+        const int32_t hl = A.abs_pos ? len : (int32_t)m;
+        const bool ok = hl >= MIN_HALFLEN;
+        uint32_t occ_carry = 0, cnt_carry = 0, c = 0, seeds = 0, mx = 0;
+        for (uint32_t b0 = 0; ok && b0 < u; b0 += WAVE) {
+            const uint32_t q = b0 + lane;
+            uint32_t occ = 0, cnt = 0, sid = 0;
+            if (q < u) {
+                const uint64_t key = kh[q];
+                sid = (uint32_t)key;
+                occ = so[sid];
+                cnt = (uint32_t)(key >> 32);
+                A.sd_ord[o + q] = sid;
+            }
+            const uint32_t oinc = wave_incl_scan(occ);
+            const bool consumed = q < u && (int64_t)(occ_carry + oinc - occ) < (int64_t)S;
+            const uint32_t cv = consumed ? cnt : 0u;
+            const uint32_t cinc = wave_incl_scan(cv);
+            if (consumed) A.sd_epre[o + q] = cnt_carry + cinc - cv;
+            const uint64_t cb = __ballot(consumed);
+            c += __popcll(cb);
+            uint32_t so_ = consumed ? occ : 0u, mm = consumed ? cnt : 0u;
+            for (int d = WAVE / 2; d > 0; d >>= 1) {
+                so_ += __shfl_xor(so_, d);
+                const uint32_t om = __shfl_xor(mm, d);
+                mm = om > mm ? om : mm;
+            }
+            seeds += so_;
+            mx = mm > mx ? mm : mx;
+            occ_carry += __shfl(oinc, WAVE - 1);
+            cnt_carry += __shfl(cinc, WAVE - 1);
+        }
+        if (!ok)
+            for (uint32_t q = lane; q < u; q += WAVE) A.sd_ord[o + q] = (uint32_t)kh[q];
+        if (lane == 0) {
+            const uint32_t E = cnt_carry;
+            A.rd_u[rd] = (int32_t)u;
+            A.rd_possible[rd] = (int32_t)pm;  // an i32 sum (src/shmap/mod.rs:689)
+            A.rd_S[rd] = S;
+            A.rd_hl[rd] = hl;
+            A.rd_ok[rd] = ok ? 1 : 0;
+            A.rd_c[rd] = (int32_t)c;
+            A.rd_bseeds[rd] = (int32_t)seeds;
+            A.rd_max_seed[rd] = (int32_t)mx;
+            A.rd_seed_matches[rd] = (int32_t)E;
+            A.rd_E[rd] = E;
+            uint64_t cap = 0;
+            if (E > 0) {
+                cap = 64;
+                while (cap < 2ull * E) cap <<= 1;
+            }
+            A.rd_cap[rd] = cap;
+        }
+        wave_sync();
+    }
+}
+
+// The seeding stages for reads with more than WAVE_SEED_CAP sketched k-mers
+// (listed by k_seed_wave), one workgroup per read, through global memory with
+// the sorts staged in LDS when they fit: the same steps as k_seed_wave --
+// sort by (hash, position) (the order of unique_elements_with_info,
+// src/shmap/seeding.rs:28-34, with every run in increasing r, so a run's first
+// element is the smallest-r occurrence whose strand the seed keeps and
+// pmatches is the run read backwards), one seed per run with the index probe
+// (SketchIndex::count), the rarest-first order as a sort by (hits_in_t,
+// seed_num) (src/shmap/seeding.rs:26-77), then the seed budget S, halflen and
+// the consumed prefix of match_seeds (src/shmap/mod.rs:726-742,
+// src/shmap/seeding.rs:101-106, 207-209).
+__global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
-    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
+    __shared__ uint32_t tmp[NWAVES + 1];
+    __shared__ uint32_t s_u, s_pm, s_c, s_seeds, s_max, s_occ_carry, s_cnt_carry;
+    const double theta2 = A.theta - A.min_diff;
+    const uint32_t nbig = *A.big_count;
+    for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
+        const int32_t rd = (int32_t)A.big_list[t];
         const uint64_t o = A.koff[rd];
         const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
-        if (m <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
+        // 1. (hash, position) order
         for (uint32_t i = threadIdx.x; i < m; i += WG) {
             A.ps_h[o + i] = A.k_h[o + i];
             A.ps_v[o + i] = i;
         }
-        __syncthreads();
-        wg_sort_pairs(A.ps_h + o, A.ps_v + o, m, sk, sv, SORT_CAP);
-    }
-}
-
-// Group runs into seeds (hash order), probe the index (SketchIndex::count), and
-// key the rarest-first order: the stable sort by hits_in_t of
-// src/shmap/seeding.rs:71 is a sort by (hits_in_t, seed_num).
-__global__ void __launch_bounds__(WG) k_group_seeds(MapArgs A, DevIndex I) {
-    __shared__ uint32_t tmp[NWAVES + 1];
-    __shared__ uint32_t s_u, s_pm;
-    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
-        const uint64_t o = A.koff[rd];
-        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
-        if (m <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
         if (threadIdx.x == 0) {
             s_u = 0;
             s_pm = 0;
+            s_c = 0;
+            s_seeds = 0;
+            s_max = 0;
+            s_occ_carry = 0;
+            s_cnt_carry = 0;
         }
         __syncthreads();
+        wg_sort_pairs(A.ps_h + o, A.ps_v + o, m, sk, sv, SORT_CAP);
+        // 2. runs -> seeds in hash order, one probe each
         for (uint32_t base = 0; base < m; base += WG) {
             const uint32_t i = base + threadIdx.x;
             const bool head = i < m && (i == 0 || A.ps_h[o + i] != A.ps_h[o + i - 1]);
@@ -599,173 +736,18 @@ __global__ void __launch_bounds__(WG) k_group_seeds(MapArgs A, DevIndex I) {
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_pm, pm);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            A.rd_u[rd] = (int32_t)u;
-            A.rd_possible[rd] = (int32_t)s_pm;  // an i32 sum (src/shmap/mod.rs:689)
-        }
-        __syncthreads();
-    }
-}
-
-// The seeding stages for one read per wave (reads with at most WAVE_SEED_CAP
-// sketched k-mers; larger reads take the workgroup kernels below):
-//   unique_elements_with_info (src/shmap/seeding.rs:26-77): sort by (hash,
-//     position), one seed per hash run keeping the smallest-r strand, index
-//     probe, stable rarest-first order as a sort by (hits_in_t, seed_num);
-//   the seed budget S, halflen and the consumed prefix of match_seeds
-//     (src/shmap/mod.rs:726-742, src/shmap/seeding.rs:101-106, 207-209).
-// Everything stays in the wave's LDS slice; seeds are written out once.
-__global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
-    extern __shared__ uint8_t lds[];
-    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    uint8_t *my = lds + (size_t)w * WAVE_SEED_LDS;
-    uint64_t *kh = (uint64_t *)my;                                  // hashes
-    uint32_t *kv = (uint32_t *)(my + WAVE_SEED_CAP * 8);            // positions
-    uint64_t *sk = (uint64_t *)(my + WAVE_SEED_CAP * 12);           // (hits, seed) keys
-    uint32_t *sv = (uint32_t *)(my + WAVE_SEED_CAP * 20);           // seed ids
-    uint32_t *so = (uint32_t *)(my + WAVE_SEED_CAP * 24);           // occs by seed id
-    const uint64_t lt = (1ull << lane) - 1ull;
-    const double theta2 = A.theta - A.min_diff;
-    for (int32_t rd = blockIdx.x * NWAVES + w; rd < A.R; rd += gridDim.x * NWAVES) {
-        const uint64_t o = A.koff[rd];
-        const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
-        if (m > WAVE_SEED_CAP) continue;
-        for (uint32_t i = lane; i < m; i += WAVE) {
-            kh[i] = A.k_h[o + i];
-            kv[i] = i;
-        }
-        wave_sync();
-        wave_bitonic(kh, kv, m);
-        // runs of equal hash -> seeds in hash order
-        uint32_t u = 0, pm = 0;
-        for (uint32_t b0 = 0; b0 < m; b0 += WAVE) {
-            const uint32_t i = b0 + lane;
-            const bool head = i < m && (i == 0 || kh[i] != kh[i - 1]);
-            const uint64_t bal = __ballot(head);
-            if (i < m) A.ps_v[o + i] = kv[i];
-            if (head) {
-                const uint32_t sid = u + __popcll(bal & lt);
-                const uint64_t h = kh[i];
-                uint32_t e = i + 1;
-                while (e < m && kh[e] == h) e++;
-                uint32_t hoff;
-                const uint32_t cnt = index_lookup(I, h, hoff);
-                A.sd_h[o + sid] = h;
-                A.sd_first[o + sid] = i;
-                A.sd_occs[o + sid] = e - i;
-                A.sd_strand[o + sid] = A.k_s[o + kv[i]];
-                A.sd_cnt[o + sid] = cnt;
-                A.sd_hoff[o + sid] = hoff;
-                sk[sid] = ((uint64_t)cnt << 32) | sid;
-                sv[sid] = sid;
-                so[sid] = e - i;
-                pm += cnt;
-            }
-            u += __popcll(bal);
-        }
-        for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
-        wave_sync();
-        wave_bitonic(sk, sv, u);
-        // budget: consume seeds while fewer than S occurrences are consumed
+        // 3. rarest-first order
+        wg_sort_pairs(A.sd_key + o, A.sd_ord + o, u, sk, sv, SORT_CAP);
+        // 4. budget: consume seeds while fewer than S occurrences are consumed
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
         const int32_t hl = A.abs_pos ? len : (int32_t)m;
         const bool ok = hl >= MIN_HALFLEN;
-        uint32_t occ_carry = 0, cnt_carry = 0, c = 0, seeds = 0, mx = 0;
-        for (uint32_t b0 = 0; ok && b0 < u; b0 += WAVE) {
-            const uint32_t q = b0 + lane;
-            uint32_t occ = 0, cnt = 0, sid = 0;
-            if (q < u) {
-                sid = sv[q];
-                occ = so[sid];
-                cnt = (uint32_t)(sk[q] >> 32);
-                A.sd_ord[o + q] = sid;
-            }
-            const uint32_t oinc = wave_incl_scan(occ);
-            const bool consumed = q < u && (int64_t)(occ_carry + oinc - occ) < (int64_t)S;
-            const uint32_t cv = consumed ? cnt : 0u;
-            const uint32_t cinc = wave_incl_scan(cv);
-            if (consumed) A.sd_epre[o + q] = cnt_carry + cinc - cv;
-            const uint64_t cb = __ballot(consumed);
-            c += __popcll(cb);
-            uint32_t so_ = consumed ? occ : 0u, mm = consumed ? cnt : 0u;
-            for (int d = WAVE / 2; d > 0; d >>= 1) {
-                so_ += __shfl_xor(so_, d);
-                const uint32_t om = __shfl_xor(mm, d);
-                mm = om > mm ? om : mm;
-            }
-            seeds += so_;
-            mx = mm > mx ? mm : mx;
-            occ_carry += __shfl(oinc, WAVE - 1);
-            cnt_carry += __shfl(cinc, WAVE - 1);
-        }
-        if (!ok)
-            for (uint32_t q = lane; q < u; q += WAVE) A.sd_ord[o + q] = sv[q];
-        if (lane == 0) {
-            const uint32_t E = cnt_carry;
-            A.rd_u[rd] = (int32_t)u;
-            A.rd_possible[rd] = (int32_t)pm;  // an i32 sum (src/shmap/mod.rs:689)
-            A.rd_S[rd] = S;
-            A.rd_hl[rd] = hl;
-            A.rd_ok[rd] = ok ? 1 : 0;
-            A.rd_c[rd] = (int32_t)c;
-            A.rd_bseeds[rd] = (int32_t)seeds;
-            A.rd_max_seed[rd] = (int32_t)mx;
-            A.rd_seed_matches[rd] = (int32_t)E;
-            A.rd_E[rd] = E;
-            uint64_t cap = 0;
-            if (E > 0) {
-                cap = 64;
-                while (cap < 2ull * E) cap <<= 1;
-            }
-            A.rd_cap[rd] = cap;
-        }
-        wave_sync();
-    }
-}
-
-__global__ void __launch_bounds__(WG) k_sort_seeds(MapArgs A) {
-    extern __shared__ uint8_t lds[];
-    uint64_t *sk = (uint64_t *)lds;
-    uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
-    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
-        const uint64_t o = A.koff[rd];
-        if (A.koff[rd + 1] - o <= WAVE_SEED_CAP) continue;  // handled by k_seed_wave
-        const uint32_t u = (uint32_t)A.rd_u[rd];
-        wg_sort_pairs(A.sd_key + o, A.sd_ord + o, u, sk, sv, SORT_CAP);
-    }
-}
-
-// Seed budget S and halflen (src/shmap/mod.rs:726-742), the consumed prefix of
-// match_seeds (src/shmap/seeding.rs:101-102: seeds are taken while fewer than S
-// occurrences have been consumed, a zero-hit seed included) and its counters
-// (:103-106, :207-209).
-__global__ void __launch_bounds__(WG) k_budget(MapArgs A) {
-    __shared__ uint32_t tmp[NWAVES + 1];
-    __shared__ uint32_t s_c, s_seeds, s_max, s_occ_carry, s_cnt_carry;
-    for (int32_t rd = blockIdx.x; rd < A.R; rd += gridDim.x) {
-        const uint64_t o = A.koff[rd];
-        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
-        if (m <= (int32_t)WAVE_SEED_CAP) continue;  // handled by k_seed_wave
-        const int32_t u = A.rd_u[rd];
-        const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
-        const double theta2 = A.theta - A.min_diff;
-        const int32_t S = A.one_sweep ? u : (int32_t)((1.0 - theta2) * (double)m) + 1;
-        const int32_t hl = A.abs_pos ? len : m;
-        const bool ok = hl >= MIN_HALFLEN;
-        if (threadIdx.x == 0) {
-            s_c = 0;
-            s_seeds = 0;
-            s_max = 0;
-            s_occ_carry = 0;
-            s_cnt_carry = 0;
-        }
-        __syncthreads();
         if (ok) {
-            for (int32_t base = 0; base < u; base += WG) {
+            for (int32_t base = 0; base < (int32_t)u; base += WG) {
                 const int32_t q = base + threadIdx.x;
                 uint32_t occs = 0, cnt = 0;
-                if (q < u) {
+                if (q < (int32_t)u) {
                     const uint32_t sid = A.sd_ord[o + q];
                     occs = A.sd_occs[o + sid];
                     cnt = A.sd_cnt[o + sid];
@@ -773,7 +755,7 @@ __global__ void __launch_bounds__(WG) k_budget(MapArgs A) {
                 uint32_t tot;
                 const uint32_t ex = wg_excl_scan(occs, tmp, &tot);
                 const uint32_t pre = s_occ_carry + ex;
-                const bool consumed = q < u && (int64_t)pre < (int64_t)S;
+                const bool consumed = q < (int32_t)u && (int64_t)pre < (int64_t)S;
                 uint32_t ctot;
                 const uint32_t cex = wg_excl_scan(consumed ? cnt : 0u, tmp, &ctot);
                 if (consumed) {
@@ -792,6 +774,8 @@ __global__ void __launch_bounds__(WG) k_budget(MapArgs A) {
         }
         if (threadIdx.x == 0) {
             const uint32_t E = s_cnt_carry;
+            A.rd_u[rd] = (int32_t)u;
+            A.rd_possible[rd] = (int32_t)s_pm;  // an i32 sum (src/shmap/mod.rs:689)
             A.rd_S[rd] = S;
             A.rd_hl[rd] = hl;
             A.rd_ok[rd] = ok ? 1 : 0;
@@ -1721,10 +1705,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     const uint32_t gg = read_grid(A.RG, max_grid);  // group per-read kernels
     switch (stage) {
     case ST_SEED_WAVE: k_seed_wave<<<read_grid((A.R + NWAVES - 1) / NWAVES, max_grid), WG, (size_t)NWAVES * WAVE_SEED_LDS, st>>>(A, I); break;
-    case ST_SORT_KMERS: k_sort_kmers<<<gb, WG, SORT_LDS, st>>>(A); break;
-    case ST_GROUP_SEEDS: k_group_seeds<<<gb, WG, 0, st>>>(A, I); break;
-    case ST_SORT_SEEDS: k_sort_seeds<<<gb, WG, SORT_LDS, st>>>(A); break;
-    case ST_BUDGET: k_budget<<<gb, WG, 0, st>>>(A); break;
+    case ST_SEED_WG: k_seed_wg<<<gb, WG, SORT_LDS, st>>>(A, I); break;
     case ST_TAB_INIT: k_tab_init<<<flat_grid(flat_n, 8192), WG, 0, st>>>(A); break;
     case ST_CONTRIB: k_contrib<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I); break;
     case ST_COUNT_BUCKETS: k_count_buckets<<<gg, WG, 0, st>>>(A); break;

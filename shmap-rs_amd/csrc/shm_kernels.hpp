@@ -19,7 +19,7 @@ constexpr int32_t REFINE_CAP = 4096;      // int32 entries per LDS array in refi
 constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
-constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 28;  // per-wave LDS of k_seed_wave
+constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
@@ -105,6 +105,8 @@ struct MapArgs {
     uint32_t task_cap;
     int32_t refine_pass;
     int32_t *rd_flag, *rd_first;
+    // reads too large for one wave's seeding slice, for k_seed_wg
+    uint32_t *big_list, *big_count;
     // per-workgroup scratch for spans/reads that overflow LDS
     int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;
@@ -122,10 +124,7 @@ enum Stage {
     ST_SKETCH = 0,
     ST_SKETCH_COMPACT,
     ST_SEED_WAVE,
-    ST_SORT_KMERS,
-    ST_GROUP_SEEDS,
-    ST_SORT_SEEDS,
-    ST_BUDGET,
+    ST_SEED_WG,
     ST_TAB_INIT,
     ST_CONTRIB,
     ST_COUNT_BUCKETS,

@@ -555,10 +555,9 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 // mapper
 // ---------------------------------------------------------------------------
 
-static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "sort_kmers", "group_seeds",
-                                            "sort_seeds",   "budget",       "tab_init",   "contrib",
-                                            "count_buckets", "extract",     "prune",      "refine",
-                                            "final"};
+static const char *kStageNames[ST_COUNT] = {"sketch",  "sketch_compact", "seed_wave", "seed_wg",
+                                            "tab_init", "contrib",        "count_buckets", "extract",
+                                            "prune",   "refine",         "final"};
 
 constexpr int SHM_N_COUNTS = 7;  // see shm_mapper_last_counts
 
@@ -599,6 +598,7 @@ struct shm_mapper {
     DevBuf<double> bs_sh, sc_j;
     DevBuf<int32_t> g_sid, g_tr, g_diff, g_tails;
     DevBuf<uint2> tasks, sel;
+    DevBuf<uint32_t> big;  // [0] count, then the reads seeded by a workgroup
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
 };
@@ -808,8 +808,12 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.RG = R;
 
     const bool any_large = mp->so.max_m > WAVE_SEED_CAP;  // a read whose sketch exceeds one wave's LDS slice
-    for (Stage s : {ST_SEED_WAVE, ST_SORT_KMERS, ST_GROUP_SEEDS, ST_SORT_SEEDS, ST_BUDGET}) {
-        if (s != ST_SEED_WAVE && !any_large) continue;
+    RCHK(mp->big.ensure((size_t)R + 2));
+    A.big_count = mp->big.p;
+    A.big_list = mp->big.p + 1;
+    HIPCHK(hipMemsetAsync(A.big_count, 0, 4, st));
+    for (Stage s : {ST_SEED_WAVE, ST_SEED_WG}) {
+        if (s == ST_SEED_WG && !any_large) continue;
         TIME_BEGIN(s);
         launch_stage(s, A, I, nullptr, 0, mp->max_grid, st);
         TIME_END(s);
