@@ -97,10 +97,9 @@ __global__ void __launch_bounds__(WG) k_chunk_seq(SeqSet S, const uint64_t *chun
 }
 
 // Codes of four packed bytes: ((c >> 1) ^ (c >> 2)) & 3 sends a/c/g/t in
-// either case to 0/1/2/3; a byte that is not one of those letters (whose ntHash
-// seed is 0, src/sketch.rs:73-88) gets bit 2 set, and table entries 4-7 are 0.
-// Validity is checked by rebuilding each letter from its code with a byte
-// permute and comparing.
+// either case to 0/1/2/3; any other byte (whose ntHash seed is 0,
+// src/sketch.rs:73-88) becomes 4.  Validity is checked by rebuilding each letter
+// from its code with a byte permute and comparing.
 __device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
     const uint32_t x = v | 0x20202020u;                               // lower case
     const uint32_t code = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
@@ -109,18 +108,21 @@ __device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
     bad |= bad >> 4;
     bad |= bad >> 2;
     bad |= bad >> 1;
-    return code | ((bad & 0x01010101u) << 2);
+    const uint32_t bb = bad & 0x01010101u;
+    return (code & ~(bb * 3u)) | (bb << 2);
 }
 
 // One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
 // are staged in LDS as one code byte per base (one dword of bank skew per 128
 // bytes, so lanes rolling runs 128 windows apart hit different banks).  The
-// ntHash seeds with the fixed rotations of the rolling update pre-applied
-// (src/sketch.rs:90-97) sit in two 8-entry LDS tables (codes 4-7 are the zero
-// seed of non-ACGT bytes): the incoming base's (fw, rotl(rc, k-1)) and the
-// outgoing base's (rotl(fw, k), rotr(rc, 1)), one 16-byte load each.  Each lane
-// rolls LANE_W consecutive windows; its first window is rolled in from a zero
-// hash, which equals the direct definition of src/sketch.rs:180-184.
+// rolling update (src/sketch.rs:90-97, 202-203) xors, per strand, the outgoing
+// base's rotated seed and the incoming base's; both come from one 25-entry LDS
+// table indexed by (outgoing code * 5 + incoming code), entry
+// (rotl(fw_out, k) ^ fw_in, rotr(rc_out, 1) ^ rotl(rc_in, k-1)), one 16-byte
+// load per window (code 4 is a non-ACGT byte, seed 0).  Each lane rolls
+// LANE_W consecutive windows; its first window is rolled in from a zero hash
+// through the outgoing-code-4 row, which equals the direct definition of
+// src/sketch.rs:180-184.
 //
 // PASS 0 visits every chunk and writes each lane's kept k-mers, in position
 // order, to lane-strided staging slots (slot i of lane l of chunk c at
@@ -135,16 +137,14 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                                                int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                                                uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
-    ulonglong2 *T_in = (ulonglong2 *)lds;  // code -> (fw, rotl(rc, k-1))
-    ulonglong2 *T_out = T_in + 8;          // code -> (rotl(fw, k), rotr(rc, 1))
+    ulonglong2 *T = (ulonglong2 *)lds;  // (out code * 5 + in code) -> xor terms of both strands
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     const uint32_t span = sketch_span(k);
     uint8_t *buf = lds + SKETCH_LUT_BYTES + (size_t)w * span;
-    if (threadIdx.x < 8) {
-        const uint8_t c = "ACGTNNNN"[threadIdx.x];  // code order of base_codes4
-        const uint64_t fw = nt_fw(c), rc = nt_rc(c);
-        T_in[threadIdx.x] = make_ulonglong2(fw, rotl64(rc, (uint32_t)(k - 1)));
-        T_out[threadIdx.x] = make_ulonglong2(rotl64(fw, (uint32_t)k), rotr64(rc, 1));
+    if (threadIdx.x < 25) {
+        const uint8_t co = "ACGTN"[threadIdx.x / 5], ci = "ACGTN"[threadIdx.x % 5];  // code order of base_codes4
+        T[threadIdx.x] = make_ulonglong2(rotl64(nt_fw(co), (uint32_t)k) ^ nt_fw(ci),
+                                         rotr64(nt_rc(co), 1) ^ rotl64(nt_rc(ci), (uint32_t)(k - 1)));
     }
     __syncthreads();
     const uint64_t total = PASS == 0 ? chunk_start[S.n] : (uint64_t)*ovf_count;
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             if (lw0 < lw1) {
                 uint64_t hf = 0, hr = 0;
                 for (int32_t i = 0; i < k; i++) {
-                    const ulonglong2 ti = T_in[SKB(lw0 + i)];
+                    const ulonglong2 ti = T[20 + SKB(lw0 + i)];  // nothing rolls out
                     hf = rotl1(hf) ^ ti.x;
                     hr = rotr1(hr) ^ ti.y;
                 }
@@ -231,37 +231,39 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                 const int64_t nsteps = lw1 - lw0 - 1;
                 const uint32_t ish = (uint32_t)(k & 3) * 8;
                 int64_t j = 0;
+                // a lane's outgoing bases stay inside its own 128-byte skew block
+                const uint8_t *obase = &SKB(lw0);
                 for (; j + 4 <= nsteps; j += 4) {
-                    const uint32_t pq = (uint32_t)(lw0 + j);
-                    const uint32_t od = *(const uint32_t *)&SKB(pq);
+                    const uint32_t od = *(const uint32_t *)(obase + j);
                     const uint32_t iq = (uint32_t)(lw0 + j + k) & ~3u;
                     const uint32_t i0 = *(const uint32_t *)&SKB(iq);
                     const uint32_t i1 = *(const uint32_t *)&SKB(iq + 4);
                     const uint32_t id = ish ? (uint32_t)((((uint64_t)i1 << 32) | i0) >> ish) : i0;
-                    uint64_t hs[4];
-                    uint32_t keep = 0, str = 0;
+                    const uint32_t pair = (od << 2) + od + id;  // per byte: out * 5 + in <= 24
+                    uint64_t hs[4], fs[4];
+                    uint32_t keep = 0;
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
-                        const ulonglong2 to = T_out[(od >> (8 * t)) & 0xff], ti = T_in[(id >> (8 * t)) & 0xff];
-                        hf = rotl1(hf) ^ to.x ^ ti.x;
-                        hr = rotr1(hr) ^ to.y ^ ti.y;
+                        const ulonglong2 tt = T[(pair >> (8 * t)) & 0xff];
+                        hf = rotl1(hf) ^ tt.x;
+                        hr = rotr1(hr) ^ tt.y;
+                        fs[t] = hf;
                         hs[t] = hr ^ hf;
                         keep |= (uint32_t)(hs[t] <= thres) << t;
-                        str |= (uint32_t)(hf > hr) << t;
                     }
                     if (keep) {
 #pragma unroll
                         for (int t = 0; t < 4; t++)
                             if (keep & (1u << t)) {
-                                put(lw0 + j + t + 1, hs[t], (str >> t) & 1u);
+                                put(lw0 + j + t + 1, hs[t], fs[t] > (hs[t] ^ fs[t]));
                                 cnt++;
                             }
                     }
                 }
                 for (; j < nsteps; j++) {
-                    const ulonglong2 to = T_out[SKB(lw0 + j)], ti = T_in[SKB(lw0 + j + k)];
-                    hf = rotl1(hf) ^ to.x ^ ti.x;
-                    hr = rotr1(hr) ^ to.y ^ ti.y;
+                    const ulonglong2 tt = T[SKB(lw0 + j) * 5u + SKB(lw0 + j + k)];
+                    hf = rotl1(hf) ^ tt.x;
+                    hr = rotr1(hr) ^ tt.y;
                     const uint64_t h = hr ^ hf;
                     if (h <= thres) {
                         put(lw0 + j + 1, h, hf > hr);
