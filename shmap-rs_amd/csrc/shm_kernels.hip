@@ -112,9 +112,16 @@ __device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
     return (code & ~(bb * 3u)) | (bb << 2);
 }
 
+// Four code bytes (each < 16) to four nibbles, first byte lowest.
+__device__ __forceinline__ uint32_t pack_nibbles4(uint32_t c) {
+    c = (c | (c >> 4)) & 0x00ff00ffu;
+    return (c | (c >> 8)) & 0x0000ffffu;
+}
+
 // One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
-// are staged in LDS as one code byte per base (one dword of bank skew per 128
-// bytes, so lanes rolling runs 128 windows apart hit different banks).  The
+// are staged in LDS as 4-bit codes, base p at nibble p & 7 of dword p >> 3, with
+// one dword of bank skew per 16 dwords (lanes rolling runs 128 windows = 16
+// dwords apart hit different banks).  The
 // rolling update (src/sketch.rs:90-97, 202-203) xors, per strand, the outgoing
 // base's rotated seed and the incoming base's; both come from one 25-entry LDS
 // table indexed by (outgoing code * 5 + incoming code), entry
@@ -148,7 +155,9 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
     }
     __syncthreads();
     const uint64_t total = PASS == 0 ? chunk_start[S.n] : (uint64_t)*ovf_count;
-#define SKB(q) buf[(uint32_t)(q) + (((uint32_t)(q) >> 7) << 2)]
+    uint32_t *b32 = (uint32_t *)buf;
+#define SKW(q) b32[(uint32_t)(q) + ((uint32_t)(q) >> 4)]
+#define SKN(p) ((SKW((uint32_t)(p) >> 3) >> (((uint32_t)(p) & 7u) * 4u)) & 0xfu)
     for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
         const bool valid = cb + w < total;
         const uint64_t c = !valid ? 0 : PASS == 0 ? cb + w : (uint64_t)ovf_list[cb + w];
@@ -165,23 +174,25 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             const uint32_t sh = (uint32_t)(src - a) * 8;
             const uintptr_t lim = (uintptr_t)(S.bases + S.off[S.n]);  // never read past the set
             const uint32_t *s32 = (const uint32_t *)a;
-            uint32_t *b32 = (uint32_t *)buf;
-            const int64_t ng = (nb + 3) >> 2;  // code dwords
-            if (a + 4 * (uintptr_t)(ng + 1) <= lim) {
+            const int64_t ng = (nb + 7) >> 3;  // code dwords, 8 bases each
+            if (a + 4 * (uintptr_t)(2 * ng + 1) <= lim) {
                 // every aligned dword this chunk needs lies inside the set
-#pragma unroll 4
+#pragma unroll 2
                 for (int64_t g = lane; g < ng; g += WAVE) {
-                    const uint32_t d0 = s32[g], d1 = s32[g + 1];
-                    const uint32_t v = sh ? (uint32_t)((((uint64_t)d1 << 32) | d0) >> sh) : d0;
-                    b32[g + (g >> 5)] = base_codes4(v);
+                    const uint32_t d0 = s32[2 * g], d1 = s32[2 * g + 1], d2 = s32[2 * g + 2];
+                    const uint32_t v0 = sh ? __builtin_amdgcn_alignbit(d1, d0, sh) : d0;
+                    const uint32_t v1 = sh ? __builtin_amdgcn_alignbit(d2, d1, sh) : d1;
+                    SKW(g) = pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
                 }
             } else {
                 for (int64_t g = lane; g < ng; g += WAVE) {
-                    const uintptr_t at = src + 4 * (uintptr_t)g;  // first byte of this group
-                    uint32_t v = 0;
-                    for (uint32_t j = 0; j < 4; j++)
-                        if (at + j < lim) v |= (uint32_t)(*(const uint8_t *)(at + j)) << (8 * j);
-                    b32[g + (g >> 5)] = base_codes4(v);
+                    const uintptr_t at = src + 8 * (uintptr_t)g;  // first byte of this group
+                    uint32_t v0 = 0, v1 = 0;
+                    for (uint32_t j = 0; j < 4; j++) {
+                        if (at + j < lim) v0 |= (uint32_t)(*(const uint8_t *)(at + j)) << (8 * j);
+                        if (at + 4 + j < lim) v1 |= (uint32_t)(*(const uint8_t *)(at + 4 + j)) << (8 * j);
+                    }
+                    SKW(g) = pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
                 }
             }
         }
@@ -199,7 +210,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
             if (lw0 < lw1) {
                 uint64_t hf = 0, hr = 0;
                 for (int32_t i = 0; i < k; i++) {
-                    const ulonglong2 ti = T[20 + SKB(lw0 + i)];  // nothing rolls out
+                    const ulonglong2 ti = T[20 + SKN(lw0 + i)];  // nothing rolls out
                     hf = rotl1(hf) ^ ti.x;
                     hr = rotr1(hr) ^ ti.y;
                 }
@@ -223,45 +234,50 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                     }
                 }
                 // windows lw0+1 .. lw1-1; the step into window lw0+j+1 rolls out
-                // base lw0+j and in base lw0+j+k.  Out positions are dword aligned
-                // (lw0 is a multiple of LANE_W), so codes are read four at a time;
-                // the in stream is re-aligned by a funnel shift of two dwords.
-                // Kept windows are rare, so the four steps only record a keep mask
-                // and the writes (pass 1) happen after them.
+                // base lw0+j and in base lw0+j+k.  Eight steps at a time: the
+                // outgoing codes are one dword (lw0 is a multiple of LANE_W, so a
+                // lane's outgoing dwords share one skew), the incoming ones a
+                // funnel shift of two; even and odd nibbles spread to bytes give
+                // the eight (out * 5 + in) table indices.  Kept windows are rare,
+                // so the steps only record a keep mask and the writes come after.
                 const int64_t nsteps = lw1 - lw0 - 1;
-                const uint32_t ish = (uint32_t)(k & 3) * 8;
+                const uint32_t ish = (uint32_t)(k & 7) * 4;
+                const uint32_t *obase = &SKW((uint32_t)lw0 >> 3);
                 int64_t j = 0;
-                // a lane's outgoing bases stay inside its own 128-byte skew block
-                const uint8_t *obase = &SKB(lw0);
-                for (; j + 4 <= nsteps; j += 4) {
-                    const uint32_t od = *(const uint32_t *)(obase + j);
-                    const uint32_t iq = (uint32_t)(lw0 + j + k) & ~3u;
-                    const uint32_t i0 = *(const uint32_t *)&SKB(iq);
-                    const uint32_t i1 = *(const uint32_t *)&SKB(iq + 4);
-                    const uint32_t id = ish ? (uint32_t)((((uint64_t)i1 << 32) | i0) >> ish) : i0;
-                    const uint32_t pair = (od << 2) + od + id;  // per byte: out * 5 + in <= 24
-                    uint64_t hs[4], fs[4];
-                    uint32_t keep = 0;
+                for (; j + 8 <= nsteps; j += 8) {
+                    const uint32_t od = obase[j >> 3];
+                    const uint32_t iq = (uint32_t)(lw0 + j + k) >> 3;
+                    const uint32_t i0 = SKW(iq), i1 = SKW(iq + 1);
+                    const uint32_t id = ish ? __builtin_amdgcn_alignbit(i1, i0, ish) : i0;
+                    const uint32_t oe = od & 0x0f0f0f0fu, oo = (od >> 4) & 0x0f0f0f0fu;
+                    const uint32_t pe = (oe << 2) + oe + (id & 0x0f0f0f0fu);          // windows 0, 2, 4, 6
+                    const uint32_t po = (oo << 2) + oo + ((id >> 4) & 0x0f0f0f0fu);   // windows 1, 3, 5, 7
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const ulonglong2 tt = T[(pair >> (8 * t)) & 0xff];
-                        hf = rotl1(hf) ^ tt.x;
-                        hr = rotr1(hr) ^ tt.y;
-                        fs[t] = hf;
-                        hs[t] = hr ^ hf;
-                        keep |= (uint32_t)(hs[t] <= thres) << t;
-                    }
-                    if (keep) {
+                    for (int half = 0; half < 2; half++) {
+                        uint64_t hs[4], fs[4];
+                        uint32_t keep = 0;
 #pragma unroll
-                        for (int t = 0; t < 4; t++)
-                            if (keep & (1u << t)) {
-                                put(lw0 + j + t + 1, hs[t], fs[t] > (hs[t] ^ fs[t]));
-                                cnt++;
-                            }
+                        for (int t4 = 0; t4 < 4; t4++) {
+                            const int t = half * 4 + t4;
+                            const ulonglong2 tt = T[(((t & 1) ? po : pe) >> (8 * (t >> 1))) & 0xff];
+                            hf = rotl1(hf) ^ tt.x;
+                            hr = rotr1(hr) ^ tt.y;
+                            fs[t4] = hf;
+                            hs[t4] = hr ^ hf;
+                            keep |= (uint32_t)(hs[t4] <= thres) << t4;
+                        }
+                        if (keep) {
+#pragma unroll
+                            for (int t4 = 0; t4 < 4; t4++)
+                                if (keep & (1u << t4)) {
+                                    put(lw0 + j + half * 4 + t4 + 1, hs[t4], fs[t4] > (hs[t4] ^ fs[t4]));
+                                    cnt++;
+                                }
+                        }
                     }
                 }
                 for (; j < nsteps; j++) {
-                    const ulonglong2 tt = T[SKB(lw0 + j) * 5u + SKB(lw0 + j + k)];
+                    const ulonglong2 tt = T[SKN(lw0 + j) * 5u + SKN(lw0 + j + k)];
                     hf = rotl1(hf) ^ tt.x;
                     hr = rotr1(hr) ^ tt.y;
                     const uint64_t h = hr ^ hf;
@@ -281,7 +297,8 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
         }
         __syncthreads();
     }
-#undef SKB
+#undef SKW
+#undef SKN
 }
 
 // Moves the staged k-mers of every chunk without a lane overflow to their

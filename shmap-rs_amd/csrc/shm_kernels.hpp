@@ -23,12 +23,12 @@ constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_see
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
-// LDS bytes staged per wave: the chunk's CHUNK_W + k - 1 bases plus up to 3
-// bytes of dword-alignment slack on each side, plus one dword of bank skew per
-// 128 bytes, rounded up to 16.
+// LDS bytes staged per wave: the chunk's CHUNK_W + k - 1 bases as 4-bit codes,
+// eight per dword, with two dwords of read-ahead, plus one dword of bank skew
+// per 16 dwords, rounded up to 16.
 __host__ __device__ inline uint32_t sketch_span(int32_t k) {
-    const uint32_t raw = (uint32_t)(CHUNK_W + k + 8);
-    return (raw + ((raw >> 7) + 1) * 4 + 15u) & ~15u;
+    const uint32_t dw = (uint32_t)(CHUNK_W + k) / 8u + 2u;
+    return (4u * (dw + dw / 16u + 1u) + 15u) & ~15u;
 }
 
 // A set of sequences: sequence i is bases[off[i], off[i+1]); bytes up to
