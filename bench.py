@@ -47,7 +47,7 @@ PROBE_COPY_BYTES = 1 << 30  # calibration copy: reads and writes exactly this ma
 # main kernel of each timed stage (for the counter passes)
 STAGE_KERNEL = {"sketch": "k_sketch<0>(", "sketch_compact": "k_sketch_compact(", "seed_wave": "k_seed_wave(",
                 "seed_wg": "k_seed_wg(",
-                "contrib": "k_contrib(", "count_buckets": "k_count_buckets(", "extract": "k_extract(",
+                "contrib": "k_contrib(", "extract": "k_extract(",
                 "prune": "k_prune(", "refine": "k_refine(", "final": "k_final("}
 
 

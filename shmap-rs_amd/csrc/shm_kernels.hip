@@ -11,6 +11,9 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <vector>
+
 #include "shm_device.hpp"
 #include "shm_kernels.hpp"
 
@@ -138,7 +141,7 @@ __device__ __forceinline__ uint32_t pack_nibbles4(uint32_t c) {
 // lcap.  PASS 1 re-rolls only the listed chunks and writes straight to the final
 // positions: the chunk's scanned offset plus the lane's exclusive prefix.
 template <int PASS>
-__global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thres,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_sketch(SeqSet S, int32_t k, uint64_t thres,
                                                const uint64_t *chunk_start, const uint32_t *chunk_seq,
                                                uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
                                                int32_t *out_r, uint8_t *out_s, uint32_t lcap,
@@ -158,11 +161,12 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
     uint32_t *b32 = (uint32_t *)buf;
 #define SKW(q) b32[(uint32_t)(q) + ((uint32_t)(q) >> 4)]
 #define SKN(p) ((SKW((uint32_t)(p) >> 3) >> (((uint32_t)(p) & 7u) * 4u)) & 0xfu)
-    for (uint64_t cb = (uint64_t)blockIdx.x * NWAVES; cb < total; cb += (uint64_t)gridDim.x * NWAVES) {
-        const bool valid = cb + w < total;
-        const uint64_t c = !valid ? 0 : PASS == 0 ? cb + w : (uint64_t)ovf_list[cb + w];
+    // Each wave stages and reads only its own slice, so the waves of a
+    // workgroup run independently (wave-level ordering only).
+    for (uint64_t ci = (uint64_t)blockIdx.x * NWAVES + w; ci < total; ci += (uint64_t)gridDim.x * NWAVES) {
+        const uint64_t c = PASS == 0 ? ci : (uint64_t)ovf_list[ci];
         int64_t w0 = 0, w1 = 0;
-        if (valid) {
+        {
             const uint32_t seq = chunk_seq[c];
             const int64_t len = S.off[seq + 1] - S.off[seq];
             const int64_t nw = len - k + 1;
@@ -196,8 +200,8 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                 }
             }
         }
-        __syncthreads();
-        if (valid) {
+        wave_sync();
+        {
             const int64_t nwin = w1 - w0;
             const int64_t lw0 = (int64_t)lane * LANE_W;
             const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
@@ -250,8 +254,11 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                     const uint32_t i0 = SKW(iq), i1 = SKW(iq + 1);
                     const uint32_t id = ish ? __builtin_amdgcn_alignbit(i1, i0, ish) : i0;
                     const uint32_t oe = od & 0x0f0f0f0fu, oo = (od >> 4) & 0x0f0f0f0fu;
-                    const uint32_t pe = (oe << 2) + oe + (id & 0x0f0f0f0fu);          // windows 0, 2, 4, 6
-                    const uint32_t po = (oo << 2) + oo + ((id >> 4) & 0x0f0f0f0fu);   // windows 1, 3, 5, 7
+                    uint32_t oe4 = oe << 2, oo4 = oo << 2;
+                    // keep the per-byte *5 as a 32-bit shift-add (not a 64-bit multiply-add)
+                    asm volatile("" : "+v"(oe4), "+v"(oo4));
+                    const uint32_t pe = oe4 + oe + (id & 0x0f0f0f0fu);          // windows 0, 2, 4, 6
+                    const uint32_t po = oo4 + oo + ((id >> 4) & 0x0f0f0f0fu);   // windows 1, 3, 5, 7
 #pragma unroll
                     for (int half = 0; half < 2; half++) {
                         uint64_t hs[4], fs[4];
@@ -295,7 +302,7 @@ __global__ void __launch_bounds__(WG) k_sketch(SeqSet S, int32_t k, uint64_t thr
                 if (__ballot(cnt > lcap) != 0 && lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)c;
             }
         }
-        __syncthreads();
+        wave_sync();  // every lane is done with the slice before it is restaged
     }
 #undef SKW
 #undef SKN
@@ -815,6 +822,8 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
 }
 
 __global__ void __launch_bounds__(WG) k_tab_init(MapArgs A) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < (uint64_t)A.RG; i += (uint64_t)gridDim.x * WG)
+        A.grp_nb[i] = 0;
     const uint64_t n = A.tab_base[A.RG];
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
         A.tb_key[i] = EMPTY_KEY;
@@ -862,14 +871,20 @@ __device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t
     return lo;
 }
 
-__device__ __forceinline__ void tab_add(const MapArgs &A, uint64_t base, uint64_t cap, uint32_t g,
+// Adds one (seed, bucket) contribution to read lr's bucket table; the insert
+// that claims an empty slot counts the read's new bucket.
+__device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t base, uint64_t cap, uint32_t g,
                                         uint32_t matches, int32_t codir, int32_t rmin, int32_t rmax) {
     const uint32_t lg = 63 - __clzll(cap);
     const uint32_t mask = (uint32_t)(cap - 1);
     uint32_t pos = ((g * 0x9E3779B1u) >> (32 - lg)) & mask;
     for (;;) {
         const uint32_t old = atomicCAS(&A.tb_key[base + pos], EMPTY_KEY, g);
-        if (old == EMPTY_KEY || old == g) break;
+        if (old == EMPTY_KEY) {
+            atomicAdd((unsigned long long *)&A.grp_nb[lr], 1ull);
+            break;
+        }
+        if (old == g) break;
         pos = (pos + 1u) & mask;
     }
     atomicAdd((unsigned long long *)&A.tb_mc[base + pos],
@@ -925,29 +940,9 @@ __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
             const uint32_t n = e - idx;
             const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx];
             const uint32_t same = sstrand ? ones : n - ones;
-            tab_add(A, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
+            tab_add(A, lr, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
                     (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
         }
-    }
-}
-
-__global__ void __launch_bounds__(WG) k_count_buckets(MapArgs A) {
-    __shared__ uint32_t s_n;
-    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
-        const int32_t rd = A.r0 + lr;
-        const uint64_t base = A.tab_base[lr], cap = A.rd_cap[rd];
-        if (threadIdx.x == 0) s_n = 0;
-        __syncthreads();
-        uint32_t n = 0;
-        for (uint64_t i = threadIdx.x; i < cap; i += WG) n += A.tb_key[base + i] != EMPTY_KEY;
-        for (int d = WAVE / 2; d > 0; d >>= 1) n += __shfl_xor(n, d);
-        if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_n, n);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            A.rd_nb[rd] = (int32_t)s_n;
-            A.grp_nb[lr] = s_n;
-        }
-        __syncthreads();
     }
 }
 
@@ -1022,7 +1017,7 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
     for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
         const int32_t rd = A.r0 + lr;
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
-        const int32_t nb = A.rd_nb[rd];
+        const int32_t nb = (int32_t)A.grp_nb[lr];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
         const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
         const int32_t seeds0 = A.rd_bseeds[rd];
@@ -1511,7 +1506,7 @@ __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
         r.seed_matches = A.rd_seed_matches[rd];
         r.total_matches = A.rd_seed_matches[rd];
         r.max_seed_matches = A.rd_max_seed[rd];
-        const int32_t nb = A.rd_nb[rd];
+        const int32_t nb = (int32_t)A.grp_nb[lr];
         const int32_t hl = A.rd_hl[rd];
         r.seeded_buckets = nb;
         const bool nbp = A.no_bucket_pruning;
@@ -1597,6 +1592,41 @@ __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
 // ============================================================================
 namespace shm {
 
+// Workgroups of kernel fn (with `lds` bytes of dynamic LDS) that are resident
+// on the current device at once.  A persistent grid-stride loop launched with
+// more would leave the extra workgroups waiting for a second round.
+static uint32_t resident_wgs(const void *fn, size_t lds) {
+    struct Entry {
+        const void *fn;
+        size_t lds;
+        int dev;
+        uint32_t n;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry &e : cache)
+        if (e.fn == fn && e.lds == lds && e.dev == dev) return e.n;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    const uint32_t n = (uint32_t)(per_cu * cus);
+    cache.push_back(Entry{fn, lds, dev, n});
+    return n;
+}
+
+// Grid of a persistent kernel: enough workgroups for `want` units of work, at
+// most the resident count and max_grid.
+static uint32_t persistent_grid(const void *fn, size_t lds, uint64_t want, uint32_t max_grid) {
+    uint64_t g = want < 1 ? 1 : want;
+    const uint32_t r = resident_wgs(fn, lds);
+    if (g > r) g = r;
+    if (g > max_grid) g = max_grid;
+    return (uint32_t)g;
+}
+
 static inline uint32_t flat_grid(uint64_t n, uint32_t cap) {
     uint64_t g = (n + WG - 1) / WG;
     if (g < 1) g = 1;
@@ -1634,27 +1664,25 @@ void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t
                    const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
                    int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list, uint32_t *ovf_count,
                    uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
-    uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
-    if (g < 1) g = 1;
-    if (g > max_grid) g = max_grid;
+    const size_t lds = sketch_lds(k);
+    const uint64_t want = (n_chunks + NWAVES - 1) / NWAVES;
     if (pass == 0)
-        k_sketch<0><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
-                                                             lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
-                                                             ovf_count);
+        k_sketch<0><<<persistent_grid((const void *)k_sketch<0>, lds, want, max_grid), WG, lds, st>>>(
+            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+            ovf_count);
     else
-        k_sketch<1><<<(uint32_t)g, WG, sketch_lds(k), st>>>(S, k, thres, chunk_start, chunk_seq, chunk_val,
-                                                             lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
-                                                             ovf_count);
+        k_sketch<1><<<persistent_grid((const void *)k_sketch<1>, lds, want, max_grid), WG, lds, st>>>(
+            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+            ovf_count);
 }
 
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint64_t *st_h, const int32_t *st_r, const uint8_t *st_s,
                            uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t max_grid, hipStream_t st) {
-    uint64_t g = (n_chunks + NWAVES - 1) / NWAVES;
-    if (g < 1) g = 1;
-    if (g > max_grid) g = max_grid;
-    k_sketch_compact<<<(uint32_t)g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, st_h, st_r, st_s, out_h,
-                                                 out_r, out_s);
+    const uint32_t g =
+        persistent_grid((const void *)k_sketch_compact, 0, (n_chunks + NWAVES - 1) / NWAVES, max_grid);
+    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, st_h, st_r, st_s, out_h, out_r,
+                                       out_s);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
@@ -1716,20 +1744,23 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
 static const size_t REFINE_LDS = ((size_t)REFINE_CAP * 3 + 2) * 4;
+static const size_t SEED_WAVE_LDS = (size_t)NWAVES * WAVE_SEED_LDS;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
 void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out,
                   uint64_t flat_n, uint32_t max_grid, hipStream_t st) {
-    const uint32_t gb = read_grid(A.R, max_grid);   // batch-wide per-read kernels
-    const uint32_t gg = read_grid(A.RG, max_grid);  // group per-read kernels
+#define PGRID(fn, lds, want) persistent_grid((const void *)(fn), (lds), (uint64_t)(want), max_grid)
     switch (stage) {
-    case ST_SEED_WAVE: k_seed_wave<<<read_grid((A.R + NWAVES - 1) / NWAVES, max_grid), WG, (size_t)NWAVES * WAVE_SEED_LDS, st>>>(A, I); break;
-    case ST_SEED_WG: k_seed_wg<<<gb, WG, SORT_LDS, st>>>(A, I); break;
+    case ST_SEED_WAVE:
+        k_seed_wave<<<PGRID(k_seed_wave, SEED_WAVE_LDS, (A.R + NWAVES - 1) / NWAVES), WG, SEED_WAVE_LDS, st>>>(A, I);
+        break;
+    case ST_SEED_WG: k_seed_wg<<<read_grid(A.R, max_grid), WG, SORT_LDS, st>>>(A, I); break;
     case ST_TAB_INIT: k_tab_init<<<flat_grid(flat_n, 8192), WG, 0, st>>>(A); break;
     case ST_CONTRIB: k_contrib<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I); break;
-    case ST_COUNT_BUCKETS: k_count_buckets<<<gg, WG, 0, st>>>(A); break;
-    case ST_EXTRACT: k_extract<<<gg, WG, SORT_LDS, st>>>(A); break;
-    case ST_PRUNE: k_prune<<<gg, WG, 0, st>>>(A, I); break;
+    // per-read workgroup kernels with data-dependent cost: a grid above the
+    // resident count lets the hardware backfill as reads finish
+    case ST_EXTRACT: k_extract<<<read_grid(A.RG, max_grid), WG, SORT_LDS, st>>>(A); break;
+    case ST_PRUNE: k_prune<<<read_grid(A.RG, max_grid), WG, 0, st>>>(A, I); break;
     case ST_REFINE: {
         // three rounds: first candidates, other first-pass candidates, second pass
         MapArgs P = A;
@@ -1743,6 +1774,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;
     default: break;
     }
+#undef PGRID
 }
 
 __global__ void __launch_bounds__(WG) k_lookup(DevIndex I, const uint64_t *hs, uint32_t n, uint32_t *cnt,

@@ -71,11 +71,11 @@ struct MapArgs {
     uint64_t *sd_key;
     // per read (whole batch)
     int32_t *rd_u, *rd_possible, *rd_S, *rd_hl, *rd_ok, *rd_c, *rd_bseeds, *rd_max_seed,
-        *rd_seed_matches, *rd_nb;
+        *rd_seed_matches;
     uint64_t *rd_E, *rd_cap;
     // per group read (local index), RG+1 entries
     const uint64_t *hit_base, *tab_base, *bk_base;
-    uint64_t *grp_nb;  // each group read's bucket count, scanned into bk_base
+    uint64_t *grp_nb;  // each group read's bucket count (counted by k_contrib's inserts), scanned into bk_base
     // per-read bucket tables (group)
     uint32_t *tb_key;
     uint64_t *tb_mc;
@@ -127,7 +127,6 @@ enum Stage {
     ST_SEED_WG,
     ST_TAB_INIT,
     ST_CONTRIB,
-    ST_COUNT_BUCKETS,
     ST_EXTRACT,
     ST_PRUNE,
     ST_REFINE,

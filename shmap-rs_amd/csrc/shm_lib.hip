@@ -555,9 +555,8 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 // mapper
 // ---------------------------------------------------------------------------
 
-static const char *kStageNames[ST_COUNT] = {"sketch",  "sketch_compact", "seed_wave", "seed_wg",
-                                            "tab_init", "contrib",        "count_buckets", "extract",
-                                            "prune",   "refine",         "final"};
+static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "seed_wg", "tab_init",
+                                            "contrib", "extract",        "prune",     "refine",  "final"};
 
 constexpr int SHM_N_COUNTS = 7;  // see shm_mapper_last_counts
 
@@ -799,7 +798,6 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_bseeds = ri + 6 * (size_t)R;
     A.rd_max_seed = ri + 7 * (size_t)R;
     A.rd_seed_matches = ri + 8 * (size_t)R;
-    A.rd_nb = ri + 9 * (size_t)R;
     A.rd_flag = ri + 10 * (size_t)R;
     A.rd_first = ri + 11 * (size_t)R;
     A.rd_E = mp->rd_u64.p;
@@ -897,9 +895,6 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         TIME_BEGIN(ST_CONTRIB);
         launch_stage(ST_CONTRIB, A, I, nullptr, E_grp, mp->max_grid, st);
         TIME_END(ST_CONTRIB);
-        TIME_BEGIN(ST_COUNT_BUCKETS);
-        launch_stage(ST_COUNT_BUCKETS, A, I, nullptr, 0, mp->max_grid, st);
-        TIME_END(ST_COUNT_BUCKETS);
         launch_scan(mp->grp_nb.p, mp->grp_base.p + 2 * (size_t)(RG + 1), nullptr, RG, mp->part.p, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(mp->h_sum.p + 3, mp->grp_base.p + 2 * (size_t)(RG + 1) + RG, 8, hipMemcpyDeviceToHost,
