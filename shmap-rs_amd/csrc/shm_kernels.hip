@@ -101,18 +101,16 @@ __global__ void __launch_bounds__(WG) k_chunk_seq(SeqSet S, const uint64_t *chun
 
 // Codes of four packed bytes: ((c >> 1) ^ (c >> 2)) & 3 sends a/c/g/t in
 // either case to 0/1/2/3; any other byte (whose ntHash seed is 0,
-// src/sketch.rs:73-88) becomes 4.  Validity is checked by rebuilding each letter
-// from its code with a byte permute and comparing.
+// src/sketch.rs:73-88) gets bit 2 set (codes 4-7).  Validity is checked by
+// rebuilding each letter from its code with a byte permute and testing the
+// difference for a nonzero byte.
 __device__ __forceinline__ uint32_t base_codes4(uint32_t v) {
     const uint32_t x = v | 0x20202020u;                               // lower case
     const uint32_t code = ((x >> 1) ^ (x >> 2)) & 0x03030303u;
     const uint32_t back = __builtin_amdgcn_perm(0x74676361u, 0x74676361u, code); // "acgt"[code]
-    uint32_t bad = x ^ back;                                          // nonzero byte = not ACGT
-    bad |= bad >> 4;
-    bad |= bad >> 2;
-    bad |= bad >> 1;
-    const uint32_t bb = bad & 0x01010101u;
-    return (code & ~(bb * 3u)) | (bb << 2);
+    const uint32_t d = x ^ back;                                      // nonzero byte = not ACGT
+    const uint32_t nz = (((d & 0x7f7f7f7fu) + 0x7f7f7f7fu) | d) & 0x80808080u;
+    return code | (nz >> 5);
 }
 
 // Four code bytes (each < 16) to four nibbles, first byte lowest.
@@ -126,10 +124,10 @@ __device__ __forceinline__ uint32_t pack_nibbles4(uint32_t c) {
 // one dword of bank skew per 16 dwords (lanes rolling runs 128 windows = 16
 // dwords apart hit different banks).  The
 // rolling update (src/sketch.rs:90-97, 202-203) xors, per strand, the outgoing
-// base's rotated seed and the incoming base's; both come from one 25-entry LDS
-// table indexed by (outgoing code * 5 + incoming code), entry
+// base's rotated seed and the incoming base's; both come from one 64-entry LDS
+// table indexed by (outgoing code << 3 | incoming code), entry
 // (rotl(fw_out, k) ^ fw_in, rotr(rc_out, 1) ^ rotl(rc_in, k-1)), one 16-byte
-// load per window (code 4 is a non-ACGT byte, seed 0).  Each lane rolls
+// load per window (codes 4-7 are non-ACGT bytes, seed 0).  Each lane rolls
 // LANE_W consecutive windows; its first window is rolled in from a zero hash
 // through the outgoing-code-4 row, which equals the direct definition of
 // src/sketch.rs:180-184.
@@ -147,12 +145,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                                int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                                                uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
-    ulonglong2 *T = (ulonglong2 *)lds;  // (out code * 5 + in code) -> xor terms of both strands
+    ulonglong2 *T = (ulonglong2 *)lds;  // (out code << 3 | in code) -> xor terms of both strands
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     const uint32_t span = sketch_span(k);
     uint8_t *buf = lds + SKETCH_LUT_BYTES + (size_t)w * span;
-    if (threadIdx.x < 25) {
-        const uint8_t co = "ACGTN"[threadIdx.x / 5], ci = "ACGTN"[threadIdx.x % 5];  // code order of base_codes4
+    if (threadIdx.x < 64) {
+        const uint8_t co = "ACGTNNNN"[threadIdx.x >> 3], ci = "ACGTNNNN"[threadIdx.x & 7];  // code order of base_codes4
         T[threadIdx.x] = make_ulonglong2(rotl64(nt_fw(co), (uint32_t)k) ^ nt_fw(ci),
                                          rotr64(nt_rc(co), 1) ^ rotl64(nt_rc(ci), (uint32_t)(k - 1)));
     }
@@ -214,7 +212,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
             if (lw0 < lw1) {
                 uint64_t hf = 0, hr = 0;
                 for (int32_t i = 0; i < k; i++) {
-                    const ulonglong2 ti = T[20 + SKN(lw0 + i)];  // nothing rolls out
+                    const ulonglong2 ti = T[32 + SKN(lw0 + i)];  // nothing rolls out
                     hf = rotl1(hf) ^ ti.x;
                     hr = rotr1(hr) ^ ti.y;
                 }
@@ -242,7 +240,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 // outgoing codes are one dword (lw0 is a multiple of LANE_W, so a
                 // lane's outgoing dwords share one skew), the incoming ones a
                 // funnel shift of two; even and odd nibbles spread to bytes give
-                // the eight (out * 5 + in) table indices.  Kept windows are rare,
+                // the eight (out << 3 | in) table indices.  Kept windows are rare,
                 // so the steps only record a keep mask and the writes come after.
                 const int64_t nsteps = lw1 - lw0 - 1;
                 const uint32_t ish = (uint32_t)(k & 7) * 4;
@@ -253,12 +251,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     const uint32_t iq = (uint32_t)(lw0 + j + k) >> 3;
                     const uint32_t i0 = SKW(iq), i1 = SKW(iq + 1);
                     const uint32_t id = ish ? __builtin_amdgcn_alignbit(i1, i0, ish) : i0;
-                    const uint32_t oe = od & 0x0f0f0f0fu, oo = (od >> 4) & 0x0f0f0f0fu;
-                    uint32_t oe4 = oe << 2, oo4 = oo << 2;
-                    // keep the per-byte *5 as a 32-bit shift-add (not a 64-bit multiply-add)
-                    asm volatile("" : "+v"(oe4), "+v"(oo4));
-                    const uint32_t pe = oe4 + oe + (id & 0x0f0f0f0fu);          // windows 0, 2, 4, 6
-                    const uint32_t po = oo4 + oo + ((id >> 4) & 0x0f0f0f0fu);   // windows 1, 3, 5, 7
+                    const uint32_t pe = ((od & 0x0f0f0f0fu) << 3) | (id & 0x0f0f0f0fu);            // windows 0, 2, 4, 6
+                    const uint32_t po = (((od >> 4) & 0x0f0f0f0fu) << 3) | ((id >> 4) & 0x0f0f0f0fu); // windows 1, 3, 5, 7
 #pragma unroll
                     for (int half = 0; half < 2; half++) {
                         uint64_t hs[4], fs[4];
@@ -284,7 +278,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     }
                 }
                 for (; j < nsteps; j++) {
-                    const ulonglong2 tt = T[SKN(lw0 + j) * 5u + SKN(lw0 + j + k)];
+                    const ulonglong2 tt = T[(SKN(lw0 + j) << 3) | SKN(lw0 + j + k)];
                     hf = rotl1(hf) ^ tt.x;
                     hr = rotr1(hr) ^ tt.y;
                     const uint64_t h = hr ^ hf;
@@ -947,27 +941,90 @@ __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
 }
 
 // propagate_seeds_to_buckets + get_sorted_buckets (src/buckets.rs:603-640,
-// 685-724): a read's buckets in bucket-id order, then ordered by (matches
-// descending, bucket-id rank ascending).
+// 685-724): a read's buckets ordered by (matches descending, bucket id
+// ascending).  Bucket ids are unique within a read, so one sort on the key
+// ((~matches) << 32 | id) gives that order directly.
+__device__ __forceinline__ uint32_t mc_matches(uint64_t mc, int32_t &codir) {
+    codir = (int32_t)(uint32_t)mc;  // the low word is the (signed) codirection sum
+    return (uint32_t)((mc - (uint64_t)(int64_t)codir) >> 32);
+}
+
+__device__ __forceinline__ void emit_bucket(const MapArgs &A, uint64_t base, uint32_t p, uint64_t key,
+                                            uint64_t tb, uint32_t slot) {
+    int32_t codir;
+    (void)mc_matches(A.tb_mc[tb + slot], codir);
+    A.bs_g[base + p] = (uint32_t)key;
+    A.bs_m[base + p] = (int32_t)(0xffffffffu - (uint32_t)(key >> 32));
+    A.bs_c[base + p] = codir;
+    A.bs_min[base + p] = A.tb_min[tb + slot];
+    A.bs_max[base + p] = A.tb_max[tb + slot];
+}
+
+// One wave per read whose table has at most EXTRACT_WAVE_CAP slots: the used
+// slots are compacted by ballot into the wave's LDS slice and sorted there.
+// Larger tables are listed for k_extract.
+__global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
+    extern __shared__ uint8_t lds[];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    uint64_t *kk = (uint64_t *)(lds + (size_t)w * EXTRACT_WAVE_LDS);
+    uint32_t *kv = (uint32_t *)(kk + EXTRACT_WAVE_CAP);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
+        const int32_t rd = A.r0 + lr;
+        const uint64_t cap = A.rd_cap[rd];
+        if (cap > EXTRACT_WAVE_CAP) {
+            if (lane == 0) A.ext_list[atomicAdd(A.ext_count, 1u)] = (uint32_t)lr;
+            continue;
+        }
+        const uint64_t tb = A.tab_base[lr], base = A.bk_base[lr];
+        uint32_t n = 0;
+        for (uint32_t b0 = 0; b0 < (uint32_t)cap; b0 += WAVE) {
+            const uint32_t i = b0 + lane;
+            const uint32_t g = A.tb_key[tb + i];  // cap is a multiple of WAVE
+            const bool used = g != EMPTY_KEY;
+            const uint64_t bal = __ballot(used);
+            if (used) {
+                int32_t codir;
+                const uint32_t matches = mc_matches(A.tb_mc[tb + i], codir);
+                const uint32_t at = n + (uint32_t)__popcll(bal & lt);
+                kk[at] = ((uint64_t)(0xffffffffu - matches) << 32) | g;
+                kv[at] = i;
+            }
+            n += (uint32_t)__popcll(bal);
+        }
+        wave_sync();
+        wave_bitonic(kk, kv, n);
+        for (uint32_t p = lane; p < n; p += WAVE) emit_bucket(A, base, p, kk[p], tb, kv[p]);
+        wave_sync();
+    }
+}
+
+// The same for the listed reads with larger tables, one workgroup each, through
+// global memory with the sort staged in LDS when it fits.
 __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
     __shared__ uint32_t tmp[NWAVES + 1];
     __shared__ uint32_t s_n;
-    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
+    const uint32_t nlist = *A.ext_count;
+    for (uint32_t t = blockIdx.x; t < nlist; t += gridDim.x) {
+        const int32_t lr = (int32_t)A.ext_list[t];
         const int32_t rd = A.r0 + lr;
         const uint64_t tb = A.tab_base[lr], cap = A.rd_cap[rd], base = A.bk_base[lr];
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         for (uint64_t b0 = 0; b0 < cap; b0 += WG) {
             const uint64_t i = b0 + threadIdx.x;
-            const bool used = i < cap && A.tb_key[tb + i] != EMPTY_KEY;
+            const uint32_t g = i < cap ? A.tb_key[tb + i] : EMPTY_KEY;
+            const bool used = g != EMPTY_KEY;
             uint32_t tot;
             const uint32_t ex = wg_excl_scan(used ? 1u : 0u, tmp, &tot);
             const uint32_t n0 = s_n;
             if (used) {
-                A.bk_key[base + n0 + ex] = A.tb_key[tb + i];
+                int32_t codir;
+                const uint32_t matches = mc_matches(A.tb_mc[tb + i], codir);
+                A.bk_key[base + n0 + ex] = ((uint64_t)(0xffffffffu - matches) << 32) | g;
                 A.bk_val[base + n0 + ex] = (uint32_t)i;
             }
             __syncthreads();
@@ -976,32 +1033,7 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
         }
         const uint32_t nb = s_n;
         wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
-        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
-            const uint32_t slot = A.bk_val[base + i];
-            const uint64_t mc = A.tb_mc[tb + slot];
-            const int32_t codir = (int32_t)(uint32_t)mc;
-            const uint32_t matches = (uint32_t)((mc - (uint64_t)(int64_t)codir) >> 32);
-            A.tmp_g[base + i] = (uint32_t)A.bk_key[base + i];
-            A.tmp_m[base + i] = (int32_t)matches;
-            A.tmp_c[base + i] = codir;
-            A.tmp_min[base + i] = A.tb_min[tb + slot];
-            A.tmp_max[base + i] = A.tb_max[tb + slot];
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
-            A.bk_key[base + i] = ((uint64_t)(0xffffffffu - (uint32_t)A.tmp_m[base + i]) << 32) | i;
-            A.bk_val[base + i] = i;
-        }
-        __syncthreads();
-        wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
-        for (uint32_t p = threadIdx.x; p < nb; p += WG) {
-            const uint32_t i = A.bk_val[base + p];
-            A.bs_g[base + p] = A.tmp_g[base + i];
-            A.bs_m[base + p] = A.tmp_m[base + i];
-            A.bs_c[base + p] = A.tmp_c[base + i];
-            A.bs_min[base + p] = A.tmp_min[base + i];
-            A.bs_max[base + p] = A.tmp_max[base + i];
-        }
+        for (uint32_t p = threadIdx.x; p < nb; p += WG) emit_bucket(A, base, p, A.bk_key[base + p], tb, A.bk_val[base + p]);
         __syncthreads();
     }
 }
@@ -1745,6 +1777,7 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
 static const size_t REFINE_LDS = ((size_t)REFINE_CAP * 3 + 2) * 4;
 static const size_t SEED_WAVE_LDS = (size_t)NWAVES * WAVE_SEED_LDS;
+static const size_t EXTRACT_LDS = (size_t)NWAVES * EXTRACT_WAVE_LDS;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
 void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out,
@@ -1759,7 +1792,10 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_CONTRIB: k_contrib<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I); break;
     // per-read workgroup kernels with data-dependent cost: a grid above the
     // resident count lets the hardware backfill as reads finish
-    case ST_EXTRACT: k_extract<<<read_grid(A.RG, max_grid), WG, SORT_LDS, st>>>(A); break;
+    case ST_EXTRACT:
+        k_extract_wave<<<PGRID(k_extract_wave, EXTRACT_LDS, (A.RG + NWAVES - 1) / NWAVES), WG, EXTRACT_LDS, st>>>(A);
+        k_extract<<<read_grid(A.RG, max_grid), WG, SORT_LDS, st>>>(A);
+        break;
     case ST_PRUNE: k_prune<<<read_grid(A.RG, max_grid), WG, 0, st>>>(A, I); break;
     case ST_REFINE: {
         // three rounds: first candidates, other first-pass candidates, second pass

@@ -11,7 +11,7 @@ namespace shm {
 
 constexpr int LANE_W = 128;               // consecutive windows rolled by one lane
 constexpr int CHUNK_W = 64 * LANE_W;      // windows per wave-chunk when sketching
-constexpr uint32_t SKETCH_LUT_BYTES = 512;   // 25-entry (outgoing, incoming) seed table of k_sketch
+constexpr uint32_t SKETCH_LUT_BYTES = 1024;  // 64-entry (outgoing, incoming) seed table of k_sketch
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
@@ -20,6 +20,8 @@ constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
+constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // bucket-table slots of a read sorted by one wave
+constexpr uint32_t EXTRACT_WAVE_LDS = EXTRACT_WAVE_CAP * 12;  // per-wave LDS of k_extract_wave
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
@@ -82,8 +84,7 @@ struct MapArgs {
     int32_t *tb_min, *tb_max;
     // buckets (group), at bk_base
     uint64_t *bk_key;
-    uint32_t *bk_val, *tmp_g;
-    int32_t *tmp_m, *tmp_c, *tmp_min, *tmp_max;
+    uint32_t *bk_val;
     uint32_t *bs_g;
     int32_t *bs_m, *bs_c, *bs_min, *bs_max;
     uint8_t *bs_alive;
@@ -107,6 +108,8 @@ struct MapArgs {
     int32_t *rd_flag, *rd_first;
     // reads too large for one wave's seeding slice, for k_seed_wg
     uint32_t *big_list, *big_count;
+    // group reads whose bucket tables exceed one wave's slice, for k_extract
+    uint32_t *ext_list, *ext_count;
     // per-workgroup scratch for spans/reads that overflow LDS
     int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;

@@ -591,13 +591,14 @@ struct shm_mapper {
     DevBuf<uint64_t> tb_mc;
     DevBuf<int32_t> tb_min, tb_max;
     DevBuf<uint64_t> bk_key;
-    DevBuf<uint32_t> bk_val, tmp_g, bs_g;
-    DevBuf<int32_t> bk_i32;  // tmp_m,c,min,max + bs_m,c,min,max + sc_* (7)
+    DevBuf<uint32_t> bk_val, bs_g;
+    DevBuf<int32_t> bk_i32;  // bs_m,c,min,max + sc_* (7)
     DevBuf<uint8_t> bs_alive;
     DevBuf<double> bs_sh, sc_j;
     DevBuf<int32_t> g_sid, g_tr, g_diff, g_tails;
     DevBuf<uint2> tasks, sel;
     DevBuf<uint32_t> big;  // [0] count, then the reads seeded by a workgroup
+    DevBuf<uint32_t> ext;  // [0] count, then the group reads extracted by a workgroup
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
 };
@@ -904,9 +905,8 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         const uint64_t B = mp->h_sum.p[3];
         RCHK(mp->bk_key.ensure(B + 1));
         RCHK(mp->bk_val.ensure(B + 1));
-        RCHK(mp->tmp_g.ensure(B + 1));
         RCHK(mp->bs_g.ensure(B + 1));
-        RCHK(mp->bk_i32.ensure(15 * (B + 1)));
+        RCHK(mp->bk_i32.ensure(11 * (B + 1)));
         RCHK(mp->bs_alive.ensure(B + 1));
         RCHK(mp->bs_sh.ensure(B + 1));
         RCHK(mp->sc_j.ensure(B + 1));
@@ -920,30 +920,29 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->tasks.ensure(B + 1));
+        RCHK(mp->ext.ensure((size_t)RG + 2));
+        A.ext_count = mp->ext.p;
+        A.ext_list = mp->ext.p + 1;
+        HIPCHK(hipMemsetAsync(A.ext_count, 0, 4, st));
         RCHK(mp->sel.ensure(B + (uint64_t)RG + 1));
         uint32_t *task_count = mp->task_count.p + 5 * (size_t)ngroups;
         HIPCHK(hipMemsetAsync(task_count, 0, 20, st));
         A.bk_key = mp->bk_key.p;
         A.bk_val = mp->bk_val.p;
-        A.tmp_g = mp->tmp_g.p;
         A.bs_g = mp->bs_g.p;
         int32_t *bi = mp->bk_i32.p;
         const size_t Bs = B + 1;
-        A.tmp_m = bi + 0 * Bs;
-        A.tmp_c = bi + 1 * Bs;
-        A.tmp_min = bi + 2 * Bs;
-        A.tmp_max = bi + 3 * Bs;
-        A.bs_m = bi + 4 * Bs;
-        A.bs_c = bi + 5 * Bs;
-        A.bs_min = bi + 6 * Bs;
-        A.bs_max = bi + 7 * Bs;
-        A.sc_i = bi + 8 * Bs;
-        A.sc_same = bi + 9 * Bs;
-        A.sc_tl = bi + 10 * Bs;
-        A.sc_tr = bi + 11 * Bs;
-        A.sc_ssz = bi + 12 * Bs;
-        A.sc_pend = bi + 13 * Bs;
-        A.sc_seg = bi + 14 * Bs;
+        A.bs_m = bi + 0 * Bs;
+        A.bs_c = bi + 1 * Bs;
+        A.bs_min = bi + 2 * Bs;
+        A.bs_max = bi + 3 * Bs;
+        A.sc_i = bi + 4 * Bs;
+        A.sc_same = bi + 5 * Bs;
+        A.sc_tl = bi + 6 * Bs;
+        A.sc_tr = bi + 7 * Bs;
+        A.sc_ssz = bi + 8 * Bs;
+        A.sc_pend = bi + 9 * Bs;
+        A.sc_seg = bi + 10 * Bs;
         A.bs_alive = mp->bs_alive.p;
         A.bs_sh = mp->bs_sh.p;
         A.sc_j = mp->sc_j.p;
