@@ -1517,6 +1517,7 @@ __device__ __forceinline__ double overlap_of(int32_t sa, int32_t la, int32_t ra,
 // refine memo's counters follow :38-109 (a Containment/Jaccard bucket refined in
 // pass 1 is a memo hit in pass 2).
 __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
+    uint64_t u_sum = 0;  // distinct k-mers of this thread's reads
     for (int32_t lr = blockIdx.x * WG + threadIdx.x; lr < A.RG; lr += gridDim.x * WG) {
         const int32_t rd = A.r0 + lr;
         shm_read_result r = {};
@@ -1525,6 +1526,7 @@ __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
         r.read_len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         r.m = m;
         r.u = A.rd_u[rd];
+        u_sum += (uint64_t)(uint32_t)r.u;
         r.S = A.rd_S[rd];
         r.possible_matches = A.rd_possible[rd];
         r.b_segm = r.b_b = r.b2_segm = r.b2_b = -1;
@@ -1613,8 +1615,11 @@ __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
                 r.b2_b = b;
             }
         }
+        if (r.mapped) A.mapped_list[atomicAdd(A.mapped_count, 1u)] = (uint32_t)rd;
         out[rd] = r;
     }
+    for (int d = WAVE / 2; d > 0; d >>= 1) u_sum += __shfl_xor(u_sum, d);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && u_sum) atomicAdd((unsigned long long *)A.u_total, (unsigned long long)u_sum);
 }
 
 }  // namespace shm

@@ -110,6 +110,10 @@ struct MapArgs {
     uint32_t *big_list, *big_count;
     // group reads whose bucket tables exceed one wave's slice, for k_extract
     uint32_t *ext_list, *ext_count;
+    // k_final's batch totals: the mapped reads (for the host's finalization)
+    // and the sum of distinct k-mers
+    uint32_t *mapped_list, *mapped_count;
+    uint64_t *u_total;
     // per-workgroup scratch for spans/reads that overflow LDS
     int32_t *g_sid, *g_tr;     // sid_stride entries each
     uint64_t sid_stride;

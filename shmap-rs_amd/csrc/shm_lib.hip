@@ -599,6 +599,9 @@ struct shm_mapper {
     DevBuf<uint2> tasks, sel;
     DevBuf<uint32_t> big;  // [0] count, then the reads seeded by a workgroup
     DevBuf<uint32_t> ext;  // [0] count, then the group reads extracted by a workgroup
+    DevBuf<uint32_t> mapped;   // [0] count, then the mapped reads
+    DevBuf<uint64_t> u_total;  // sum of distinct k-mers
+    PinnedBuf<uint32_t> h_mapped;
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
 };
@@ -847,6 +850,13 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    RCHK(mp->mapped.ensure((size_t)R + 2));
+    RCHK(mp->u_total.ensure(1));
+    A.mapped_count = mp->mapped.p;
+    A.mapped_list = mp->mapped.p + 1;
+    A.u_total = mp->u_total.p;
+    HIPCHK(hipMemsetAsync(A.mapped_count, 0, 4, st));
+    HIPCHK(hipMemsetAsync(A.u_total, 0, 8, st));
     // bucketing, in groups of reads whose tables fit the budget
     RCHK(mp->task_count.ensure(5 * (size_t)R + 5));  // five refine-task counters per group
     int32_t ngroups = 0;
@@ -969,6 +979,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     std::vector<uint32_t> ntask(5 * (size_t)ngroups);
     HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    RCHK(mp->h_mapped.ensure((size_t)R + 1));
+    RCHK(mp->h_sum.ensure(6));
+    HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mp->h_sum.p + 4, A.u_total, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(t1, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
@@ -977,14 +991,16 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     mp->total_ms = ms;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    for (int32_t i = 0; i < R; i++) finalize_result(mp->prm, out[i]);
+    // only mapped reads need the host's double-precision finalization
+    const uint32_t n_mapped = mp->h_mapped.p[0];
+    for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
     int64_t cand = 0, deferred = 0, refined = 0, seeds = 0;
     for (int32_t g = 0; g < ngroups; g++) {
         cand += ntask[5 * g];
         deferred += ntask[5 * g + 1];
         refined += ntask[5 * g + 2] + ntask[5 * g + 3] + ntask[5 * g + 4];
     }
-    for (int32_t i = 0; i < R; i++) seeds += out[i].u;
+    seeds = (int64_t)mp->h_sum.p[4];
     mp->counts[0] = (int64_t)M;
     mp->counts[1] = seeds;
     mp->counts[2] = nbuckets;
