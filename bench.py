@@ -85,6 +85,10 @@ WORKLOADS = {
     "genome": dict(lens=None, repeat=0.08, div=(0.01, 0.03), n_units=64, n_reads=92_000,
                    length=(24_000, 12_000), lognormal=True, clip=(1_000, 200_000),
                    errors=((0.07, 0.4, 0.3, 0.3),), t=0.4),
+    # the same reference sketched at r=0.10, 125,000 HiFi-like reads (mean 23 kb, 0.1%)
+    "dense": dict(lens=None, repeat=0.08, div=(0.01, 0.03), n_units=64, n_reads=125_000,
+                  length=(23_000, 10_000), lognormal=True, clip=(1_000, 200_000),
+                  errors=((0.001, 0.6, 0.2, 0.2),), t=0.4, r=0.10),
 }
 
 
@@ -282,7 +286,7 @@ def main():
 
     lens, ref, bases, off, truth, w = make_workload(args.workload, n_reads=args.reads)
     n_reads = len(off) - 1
-    k, h_frac, theta, d, o = 25, 0.01, w["t"], 0.075, 0.3
+    k, h_frac, theta, d, o = 25, w.get("r", 0.01), w["t"], 0.075, 0.3
     names, at = [], 0
     segs = []
     for i, L in enumerate(lens):

@@ -1202,9 +1202,8 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
     }
 }
 
-// find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket that can
-// survive either match_rest pass, one workgroup per bucket from the flat task
-// list the prune pass appended.
+// find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket a
+// refinement round selected, one workgroup per bucket.
 //
 // The span's matched reference k-mers come from the read's own seeds: every
 // hit of every seed inside [l0, end) of the segment marks its position with
@@ -1217,18 +1216,20 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // [max(P(x)+1, x-m+1, 0), x] and its strand agreement to every l in
 // [max(x-m+1, 0), x]; two difference arrays and scans give intersection(l) and
 // same_strand(l), and the first l of the greatest score wins, as the strict
-// `score > best` of the sequential sweep picks.  With abs_pos the window edge
-// depends on positions, so that mode keeps the sequential sweep; so does LCS
-// (:114-170), whose patience LIS is sequential by nature.
+// `score > best` of the sequential sweep picks.  A seed's hits inside the span
+// are a position-ordered run of the index, so P(x) of its k-th hit there is its
+// (k - occs)-th: no sort is needed.  The two arrays live in LDS when the span
+// fits and in the workgroup's global scratch otherwise.  With abs_pos the
+// window edge depends on positions, so that mode keeps the sequential sweep;
+// so does LCS (:114-170), whose patience LIS is sequential by nature.
 __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     extern __shared__ uint8_t lds[];
     int32_t *l_sid = (int32_t *)lds;           // REFINE_CAP
     int32_t *l_b = l_sid + REFINE_CAP;         // REFINE_CAP (+1)
     int32_t *l_c = l_b + REFINE_CAP + 1;       // REFINE_CAP + 1
     __shared__ uint32_t tmp[NWAVES + 1];
-    __shared__ uint32_t s_pre[WG], s_lo[WG];
+    __shared__ uint32_t s_pre[WG], s_lo[WG], s_occ[WG];
     __shared__ uint8_t s_str[WG];
-    __shared__ uint32_t s_nm;
     __shared__ double s_bj[NWAVES];
     __shared__ int32_t s_bl[NWAVES];
     const int ap = A.abs_pos;
@@ -1277,22 +1278,32 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         const int64_t end = be < tlen ? be : tlen;
         const int64_t n = end > l0 ? end - l0 : 0;
         const bool fits = n <= REFINE_CAP;
-        const bool parallel = fits && n > 0 && !ap && A.metric != METRIC_BUCKET_LCS && u < (1 << 19);
-        int32_t *sid = fits ? l_sid : A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
-        int32_t *keys = fits ? l_b : A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
-        // 1. mark every matched position of the span with its seed.  WG seeds
-        //    at a time: each thread finds its seed's hit range inside the span,
-        //    a scan lays the ranges end to end, and the whole workgroup marks
-        //    the hits, so a repeat k-mer with many hits in the span is not left
-        //    to one thread.
-        for (int64_t x = threadIdx.x; x < n; x += WG) sid[x] = -1;
-        if (parallel)
-            for (int64_t x = threadIdx.x; x <= n; x += WG) l_c[x] = 0;
-        if (threadIdx.x == 0) s_nm = 0;
+        const bool parallel = n > 0 && !ap && A.metric != METRIC_BUCKET_LCS;
+        // LDS when the span fits, else this workgroup's global scratch
+        int32_t *const wsid = A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
+        int32_t *const wtr = A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
+        int32_t *const cdf = fits ? l_c : wsid;   // parallel: intersection difference array (n+1)
+        int32_t *const sdf = fits ? l_b : wtr;    // parallel: strand difference array (n+1)
+        int32_t *const sid = fits ? l_sid : wsid; // sequential: seed of each position (n)
+        int32_t *const keys = fits ? l_b : wtr;   // sequential: t_r of each position (n)
+        // 1. every matched position of the span, WG seeds at a time: each thread
+        //    finds its seed's hit range inside the span, a scan lays the ranges
+        //    end to end, and the whole workgroup visits the hits, so a repeat
+        //    k-mer with many hits in the span is not left to one thread.
+        //    A seed's hits in the span are a position-ordered run of the index,
+        //    so the occs(s)-th previous occurrence P(x) of hit j is hit j - occs.
+        if (parallel) {
+            for (int64_t x = threadIdx.x; x <= n; x += WG) {
+                cdf[x] = 0;
+                sdf[x] = 0;
+            }
+        } else {
+            for (int64_t x = threadIdx.x; x < n; x += WG) sid[x] = -1;
+        }
         __syncthreads();
         for (int32_t s0 = 0; s0 < u; s0 += WG) {
             const int32_t sd = s0 + (int32_t)threadIdx.x;
-            uint32_t lo = 0, nin = 0, sstr = 0;
+            uint32_t lo = 0, nin = 0, sstr = 0, occ = 0;
             if (sd < u) {
                 const uint32_t cnt = A.sd_cnt[o + sd];
                 if (cnt) {
@@ -1300,69 +1311,55 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                     lo = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0);
                     nin = hits_lower_bound(I, lo, hoff + cnt, seg, l0 + n, 0) - lo;
                     sstr = A.sd_strand[o + sd];
+                    occ = A.sd_occs[o + sd];
                 }
             }
             uint32_t tot;
             const uint32_t ex = wg_excl_scan(nin, tmp, &tot);  // ends with a barrier
             s_pre[threadIdx.x] = ex;
             s_lo[threadIdx.x] = lo;
+            s_occ[threadIdx.x] = occ;
             s_str[threadIdx.x] = (uint8_t)sstr;
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < tot; i += WG) {
                 // the last seed slot whose range starts at or before i
-                uint32_t a = 0, b = WG;
-                while (b - a > 1) {
-                    const uint32_t mid = (a + b) >> 1;
-                    if (s_pre[mid] <= i) a = mid; else b = mid;
+                uint32_t a = 0, b2 = WG;
+                while (b2 - a > 1) {
+                    const uint32_t mid = (a + b2) >> 1;
+                    if (s_pre[mid] <= i) a = mid; else b2 = mid;
                 }
-                const uint32_t j = s_lo[a] + (i - s_pre[a]);
+                const uint32_t kk = i - s_pre[a];
+                const uint32_t j = s_lo[a] + kk;
                 const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
-                const uint32_t hstr = I.hit_r[j] >> 31;
-                const int32_t sdj = s0 + (int32_t)a;
-                sid[x] = sdj | (s_str[a] == hstr ? 0 : 0x40000000);
-                if (parallel) keys[atomicAdd(&s_nm, 1u)] = ((uint32_t)sdj << 12) | (uint32_t)x;
+                const bool same = s_str[a] == (I.hit_r[j] >> 31);
+                if (parallel) {
+                    // x counts in windows [max(P(x)+1, x-m+1, 0), x]; its strand
+                    // agreement in [max(x-m+1, 0), x]
+                    const uint32_t oc = s_occ[a];
+                    const int64_t P = kk >= oc ? (int64_t)I.hit_g[j - oc] - ks - l0 : -1;
+                    int64_t lw = x - m + 1;
+                    if (lw < 0) lw = 0;
+                    const int64_t lc = P + 1 > lw ? P + 1 : lw;
+                    atomicAdd(&cdf[lc], 1);
+                    atomicAdd(&cdf[x + 1], -1);
+                    const int32_t cd = same ? 1 : -1;
+                    atomicAdd(&sdf[lw], cd);
+                    atomicAdd(&sdf[x + 1], -cd);
+                } else {
+                    sid[x] = (s0 + (int32_t)a) | (same ? 0 : 0x40000000);
+                }
             }
             __syncthreads();
         }
         if (parallel) {
-            // 2. occurrences grouped by seed, in position order
-            const uint32_t nm = s_nm;
-            bitonic_keys((uint32_t *)keys, nm);
-            // 3. intersection difference array
-            for (uint32_t i = threadIdx.x; i < nm; i += WG) {
-                const uint32_t key = (uint32_t)keys[i];
-                const int32_t sd = (int32_t)(key >> 12), x = (int32_t)(key & 0xfff);
-                const uint32_t occ = A.sd_occs[o + sd];
-                int32_t P = -1;
-                if (i >= occ && ((uint32_t)keys[i - occ] >> 12) == (uint32_t)sd) P = (int32_t)((uint32_t)keys[i - occ] & 0xfff);
-                int32_t lo = x - m + 1;
-                if (P + 1 > lo) lo = P + 1;
-                if (lo < 0) lo = 0;
-                atomicAdd(&l_c[lo], 1);
-                atomicAdd(&l_c[x + 1], -1);
-            }
-            __syncthreads();
-            // 4. strand agreement difference array (reusing the key space)
-            for (int64_t x = threadIdx.x; x <= n; x += WG) keys[x] = 0;
-            __syncthreads();
-            for (int64_t x = threadIdx.x; x < n; x += WG) {
-                const int32_t v = sid[x];
-                if (v < 0) continue;
-                const int32_t cd = (v & 0x40000000) ? -1 : 1;
-                int64_t lo = x - m + 1;
-                if (lo < 0) lo = 0;
-                atomicAdd(&keys[lo], cd);
-                atomicAdd(&keys[x + 1], -cd);
-            }
-            __syncthreads();
-            wg_scan_inplace(l_c, n, tmp);
-            wg_scan_inplace(keys, n, tmp);
-            // 5. first window with the greatest score
+            wg_scan_inplace(cdf, n, tmp);
+            wg_scan_inplace(sdf, n, tmp);
+            // first window with the greatest score
             const bool jac = A.metric == METRIC_JACCARD;
             double bj = -1.0;
             int32_t bl = 0x7fffffff;
             for (int64_t l = threadIdx.x; l < n; l += WG) {
-                const int32_t I2 = l_c[l];
+                const int32_t I2 = cdf[l];
                 const int64_t r = l + m < n ? l + m : n;
                 const double score = jac ? (double)I2 / (double)(m + (int32_t)(r - l) - I2) : (double)I2 / (double)m;
                 if (score > bj) {
@@ -1395,8 +1392,8 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                 const int32_t tl = I.t_r[ks + l0 + L], tr = I.t_r[ks + l0 + r - 1];
                 if (J > A.theta) atomicMin(&A.rd_flag[rd], p);
                 A.sc_j[base + p] = J;
-                A.sc_i[base + p] = l_c[L];
-                A.sc_same[base + p] = keys[L];
+                A.sc_i[base + p] = cdf[L];
+                A.sc_same[base + p] = sdf[L];
                 A.sc_tl[base + p] = tl;
                 A.sc_tr[base + p] = tr;
                 A.sc_ssz[base + p] = tr - tl + 1;
