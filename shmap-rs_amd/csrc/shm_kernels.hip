@@ -866,7 +866,7 @@ __device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t
 }
 
 // Adds one (seed, bucket) contribution to read lr's bucket table; the insert
-// that claims an empty slot counts the read's new bucket.
+// that claims an empty slot counts the read's new bucket and lists its slot.
 __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t base, uint64_t cap, uint32_t g,
                                         uint32_t matches, int32_t codir, int32_t rmin, int32_t rmax) {
     const uint32_t lg = 63 - __clzll(cap);
@@ -875,7 +875,8 @@ __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t 
     for (;;) {
         const uint32_t old = atomicCAS(&A.tb_key[base + pos], EMPTY_KEY, g);
         if (old == EMPTY_KEY) {
-            atomicAdd((unsigned long long *)&A.grp_nb[lr], 1ull);
+            const uint64_t at = atomicAdd((unsigned long long *)&A.grp_nb[lr], 1ull);
+            A.tb_list[base + at] = pos;  // the read's used slots, for k_extract
             break;
         }
         if (old == g) break;
@@ -960,37 +961,27 @@ __device__ __forceinline__ void emit_bucket(const MapArgs &A, uint64_t base, uin
     A.bs_max[base + p] = A.tb_max[tb + slot];
 }
 
-// One wave per read whose table has at most EXTRACT_WAVE_CAP slots: the used
-// slots are compacted by ballot into the wave's LDS slice and sorted there.
-// Larger tables are listed for k_extract.
+// One wave per read with at most EXTRACT_WAVE_CAP buckets: the listed slots are
+// keyed into the wave's LDS slice and sorted there.  Reads with more buckets are
+// listed for k_extract.
 __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
     extern __shared__ uint8_t lds[];
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     uint64_t *kk = (uint64_t *)(lds + (size_t)w * EXTRACT_WAVE_LDS);
     uint32_t *kv = (uint32_t *)(kk + EXTRACT_WAVE_CAP);
-    const uint64_t lt = (1ull << lane) - 1ull;
     for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
-        const int32_t rd = A.r0 + lr;
-        const uint64_t cap = A.rd_cap[rd];
-        if (cap > EXTRACT_WAVE_CAP) {
+        const uint32_t n = (uint32_t)A.grp_nb[lr];
+        if (n > EXTRACT_WAVE_CAP) {
             if (lane == 0) A.ext_list[atomicAdd(A.ext_count, 1u)] = (uint32_t)lr;
             continue;
         }
         const uint64_t tb = A.tab_base[lr], base = A.bk_base[lr];
-        uint32_t n = 0;
-        for (uint32_t b0 = 0; b0 < (uint32_t)cap; b0 += WAVE) {
-            const uint32_t i = b0 + lane;
-            const uint32_t g = A.tb_key[tb + i];  // cap is a multiple of WAVE
-            const bool used = g != EMPTY_KEY;
-            const uint64_t bal = __ballot(used);
-            if (used) {
-                int32_t codir;
-                const uint32_t matches = mc_matches(A.tb_mc[tb + i], codir);
-                const uint32_t at = n + (uint32_t)__popcll(bal & lt);
-                kk[at] = ((uint64_t)(0xffffffffu - matches) << 32) | g;
-                kv[at] = i;
-            }
-            n += (uint32_t)__popcll(bal);
+        for (uint32_t i = lane; i < n; i += WAVE) {
+            const uint32_t slot = A.tb_list[tb + i];
+            int32_t codir;
+            const uint32_t matches = mc_matches(A.tb_mc[tb + slot], codir);
+            kk[i] = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
+            kv[i] = slot;
         }
         wave_sync();
         wave_bitonic(kk, kv, n);
@@ -999,39 +990,25 @@ __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
     }
 }
 
-// The same for the listed reads with larger tables, one workgroup each, through
+// The same for the listed reads with more buckets, one workgroup each, through
 // global memory with the sort staged in LDS when it fits.
 __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
-    __shared__ uint32_t tmp[NWAVES + 1];
-    __shared__ uint32_t s_n;
     const uint32_t nlist = *A.ext_count;
     for (uint32_t t = blockIdx.x; t < nlist; t += gridDim.x) {
         const int32_t lr = (int32_t)A.ext_list[t];
-        const int32_t rd = A.r0 + lr;
-        const uint64_t tb = A.tab_base[lr], cap = A.rd_cap[rd], base = A.bk_base[lr];
-        if (threadIdx.x == 0) s_n = 0;
-        __syncthreads();
-        for (uint64_t b0 = 0; b0 < cap; b0 += WG) {
-            const uint64_t i = b0 + threadIdx.x;
-            const uint32_t g = i < cap ? A.tb_key[tb + i] : EMPTY_KEY;
-            const bool used = g != EMPTY_KEY;
-            uint32_t tot;
-            const uint32_t ex = wg_excl_scan(used ? 1u : 0u, tmp, &tot);
-            const uint32_t n0 = s_n;
-            if (used) {
-                int32_t codir;
-                const uint32_t matches = mc_matches(A.tb_mc[tb + i], codir);
-                A.bk_key[base + n0 + ex] = ((uint64_t)(0xffffffffu - matches) << 32) | g;
-                A.bk_val[base + n0 + ex] = (uint32_t)i;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) s_n = n0 + tot;
-            __syncthreads();
+        const uint64_t tb = A.tab_base[lr], base = A.bk_base[lr];
+        const uint32_t nb = (uint32_t)A.grp_nb[lr];
+        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
+            const uint32_t slot = A.tb_list[tb + i];
+            int32_t codir;
+            const uint32_t matches = mc_matches(A.tb_mc[tb + slot], codir);
+            A.bk_key[base + i] = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
+            A.bk_val[base + i] = slot;
         }
-        const uint32_t nb = s_n;
+        __syncthreads();
         wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
         for (uint32_t p = threadIdx.x; p < nb; p += WG) emit_bucket(A, base, p, A.bk_key[base + p], tb, A.bk_val[base + p]);
         __syncthreads();

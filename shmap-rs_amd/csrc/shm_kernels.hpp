@@ -20,7 +20,7 @@ constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
-constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // bucket-table slots of a read sorted by one wave
+constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave
 constexpr uint32_t EXTRACT_WAVE_LDS = EXTRACT_WAVE_CAP * 12;  // per-wave LDS of k_extract_wave
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
@@ -78,8 +78,8 @@ struct MapArgs {
     // per group read (local index), RG+1 entries
     const uint64_t *hit_base, *tab_base, *bk_base;
     uint64_t *grp_nb;  // each group read's bucket count (counted by k_contrib's inserts), scanned into bk_base
-    // per-read bucket tables (group)
-    uint32_t *tb_key;
+    // per-read bucket tables (group); tb_list holds each read's used slots
+    uint32_t *tb_key, *tb_list;
     uint64_t *tb_mc;
     int32_t *tb_min, *tb_max;
     // buckets (group), at bk_base

@@ -566,7 +566,7 @@ struct shm_mapper {
     shm_sketcher sk{};
     hipStream_t st = nullptr;
     uint32_t max_grid = 1024;
-    uint64_t tab_budget = 1ull << 27;  // table slots per group (20 B each)
+    uint64_t tab_budget = 1ull << 30;  // bucket-table slots per group (24 B each, ~26 GB)
     bool timing = false;
     hipEvent_t ev[ST_COUNT][2];
     double stage_ms[ST_COUNT] = {};
@@ -587,7 +587,7 @@ struct shm_mapper {
     DevBuf<uint64_t> grp_base; // 3 x (RG+1): hit, table and bucket offsets of a group's reads
     DevBuf<uint64_t> grp_nb, part, stat;
     PinnedBuf<uint64_t> h_sum;  // totals read back between stages
-    DevBuf<uint32_t> tb_key;
+    DevBuf<uint32_t> tb_key, tb_list;
     DevBuf<uint64_t> tb_mc;
     DevBuf<int32_t> tb_min, tb_max;
     DevBuf<uint64_t> bk_key;
@@ -887,6 +887,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         }
         const int32_t RG = r1 - r0;
         RCHK(mp->tb_key.ensure(T + 1));
+        RCHK(mp->tb_list.ensure(T + 1));
         RCHK(mp->tb_mc.ensure(T + 1));
         RCHK(mp->tb_min.ensure(T + 1));
         RCHK(mp->tb_max.ensure(T + 1));
@@ -897,6 +898,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.bk_base = mp->grp_base.p + 2 * (size_t)(RG + 1);
         A.grp_nb = mp->grp_nb.p;
         A.tb_key = mp->tb_key.p;
+        A.tb_list = mp->tb_list.p;
         A.tb_mc = mp->tb_mc.p;
         A.tb_min = mp->tb_min.p;
         A.tb_max = mp->tb_max.p;

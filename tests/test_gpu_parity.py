@@ -171,3 +171,25 @@ def test_empty_and_degenerate_batches():
     res = mp.map_batch([b"", b"ACGT", b"N" * 5000, reads[0]])
     assert list(res["mapped"][:3]) == [0, 0, 0]
     assert res["unmappable_short"][0] == 1
+
+
+def test_table_budget_groups_give_the_same_results():
+    """A bucket-table budget far below the batch's need splits the reads into
+    many groups on the host; every record stays the same."""
+    segs, reads = _workload(**WORKLOADS["repeats"])
+    prm = dict(theta=0.4, min_diff=0.075, max_overlap=0.3)
+    gi, gp, mp, whole = P.gpu_map(segs, reads, 15, 0.1, **prm)
+    old = os.environ.get("SHM_TAB_BUDGET")
+    os.environ["SHM_TAB_BUDGET"] = "4096"
+    try:
+        small = G.Mapper(gi, gp)
+    finally:
+        if old is None:
+            del os.environ["SHM_TAB_BUDGET"]
+        else:
+            os.environ["SHM_TAB_BUDGET"] = old
+    grouped = small.map_batch(reads)
+    assert small.last_counts()["groups"] > 5
+    bad = P.diff_results(whole, grouped)
+    assert not bad, bad
+
