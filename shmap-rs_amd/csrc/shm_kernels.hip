@@ -119,6 +119,27 @@ __device__ __forceinline__ uint32_t pack_nibbles4(uint32_t c) {
     return (c | (c >> 8)) & 0x0000ffffu;
 }
 
+// Eight bases (v0 then v1, first base lowest) to eight 4-bit codes as
+// base_codes4 gives them.  A, C, G, T (either case) differ in their low three
+// bits (1, 3, 7, 4), so byte permutes over 8-entry tables indexed by those bits
+// give each byte's code, its code times 16, and the lower-case letter the code
+// stands for; when every byte is that letter (in either case) the codes pack
+// with one shift-or per dword and one byte gather.  Any other byte takes the
+// general path.
+__device__ __forceinline__ uint32_t stage_codes8(uint32_t v0, uint32_t v1) {
+    constexpr uint32_t LET_LO = 0x63006100u, LET_HI = 0x67000074u;  // low 3 bits -> letter: 1 a, 3 c, 4 t, 7 g
+    constexpr uint32_t COD_LO = 0x01000000u, COD_HI = 0x02000003u;  // -> code
+    constexpr uint32_t C16_LO = 0x10000000u, C16_HI = 0x20000030u;  // -> code << 4
+    const uint32_t s0 = v0 & 0x07070707u, s1 = v1 & 0x07070707u;
+    const uint32_t d = ((v0 | 0x20202020u) ^ __builtin_amdgcn_perm(LET_HI, LET_LO, s0)) |
+                       ((v1 | 0x20202020u) ^ __builtin_amdgcn_perm(LET_HI, LET_LO, s1));
+    if (__builtin_expect(d != 0u, 0))
+        return pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
+    const uint32_t t0 = (__builtin_amdgcn_perm(COD_HI, COD_LO, s0) << 8) | __builtin_amdgcn_perm(C16_HI, C16_LO, s0);
+    const uint32_t t1 = (__builtin_amdgcn_perm(COD_HI, COD_LO, s1) << 8) | __builtin_amdgcn_perm(C16_HI, C16_LO, s1);
+    return __builtin_amdgcn_perm(t1, t0, 0x07050301u);  // bytes 1 and 3 of t0, then of t1
+}
+
 // One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
 // are staged in LDS as 4-bit codes, base p at nibble p & 7 of dword p >> 3, with
 // one dword of bank skew per 16 dwords (lanes rolling runs 128 windows = 16
@@ -129,8 +150,9 @@ __device__ __forceinline__ uint32_t pack_nibbles4(uint32_t c) {
 // (rotl(fw_out, k) ^ fw_in, rotr(rc_out, 1) ^ rotl(rc_in, k-1)), one 16-byte
 // load per window (codes 4-7 are non-ACGT bytes, seed 0).  Each lane rolls
 // LANE_W consecutive windows; its first window is rolled in from a zero hash
-// through the outgoing-code-4 row, which equals the direct definition of
-// src/sketch.rs:180-184.
+// two bases per step (a second 64-entry table of base pairs) and, for odd k,
+// one more through the outgoing-code-4 row, which equals the direct definition
+// of src/sketch.rs:180-184.
 //
 // PASS 0 visits every chunk and writes each lane's kept k-mers, in position
 // order, to lane-strided staging slots (slot i of lane l of chunk c at
@@ -145,16 +167,21 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                                int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                                                uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
-    ulonglong2 *T = (ulonglong2 *)lds;  // (out code << 3 | in code) -> xor terms of both strands
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    // T[(out code << 3) | in code]: the xor terms of both strands for one step
+    // (row 4, an N rolling out, adds nothing: the lane starts use it).
+    // T2[(a << 3) | b]: two bases a, b rolled in from nothing, for lane starts.
+    __shared__ ulonglong2 T[64], T2[64];
+    const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const uint32_t span = sketch_span(k);
-    uint8_t *buf = lds + SKETCH_LUT_BYTES + (size_t)w * span;
+    uint8_t *buf = lds + (size_t)w * span;
     if (threadIdx.x < 64) {
         const uint8_t co = "ACGTNNNN"[threadIdx.x >> 3], ci = "ACGTNNNN"[threadIdx.x & 7];  // code order of base_codes4
-        T[threadIdx.x] = make_ulonglong2(rotl64(nt_fw(co), (uint32_t)k) ^ nt_fw(ci),
-                                         rotr64(nt_rc(co), 1) ^ rotl64(nt_rc(ci), (uint32_t)(k - 1)));
+        const uint64_t ri = rotl64(nt_rc(ci), (uint32_t)(k - 1)), rl = rotl64(nt_rc(co), (uint32_t)(k - 1));
+        T[threadIdx.x] = make_ulonglong2(rotl64(nt_fw(co), (uint32_t)k) ^ nt_fw(ci), rotr64(nt_rc(co), 1) ^ ri);
+        T2[threadIdx.x] = make_ulonglong2(rotl64(nt_fw(co), 1) ^ nt_fw(ci), rotr64(rl, 1) ^ ri);
     }
     __syncthreads();
+    const uint32_t thres_hi = (uint32_t)(thres >> 32);
     const uint64_t total = PASS == 0 ? chunk_start[S.n] : (uint64_t)*ovf_count;
     uint32_t *b32 = (uint32_t *)buf;
 #define SKW(q) b32[(uint32_t)(q) + ((uint32_t)(q) >> 4)]
@@ -184,7 +211,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     const uint32_t d0 = s32[2 * g], d1 = s32[2 * g + 1], d2 = s32[2 * g + 2];
                     const uint32_t v0 = sh ? __builtin_amdgcn_alignbit(d1, d0, sh) : d0;
                     const uint32_t v1 = sh ? __builtin_amdgcn_alignbit(d2, d1, sh) : d1;
-                    SKW(g) = pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
+                    SKW(g) = stage_codes8(v0, v1);
                 }
             } else {
                 for (int64_t g = lane; g < ng; g += WAVE) {
@@ -210,23 +237,52 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 base_out = chunk_val[c] + (wave_incl_scan(mine) - mine);
             }
             if (lw0 < lw1) {
-                uint64_t hf = 0, hr = 0;
-                for (int32_t i = 0; i < k; i++) {
-                    const ulonglong2 ti = T[32 + SKN(lw0 + i)];  // nothing rolls out
+                // the lane's first window: its k bases rolled in from nothing,
+                // two at a time (pairs are the bytes of the code stream; the
+                // lane start is dword aligned since LANE_W is a multiple of 8)
+                uint32_t fl = 0, fh = 0, rl = 0, rh = 0;  // the strands as 32-bit halves
+                const uint32_t q0 = (uint32_t)lw0 >> 3;
+                for (int32_t p = 0; p < k / 2; p += 4) {
+                    const uint32_t d = SKW(q0 + ((uint32_t)p >> 2));
+                    const uint32_t e = ((d & 0x07070707u) << 3) | ((d >> 4) & 0x07070707u);
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (p + t < k / 2) {
+                            const uint4 tt = ((const uint4 *)T2)[__builtin_amdgcn_perm(0u, e, 0x0c0c0c00u | (uint32_t)t)];
+                            const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 30) ^ tt.y;  // rotl2
+                            const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 30) ^ tt.x;
+                            const uint32_t nrh = __builtin_amdgcn_alignbit(rl, rh, 2) ^ tt.w;   // rotr2
+                            const uint32_t nrl = __builtin_amdgcn_alignbit(rh, rl, 2) ^ tt.z;
+                            fh = nfh; fl = nfl; rh = nrh; rl = nrl;
+                        }
+                    }
+                }
+                uint64_t hf = ((uint64_t)fh << 32) | fl, hr = ((uint64_t)rh << 32) | rl;
+                if (k & 1) {
+                    const ulonglong2 ti = T[32 + SKN(lw0 + k - 1)];
                     hf = rotl1(hf) ^ ti.x;
                     hr = rotr1(hr) ^ ti.y;
                 }
+                // A kept k-mer goes to the lane's next staging slot: chunk c's
+                // slots (PASS 0) are uniform bases plus a 32-bit lane offset.
+                const uint64_t cb = (uint64_t)c * lcap * WAVE;
+                uint64_t *oh_c = out_h + (PASS == 0 ? cb : 0);
+                int32_t *or_c = out_r + (PASS == 0 ? cb : 0);
+                uint8_t *os_c = out_s + (PASS == 0 ? cb : 0);
+                const int32_t r0 = (int32_t)(w0 + k - 1);  // window wi ends at base w0 + wi + k - 1
                 auto put = [&](int64_t wi, uint64_t h, bool strand) {
-                    uint64_t o;
                     if (PASS == 1) {
-                        o = base_out + cnt;
+                        const uint64_t o = base_out + cnt;
+                        out_h[o] = h;
+                        out_r[o] = r0 + (int32_t)wi;
+                        out_s[o] = strand;
                     } else {
                         if (cnt >= lcap) return;
-                        o = ((uint64_t)c * lcap + cnt) * WAVE + lane;
+                        const uint32_t o = cnt * WAVE + (uint32_t)lane;
+                        oh_c[o] = h;
+                        or_c[o] = r0 + (int32_t)wi;
+                        os_c[o] = strand;
                     }
-                    out_h[o] = h;
-                    out_r[o] = (int32_t)(w0 + wi + k - 1);
-                    out_s[o] = strand;
                 };
                 {
                     const uint64_t h = hr ^ hf;
@@ -240,11 +296,15 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 // outgoing codes are one dword (lw0 is a multiple of LANE_W, so a
                 // lane's outgoing dwords share one skew), the incoming ones a
                 // funnel shift of two; even and odd nibbles spread to bytes give
-                // the eight (out << 3 | in) table indices.  Kept windows are rare,
-                // so the steps only record a keep mask and the writes come after.
+                // the eight (out << 3 | in) table indices.  The strands roll as
+                // 32-bit halves.  Kept windows are rare, so the steps only test
+                // the high word of the hash against the threshold's, and the
+                // exact test and the writes come after.
                 const int64_t nsteps = lw1 - lw0 - 1;
                 const uint32_t ish = (uint32_t)(k & 7) * 4;
                 const uint32_t *obase = &SKW((uint32_t)lw0 >> 3);
+                const uint4 *T4 = (const uint4 *)T;  // (fw lo, fw hi, rc lo, rc hi)
+                fl = (uint32_t)hf; fh = (uint32_t)(hf >> 32); rl = (uint32_t)hr; rh = (uint32_t)(hr >> 32);
                 int64_t j = 0;
                 for (; j + 8 <= nsteps; j += 8) {
                     const uint32_t od = obase[j >> 3];
@@ -255,28 +315,38 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     const uint32_t po = (((od >> 4) & 0x0f0f0f0fu) << 3) | ((id >> 4) & 0x0f0f0f0fu); // windows 1, 3, 5, 7
 #pragma unroll
                     for (int half = 0; half < 2; half++) {
-                        uint64_t hs[4], fs[4];
-                        uint32_t keep = 0;
+                        uint32_t sfl[4], sfh[4], srl[4], srh[4];
+                        uint32_t cand = 0;
 #pragma unroll
                         for (int t4 = 0; t4 < 4; t4++) {
                             const int t = half * 4 + t4;
-                            const ulonglong2 tt = T[(((t & 1) ? po : pe) >> (8 * (t >> 1))) & 0xff];
-                            hf = rotl1(hf) ^ tt.x;
-                            hr = rotr1(hr) ^ tt.y;
-                            fs[t4] = hf;
-                            hs[t4] = hr ^ hf;
-                            keep |= (uint32_t)(hs[t4] <= thres) << t4;
+                            const uint4 tt =
+                                T4[__builtin_amdgcn_perm(0u, (t & 1) ? po : pe, 0x0c0c0c00u | (uint32_t)(t >> 1))];
+                            const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 31) ^ tt.y;  // rotl1
+                            const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 31) ^ tt.x;
+                            const uint32_t nrh = __builtin_amdgcn_alignbit(rl, rh, 1) ^ tt.w;   // rotr1
+                            const uint32_t nrl = __builtin_amdgcn_alignbit(rh, rl, 1) ^ tt.z;
+                            fh = nfh; fl = nfl; rh = nrh; rl = nrl;
+                            sfl[t4] = fl; sfh[t4] = fh; srl[t4] = rl; srh[t4] = rh;
+                            cand |= (uint32_t)((fh ^ rh) <= thres_hi) << t4;
                         }
-                        if (keep) {
+                        if (cand) {
 #pragma unroll
                             for (int t4 = 0; t4 < 4; t4++)
-                                if (keep & (1u << t4)) {
-                                    put(lw0 + j + half * 4 + t4 + 1, hs[t4], fs[t4] > (hs[t4] ^ fs[t4]));
-                                    cnt++;
+                                if (cand & (1u << t4)) {
+                                    const uint64_t f = ((uint64_t)sfh[t4] << 32) | sfl[t4];
+                                    const uint64_t r = ((uint64_t)srh[t4] << 32) | srl[t4];
+                                    const uint64_t h = f ^ r;
+                                    if (h <= thres) {
+                                        put(lw0 + j + half * 4 + t4 + 1, h, f > r);
+                                        cnt++;
+                                    }
                                 }
                         }
                     }
                 }
+                hf = ((uint64_t)fh << 32) | fl;
+                hr = ((uint64_t)rh << 32) | rl;
                 for (; j < nsteps; j++) {
                     const ulonglong2 tt = T[(SKN(lw0 + j) << 3) | SKN(lw0 + j + k)];
                     hf = rotl1(hf) ^ tt.x;
@@ -896,12 +966,10 @@ __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t 
 // so the first hit of each (segment, half-bucket) run emits the clamped
 // per-(seed, bucket) contribution of bucket b (runs b and b+1) and -- unless the
 // previous run, at b-1, already did -- of bucket b-1 (run b alone).  Counts come
-// from binary searches, strand agreement from the strand prefix.
-__global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
-    const uint64_t E = A.hit_base[A.RG];
-    const int ap = A.abs_pos;
-    for (uint64_t f = (uint64_t)blockIdx.x * WG + threadIdx.x; f < E; f += (uint64_t)gridDim.x * WG) {
-        const uint32_t lr = upper_bound_u64(A.hit_base, (uint32_t)A.RG + 1, f) - 1;
+// from binary searches, strand agreement from the strand prefix.  This is the
+// work of one examined hit f of group read lr.
+__device__ __forceinline__ void contrib_hit(const MapArgs &A, const DevIndex &I, uint64_t f, uint32_t lr, int ap) {
+    {
         const int32_t rd = A.r0 + (int32_t)lr;
         const uint32_t local = (uint32_t)(f - A.hit_base[lr]);
         const uint64_t o = A.koff[rd];
@@ -925,7 +993,7 @@ __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
             const uint32_t seg0 = I.hit_seg[idx - 1];
             if (seg0 == seg) {
                 const int32_t b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
-                if (b0 == b) continue;  // inside a run
+                if (b0 == b) return;  // inside a run
                 prev_bm1 = b0 == b - 1;
             }
         }
@@ -941,6 +1009,39 @@ __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
             tab_add(A, lr, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
                     (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
         }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
+    // A block's WG consecutive hits lie in a run of reads [r0, r0 + nr): one
+    // binary search over the group's hit offsets finds the run, and when it is
+    // short (reads without hits can lengthen it) its offsets are staged in LDS
+    // for every thread's search.
+    __shared__ uint64_t s_hb[WG + 1];
+    __shared__ uint32_t s_r0, s_nr;
+    const uint64_t E = A.hit_base[A.RG];
+    const int ap = A.abs_pos;
+    for (uint64_t f0 = (uint64_t)blockIdx.x * WG; f0 < E; f0 += (uint64_t)gridDim.x * WG) {
+        if (threadIdx.x == 0) {
+            const uint64_t fl = f0 + WG - 1 < E ? f0 + WG - 1 : E - 1;
+            const uint32_t a = upper_bound_u64(A.hit_base, (uint32_t)A.RG + 1, f0) - 1;
+            const uint32_t b = upper_bound_u64(A.hit_base + a, (uint32_t)A.RG + 1 - a, fl) - 1 + a;
+            s_r0 = a;
+            s_nr = b - a + 1;
+        }
+        __syncthreads();
+        const uint32_t r0 = s_r0, nr = s_nr;
+        const bool staged = nr + 1 <= (uint32_t)WG + 1;
+        if (staged)
+            for (uint32_t i = threadIdx.x; i <= nr; i += WG) s_hb[i] = A.hit_base[r0 + i];
+        __syncthreads();
+        const uint64_t f = f0 + threadIdx.x;
+        if (f < E) {
+            const uint32_t lr = r0 - 1 + (staged ? upper_bound_u64(s_hb, nr + 1, f)
+                                                 : upper_bound_u64(A.hit_base + r0, nr + 1, f));
+            contrib_hit(A, I, f, lr, ap);
+        }
+        __syncthreads();
     }
 }
 
@@ -1756,7 +1857,7 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
     k_chunk_counts<<<flat_grid((uint64_t)S.n, 1024), WG, 0, st>>>(S, k, nch);
 }
 
-size_t sketch_lds(int32_t k) { return SKETCH_LUT_BYTES + (size_t)NWAVES * sketch_span(k); }
+size_t sketch_lds(int32_t k) { return (size_t)NWAVES * sketch_span(k); }
 
 uint32_t sketch_lane_cap(uint64_t thres) {
     const double frac = ((double)thres + 1.0) / 18446744073709551616.0;  // keep probability

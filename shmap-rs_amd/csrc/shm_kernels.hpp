@@ -11,7 +11,6 @@ namespace shm {
 
 constexpr int LANE_W = 128;               // consecutive windows rolled by one lane
 constexpr int CHUNK_W = 64 * LANE_W;      // windows per wave-chunk when sketching
-constexpr uint32_t SKETCH_LUT_BYTES = 1024;  // 64-entry (outgoing, incoming) seed table of k_sketch
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
