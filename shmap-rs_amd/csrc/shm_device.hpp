@@ -240,6 +240,123 @@ __device__ void bitonic_keys(K *k, uint32_t n) {
     }
 }
 
+// Register-resident bitonic sort of one wave's 64*E elements, ascending by
+// (key, val) (by key alone when !PAIRS); element x = lane*E + e is k[e], v[e].
+// Every step pairs x with x ^ m (m = size-1 for the first step of a merge,
+// whose partner mirrors x inside its block, then the strides size/4 .. 1);
+// the lower index keeps the smaller element.  Masks below E pair registers of
+// one lane; larger ones pair lanes through shuffles.
+template <bool PAIRS>
+__device__ __forceinline__ bool sort_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+    return ka < kb || (PAIRS && ka == kb && va < vb);
+}
+
+template <int E, bool PAIRS, int M>
+__device__ __forceinline__ void sort_reg_step(uint64_t (&k)[E], uint32_t (&v)[E]) {
+    // in-register pairs (e, e ^ M), M < E; the lower register keeps the min
+    constexpr int HB = (M & (M + 1)) == 0 ? (M + 1) / 2 : M;  // highest bit of M (2^j - 1 or 2^j)
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        if (e & HB) continue;
+        const int p = e ^ M;
+        const bool t = sort_less<PAIRS>(k[p], v[p], k[e], v[e]);
+        const uint64_t ke = k[e], kp = k[p];
+        const uint32_t ve = v[e], vp = v[p];
+        k[e] = t ? kp : ke;
+        k[p] = t ? ke : kp;
+        if (PAIRS) {
+            v[e] = t ? vp : ve;
+            v[p] = t ? ve : vp;
+        }
+    }
+}
+
+template <int E, bool PAIRS, int ST>
+__device__ __forceinline__ void sort_reg_strides(uint64_t (&k)[E], uint32_t (&v)[E]) {
+    if constexpr (ST > 0) {
+        sort_reg_step<E, PAIRS, ST>(k, v);
+        sort_reg_strides<E, PAIRS, ST / 2>(k, v);
+    }
+}
+
+template <int E, bool PAIRS, int SIZE>
+__device__ __forceinline__ void sort_reg_merges(uint64_t (&k)[E], uint32_t (&v)[E]) {
+    if constexpr (SIZE <= E) {
+        sort_reg_step<E, PAIRS, SIZE - 1>(k, v);
+        sort_reg_strides<E, PAIRS, SIZE / 4>(k, v);
+        sort_reg_merges<E, PAIRS, SIZE * 2>(k, v);
+    }
+}
+
+// Cross-lane step: this lane's register e meets register (MIRROR ? E-1-e : e)
+// of the lane whose byte address is pa; lower says whether this lane holds the
+// lower indices.  Elements are distinct or identical, so the partner is taken
+// exactly when (partner < mine) == lower.
+template <int E, bool PAIRS, bool MIRROR>
+__device__ __forceinline__ void sort_lane_step(uint64_t (&k)[E], uint32_t (&v)[E], int pa, bool lower) {
+    uint32_t pl[E], ph[E], pv[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int q = MIRROR ? E - 1 - e : e;
+        pl[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)k[q]);
+        ph[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)(k[q] >> 32));
+        pv[e] = PAIRS ? (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)v[q]) : 0u;
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint64_t pk = ((uint64_t)ph[e] << 32) | pl[e];
+        const bool take = sort_less<PAIRS>(pk, pv[e], k[e], v[e]) == lower;
+        k[e] = take ? pk : k[e];
+        if (PAIRS) v[e] = take ? pv[e] : v[e];
+    }
+}
+
+template <int E, bool PAIRS>
+__device__ void wave_sort_regs(uint64_t (&k)[E], uint32_t (&v)[E]) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    sort_reg_merges<E, PAIRS, 2>(k, v);  // blocks of up to E elements: within each lane
+#pragma unroll 1
+    for (uint32_t lsz = 2; lsz <= WAVE; lsz <<= 1) {  // blocks of lsz lanes
+        sort_lane_step<E, PAIRS, true>(k, v, (int)((lane ^ (lsz - 1)) << 2), (lane & (lsz >> 1)) == 0);
+#pragma unroll 1
+        for (uint32_t ls = lsz >> 2; ls > 0; ls >>= 1)
+            sort_lane_step<E, PAIRS, false>(k, v, (int)((lane ^ ls) << 2), (lane & ls) == 0);
+        sort_reg_strides<E, PAIRS, E / 2>(k, v);
+    }
+}
+
+// One wave sorts n <= 64*E elements held in registers between load(x, key, val)
+// and store(x, key, val) over x < n (lane-major: lane l holds x = l*E .. l*E+E-1).
+template <int E, bool PAIRS, typename LoadF, typename StoreF>
+__device__ __forceinline__ void wave_sort_e(uint32_t n, LoadF load, StoreF store) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint64_t k[E];
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = lane * E + (uint32_t)e;
+        k[e] = ~0ull;
+        v[e] = ~0u;
+        if (x < n) load(x, k[e], v[e]);
+    }
+    wave_sort_regs<E, PAIRS>(k, v);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = lane * E + (uint32_t)e;
+        if (x < n) store(x, k[e], v[e]);
+    }
+}
+
+// n <= 4*WAVE: in registers; returns false (nothing done) for larger n.
+template <bool PAIRS, typename LoadF, typename StoreF>
+__device__ bool wave_sort_n(uint32_t n, LoadF load, StoreF store) {
+    if (n <= WAVE) wave_sort_e<1, PAIRS>(n, load, store);
+    else if (n <= 2 * WAVE) wave_sort_e<2, PAIRS>(n, load, store);
+    else if (n <= 4 * WAVE) wave_sort_e<4, PAIRS>(n, load, store);
+    else return false;
+    return true;
+}
+
 // One wave sorts n (key, val) pairs in LDS, ascending by (key, val).
 template <typename K, typename V>
 __device__ void wave_bitonic(K *k, V *v, uint32_t n) {

@@ -374,29 +374,38 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 
 // Moves the staged k-mers of every chunk without a lane overflow to their
 // final positions (the chunk's scanned offset plus the lane's exclusive
-// prefix); the listed overflow chunks are written by k_sketch<1>.
+// prefix); the listed overflow chunks are written by k_sketch<1>.  Output
+// element t of a chunk is found from the lane prefix (in LDS), so each wave
+// writes its chunk's k-mers contiguously and gathers from the staging slots.
 __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
                                                        const uint64_t *chunk_koff, uint32_t lcap,
                                                        const uint64_t *st_h, const int32_t *st_r,
                                                        const uint8_t *st_s, uint64_t *out_h, int32_t *out_r,
                                                        uint8_t *out_s) {
-    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    __shared__ uint32_t s_pre[NWAVES][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    uint32_t *pre_w = s_pre[w];
     for (uint64_t c = (uint64_t)blockIdx.x * NWAVES + w; c < n_chunks; c += (uint64_t)gridDim.x * NWAVES) {
         const uint32_t cnt = lane_cnt[c * WAVE + lane];
         if (__ballot(cnt > lcap) != 0) continue;
-        const uint32_t pre = wave_incl_scan(cnt) - cnt;
-        uint32_t mx = cnt;
-        for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d));
-        const uint64_t ob = chunk_koff[c] + pre;
-        const uint64_t sb = (uint64_t)c * lcap * WAVE + lane;
-        for (uint32_t i = 0; i < mx; i++) {
-            if (i < cnt) {
-                const uint64_t si = sb + (uint64_t)i * WAVE;
-                out_h[ob + i] = st_h[si];
-                out_r[ob + i] = st_r[si];
-                out_s[ob + i] = st_s[si];
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = __shfl(incl, WAVE - 1);
+        pre_w[lane] = incl - cnt;
+        wave_sync();
+        const uint64_t ob = chunk_koff[c];
+        const uint64_t sb = (uint64_t)c * lcap * WAVE;
+        for (uint32_t t = lane; t < total; t += WAVE) {
+            uint32_t lo = 0, hi = WAVE;  // the last lane whose prefix is <= t
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pre_w[mid] <= t) lo = mid; else hi = mid;
             }
+            const uint64_t si = sb + (uint64_t)(t - pre_w[lo]) * WAVE + lo;
+            out_h[ob + t] = st_h[si];
+            out_r[ob + t] = st_r[si];
+            out_s[ob + t] = st_s[si];
         }
+        wave_sync();  // the prefix slice is rewritten by the wave's next chunk
     }
 }
 
@@ -651,12 +660,18 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)rd;
             continue;
         }
-        for (uint32_t i = lane; i < m; i += WAVE) {
-            kh[i] = A.k_h[o + i];
-            kv[i] = i;
+        // (hash, position) order: in registers up to 4*WAVE k-mers, else in LDS
+        if (!wave_sort_n<true>(
+                m, [&](uint32_t x, uint64_t &key, uint32_t &val) { key = A.k_h[o + x]; val = x; },
+                [&](uint32_t x, uint64_t key, uint32_t val) { kh[x] = key; kv[x] = val; })) {
+            for (uint32_t i = lane; i < m; i += WAVE) {
+                kh[i] = A.k_h[o + i];
+                kv[i] = i;
+            }
+            wave_sync();
+            wave_bitonic(kh, kv, m);
         }
         wave_sync();
-        wave_bitonic(kh, kv, m);
         // runs of equal hash -> seeds in hash order.  A seed id never exceeds
         // the position of its run head, so kv[sid] can take the seed's hit
         // count once this chunk has read its own positions (kv of later chunks
@@ -693,9 +708,15 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         wave_sync();
-        for (uint32_t q = lane; q < u; q += WAVE) kh[q] = ((uint64_t)kv[q] << 32) | q;
+        // rarest-first order: keys (hits << 32 | seed id), distinct
+        if (!wave_sort_n<false>(
+                u, [&](uint32_t x, uint64_t &key, uint32_t &) { key = ((uint64_t)kv[x] << 32) | x; },
+                [&](uint32_t x, uint64_t key, uint32_t) { kh[x] = key; })) {
+            for (uint32_t q = lane; q < u; q += WAVE) kh[q] = ((uint64_t)kv[q] << 32) | q;
+            wave_sync();
+            wave_bitonic_keys(kh, u);
+        }
         wave_sync();
-        wave_bitonic_keys(kh, u);
         // budget: consume seeds while fewer than S occurrences are consumed
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;

@@ -178,7 +178,7 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
     DevBuf<uint64_t> nch, chunk_start, chunk_cnt, chunk_koff, koff, part, stat;  // stat: [0] largest sketch
-    DevBuf<uint32_t> lane_cnt, chunk_seq, ovf;  // ovf: [0] overflow count, then the chunk list
+    DevBuf<uint32_t> lane_cnt, chunk_seq, ovf, ovf_n;  // overflow chunk list and its count
     DevBuf<uint64_t> st_h;                      // lane-strided staging of the rolling pass
     DevBuf<int32_t> st_r;
     DevBuf<uint8_t> st_s;
@@ -199,9 +199,11 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.koff.ensure(n + 1));
     RCHK(so.part.ensure(SCAN_PARTS + 1));
     RCHK(so.h_sum.ensure(3));
+    RCHK(so.ovf_n.ensure(1));
     launch_chunk_counts(S, sk->k, so.nch.p, st);
     launch_scan(so.nch.p, so.chunk_start.p, nullptr, n, so.part.p, st);
     HIPCHK(hipMemcpyAsync(so.h_sum.p, so.chunk_start.p + n, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemsetAsync(so.ovf_n.p, 0, 4, st));  // cleared while the host waits
     HIPCHK(hipStreamSynchronize(st));
     const uint64_t C = so.h_sum.p[0];
     RCHK(so.chunk_cnt.ensure(C + 1));
@@ -214,9 +216,8 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.st_h.ensure(slots + 1));
     RCHK(so.st_r.ensure(slots + 1));
     RCHK(so.st_s.ensure(slots + 1));
-    RCHK(so.ovf.ensure(C + 2));
-    uint32_t *ovf_count = so.ovf.p, *ovf_list = so.ovf.p + 1;
-    HIPCHK(hipMemsetAsync(ovf_count, 0, 4, st));
+    RCHK(so.ovf.ensure(C + 1));
+    uint32_t *ovf_count = so.ovf_n.p, *ovf_list = so.ovf.p;
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
         launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_cnt.p, so.lane_cnt.p,
@@ -835,6 +836,11 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     HIPCHK(hipMemcpyAsync(mp->h_sum.p, mp->grp_base.p + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 1, mp->grp_base.p + (R + 1) + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 2, mp->stat.p, 8, hipMemcpyDeviceToHost, st));
+    // batch counters of k_final, cleared while the host waits
+    RCHK(mp->mapped.ensure((size_t)R + 2));
+    RCHK(mp->u_total.ensure(1));
+    HIPCHK(hipMemsetAsync(mp->mapped.p, 0, 4, st));
+    HIPCHK(hipMemsetAsync(mp->u_total.p, 0, 8, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
     const uint64_t E_all = mp->h_sum.p[0], T_all = mp->h_sum.p[1];
@@ -850,13 +856,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipStreamSynchronize(st));
     }
 
-    RCHK(mp->mapped.ensure((size_t)R + 2));
-    RCHK(mp->u_total.ensure(1));
     A.mapped_count = mp->mapped.p;
     A.mapped_list = mp->mapped.p + 1;
     A.u_total = mp->u_total.p;
-    HIPCHK(hipMemsetAsync(A.mapped_count, 0, 4, st));
-    HIPCHK(hipMemsetAsync(A.u_total, 0, 8, st));
     // bucketing, in groups of reads whose tables fit the budget
     RCHK(mp->task_count.ensure(5 * (size_t)R + 5));  // five refine-task counters per group
     int32_t ngroups = 0;
@@ -912,6 +914,11 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(mp->h_sum.p + 3, mp->grp_base.p + 2 * (size_t)(RG + 1) + RG, 8, hipMemcpyDeviceToHost,
                               st));
+        // the group's list and task counters, cleared while the host waits
+        RCHK(mp->ext.ensure((size_t)RG + 2));
+        uint32_t *task_count = mp->task_count.p + 5 * (size_t)ngroups;
+        HIPCHK(hipMemsetAsync(mp->ext.p, 0, 4, st));
+        HIPCHK(hipMemsetAsync(task_count, 0, 20, st));
         HIPCHK(hipStreamSynchronize(st));
         RCHK(flush_timing(mp, pending));
         const uint64_t B = mp->h_sum.p[3];
@@ -932,13 +939,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->tasks.ensure(B + 1));
-        RCHK(mp->ext.ensure((size_t)RG + 2));
         A.ext_count = mp->ext.p;
         A.ext_list = mp->ext.p + 1;
-        HIPCHK(hipMemsetAsync(A.ext_count, 0, 4, st));
         RCHK(mp->sel.ensure(B + (uint64_t)RG + 1));
-        uint32_t *task_count = mp->task_count.p + 5 * (size_t)ngroups;
-        HIPCHK(hipMemsetAsync(task_count, 0, 20, st));
         A.bk_key = mp->bk_key.p;
         A.bk_val = mp->bk_val.p;
         A.bs_g = mp->bs_g.p;
