@@ -291,23 +291,35 @@ __device__ __forceinline__ void sort_reg_merges(uint64_t (&k)[E], uint32_t (&v)[
 // Cross-lane step: this lane's register e meets register (MIRROR ? E-1-e : e)
 // of the lane whose byte address is pa; lower says whether this lane holds the
 // lower indices.  Elements are distinct or identical, so the partner is taken
-// exactly when (partner < mine) == lower.
+// exactly when (partner < mine) == lower.  Registers go in groups of at most
+// four (a mirror group holds e and E-1-e together, so no group reads a
+// register an earlier group has replaced).
 template <int E, bool PAIRS, bool MIRROR>
 __device__ __forceinline__ void sort_lane_step(uint64_t (&k)[E], uint32_t (&v)[E], int pa, bool lower) {
-    uint32_t pl[E], ph[E], pv[E];
+    constexpr int G = E < 4 ? E : 4;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int q = MIRROR ? E - 1 - e : e;
-        pl[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)k[q]);
-        ph[e] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)(k[q] >> 32));
-        pv[e] = PAIRS ? (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)v[q]) : 0u;
-    }
+    for (int j = 0; j < E / G; j++) {
+        int es[G];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const uint64_t pk = ((uint64_t)ph[e] << 32) | pl[e];
-        const bool take = sort_less<PAIRS>(pk, pv[e], k[e], v[e]) == lower;
-        k[e] = take ? pk : k[e];
-        if (PAIRS) v[e] = take ? pv[e] : v[e];
+        for (int i = 0; i < G; i++)
+            es[i] = !MIRROR ? j * G + i : (i < G / 2 ? j * (G / 2) + i : E - 1 - (j * (G / 2) + i - G / 2));
+        if (MIRROR && G == 1) es[0] = 0;
+        uint32_t pl[G], ph[G], pv[G];
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int q = MIRROR ? E - 1 - es[i] : es[i];
+            pl[i] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)k[q]);
+            ph[i] = (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)(uint32_t)(k[q] >> 32));
+            pv[i] = PAIRS ? (uint32_t)__builtin_amdgcn_ds_bpermute(pa, (int)v[q]) : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int e = es[i];
+            const uint64_t pk = ((uint64_t)ph[i] << 32) | pl[i];
+            const bool take = sort_less<PAIRS>(pk, pv[i], k[e], v[e]) == lower;
+            k[e] = take ? pk : k[e];
+            if (PAIRS) v[e] = take ? pv[i] : v[e];
+        }
     }
 }
 
@@ -347,12 +359,13 @@ __device__ __forceinline__ void wave_sort_e(uint32_t n, LoadF load, StoreF store
     }
 }
 
-// n <= 4*WAVE: in registers; returns false (nothing done) for larger n.
+// n <= 8*WAVE: in registers; returns false (nothing done) for larger n.
 template <bool PAIRS, typename LoadF, typename StoreF>
 __device__ bool wave_sort_n(uint32_t n, LoadF load, StoreF store) {
     if (n <= WAVE) wave_sort_e<1, PAIRS>(n, load, store);
     else if (n <= 2 * WAVE) wave_sort_e<2, PAIRS>(n, load, store);
     else if (n <= 4 * WAVE) wave_sort_e<4, PAIRS>(n, load, store);
+    else if (n <= 8 * WAVE) wave_sort_e<8, PAIRS>(n, load, store);
     else return false;
     return true;
 }

@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)rd;
             continue;
         }
-        // (hash, position) order: in registers up to 4*WAVE k-mers, else in LDS
+        // (hash, position) order: in registers up to 8*WAVE k-mers, else in LDS
         if (!wave_sort_n<true>(
                 m, [&](uint32_t x, uint64_t &key, uint32_t &val) { key = A.k_h[o + x]; val = x; },
                 [&](uint32_t x, uint64_t key, uint32_t val) { kh[x] = key; kv[x] = val; })) {
