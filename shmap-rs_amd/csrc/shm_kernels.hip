@@ -1710,10 +1710,22 @@ __device__ __forceinline__ double overlap_of(int32_t sa, int32_t la, int32_t ra,
 // match_rest, both passes (src/shmap/scoring.rs:367-489 called from
 // src/shmap/mod.rs:809-861), over the precomputed survival and scores; the
 // refine memo's counters follow :38-109 (a Containment/Jaccard bucket refined in
-// pass 1 is a memo hit in pass 2).
+// pass 1 is a memo hit in pass 2).  One wave per read.  Each pass is a scan in
+// sorted order whose threshold rises only when a bucket is taken (its score
+// exceeds the threshold -- and, in pass 2, it overlaps the best mapping by less
+// than max_overlap -- so the taken score is above every earlier one): a chunk
+// of 64 buckets is tested against the current threshold, the first taken lane
+// settles every lane up to it, and the lanes after it are tested again with
+// the raised threshold.  The result is staged in LDS and written by the wave.
 __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
-    uint64_t u_sum = 0;  // distinct k-mers of this thread's reads
-    for (int32_t lr = blockIdx.x * WG + threadIdx.x; lr < A.RG; lr += gridDim.x * WG) {
+    static_assert(sizeof(shm_read_result) % 4 == 0 && sizeof(shm_read_result) / 4 <= WAVE, "result layout");
+    constexpr uint32_t RES_DW = sizeof(shm_read_result) / 4;
+    __shared__ shm_read_result s_res[NWAVES];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const bool nbp = A.no_bucket_pruning;
+    const bool memoizable = A.metric == METRIC_CONTAINMENT || A.metric == METRIC_JACCARD;
+    uint64_t u_sum = 0;  // distinct k-mers of this wave's reads (lane 0)
+    for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
         const int32_t rd = A.r0 + lr;
         shm_read_result r = {};
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
@@ -1729,92 +1741,125 @@ __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
         r.J2 = -1.0;
         if (!A.rd_ok[rd]) {
             r.unmappable_short = 1;
-            out[rd] = r;
-            continue;
-        }
-        r.seed_matches = A.rd_seed_matches[rd];
-        r.total_matches = A.rd_seed_matches[rd];
-        r.max_seed_matches = A.rd_max_seed[rd];
-        const int32_t nb = (int32_t)A.grp_nb[lr];
-        const int32_t hl = A.rd_hl[rd];
-        r.seeded_buckets = nb;
-        const bool nbp = A.no_bucket_pruning;
-        const bool memoizable = A.metric == METRIC_CONTAINMENT || A.metric == METRIC_JACCARD;
-        int32_t refined = 0, memo = 0, fin = 0;
-        double thr = A.theta;
-        int32_t best = -1;
-        for (int32_t p = 0; p < nb; p++) {
-            if (!A.bs_alive[base + p]) continue;
-            if (!nbp && A.bs_sh[base + p] < thr) continue;
-            refined++;
-            A.bs_alive[base + p] = 2;  // refined in pass 1: a memo entry
-            const double sc = A.sc_j[base + p];
-            if (sc > thr) {
-                fin++;
-                if (best < 0 || sc > A.sc_j[base + best]) {
-                    thr = sc;
-                    best = p;
+        } else {
+            r.seed_matches = A.rd_seed_matches[rd];
+            r.total_matches = A.rd_seed_matches[rd];
+            r.max_seed_matches = A.rd_max_seed[rd];
+            const int32_t nb = (int32_t)A.grp_nb[lr];
+            const int32_t hl = A.rd_hl[rd];
+            r.seeded_buckets = nb;
+            int32_t refined = 0, memo = 0, fin = 0;
+            // pass 1: threshold theta, then the best score so far
+            double thr = A.theta;
+            int32_t best = -1;
+            for (int32_t c0 = 0; c0 < nb; c0 += WAVE) {
+                const int32_t p = c0 + (int32_t)lane;
+                uint8_t al = 0;
+                double sh = 0.0, sc = 0.0;
+                if (p < nb) {
+                    al = A.bs_alive[base + p];
+                    if (al) {
+                        sh = A.bs_sh[base + p];
+                        sc = A.sc_j[base + p];
+                    }
+                }
+                uint32_t start = 0;
+                for (;;) {
+                    const bool ref = al && lane >= start && (nbp || !(sh < thr));
+                    const uint64_t am = __ballot(ref && sc > thr), rm = __ballot(ref);
+                    const uint32_t f = am ? (uint32_t)__builtin_ctzll(am) : (uint32_t)WAVE;
+                    const uint64_t upto = f >= WAVE - 1 ? ~0ull : ((2ull << f) - 1ull);
+                    refined += __popcll(rm & upto);
+                    if (ref && lane <= f) A.bs_alive[base + p] = 2;  // refined in pass 1: a memo entry
+                    if (!am) break;
+                    fin++;
+                    thr = __shfl(sc, (int)f);
+                    best = c0 + (int32_t)f;
+                    start = f + 1;
+                    if (start >= WAVE) break;
                 }
             }
-        }
-        int32_t best2 = -1;
-        if (best >= 0) {
-            double thr2 = A.sc_j[base + best] * (1.0 - A.min_diff);
-            const int32_t fs = A.sc_seg[base + best], fl = A.sc_tl[base + best], fr = A.sc_tr[base + best];
-            for (int32_t p = 0; p < nb; p++) {
-                const uint8_t al = A.bs_alive[base + p];
-                if (!al) continue;
-                if (!nbp && A.bs_sh[base + p] < thr2) continue;
-                if (memoizable && al == 2) memo++; else refined++;
-                const double sc = A.sc_j[base + p];
-                if (sc > thr2) {
-                    fin++;
-                    const double ov =
-                        overlap_of(A.sc_seg[base + p], A.sc_tl[base + p], A.sc_tr[base + p], fs, fl, fr);
-                    if (ov < A.max_overlap && (best2 < 0 || sc > A.sc_j[base + best2])) {
-                        if (sc > thr2) thr2 = sc;
-                        best2 = p;
+            int32_t best2 = -1;
+            if (best >= 0) {
+                const uint64_t bb = base + best;
+                double thr2 = A.sc_j[bb] * (1.0 - A.min_diff);
+                const double thr2_0 = thr2;
+                const int32_t fs = A.sc_seg[bb], fl = A.sc_tl[bb], fr = A.sc_tr[bb];
+                for (int32_t c0 = 0; c0 < nb; c0 += WAVE) {
+                    const int32_t p = c0 + (int32_t)lane;
+                    uint8_t al = 0;
+                    double sh = 0.0, sc = 0.0, ov = 1.0;
+                    if (p < nb) {
+                        al = A.bs_alive[base + p];
+                        if (al) {
+                            sh = A.bs_sh[base + p];
+                            sc = A.sc_j[base + p];
+                            if (sc > thr2_0)  // the threshold only rises: others never score above it
+                                ov = overlap_of(A.sc_seg[base + p], A.sc_tl[base + p], A.sc_tr[base + p], fs, fl,
+                                                fr);
+                        }
+                    }
+                    const bool mh = memoizable && al == 2;
+                    uint32_t start = 0;
+                    for (;;) {
+                        const bool ref = al && lane >= start && (nbp || !(sh < thr2));
+                        const bool gt = ref && sc > thr2;
+                        const uint64_t am = __ballot(gt && ov < A.max_overlap);
+                        const uint32_t f = am ? (uint32_t)__builtin_ctzll(am) : (uint32_t)WAVE;
+                        const uint64_t upto = f >= WAVE - 1 ? ~0ull : ((2ull << f) - 1ull);
+                        memo += __popcll(__ballot(ref && mh) & upto);
+                        refined += __popcll(__ballot(ref && !mh) & upto);
+                        fin += __popcll(__ballot(gt) & upto);
+                        if (!am) break;
+                        thr2 = __shfl(sc, (int)f);
+                        best2 = c0 + (int32_t)f;
+                        start = f + 1;
+                        if (start >= WAVE) break;
                     }
                 }
             }
-        }
-        r.refined_buckets = refined;
-        r.refine_memo_hits = memo;
-        r.final_buckets = fin;
-        if (best >= 0) {
-            const uint64_t bp = base + best;
-            uint32_t seg;
-            int32_t b;
-            r.mapped = 1;
-            r.p_start = 0;
-            r.p_end = A.sc_pend[bp];
-            r.strand = A.sc_same[bp] > 0 ? '+' : '-';
-            r.segm_id = A.sc_seg[bp];
-            r.t_l = A.sc_tl[bp];
-            r.t_r = A.sc_tr[bp];
-            r.s_sz = A.sc_ssz[bp];
-            r.intersection = A.sc_i[bp];
-            r.same_strand = A.sc_same[bp];
-            r.J = A.sc_j[bp];
-            r.sh = nbp ? 1.0 : A.bs_sh[bp];
-            decode_bucket(A, A.bs_g[bp], hl, seg, b);
-            r.b_segm = (int32_t)seg;
-            r.b_b = b;
-            if (best2 >= 0) {
-                const uint64_t bq = base + best2;
-                r.has_second = 1;
-                r.intersection2 = A.sc_i[bq];
-                r.J2 = A.sc_j[bq];
-                decode_bucket(A, A.bs_g[bq], hl, seg, b);
-                r.b2_segm = (int32_t)seg;
-                r.b2_b = b;
+            r.refined_buckets = refined;
+            r.refine_memo_hits = memo;
+            r.final_buckets = fin;
+            if (best >= 0) {
+                const uint64_t bp = base + best;
+                uint32_t seg;
+                int32_t b;
+                r.mapped = 1;
+                r.p_start = 0;
+                r.p_end = A.sc_pend[bp];
+                r.strand = A.sc_same[bp] > 0 ? '+' : '-';
+                r.segm_id = A.sc_seg[bp];
+                r.t_l = A.sc_tl[bp];
+                r.t_r = A.sc_tr[bp];
+                r.s_sz = A.sc_ssz[bp];
+                r.intersection = A.sc_i[bp];
+                r.same_strand = A.sc_same[bp];
+                r.J = A.sc_j[bp];
+                r.sh = nbp ? 1.0 : A.bs_sh[bp];
+                decode_bucket(A, A.bs_g[bp], hl, seg, b);
+                r.b_segm = (int32_t)seg;
+                r.b_b = b;
+                if (best2 >= 0) {
+                    const uint64_t bq = base + best2;
+                    r.has_second = 1;
+                    r.intersection2 = A.sc_i[bq];
+                    r.J2 = A.sc_j[bq];
+                    decode_bucket(A, A.bs_g[bq], hl, seg, b);
+                    r.b2_segm = (int32_t)seg;
+                    r.b2_b = b;
+                }
             }
         }
-        if (r.mapped) A.mapped_list[atomicAdd(A.mapped_count, 1u)] = (uint32_t)rd;
-        out[rd] = r;
+        if (lane == 0) {
+            s_res[w] = r;
+            if (r.mapped) A.mapped_list[atomicAdd(A.mapped_count, 1u)] = (uint32_t)rd;
+        }
+        wave_sync();
+        if (lane < RES_DW) ((uint32_t *)(out + rd))[lane] = ((const uint32_t *)&s_res[w])[lane];
+        wave_sync();  // the slot is rewritten for the wave's next read
     }
-    for (int d = WAVE / 2; d > 0; d >>= 1) u_sum += __shfl_xor(u_sum, d);
-    if ((threadIdx.x & (WAVE - 1)) == 0 && u_sum) atomicAdd((unsigned long long *)A.u_total, (unsigned long long)u_sum);
+    if (lane == 0 && u_sum) atomicAdd((unsigned long long *)A.u_total, (unsigned long long)u_sum);
 }
 
 }  // namespace shm
@@ -2007,7 +2052,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
         }
         break;
     }
-    case ST_FINAL: k_final<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A, d_out); break;
+    case ST_FINAL: k_final<<<read_grid((A.RG + NWAVES - 1) / NWAVES, max_grid), WG, 0, st>>>(A, d_out); break;
     default: break;
     }
 #undef PGRID
