@@ -180,8 +180,9 @@ const char *shm_mapper_stage_name(int32_t stage);
  * distinct k-mers (seeds), [2] buckets counted, [3] buckets refined (a superset
  * of what the match_rest passes read), [4] read groups (table-budget batches),
  * [5] first-pass candidates (bound at least theta after the full-extension
- * prune), [6] buckets only a second pass can reach.  Writes up to n values and
- * their number to *n_written. */
+ * prune), [6] buckets only a second pass can reach, [7] examined hits (the
+ * index hits of every consumed seed).  Writes up to n values and their number
+ * to *n_written. */
 int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */

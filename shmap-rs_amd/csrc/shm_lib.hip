@@ -559,7 +559,7 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "seed_wg", "tab_init",
                                             "contrib", "extract",        "prune",     "refine",  "final"};
 
-constexpr int SHM_N_COUNTS = 7;  // see shm_mapper_last_counts
+constexpr int SHM_N_COUNTS = 8;  // see shm_mapper_last_counts
 
 struct shm_mapper {
     const shm_index *idx = nullptr;
@@ -1013,6 +1013,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     mp->counts[4] = ngroups;
     mp->counts[5] = cand;
     mp->counts[6] = deferred;
+    mp->counts[7] = (int64_t)E_all;
     return SHM_OK;
 }
 

@@ -381,7 +381,8 @@ def host_array(n: int, dtype) -> np.ndarray:
     return np.frombuffer(raw, dtype=dt, count=n)
 
 
-COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band")
+COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band",
+               "examined_hits")
 
 
 def format_paf(index: SketchIndex, params: MapParams, qid: str, rec, map_time: float = 0.0) -> str:
