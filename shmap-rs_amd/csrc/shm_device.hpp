@@ -102,6 +102,47 @@ __device__ __forceinline__ uint32_t index_lookup(const DevIndex &I, uint64_t h, 
     }
 }
 
+// index_lookup of up to four hashes per lane (act[k]) probed in lockstep, so
+// the table loads of all four are in flight together.
+__device__ __forceinline__ void index_lookup4(const DevIndex &I, const uint64_t (&h)[4], const bool (&act)[4],
+                                              uint32_t (&cnt)[4], uint32_t (&off)[4]) {
+    const uint4 *T[4];
+    uint32_t pos[4], mask[4];
+    bool live[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t sh = (uint32_t)(h[k] & 7u);
+        const uint32_t bits = I.shard_bits[sh];
+        mask[k] = (1u << bits) - 1u;
+        T[k] = I.table + I.shard_base[sh];
+        pos[k] = table_pos(h[k], bits);
+        live[k] = act[k];
+        cnt[k] = 0;
+        off[k] = 0;
+    }
+    for (;;) {
+        uint4 s[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = live[k] ? T[k][pos[k]] : make_uint4(0, 0, 0, 0);
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!live[k]) continue;
+            if (s[k].w == 0) {
+                live[k] = false;
+            } else if (s[k].x == (uint32_t)h[k] && s[k].y == (uint32_t)(h[k] >> 32)) {
+                cnt[k] = s[k].w;
+                off[k] = s[k].z;
+                live[k] = false;
+            } else {
+                pos[k] = (pos[k] + 1u) & mask[k];
+                any = true;
+            }
+        }
+        if (!any) break;
+    }
+}
+
 // ---- wave / workgroup scans --------------------------------------------------
 
 // Orders the wave's LDS accesses around a point every lane reaches.

@@ -673,38 +673,61 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
         }
         wave_sync();
         // runs of equal hash -> seeds in hash order.  A seed id never exceeds
-        // the position of its run head, so kv[sid] can take the seed's hit
-        // count once this chunk has read its own positions (kv of later chunks
-        // is untouched).
-        uint32_t u = 0, pm = 0;
+        // the position of its run head (and equals it only while every earlier
+        // element heads its own run, when kh[sid] already holds the hash), so
+        // kh[sid] can take the seed's hash once this chunk has read its own
+        // hashes: later chunks read only kh from the chunk's last element on.
+        uint32_t u = 0;
         for (uint32_t b0 = 0; b0 < m; b0 += WAVE) {
             const uint32_t i = b0 + lane;
             const uint32_t pos = i < m ? kv[i] : 0u;
             const bool head = i < m && (i == 0 || kh[i] != kh[i - 1]);
             const uint64_t bal = __ballot(head);
             if (i < m) A.ps_v[o + i] = pos;
-            uint32_t sid = 0, cnt = 0, e = 0;
+            uint32_t sid = 0, e = 0;
+            uint64_t h = 0;
             if (head) {
                 sid = u + __popcll(bal & lt);
-                const uint64_t h = kh[i];
+                h = kh[i];
                 e = i + 1;
                 while (e < m && kh[e] == h) e++;
-                uint32_t hoff;
-                cnt = index_lookup(I, h, hoff);
                 A.sd_h[o + sid] = h;
                 A.sd_first[o + sid] = i;
                 A.sd_occs[o + sid] = e - i;
                 A.sd_strand[o + sid] = A.k_s[o + pos];
-                A.sd_cnt[o + sid] = cnt;
-                A.sd_hoff[o + sid] = hoff;
-                pm += cnt;
             }
             wave_sync();  // every lane has read kv[b0, b0+64) and kh
             if (head) {
-                kv[sid] = cnt;
+                kh[sid] = h;
                 so[sid] = (uint16_t)(e - i);
             }
             u += __popcll(bal);
+        }
+        wave_sync();
+        // index probes of every seed, four per lane in flight together; kv[sid]
+        // takes the seed's hit count (kv is no longer read as positions)
+        uint32_t pm = 0;
+        for (uint32_t q0 = 0; q0 < u; q0 += 4 * WAVE) {
+            uint64_t hq[4];
+            bool act[4];
+            uint32_t cnt[4], hoff[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t q = q0 + (uint32_t)t * WAVE + lane;
+                act[t] = q < u;
+                hq[t] = act[t] ? kh[q] : 0ull;
+            }
+            index_lookup4(I, hq, act, cnt, hoff);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t q = q0 + (uint32_t)t * WAVE + lane;
+                if (act[t]) {
+                    A.sd_cnt[o + q] = cnt[t];
+                    A.sd_hoff[o + q] = hoff[t];
+                    kv[q] = cnt[t];
+                    pm += cnt[t];
+                }
+            }
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         wave_sync();
