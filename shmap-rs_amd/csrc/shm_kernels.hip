@@ -852,20 +852,33 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
         }
         const uint32_t u = s_u;
         uint32_t pm = 0;
-        for (uint32_t sid = threadIdx.x; sid < u; sid += WG) {
-            const uint32_t first = A.sd_first[o + sid];
-            const uint32_t next = sid + 1 < u ? A.sd_first[o + sid + 1] : m;
-            const uint64_t h = A.ps_h[o + first];
-            uint32_t hoff;
-            const uint32_t cnt = index_lookup(I, h, hoff);
-            A.sd_h[o + sid] = h;
-            A.sd_occs[o + sid] = next - first;
-            A.sd_strand[o + sid] = A.k_s[o + A.ps_v[o + first]];
-            A.sd_cnt[o + sid] = cnt;
-            A.sd_hoff[o + sid] = hoff;
-            A.sd_key[o + sid] = ((uint64_t)cnt << 32) | sid;
-            A.sd_ord[o + sid] = sid;
-            pm += cnt;
+        // one probe per seed, four per thread in flight together
+        for (uint32_t s0 = 0; s0 < u; s0 += 4 * WG) {
+            uint64_t hq[4];
+            bool act[4];
+            uint32_t cnt[4], hoff[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t sid = s0 + (uint32_t)t * WG + threadIdx.x;
+                act[t] = sid < u;
+                hq[t] = act[t] ? A.ps_h[o + A.sd_first[o + sid]] : 0ull;
+            }
+            index_lookup4(I, hq, act, cnt, hoff);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t sid = s0 + (uint32_t)t * WG + threadIdx.x;
+                if (!act[t]) continue;
+                const uint32_t first = A.sd_first[o + sid];
+                const uint32_t next = sid + 1 < u ? A.sd_first[o + sid + 1] : m;
+                A.sd_h[o + sid] = hq[t];
+                A.sd_occs[o + sid] = next - first;
+                A.sd_strand[o + sid] = A.k_s[o + A.ps_v[o + first]];
+                A.sd_cnt[o + sid] = cnt[t];
+                A.sd_hoff[o + sid] = hoff[t];
+                A.sd_key[o + sid] = ((uint64_t)cnt[t] << 32) | sid;
+                A.sd_ord[o + sid] = sid;
+                pm += cnt[t];
+            }
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_pm, pm);
