@@ -271,6 +271,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--no-metrics", action="store_true", help="skip the other scoring metrics' timings")
+    ap.add_argument("--all-metrics", action="store_true",
+                    help="time the other scoring metrics on any workload (default: genome only)")
     ap.add_argument("--serve-reads", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=1_000_000.0, help="offered reads/s (Poisson arrivals)")
     ap.add_argument("--serve-workers", type=int, default=2)
@@ -367,6 +370,39 @@ def main():
     stage_rates = {s: (round(sb[s] / (stage_ms[s] * 1e-3) / 1e9, 2) if s in sb and stage_ms[s] > 0 else None)
                    for s in stage_ms}
     log("[bench] stage ms: " + ", ".join(f"{s}={v:.2f}" for s, v in stage_ms.items()))
+
+    # the same batch under each of the other scoring metrics (two timed steps each)
+    per_metric = {args.metric: {"reads_per_s": round(value, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+                                "mapped_reads": n_mapped}}
+    # (the genome workload's configuration names all four; elsewhere on request)
+    if not args.no_metrics and (args.workload == "genome" or args.all_metrics):
+        for mname in ("Containment", "Jaccard", "bucket_SH", "bucket_LCS"):
+            if mname == args.metric:
+                continue
+            pm = G.Mapper(idx, G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o,
+                                             metric=mname))
+            rm = pm.map_device(d_bases, d_off, n_reads, out=out)
+            hip.sync()
+            log(f"[bench] {mname}: warmup step done")
+            tm = time.time()
+            for _ in range(2):
+                rm = pm.map_device(d_bases, d_off, n_reads, out=out)
+                log(f"[bench] {mname}: step done")
+            hip.sync()
+            ms = (time.time() - tm) / 2 * 1e3
+            per_metric[mname] = {"reads_per_s": round(n_reads / ms * 1e3, 2), "ms_per_step": round(ms, 3),
+                                 "mapped_reads": int(rm["mapped"].sum())}
+            del pm
+        log(f"[bench] per metric: {per_metric}")
+        res = mp.map_device(d_bases, d_off, n_reads, out=out)  # the headline metric's records again
+
+    # placement: mapped reads against the generator's true origin (contig, start, end)
+    mk = res["mapped"] == 1
+    hit = mk & (res["segm_id"] == truth[:n_reads, 0]) & (res["t_l"] < truth[:n_reads, 2]) & \
+        (res["t_r"] > truth[:n_reads, 1])
+    placement = {"mapped": int(mk.sum()), "overlapping_true_origin": int(hit.sum()),
+                 "frac": round(float(hit.sum()) / max(int(mk.sum()), 1), 5)}
+    log(f"[bench] placement: {placement}")
 
     # plain form: one read per launch
     n_plain = min(args.plain_reads, n_reads)
@@ -468,6 +504,8 @@ def main():
                       "same_records": plain_same},
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
         "pcie_inclusive": pcie,
+        "per_metric": per_metric,
+        "placement": placement,
         "serving": serving,
         "path_roofline": path_roof,
         "index_build_s": round(t_index, 3),
