@@ -1522,6 +1522,71 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
     }
 }
 
+// bucket_LCS (src/shmap/scoring.rs:114-170) for one wave: the longest strictly
+// increasing subsequence of the pmatches values of span positions [0, n) --
+// for each matched position in order, the read positions of its seed's
+// occurrences, latest first -- and of the reversed sequence, by patience
+// sorting (each value replaces the first tail at or above it, or is appended).
+// The wave walks 64 positions at a time: a scan of their occurrence counts
+// places each of the next 64 values on a lane, which loads it, and the values
+// are then taken one by one.  Up to LCS_TAILS tails live across the wave's
+// registers (tail i in register i / 64 of lane i % 64); a value's place is
+// the number of smaller tails, one ballot per 64 of them.  Returns -1 when the
+// read's sketch (which bounds the tails) is larger.
+constexpr int32_t LCS_TAILS = 32 * WAVE;
+__device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m) {
+    if (m > LCS_TAILS) return -1;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    int32_t best = 0;
+    for (int dir = 0; dir < 2; dir++) {
+        int32_t t[LCS_TAILS / WAVE];
+#pragma unroll
+        for (int r = 0; r < LCS_TAILS / WAVE; r++) t[r] = 0;
+        int32_t nt = 0;
+        for (int64_t g0 = 0; g0 < n; g0 += WAVE) {
+            const int64_t gi = g0 + (int64_t)lane;
+            const int64_t x = dir == 0 ? gi : n - 1 - gi;
+            uint32_t c = 0, first = 0;
+            if (gi < n) {
+                const int32_t v = sid[x];
+                if (v >= 0) {
+                    const int32_t sd = v & 0x3fffffff;
+                    c = A.sd_occs[o + sd];
+                    first = A.sd_first[o + sd];
+                }
+            }
+            const uint32_t incl = wave_incl_scan(c);
+            const uint32_t T = __shfl(incl, WAVE - 1);
+            for (uint32_t v0 = 0; v0 < T; v0 += WAVE) {
+                // value vi belongs to the first lane a with incl > vi (found by
+                // binary lifting, every lane shuffling in step; idle lanes
+                // search for the last value)
+                const uint32_t vi = v0 + lane;
+                const uint32_t vs = vi < T ? vi : T - 1;
+                uint32_t a = 0;
+#pragma unroll
+                for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
+                    if ((uint32_t)__shfl(incl, (int)(a + step - 1)) <= vs) a += step;
+                const uint32_t ca = __shfl(c, (int)a), fa = __shfl(first, (int)a);
+                const uint32_t q = vs - ((uint32_t)__shfl(incl, (int)a) - ca);
+                int32_t val = 0;
+                if (vi < T) val = A.k_r[o + A.ps_v[o + (dir == 0 ? fa + ca - 1 - q : fa + q)]];
+                const uint32_t cnt = T - v0 < (uint32_t)WAVE ? T - v0 : (uint32_t)WAVE;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    const int32_t v = __builtin_amdgcn_readlane(val, (int)j);
+                    uint32_t pos = 0;
+                    for (int32_t r = 0; r * WAVE < nt; r++)
+                        pos += __popcll(__ballot(r * WAVE + (int32_t)lane < nt && t[r] < v));
+                    if (pos == (uint32_t)nt) nt++;
+                    if (lane == pos % WAVE) t[pos / WAVE] = v;
+                }
+            }
+        }
+        best = nt > best ? nt : best;
+    }
+    return best;
+}
+
 // find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket a
 // refinement round selected, one workgroup per bucket.
 //
@@ -1552,6 +1617,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     __shared__ uint8_t s_str[WG];
     __shared__ double s_bj[NWAVES];
     __shared__ int32_t s_bl[NWAVES];
+    __shared__ int32_t s_lcs;
     const int ap = A.abs_pos;
     // Rounds (see the task-list comment of MapArgs):
     //   0  each read's first first-pass candidate in sorted order -- the first
@@ -1726,8 +1792,16 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         // sequential forms: LCS, abs_pos windows, or spans too large for LDS
         int32_t *tr_arr = keys;
         int32_t *diff = u <= REFINE_CAP ? l_c : A.g_diff + (uint64_t)blockIdx.x * A.diff_stride;
-        for (int64_t x = threadIdx.x; x < n; x += WG) tr_arr[x] = I.t_r[ks + l0 + x];
-        for (int32_t x = threadIdx.x; x < u; x += WG) diff[x] = (int32_t)A.sd_occs[o + x];
+        if (A.metric == METRIC_BUCKET_LCS) {
+            // one wave streams the LIS (s_lcs < 0: thread 0 sweeps it below)
+            if (threadIdx.x < WAVE) {
+                const int32_t best = wave_lcs(A, sid, n, o, m);
+                if (threadIdx.x == 0) s_lcs = best;
+            }
+        } else {
+            for (int64_t x = threadIdx.x; x < n; x += WG) tr_arr[x] = I.t_r[ks + l0 + x];
+            for (int32_t x = threadIdx.x; x < u; x += WG) diff[x] = (int32_t)A.sd_occs[o + x];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             double J = -1.0;
@@ -1736,8 +1810,8 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                 // Patience LIS of the pmatches sequence and of its reverse; the
                 // tails hold distinct read positions, so at most m of them.
                 int32_t *tails = A.g_tails + (uint64_t)blockIdx.x * A.diff_stride;
-                int32_t best = 0;
-                for (int dir = 0; dir < 2; dir++) {
+                int32_t best = s_lcs > 0 ? s_lcs : 0;
+                for (int dir = 0; s_lcs < 0 && dir < 2; dir++) {
                     int32_t nt = 0;
                     for (int64_t ii = 0; ii < n; ii++) {
                         const int64_t x = dir == 0 ? ii : n - 1 - ii;

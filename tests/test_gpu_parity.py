@@ -123,6 +123,21 @@ def test_map_matches_oracle(wl, metric):
             assert a == b
 
 
+@pytest.mark.parametrize("h_frac", [0.05, 0.25])
+def test_lcs_long_sketches_match_oracle(h_frac):
+    """bucket_LCS on 20 kb reads: value sequences of about 1,000 (r=0.05) and
+    5,000 (r=0.25) per bucket, beyond the register-resident tails and, at
+    r=0.25, beyond the LDS slice."""
+    segs, reads = _workload(seed=41, ref_len=300_000, n_reads=16, read_len=20_000, err=((0.001, 0.6, 0.2, 0.2),))
+    prm = dict(theta=0.4, min_diff=0.075, max_overlap=0.3, metric="bucket_LCS")
+    oi, op, ores = P.oracle_map(segs, reads, 15, h_frac, **prm)
+    gi, gp, mp, gres = P.gpu_map(segs, reads, 15, h_frac, **prm)
+    assert ores["mapped"].sum() > 0
+    assert int(ores["m"].max()) > (4096 if h_frac > 0.1 else 512)
+    bad = P.diff_results(ores, gres)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("variant", [dict(no_bucket_pruning=True), dict(one_sweep=True),
                                      dict(no_bucket_pruning=True, one_sweep=True), dict(abs_pos=True)])
 def test_variants_match_oracle(variant):
