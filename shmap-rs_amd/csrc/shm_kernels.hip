@@ -1530,9 +1530,11 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // The wave walks 64 positions at a time: a scan of their occurrence counts
 // places each of the next 64 values on a lane, which loads it, and the values
 // are then taken one by one.  Up to LCS_TAILS tails live across the wave's
-// registers (tail i in register i / 64 of lane i % 64); a value's place is
-// the number of smaller tails, one ballot per 64 of them.  Returns -1 when the
-// read's sketch (which bounds the tails) is larger.
+// registers (tail i in register i / 64 of lane i % 64, and each chunk's first
+// tail in one more register, lane = chunk); a value's place is the number of
+// smaller tails: one ballot over the chunk heads picks the chunk, one more
+// counts inside it.  Returns -1 when the read's sketch (which bounds the
+// tails) is larger.
 constexpr int32_t LCS_TAILS = 32 * WAVE;
 __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m) {
     if (m > LCS_TAILS) return -1;
@@ -1542,6 +1544,7 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
         int32_t t[LCS_TAILS / WAVE];
 #pragma unroll
         for (int r = 0; r < LCS_TAILS / WAVE; r++) t[r] = 0;
+        int32_t head = INT32_MAX;  // lane r: the first tail of chunk r (tails 64r ..), INT32_MAX if none
         int32_t nt = 0;
         for (int64_t g0 = 0; g0 < n; g0 += WAVE) {
             const int64_t gi = g0 + (int64_t)lane;
@@ -1574,11 +1577,18 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
                 const uint32_t cnt = T - v0 < (uint32_t)WAVE ? T - v0 : (uint32_t)WAVE;
                 for (uint32_t j = 0; j < cnt; j++) {
                     const int32_t v = __builtin_amdgcn_readlane(val, (int)j);
+                    // chunks whose first tail is below v, then the tails below v
+                    // in the last of them
+                    const uint32_t k = (uint32_t)__popcll(__ballot(head < v));
                     uint32_t pos = 0;
-                    for (int32_t r = 0; r * WAVE < nt; r++)
-                        pos += __popcll(__ballot(r * WAVE + (int32_t)lane < nt && t[r] < v));
+                    if (k > 0) {
+                        const int32_t ck = (int32_t)k - 1;
+                        pos = (uint32_t)ck * WAVE +
+                              (uint32_t)__popcll(__ballot(ck * WAVE + (int32_t)lane < nt && t[ck] < v));
+                    }
                     if (pos == (uint32_t)nt) nt++;
                     if (lane == pos % WAVE) t[pos / WAVE] = v;
+                    if (pos % WAVE == 0 && lane == pos / WAVE) head = v;
                 }
             }
         }
