@@ -995,26 +995,6 @@ __device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t
     return lo;
 }
 
-// hits_lower_bound over [lo, hi) when hit lo is known to lie before (seg, pos):
-// runs are mostly short, so the next four hits are tested with independent
-// loads first and the binary search covers only what lies beyond them.
-__device__ __forceinline__ uint32_t hits_run_end(const DevIndex &I, uint32_t lo, uint32_t hi, uint32_t seg,
-                                                 int64_t pos, int ap) {
-    uint32_t bef = 0;  // bit t: hit lo+1+t lies before (seg, pos)
-#pragma unroll
-    for (uint32_t t = 0; t < 4; t++) {
-        const uint32_t i = lo + 1 + t;
-        if (i < hi) {
-            const uint32_t s = I.hit_seg[i];
-            const bool before = s != seg ? s < seg : (int64_t)hit_pos(I, i, s, ap) < pos;
-            bef |= (uint32_t)before << t;
-        }
-    }
-    const uint32_t run = (uint32_t)__builtin_ctz(~bef);  // leading hits before the bound
-    if (run < 4 || lo + 5 >= hi) return lo + 1 + run < hi ? lo + 1 + run : hi;
-    return hits_lower_bound(I, lo + 5, hi, seg, pos, ap);
-}
-
 // Adds one (seed, bucket) contribution to read lr's bucket table; the insert
 // that claims an empty slot counts the read's new bucket and lists its slot.
 __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t base, uint64_t cap, uint32_t g,
@@ -1043,75 +1023,59 @@ __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t 
 // so the first hit of each (segment, half-bucket) run emits the clamped
 // per-(seed, bucket) contribution of bucket b (runs b and b+1) and -- unless the
 // previous run, at b-1, already did -- of bucket b-1 (run b alone).  Counts come
-// from the run ends, strand agreement from the strand prefix.
-//
-// The contributions of run head idx (hit of a seed with hits [hoff, hoff+cnt))
-// of group read lr, in half-bucket b of segment seg.
-__device__ __forceinline__ void contrib_head(const MapArgs &A, const DevIndex &I, uint32_t lr, int32_t rd,
-                                             uint32_t idx, uint32_t seg, int32_t b, bool prev_bm1, uint32_t end,
-                                             uint32_t occs, uint32_t sstrand, int32_t hl, int ap) {
-    const uint64_t tbase = A.tab_base[lr], tcap = A.rd_cap[rd];
-    const int32_t rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
-    for (int side = 0; side < 2; side++) {
-        if (side == 1 && (b == 0 || prev_bm1)) break;
-        const int32_t bb = side == 0 ? b : b - 1;
-        const uint32_t e = hits_run_end(I, idx, end, seg, (int64_t)(bb + 2) * hl, ap);
-        const uint32_t n = e - idx;
-        const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx];
-        const uint32_t same = sstrand ? ones : n - ones;
-        tab_add(A, lr, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
-                (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
-    }
-}
-
-// Consumed seed q of read rd (slot o + q) holding the read's examined hit `local`.
-__device__ __forceinline__ uint32_t seed_of_hit(const MapArgs &A, uint64_t o, int32_t rd, uint32_t local) {
-    uint32_t lo = 0, hi = (uint32_t)A.rd_c[rd];
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (A.sd_epre[o + mid] <= local) lo = mid + 1; else hi = mid;
-    }
-    return lo - 1;
-}
-
-// One examined hit f of group read lr, located on its own.
+// from binary searches, strand agreement from the strand prefix.  This is the
+// work of one examined hit f of group read lr.
 __device__ __forceinline__ void contrib_hit(const MapArgs &A, const DevIndex &I, uint64_t f, uint32_t lr, int ap) {
-    const int32_t rd = A.r0 + (int32_t)lr;
-    const uint32_t local = (uint32_t)(f - A.hit_base[lr]);
-    const uint64_t o = A.koff[rd];
-    const uint32_t q = seed_of_hit(A, o, rd, local);
-    const uint32_t sid = A.sd_ord[o + q];
-    const uint32_t j = local - A.sd_epre[o + q];
-    const uint32_t hoff = A.sd_hoff[o + sid], cnt = A.sd_cnt[o + sid];
-    const int32_t hl = A.rd_hl[rd];
-    const uint32_t idx = hoff + j;
-    const uint32_t seg = I.hit_seg[idx];
-    const int32_t b = hit_pos(I, idx, seg, ap) / hl;
-    bool prev_bm1 = false;
-    if (j > 0) {
-        const uint32_t seg0 = I.hit_seg[idx - 1];
-        if (seg0 == seg) {
-            const int32_t b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
-            if (b0 == b) return;  // inside a run
-            prev_bm1 = b0 == b - 1;
+    {
+        const int32_t rd = A.r0 + (int32_t)lr;
+        const uint32_t local = (uint32_t)(f - A.hit_base[lr]);
+        const uint64_t o = A.koff[rd];
+        uint32_t lo = 0, hi = (uint32_t)A.rd_c[rd];
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (A.sd_epre[o + mid] <= local) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t q = lo - 1;
+        const uint32_t sid = A.sd_ord[o + q];
+        const uint32_t j = local - A.sd_epre[o + q];
+        const uint32_t hoff = A.sd_hoff[o + sid], cnt = A.sd_cnt[o + sid];
+        const uint32_t occs = A.sd_occs[o + sid];
+        const uint32_t sstrand = A.sd_strand[o + sid];
+        const int32_t hl = A.rd_hl[rd];
+        const uint32_t idx = hoff + j;
+        const uint32_t seg = I.hit_seg[idx];
+        const int32_t b = hit_pos(I, idx, seg, ap) / hl;
+        bool prev_bm1 = false;
+        if (j > 0) {
+            const uint32_t seg0 = I.hit_seg[idx - 1];
+            if (seg0 == seg) {
+                const int32_t b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
+                if (b0 == b) return;  // inside a run
+                prev_bm1 = b0 == b - 1;
+            }
+        }
+        const uint64_t tbase = A.tab_base[lr], tcap = A.rd_cap[rd];
+        const int32_t rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
+        for (int side = 0; side < 2; side++) {
+            if (side == 1 && (b == 0 || prev_bm1)) break;
+            const int32_t bb = side == 0 ? b : b - 1;
+            const uint32_t e = hits_lower_bound(I, idx, hoff + cnt, seg, (int64_t)(bb + 2) * hl, ap);
+            const uint32_t n = e - idx;
+            const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx];
+            const uint32_t same = sstrand ? ones : n - ones;
+            tab_add(A, lr, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
+                    (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
         }
     }
-    contrib_head(A, I, lr, rd, idx, seg, b, prev_bm1, hoff + cnt, A.sd_occs[o + sid], A.sd_strand[o + sid], hl, ap);
 }
 
-// Blocks of WG consecutive hits.  Most examined hits belong to seeds with many
-// hits, so a block usually lies inside one seed of one read: then the seed is
-// found once, each thread loads only its own hit (consecutive across the
-// block) and takes the previous hit's half-bucket from its neighbour through
-// LDS.  Other blocks locate each hit on its own, the block's reads [r0, r0+nr)
-// found by one binary search and their offsets staged in LDS when few (reads
-// without hits can make the run long).
 __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
+    // A block's WG consecutive hits lie in a run of reads [r0, r0 + nr): one
+    // binary search over the group's hit offsets finds the run, and when it is
+    // short (reads without hits can lengthen it) its offsets are staged in LDS
+    // for every thread's search.
     __shared__ uint64_t s_hb[WG + 1];
-    __shared__ uint32_t s_r0, s_nr, s_one, s_hoff, s_idx0, s_end, s_occs, s_strand;
-    __shared__ int32_t s_hl;
-    __shared__ uint32_t s_seg[WG];
-    __shared__ int32_t s_b[WG];
+    __shared__ uint32_t s_r0, s_nr;
     const uint64_t E = A.hit_base[A.RG];
     const int ap = A.abs_pos;
     for (uint64_t f0 = (uint64_t)blockIdx.x * WG; f0 < E; f0 += (uint64_t)gridDim.x * WG) {
@@ -1121,68 +1085,18 @@ __global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
             const uint32_t b = upper_bound_u64(A.hit_base + a, (uint32_t)A.RG + 1 - a, fl) - 1 + a;
             s_r0 = a;
             s_nr = b - a + 1;
-            uint32_t one = 0;
-            if (a == b) {
-                const int32_t rd = A.r0 + (int32_t)a;
-                const uint64_t o = A.koff[rd], hb = A.hit_base[a];
-                const uint32_t local0 = (uint32_t)(f0 - hb), localL = (uint32_t)(fl - hb);
-                const uint32_t q = seed_of_hit(A, o, rd, local0);
-                const uint32_t c = (uint32_t)A.rd_c[rd];
-                const uint32_t qend = q + 1 < c ? A.sd_epre[o + q + 1] : (uint32_t)(A.hit_base[a + 1] - hb);
-                if (localL < qend) {  // the whole block is seed q's
-                    const uint32_t sid = A.sd_ord[o + q];
-                    const uint32_t hoff = A.sd_hoff[o + sid];
-                    one = 1;
-                    s_hoff = hoff;
-                    s_idx0 = hoff + (local0 - A.sd_epre[o + q]);
-                    s_end = hoff + A.sd_cnt[o + sid];
-                    s_occs = A.sd_occs[o + sid];
-                    s_strand = A.sd_strand[o + sid];
-                    s_hl = A.rd_hl[rd];
-                }
-            }
-            s_one = one;
         }
         __syncthreads();
         const uint32_t r0 = s_r0, nr = s_nr;
+        const bool staged = nr + 1 <= (uint32_t)WG + 1;
+        if (staged)
+            for (uint32_t i = threadIdx.x; i <= nr; i += WG) s_hb[i] = A.hit_base[r0 + i];
+        __syncthreads();
         const uint64_t f = f0 + threadIdx.x;
-        if (s_one) {
-            const uint32_t idx = s_idx0 + threadIdx.x, hoff = s_hoff;
-            const int32_t hl = s_hl;
-            uint32_t seg = 0;
-            int32_t b = 0;
-            if (f < E) {
-                seg = I.hit_seg[idx];
-                b = hit_pos(I, idx, seg, ap) / hl;
-            }
-            s_seg[threadIdx.x] = seg;
-            s_b[threadIdx.x] = b;
-            __syncthreads();
-            if (f < E) {
-                uint32_t seg0 = UINT32_MAX;  // no previous hit in the seed
-                int32_t b0 = 0;
-                if (threadIdx.x > 0) {
-                    seg0 = s_seg[threadIdx.x - 1];
-                    b0 = s_b[threadIdx.x - 1];
-                } else if (idx > hoff) {
-                    seg0 = I.hit_seg[idx - 1];
-                    if (seg0 == seg) b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
-                }
-                const bool inside = seg0 == seg && b0 == b;  // inside a run
-                if (!inside)
-                    contrib_head(A, I, r0, A.r0 + (int32_t)r0, idx, seg, b, seg0 == seg && b0 == b - 1, s_end,
-                                 s_occs, s_strand, hl, ap);
-            }
-        } else {
-            const bool staged = nr + 1 <= (uint32_t)WG + 1;
-            if (staged)
-                for (uint32_t i = threadIdx.x; i <= nr; i += WG) s_hb[i] = A.hit_base[r0 + i];
-            __syncthreads();
-            if (f < E) {
-                const uint32_t lr = r0 - 1 + (staged ? upper_bound_u64(s_hb, nr + 1, f)
-                                                     : upper_bound_u64(A.hit_base + r0, nr + 1, f));
-                contrib_hit(A, I, f, lr, ap);
-            }
+        if (f < E) {
+            const uint32_t lr = r0 - 1 + (staged ? upper_bound_u64(s_hb, nr + 1, f)
+                                                 : upper_bound_u64(A.hit_base + r0, nr + 1, f));
+            contrib_hit(A, I, f, lr, ap);
         }
         __syncthreads();
     }
@@ -1209,13 +1123,10 @@ __device__ __forceinline__ void emit_bucket(const MapArgs &A, uint64_t base, uin
 }
 
 // One wave per read with at most EXTRACT_WAVE_CAP buckets: the listed slots are
-// keyed into the wave's LDS slice and sorted there.  Reads with more buckets are
-// listed for k_extract.
+// keyed, sorted in registers across the wave and emitted straight from them.
+// Reads with more buckets are listed for k_extract.
 __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
-    extern __shared__ uint8_t lds[];
-    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-    uint64_t *kk = (uint64_t *)(lds + (size_t)w * EXTRACT_WAVE_LDS);
-    uint32_t *kv = (uint32_t *)(kk + EXTRACT_WAVE_CAP);
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
         const uint32_t n = (uint32_t)A.grp_nb[lr];
         if (n > EXTRACT_WAVE_CAP) {
@@ -1223,17 +1134,15 @@ __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
             continue;
         }
         const uint64_t tb = A.tab_base[lr], base = A.bk_base[lr];
-        for (uint32_t i = lane; i < n; i += WAVE) {
-            const uint32_t slot = A.tb_list[tb + i];
-            int32_t codir;
-            const uint32_t matches = mc_matches(A.tb_mc[tb + slot], codir);
-            kk[i] = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
-            kv[i] = slot;
-        }
-        wave_sync();
-        wave_bitonic(kk, kv, n);
-        for (uint32_t p = lane; p < n; p += WAVE) emit_bucket(A, base, p, kk[p], tb, kv[p]);
-        wave_sync();
+        wave_sort_n<true>(
+            n,
+            [&](uint32_t i, uint64_t &key, uint32_t &slot) {
+                slot = A.tb_list[tb + i];
+                int32_t codir;
+                const uint32_t matches = mc_matches(A.tb_mc[tb + slot], codir);
+                key = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
+            },
+            [&](uint32_t p, uint64_t key, uint32_t slot) { emit_bucket(A, base, p, key, tb, slot); });
     }
 }
 
@@ -2228,7 +2137,6 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
 static const size_t REFINE_LDS = ((size_t)REFINE_CAP * 3 + 2) * 4;
 static const size_t SEED_WAVE_LDS = (size_t)NWAVES * WAVE_SEED_LDS;
-static const size_t EXTRACT_LDS = (size_t)NWAVES * EXTRACT_WAVE_LDS;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
 void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out,
@@ -2244,7 +2152,7 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     // per-read workgroup kernels with data-dependent cost: a grid above the
     // resident count lets the hardware backfill as reads finish
     case ST_EXTRACT:
-        k_extract_wave<<<PGRID(k_extract_wave, EXTRACT_LDS, (A.RG + NWAVES - 1) / NWAVES), WG, EXTRACT_LDS, st>>>(A);
+        k_extract_wave<<<PGRID(k_extract_wave, 0, (A.RG + NWAVES - 1) / NWAVES), WG, 0, st>>>(A);
         k_extract<<<read_grid(A.RG, max_grid), WG, SORT_LDS, st>>>(A);
         break;
     case ST_PRUNE: k_prune<<<read_grid(A.RG, max_grid), WG, 0, st>>>(A, I); break;

@@ -20,8 +20,7 @@ constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
 constexpr uint32_t PRUNE_SEQ_SEEDS = 128;   // remaining seeds a thread extends a bucket with alone
-constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave
-constexpr uint32_t EXTRACT_WAVE_LDS = EXTRACT_WAVE_CAP * 12;  // per-wave LDS of k_extract_wave
+constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
