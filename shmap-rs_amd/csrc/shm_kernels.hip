@@ -1174,177 +1174,192 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
 // seed_heuristic_pass + matches_in_bucket (src/shmap/pruning.rs:10-119).  Each
 // extension adds occs to seeds and at most occs to matches, so sh never
 // increases: survival under any threshold is decided by the fully extended
-// value, and a bucket the reference refines is always fully extended.  One
-// workgroup per read, WG buckets at a time, against theta*(1-min_diff) (the
-// lowest threshold either match_rest pass can use).  With few remaining seeds
-// each thread extends its bucket seed by seed (the two binary searches of
-// matches_in_bucket for the bucket's span in the seed's hit run), stopping as
-// soon as sh falls below the threshold; with many, each thread only tests its
-// bucket's starting sh and the buckets still alive are extended one per wave,
-// 64 seeds at a time, stopping at the first chunk that kills them.  Survivors
-// are listed for refinement.
+// value, and a bucket the reference refines is always fully extended.  Every
+// bucket of the group is tested against theta*(1-min_diff) (the lowest
+// threshold either match_rest pass can use).  k_prune takes one thread per
+// bucket: with few remaining seeds the thread extends its bucket seed by seed
+// (the two binary searches of matches_in_bucket for the bucket's span in the
+// seed's hit run), stopping as soon as sh falls below the threshold; with many
+// it only tests the starting sh and lists the buckets still alive for
+// k_prune_ext, where one wave extends each, 64 seeds at a time, stopping at
+// the first chunk that kills it.  Survivors are listed for refinement, and
+// each read's first first-pass candidate is kept in rd_first.
+
+// A surviving bucket's refine task: first-pass list when sh reaches theta,
+// else the second-pass band at the end of the task array.
+__device__ __forceinline__ void prune_keep(const MapArgs &A, int32_t lr, int32_t p, double sh) {
+    if (!(sh < A.theta)) {
+        const uint32_t t = atomicAdd(&A.task_count[0], 1u);
+        A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+        atomicMin(&A.rd_first[A.r0 + lr], p);
+    } else {
+        const uint32_t t = atomicAdd(&A.task_count[1], 1u);
+        A.tasks[A.task_cap - 1 - t] = make_uint2((uint32_t)lr, (uint32_t)p);
+    }
+}
+
 __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
-    __shared__ int32_t s_first;
-    __shared__ uint32_t s_na;
-    __shared__ int32_t s_alive[WG];
-    const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    __shared__ uint64_t s_bb[WG + 1];
+    __shared__ uint32_t s_r0, s_nr;
+    const uint64_t B = A.bk_base[A.RG];
     const int ap = A.abs_pos;
-    for (int32_t lr = blockIdx.x; lr < A.RG; lr += gridDim.x) {
-        const int32_t rd = A.r0 + lr;
-        const uint64_t base = A.bk_base[lr], o = A.koff[rd];
-        const int32_t nb = (int32_t)A.grp_nb[lr];
-        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
-        const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
-        const int32_t seeds0 = A.rd_bseeds[rd];
-        if (threadIdx.x == 0) s_first = INT32_MAX;
-        for (int32_t p0 = 0; p0 < nb; p0 += WG) {
-            if (threadIdx.x == 0) s_na = 0;
-            __syncthreads();
-            const int32_t p = p0 + (int32_t)threadIdx.x;
-            if (p < nb) {
-                if (A.no_bucket_pruning) {
-                    A.bs_alive[base + p] = 1;
-                    A.bs_sh[base + p] = 1.0;
-                    const uint32_t t = atomicAdd(&A.task_count[0], 1u);
-                    A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
-                    atomicMin(&s_first, p);
-                } else if (u - c <= (int32_t)PRUNE_SEQ_SEEDS) {
-                    // few seeds to extend with: this thread extends its bucket
-                    // seed by seed, stopping as soon as it dies
-                    int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
-                    int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
-                    int32_t seeds = seeds0;
-                    uint32_t seg;
-                    int32_t b;
-                    decode_bucket(A, A.bs_g[base + p], hl, seg, b);
-                    const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
-                    uint8_t alive = 1;
-                    double sh;
-                    for (int32_t q = c;; q++) {
-                        sh = 1.0 - (double)(seeds - matches) / (double)m;
-                        if (sh < A.thr_min) {
-                            alive = 0;
-                            break;
-                        }
-                        if (q >= u) break;
-                        const uint32_t sid = A.sd_ord[o + q];
-                        const uint32_t occs = A.sd_occs[o + sid];
-                        const uint32_t cnt = A.sd_cnt[o + sid];
-                        seeds += (int32_t)occs;
-                        if (cnt == 0) continue;
-                        const uint32_t hoff = A.sd_hoff[o + sid];
-                        const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
-                        const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
-                        const uint32_t n = e - lo;
-                        if (n == 0) continue;
-                        const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
-                        const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
-                        codir += (int32_t)(2 * same) - (int32_t)n;
-                        const int32_t r0 = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
-                        const int32_t r1 = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
-                        if (r0 < rmin) rmin = r0;
-                        if (r1 > rmax) rmax = r1;
-                        matches += (int32_t)(n < occs ? n : occs);
-                    }
-                    A.bs_m[base + p] = matches;
-                    A.bs_c[base + p] = codir;
-                    A.bs_min[base + p] = rmin;
-                    A.bs_max[base + p] = rmax;
-                    A.bs_alive[base + p] = alive;
-                    A.bs_sh[base + p] = sh;
-                    if (alive) {
-                        if (!(sh < A.theta)) {
-                            const uint32_t t = atomicAdd(&A.task_count[0], 1u);
-                            A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
-                            atomicMin(&s_first, p);
-                        } else {
-                            const uint32_t t = atomicAdd(&A.task_count[1], 1u);
-                            A.tasks[A.task_cap - 1 - t] = make_uint2((uint32_t)lr, (uint32_t)p);
-                        }
-                    }
-                } else {
-                    const double sh = 1.0 - (double)(seeds0 - A.bs_m[base + p]) / (double)m;
-                    if (sh < A.thr_min) {
-                        A.bs_alive[base + p] = 0;
-                        A.bs_sh[base + p] = sh;
-                    } else {
-                        s_alive[atomicAdd(&s_na, 1u)] = p;  // extended by a wave below
-                    }
-                }
-            }
-            __syncthreads();
-            for (uint32_t k = w; k < s_na; k += NWAVES) {
-                const int32_t pb = s_alive[k];
-                int32_t matches = A.bs_m[base + pb], codir = A.bs_c[base + pb];
-                int32_t rmin = A.bs_min[base + pb], rmax = A.bs_max[base + pb];
+    for (uint64_t f0 = (uint64_t)blockIdx.x * WG; f0 < B; f0 += (uint64_t)gridDim.x * WG) {
+        // the block's buckets belong to group reads [r0, r0 + nr)
+        if (threadIdx.x == 0) {
+            const uint64_t fl = f0 + WG - 1 < B ? f0 + WG - 1 : B - 1;
+            const uint32_t a = upper_bound_u64(A.bk_base, (uint32_t)A.RG + 1, f0) - 1;
+            const uint32_t b = upper_bound_u64(A.bk_base + a, (uint32_t)A.RG + 1 - a, fl) - 1 + a;
+            s_r0 = a;
+            s_nr = b - a + 1;
+        }
+        __syncthreads();
+        const uint32_t r0 = s_r0, nr = s_nr;
+        const bool staged = nr + 1 <= (uint32_t)WG + 1;
+        if (staged)
+            for (uint32_t i = threadIdx.x; i <= nr; i += WG) s_bb[i] = A.bk_base[r0 + i];
+        __syncthreads();
+        const uint64_t f = f0 + threadIdx.x;
+        if (f < B) {
+            const int32_t lr = (int32_t)(r0 - 1 + (staged ? upper_bound_u64(s_bb, nr + 1, f)
+                                                          : upper_bound_u64(A.bk_base + r0, nr + 1, f)));
+            const int32_t rd = A.r0 + lr;
+            const uint64_t base = A.bk_base[lr], o = A.koff[rd];
+            const int32_t p = (int32_t)(f - base);
+            const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+            const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
+            const int32_t seeds0 = A.rd_bseeds[rd];
+            if (A.no_bucket_pruning) {
+                A.bs_alive[base + p] = 1;
+                A.bs_sh[base + p] = 1.0;
+                const uint32_t t = atomicAdd(&A.task_count[0], 1u);
+                A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
+                atomicMin(&A.rd_first[rd], p);
+            } else if (u - c <= (int32_t)PRUNE_SEQ_SEEDS) {
+                // few seeds to extend with: this thread extends its bucket
+                // seed by seed, stopping as soon as it dies
+                int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
+                int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
                 int32_t seeds = seeds0;
                 uint32_t seg;
                 int32_t b;
-                decode_bucket(A, A.bs_g[base + pb], hl, seg, b);
+                decode_bucket(A, A.bs_g[base + p], hl, seg, b);
                 const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
-                double sh = 1.0 - (double)(seeds - matches) / (double)m;
-                bool alive = true;
-                for (int32_t q0 = c; alive && q0 < u; q0 += WAVE) {
-                    const int32_t q = q0 + (int32_t)lane;
-                    int32_t d_seeds = 0, d_matches = 0, d_codir = 0, d_min = INT32_MAX, d_max = -1;
-                    if (q < u) {
-                        const uint32_t sid = A.sd_ord[o + q];
-                        const uint32_t occs = A.sd_occs[o + sid];
-                        const uint32_t cnt = A.sd_cnt[o + sid];
-                        d_seeds = (int32_t)occs;
-                        if (cnt) {
-                            const uint32_t hoff = A.sd_hoff[o + sid];
-                            const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
-                            const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
-                            const uint32_t n = e - lo;
-                            if (n) {
-                                const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
-                                const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
-                                d_codir = (int32_t)(2 * same) - (int32_t)n;
-                                d_min = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
-                                d_max = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
-                                d_matches = (int32_t)(n < occs ? n : occs);
-                            }
-                        }
-                    }
-                    for (int d = WAVE / 2; d > 0; d >>= 1) {
-                        d_seeds += __shfl_xor(d_seeds, d);
-                        d_matches += __shfl_xor(d_matches, d);
-                        d_codir += __shfl_xor(d_codir, d);
-                        d_min = min(d_min, (int32_t)__shfl_xor(d_min, d));
-                        d_max = max(d_max, (int32_t)__shfl_xor(d_max, d));
-                    }
-                    seeds += d_seeds;
-                    matches += d_matches;
-                    codir += d_codir;
-                    rmin = min(rmin, d_min);
-                    rmax = max(rmax, d_max);
+                uint8_t alive = 1;
+                double sh;
+                for (int32_t q = c;; q++) {
                     sh = 1.0 - (double)(seeds - matches) / (double)m;
-                    alive = !(sh < A.thr_min);
+                    if (sh < A.thr_min) {
+                        alive = 0;
+                        break;
+                    }
+                    if (q >= u) break;
+                    const uint32_t sid = A.sd_ord[o + q];
+                    const uint32_t occs = A.sd_occs[o + sid];
+                    const uint32_t cnt = A.sd_cnt[o + sid];
+                    seeds += (int32_t)occs;
+                    if (cnt == 0) continue;
+                    const uint32_t hoff = A.sd_hoff[o + sid];
+                    const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
+                    const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
+                    const uint32_t n = e - lo;
+                    if (n == 0) continue;
+                    const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
+                    const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
+                    codir += (int32_t)(2 * same) - (int32_t)n;
+                    const int32_t x0 = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
+                    const int32_t x1 = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
+                    if (x0 < rmin) rmin = x0;
+                    if (x1 > rmax) rmax = x1;
+                    matches += (int32_t)(n < occs ? n : occs);
                 }
-                if (lane == 0) {
-                    A.bs_m[base + pb] = matches;
-                    A.bs_c[base + pb] = codir;
-                    A.bs_min[base + pb] = rmin;
-                    A.bs_max[base + pb] = rmax;
-                    A.bs_alive[base + pb] = alive ? 1 : 0;
-                    A.bs_sh[base + pb] = sh;
-                    if (alive) {
-                        if (!(sh < A.theta)) {
-                            const uint32_t t = atomicAdd(&A.task_count[0], 1u);
-                            A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)pb);
-                            atomicMin(&s_first, pb);
-                        } else {
-                            const uint32_t t = atomicAdd(&A.task_count[1], 1u);
-                            A.tasks[A.task_cap - 1 - t] = make_uint2((uint32_t)lr, (uint32_t)pb);
-                        }
+                A.bs_m[base + p] = matches;
+                A.bs_c[base + p] = codir;
+                A.bs_min[base + p] = rmin;
+                A.bs_max[base + p] = rmax;
+                A.bs_alive[base + p] = alive;
+                A.bs_sh[base + p] = sh;
+                if (alive) prune_keep(A, lr, p, sh);
+            } else {
+                const double sh = 1.0 - (double)(seeds0 - A.bs_m[base + p]) / (double)m;
+                if (sh < A.thr_min) {
+                    A.bs_alive[base + p] = 0;
+                    A.bs_sh[base + p] = sh;
+                } else {
+                    A.sel[atomicAdd(&A.task_count[5], 1u)] = make_uint2((uint32_t)lr, (uint32_t)p);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_prune_ext(MapArgs A, DevIndex I) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ap = A.abs_pos;
+    const uint32_t nl = A.task_count[5];
+    for (uint32_t k = blockIdx.x * NWAVES + w; k < nl; k += gridDim.x * NWAVES) {
+        const uint2 e2 = A.sel[k];
+        const int32_t lr = (int32_t)e2.x, pb = (int32_t)e2.y;
+        const int32_t rd = A.r0 + lr;
+        const uint64_t base = A.bk_base[lr], o = A.koff[rd];
+        const int32_t m = (int32_t)(A.koff[rd + 1] - o);
+        const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
+        int32_t matches = A.bs_m[base + pb], codir = A.bs_c[base + pb];
+        int32_t rmin = A.bs_min[base + pb], rmax = A.bs_max[base + pb];
+        int32_t seeds = A.rd_bseeds[rd];
+        uint32_t seg;
+        int32_t b;
+        decode_bucket(A, A.bs_g[base + pb], hl, seg, b);
+        const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
+        double sh = 1.0 - (double)(seeds - matches) / (double)m;
+        bool alive = true;
+        for (int32_t q0 = c; alive && q0 < u; q0 += WAVE) {
+            const int32_t q = q0 + (int32_t)lane;
+            int32_t d_seeds = 0, d_matches = 0, d_codir = 0, d_min = INT32_MAX, d_max = -1;
+            if (q < u) {
+                const uint32_t sid = A.sd_ord[o + q];
+                const uint32_t occs = A.sd_occs[o + sid];
+                const uint32_t cnt = A.sd_cnt[o + sid];
+                d_seeds = (int32_t)occs;
+                if (cnt) {
+                    const uint32_t hoff = A.sd_hoff[o + sid];
+                    const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
+                    const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
+                    const uint32_t n = e - lo;
+                    if (n) {
+                        const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
+                        const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
+                        d_codir = (int32_t)(2 * same) - (int32_t)n;
+                        d_min = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
+                        d_max = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
+                        d_matches = (int32_t)(n < occs ? n : occs);
                     }
                 }
             }
-            __syncthreads();
+            for (int d = WAVE / 2; d > 0; d >>= 1) {
+                d_seeds += __shfl_xor(d_seeds, d);
+                d_matches += __shfl_xor(d_matches, d);
+                d_codir += __shfl_xor(d_codir, d);
+                d_min = min(d_min, (int32_t)__shfl_xor(d_min, d));
+                d_max = max(d_max, (int32_t)__shfl_xor(d_max, d));
+            }
+            seeds += d_seeds;
+            matches += d_matches;
+            codir += d_codir;
+            rmin = min(rmin, d_min);
+            rmax = max(rmax, d_max);
+            sh = 1.0 - (double)(seeds - matches) / (double)m;
+            alive = !(sh < A.thr_min);
         }
-        if (threadIdx.x == 0) A.rd_first[rd] = s_first;
-        __syncthreads();
+        if (lane == 0) {
+            A.bs_m[base + pb] = matches;
+            A.bs_c[base + pb] = codir;
+            A.bs_min[base + pb] = rmin;
+            A.bs_max[base + pb] = rmax;
+            A.bs_alive[base + pb] = alive ? 1 : 0;
+            A.bs_sh[base + pb] = sh;
+            if (alive) prune_keep(A, lr, pb, sh);
+        }
     }
 }
 
@@ -2155,7 +2170,10 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
         k_extract_wave<<<PGRID(k_extract_wave, 0, (A.RG + NWAVES - 1) / NWAVES), WG, 0, st>>>(A);
         k_extract<<<read_grid(A.RG, max_grid), WG, SORT_LDS, st>>>(A);
         break;
-    case ST_PRUNE: k_prune<<<read_grid(A.RG, max_grid), WG, 0, st>>>(A, I); break;
+    case ST_PRUNE:
+        k_prune<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I);
+        k_prune_ext<<<max_grid, WG, 0, st>>>(A, I);
+        break;
     case ST_REFINE: {
         // three rounds: first candidates, other first-pass candidates, second pass
         MapArgs P = A;

@@ -20,6 +20,7 @@ constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
 constexpr uint32_t PRUNE_SEQ_SEEDS = 128;   // remaining seeds a thread extends a bucket with alone
+constexpr uint32_t TASK_COUNTERS = 6;       // per group: first-pass, second band, 3 refine rounds, prune list
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
@@ -99,7 +100,8 @@ struct MapArgs {
     // scoring above theta (INT32_MAX if none).  Refinement runs in three rounds
     // (refine_pass 0-2) that refine a superset of what the two match_rest
     // passes read; each round's buckets are selected into sel, counted in
-    // task_count[2 + round].
+    // task_count[2 + round].  Before that, sel lists the buckets k_prune_ext
+    // extends (task_count[5]).
     uint2 *tasks, *sel;
     uint32_t *task_count;
     uint32_t task_cap;

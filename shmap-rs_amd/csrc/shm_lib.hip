@@ -860,7 +860,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.mapped_list = mp->mapped.p + 1;
     A.u_total = mp->u_total.p;
     // bucketing, in groups of reads whose tables fit the budget
-    RCHK(mp->task_count.ensure(5 * (size_t)R + 5));  // five refine-task counters per group
+    RCHK(mp->task_count.ensure(TASK_COUNTERS * ((size_t)R + 1)));  // refine-task counters per group
     int32_t ngroups = 0;
     int64_t nbuckets = 0;
     int32_t r0 = 0;
@@ -916,9 +916,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
                               st));
         // the group's list and task counters, cleared while the host waits
         RCHK(mp->ext.ensure((size_t)RG + 2));
-        uint32_t *task_count = mp->task_count.p + 5 * (size_t)ngroups;
+        uint32_t *task_count = mp->task_count.p + TASK_COUNTERS * (size_t)ngroups;
         HIPCHK(hipMemsetAsync(mp->ext.p, 0, 4, st));
-        HIPCHK(hipMemsetAsync(task_count, 0, 20, st));
+        HIPCHK(hipMemsetAsync(task_count, 0, TASK_COUNTERS * 4, st));
         HIPCHK(hipStreamSynchronize(st));
         RCHK(flush_timing(mp, pending));
         const uint64_t B = mp->h_sum.p[3];
@@ -973,7 +973,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.task_cap = (uint32_t)(B + 1);
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
             TIME_BEGIN(s);
-            launch_stage(s, A, I, mp->d_out.p, 0, mp->max_grid, st);
+            launch_stage(s, A, I, mp->d_out.p, B, mp->max_grid, st);
             TIME_END(s);
         }
         HIPCHK(hipGetLastError());
@@ -981,7 +981,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         ngroups++;
         r0 = r1;
     }
-    std::vector<uint32_t> ntask(5 * (size_t)ngroups);
+    std::vector<uint32_t> ntask(TASK_COUNTERS * (size_t)ngroups);
     HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     RCHK(mp->h_mapped.ensure((size_t)R + 1));
@@ -1001,9 +1001,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
     int64_t cand = 0, deferred = 0, refined = 0, seeds = 0;
     for (int32_t g = 0; g < ngroups; g++) {
-        cand += ntask[5 * g];
-        deferred += ntask[5 * g + 1];
-        refined += ntask[5 * g + 2] + ntask[5 * g + 3] + ntask[5 * g + 4];
+        const uint32_t *tc = &ntask[TASK_COUNTERS * (size_t)g];
+        cand += tc[0];
+        deferred += tc[1];
+        refined += tc[2] + tc[3] + tc[4];
     }
     seeds = (int64_t)mp->h_sum.p[4];
     mp->counts[0] = (int64_t)M;
