@@ -761,7 +761,22 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->sd_epre.ensure(M + 1));
     RCHK(mp->rd_i32.ensure((size_t)R * 12 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
-    RCHK(mp->d_out.ensure(R));
+    // Results go straight into the caller's buffer when it is page-locked
+    // memory the device can address (k_final's stores cross the link while it
+    // runs), else into d_out and one copy at the end.
+    shm_read_result *res_dev = nullptr;
+    {
+        hipPointerAttribute_t pa;
+        if (hipPointerGetAttributes(&pa, out) == hipSuccess && pa.type == hipMemoryTypeHost && pa.devicePointer)
+            res_dev = (shm_read_result *)pa.devicePointer;
+        else
+            (void)hipGetLastError();  // pageable memory: the query's error is not ours
+    }
+    const bool direct = res_dev != nullptr;
+    if (!direct) {
+        RCHK(mp->d_out.ensure(R));
+        res_dev = mp->d_out.p;
+    }
 
     MapArgs A{};
     A.R = R;
@@ -973,7 +988,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.task_cap = (uint32_t)(B + 1);
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
             TIME_BEGIN(s);
-            launch_stage(s, A, I, mp->d_out.p, B, mp->max_grid, st);
+            launch_stage(s, A, I, res_dev, B, mp->max_grid, st);
             TIME_END(s);
         }
         HIPCHK(hipGetLastError());
@@ -983,7 +998,8 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     }
     std::vector<uint32_t> ntask(TASK_COUNTERS * (size_t)ngroups);
     HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(out, mp->d_out.p, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    if (!direct)
+        HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     RCHK(mp->h_mapped.ensure((size_t)R + 1));
     RCHK(mp->h_sum.ensure(6));
     HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
