@@ -163,8 +163,8 @@ __device__ __forceinline__ uint32_t stage_codes8(uint32_t v0, uint32_t v1) {
 template <int PASS>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_sketch(SeqSet S, int32_t k, uint64_t thres,
                                                const uint64_t *chunk_start, const uint32_t *chunk_seq,
-                                               uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                                               int32_t *out_r, uint8_t *out_s, uint32_t lcap,
+                                               uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
+                                               uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                                                uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
     // T[(out code << 3) | in code]: the xor terms of both strands for one step
@@ -263,12 +263,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     hf = rotl1(hf) ^ ti.x;
                     hr = rotr1(hr) ^ ti.y;
                 }
-                // A kept k-mer goes to the lane's next staging slot: chunk c's
-                // slots (PASS 0) are uniform bases plus a 32-bit lane offset.
-                const uint64_t cb = (uint64_t)c * lcap * WAVE;
-                uint64_t *oh_c = out_h + (PASS == 0 ? cb : 0);
-                int32_t *or_c = out_r + (PASS == 0 ? cb : 0);
-                uint8_t *os_c = out_s + (PASS == 0 ? cb : 0);
+                // A kept k-mer goes to the lane's next staging slot (PASS 0):
+                // one 16-byte record (hash, position, strand) at chunk c's
+                // uniform base plus a 32-bit lane offset, so one store.
+                uint4 *st_c = stage + (PASS == 0 ? (uint64_t)c * lcap * WAVE : 0);
                 const int32_t r0 = (int32_t)(w0 + k - 1);  // window wi ends at base w0 + wi + k - 1
                 auto put = [&](int64_t wi, uint64_t h, bool strand) {
                     if (PASS == 1) {
@@ -278,10 +276,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                         out_s[o] = strand;
                     } else {
                         if (cnt >= lcap) return;
-                        const uint32_t o = cnt * WAVE + (uint32_t)lane;
-                        oh_c[o] = h;
-                        or_c[o] = r0 + (int32_t)wi;
-                        os_c[o] = strand;
+                        st_c[cnt * WAVE + (uint32_t)lane] =
+                            make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)(r0 + (int32_t)wi), strand ? 1u : 0u);
                     }
                 };
                 {
@@ -379,8 +375,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 // writes its chunk's k-mers contiguously and gathers from the staging slots.
 __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
                                                        const uint64_t *chunk_koff, uint32_t lcap,
-                                                       const uint64_t *st_h, const int32_t *st_r,
-                                                       const uint8_t *st_s, uint64_t *out_h, int32_t *out_r,
+                                                       const uint4 *stage, uint64_t *out_h, int32_t *out_r,
                                                        uint8_t *out_s) {
     __shared__ uint32_t s_pre[NWAVES][WAVE];
     const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
@@ -401,9 +396,10 @@ __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const 
                 if (pre_w[mid] <= t) lo = mid; else hi = mid;
             }
             const uint64_t si = sb + (uint64_t)(t - pre_w[lo]) * WAVE + lo;
-            out_h[ob + t] = st_h[si];
-            out_r[ob + t] = st_r[si];
-            out_s[ob + t] = st_s[si];
+            const uint4 rec = stage[si];
+            out_h[ob + t] = ((uint64_t)rec.y << 32) | rec.x;
+            out_r[ob + t] = (int32_t)rec.z;
+            out_s[ob + t] = (uint8_t)rec.w;
         }
         wave_sync();  // the prefix slice is rewritten by the wave's next chunk
     }
@@ -2068,28 +2064,27 @@ void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq
 }
 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                   int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list, uint32_t *ovf_count,
-                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
+                   uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list,
+                   uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
     const size_t lds = sketch_lds(k);
     const uint64_t want = (n_chunks + NWAVES - 1) / NWAVES;
     if (pass == 0)
         k_sketch<0><<<persistent_grid((const void *)k_sketch<0>, lds, want, max_grid), WG, lds, st>>>(
-            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, stage, out_h, out_r, out_s, lcap, ovf_list,
             ovf_count);
     else
         k_sketch<1><<<persistent_grid((const void *)k_sketch<1>, lds, want, max_grid), WG, lds, st>>>(
-            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, out_h, out_r, out_s, lcap, ovf_list,
+            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, stage, out_h, out_r, out_s, lcap, ovf_list,
             ovf_count);
 }
 
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
-                           uint32_t lcap, const uint64_t *st_h, const int32_t *st_r, const uint8_t *st_s,
-                           uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t max_grid, hipStream_t st) {
+                           uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
+                           uint32_t max_grid, hipStream_t st) {
     const uint32_t g =
         persistent_grid((const void *)k_sketch_compact, 0, (n_chunks + NWAVES - 1) / NWAVES, max_grid);
-    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, st_h, st_r, st_s, out_h, out_r,
-                                       out_s);
+    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,

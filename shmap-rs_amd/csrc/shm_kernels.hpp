@@ -149,16 +149,16 @@ void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
 size_t sketch_lds(int32_t k);
 void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
                       hipStream_t st);
-// pass 0: roll every chunk into lane-strided staging (lcap slots per lane),
+// pass 0: roll every chunk into lane-strided staging (lcap 16-byte records per lane),
 // listing chunks with a lane over lcap; pass 1: re-roll the listed chunks
 // straight to their final positions (chunk_val = scanned chunk offsets).
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint64_t *out_h,
-                   int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list, uint32_t *ovf_count,
-                   uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
+                   uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list,
+                   uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
-                           uint32_t lcap, const uint64_t *st_h, const int32_t *st_r, const uint8_t *st_s,
-                           uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t max_grid, hipStream_t st);
+                           uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
+                           uint32_t max_grid, hipStream_t st);
 // Staging slots per lane for a keep threshold: the expected kept count of
 // LANE_W windows plus five standard deviations and a margin, at most LANE_W.
 uint32_t sketch_lane_cap(uint64_t thres);

@@ -179,9 +179,7 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 struct SketchOut {
     DevBuf<uint64_t> nch, chunk_start, chunk_cnt, chunk_koff, koff, part, stat;  // stat: [0] largest sketch
     DevBuf<uint32_t> lane_cnt, chunk_seq, ovf, ovf_n;  // overflow chunk list and its count
-    DevBuf<uint64_t> st_h;                      // lane-strided staging of the rolling pass
-    DevBuf<int32_t> st_r;
-    DevBuf<uint8_t> st_s;
+    DevBuf<uint4> stage;                        // lane-strided staging records of the rolling pass
     std::vector<uint64_t> h_koff;  // per-sequence offsets, when asked for
     PinnedBuf<uint64_t> h_sum;     // total, largest sketch, overflow chunks
     uint64_t M = 0, max_m = 0;
@@ -213,15 +211,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     if (C > 0) launch_chunk_seq(S, so.chunk_start.p, so.chunk_seq.p, C, st);
     const uint32_t lcap = sketch_lane_cap(sk->thres);
     const uint64_t slots = C * WAVE * (uint64_t)lcap;
-    RCHK(so.st_h.ensure(slots + 1));
-    RCHK(so.st_r.ensure(slots + 1));
-    RCHK(so.st_s.ensure(slots + 1));
+    RCHK(so.stage.ensure(slots + 1));
     RCHK(so.ovf.ensure(C + 1));
     uint32_t *ovf_count = so.ovf_n.p, *ovf_list = so.ovf.p;
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
         launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_cnt.p, so.lane_cnt.p,
-                      so.st_h.p, so.st_r.p, so.st_s.p, lcap, ovf_list, ovf_count, C, max_grid, st);
+                      so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
     RCHK(so.stat.ensure(1));
@@ -246,11 +242,11 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(out_s.ensure(M + 1));
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
-        launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.st_h.p, so.st_r.p, so.st_s.p, out_h.p,
+        launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.stage.p, out_h.p,
                               out_r.p, out_s.p, max_grid, st);
     if (so.n_overflow > 0)
         launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_koff.p, so.lane_cnt.p,
-                      out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
+                      nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
