@@ -237,17 +237,18 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 base_out = chunk_val[c] + (wave_incl_scan(mine) - mine);
             }
             if (lw0 < lw1) {
-                // the lane's first window: its k bases rolled in from nothing,
+                // the lane's start: bases lw0 .. lw0+k-2 rolled in from nothing,
                 // two at a time (pairs are the bytes of the code stream; the
                 // lane start is dword aligned since LANE_W is a multiple of 8)
                 uint32_t fl = 0, fh = 0, rl = 0, rh = 0;  // the strands as 32-bit halves
                 const uint32_t q0 = (uint32_t)lw0 >> 3;
-                for (int32_t p = 0; p < k / 2; p += 4) {
+                const int32_t npair = (k - 1) / 2;
+                for (int32_t p = 0; p < npair; p += 4) {
                     const uint32_t d = SKW(q0 + ((uint32_t)p >> 2));
                     const uint32_t e = ((d & 0x07070707u) << 3) | ((d >> 4) & 0x07070707u);
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
-                        if (p + t < k / 2) {
+                        if (p + t < npair) {
                             const uint4 tt = ((const uint4 *)T2)[__builtin_amdgcn_perm(0u, e, 0x0c0c0c00u | (uint32_t)t)];
                             const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 30) ^ tt.y;  // rotl2
                             const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 30) ^ tt.x;
@@ -257,11 +258,13 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                         }
                     }
                 }
-                uint64_t hf = ((uint64_t)fh << 32) | fl, hr = ((uint64_t)rh << 32) | rl;
-                if (k & 1) {
-                    const ulonglong2 ti = T[32 + SKN(lw0 + k - 1)];
-                    hf = rotl1(hf) ^ ti.x;
-                    hr = rotr1(hr) ^ ti.y;
+                if ((k - 1) & 1) {
+                    const uint4 tt = ((const uint4 *)T)[32 + SKN(lw0 + k - 2)];  // row 4: nothing rolls out
+                    const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 31) ^ tt.y;
+                    const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 31) ^ tt.x;
+                    const uint32_t nrh = __builtin_amdgcn_alignbit(rl, rh, 1) ^ tt.w;
+                    const uint32_t nrl = __builtin_amdgcn_alignbit(rh, rl, 1) ^ tt.z;
+                    fh = nfh; fl = nfl; rh = nrh; rl = nrl;
                 }
                 // A kept k-mer goes to the lane's next staging slot (PASS 0):
                 // one 16-byte record (hash, position, strand) at chunk c's
@@ -280,31 +283,25 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                             make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)(r0 + (int32_t)wi), strand ? 1u : 0u);
                     }
                 };
-                {
-                    const uint64_t h = hr ^ hf;
-                    if (h <= thres) {
-                        put(lw0, h, hf > hr);
-                        cnt++;
-                    }
-                }
-                // windows lw0+1 .. lw1-1; the step into window lw0+j+1 rolls out
-                // base lw0+j and in base lw0+j+k.  Eight steps at a time: the
-                // outgoing codes are one dword (lw0 is a multiple of LANE_W, so a
-                // lane's outgoing dwords share one skew), the incoming ones a
-                // funnel shift of two; even and odd nibbles spread to bytes give
-                // the eight (out << 3 | in) table indices.  The strands roll as
-                // 32-bit halves.  Kept windows are rare, so the steps only test
-                // the high word of the hash against the threshold's, and the
-                // exact test and the writes come after.
-                const int64_t nsteps = lw1 - lw0 - 1;
-                const uint32_t ish = (uint32_t)(k & 7) * 4;
+                // windows lw0 .. lw1-1; the step into window lw0+j rolls out
+                // base lw0+j-1 (nothing for j = 0: code 4) and in base lw0+j+k-1.
+                // Eight steps at a time: both code runs are funnel shifts of two
+                // dwords (lw0 is a multiple of LANE_W, so a lane's outgoing
+                // dwords share one skew); even and odd nibbles spread to bytes
+                // give the eight (out << 3 | in) table indices.  A full lane is
+                // sixteen such groups.  The strands roll as 32-bit halves.
+                // Kept windows are rare, so the steps only test the high word of
+                // the hash against the threshold's, and the exact test and the
+                // writes come after.
+                const int64_t nwin_l = lw1 - lw0;
+                const uint32_t ish = (uint32_t)((k - 1) & 7) * 4;
                 const uint32_t *obase = &SKW((uint32_t)lw0 >> 3);
                 const uint4 *T4 = (const uint4 *)T;  // (fw lo, fw hi, rc lo, rc hi)
-                fl = (uint32_t)hf; fh = (uint32_t)(hf >> 32); rl = (uint32_t)hr; rh = (uint32_t)(hr >> 32);
                 int64_t j = 0;
-                for (; j + 8 <= nsteps; j += 8) {
-                    const uint32_t od = obase[j >> 3];
-                    const uint32_t iq = (uint32_t)(lw0 + j + k) >> 3;
+                for (; j + 8 <= nwin_l; j += 8) {
+                    const uint32_t oc = obase[j >> 3], op = j ? obase[(j >> 3) - 1] : 0x44444444u;
+                    const uint32_t od = __builtin_amdgcn_alignbit(oc, op, 28);  // bases lw0+j-1 .. lw0+j+6
+                    const uint32_t iq = (uint32_t)(lw0 + j + k - 1) >> 3;
                     const uint32_t i0 = SKW(iq), i1 = SKW(iq + 1);
                     const uint32_t id = ish ? __builtin_amdgcn_alignbit(i1, i0, ish) : i0;
                     const uint32_t pe = ((od & 0x0f0f0f0fu) << 3) | (id & 0x0f0f0f0fu);            // windows 0, 2, 4, 6
@@ -334,22 +331,23 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                     const uint64_t r = ((uint64_t)srh[t4] << 32) | srl[t4];
                                     const uint64_t h = f ^ r;
                                     if (h <= thres) {
-                                        put(lw0 + j + half * 4 + t4 + 1, h, f > r);
+                                        put(lw0 + j + half * 4 + t4, h, f > r);
                                         cnt++;
                                     }
                                 }
                         }
                     }
                 }
-                hf = ((uint64_t)fh << 32) | fl;
-                hr = ((uint64_t)rh << 32) | rl;
-                for (; j < nsteps; j++) {
-                    const ulonglong2 tt = T[(SKN(lw0 + j) << 3) | SKN(lw0 + j + k)];
+                // a partial lane's last windows (the read's end)
+                uint64_t hf = ((uint64_t)fh << 32) | fl, hr = ((uint64_t)rh << 32) | rl;
+                for (; j < nwin_l; j++) {
+                    const uint32_t oc = j ? SKN(lw0 + j - 1) : 4u;
+                    const ulonglong2 tt = T[(oc << 3) | SKN(lw0 + j + k - 1)];
                     hf = rotl1(hf) ^ tt.x;
                     hr = rotr1(hr) ^ tt.y;
                     const uint64_t h = hr ^ hf;
                     if (h <= thres) {
-                        put(lw0 + j + 1, h, hf > hr);
+                        put(lw0 + j, h, hf > hr);
                         cnt++;
                     }
                 }
