@@ -11,7 +11,11 @@
 #include <string.h>
 #include <sys/time.h>
 
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/shmap_hip.h"
@@ -32,6 +36,7 @@ struct Params {
     unsigned long long per_read_stats_sample = 1;
     int device = 0;
     long batch = 16384;
+    bool dump_reads = false;
 };
 
 const char *metric_name(int32_t m) {
@@ -105,7 +110,8 @@ void usage(FILE *f) {
             "      --per-read-stats <FILE>   One TSV row per read\n"
             "      --per-read-stats-sample <N>  Keep every Nth read [default: 1]\n"
             "      --device <ID>             GPU device [default: 0]\n"
-            "      --batch <N>               Reads per GPU batch [default: 16384]\n");
+            "      --batch <N>               Reads per GPU batch [default: 16384]\n"
+            "      --dump-reads              Print the reads of -p as parsed (FASTA) and exit\n");
 }
 
 struct Record {
@@ -113,65 +119,184 @@ struct Record {
     std::vector<uint8_t> seq;
 };
 
+// A batch of reads laid out contiguously: read i is bases[off[i], off[i+1])
+// with id names[name_off[i], name_off[i+1]).
+struct Batch {
+    std::vector<uint8_t> bases;
+    std::vector<int64_t> off{0};
+    std::string names;
+    std::vector<size_t> name_off{0};
+    size_t size() const { return off.size() - 1; }
+    void clear() {
+        bases.clear();
+        off.assign(1, 0);
+        names.clear();
+        name_off.assign(1, 0);
+    }
+    std::string name(size_t i) const { return names.substr(name_off[i], name_off[i + 1] - name_off[i]); }
+};
+
 // FASTA/FASTQ records as needletail delivers them (src/io/mod.rs:383-427): the
-// id cut at the first space or tab, line breaks and '\r' removed.
+// id cut at the first space or tab, line breaks and '\r' removed.  The input
+// is read in large blocks and split into lines with memchr; a record's bases
+// are copied straight into the caller's storage.
 class FastxReader {
   public:
-    explicit FastxReader(const std::string &path) : f_(fopen(path.c_str(), "rb")) {}
+    explicit FastxReader(const std::string &path) : f_(fopen(path.c_str(), "rb")), buf_(1u << 24) {}
     ~FastxReader() {
         if (f_) fclose(f_);
     }
     bool ok() const { return f_ != nullptr; }
     bool next(Record &r) {
-        std::string line;
-        if (!have_pending_) {
-            do {
-                if (!getline(line)) return false;
-            } while (line.empty() || (line[0] != '>' && line[0] != '@'));
-        } else {
-            line.swap(pending_);
-            have_pending_ = false;
-        }
-        const bool fastq = line[0] == '@';
-        size_t e = 1;
-        while (e < line.size() && line[e] != ' ' && line[e] != '\t') e++;
-        r.name.assign(line, 1, e - 1);
+        r.name.clear();
         r.seq.clear();
-        if (fastq) {
-            if (getline(line)) r.seq.assign(line.begin(), line.end());
-            std::string skip;
-            getline(skip);
-            getline(skip);
-            return true;
+        std::vector<uint8_t> *dst = &r.seq;
+        return record(
+            [&](const char *s, size_t n) { r.name.assign(s, n); },
+            [&](const char *s, size_t n) { dst->insert(dst->end(), (const uint8_t *)s, (const uint8_t *)s + n); });
+    }
+    // Appends up to max_reads records; false when none was left.
+    bool next_batch(Batch &b, long max_reads) {
+        b.clear();
+        while ((long)b.size() < max_reads) {
+            const bool got = record([&](const char *s, size_t n) { b.names.append(s, n); },
+                                    [&](const char *s, size_t n) { b.bases.insert(b.bases.end(), s, s + n); });
+            if (!got) break;
+            b.off.push_back((int64_t)b.bases.size());
+            b.name_off.push_back(b.names.size());
         }
-        while (getline(line)) {
-            if (!line.empty() && line[0] == '>') {
-                pending_.swap(line);
-                have_pending_ = true;
-                break;
-            }
-            r.seq.insert(r.seq.end(), line.begin(), line.end());
-        }
-        return true;
+        return b.size() > 0;
     }
 
   private:
-    bool getline(std::string &out) {
-        out.clear();
-        int c;
-        bool any = false;
-        while ((c = fgetc_unlocked(f_)) != EOF) {
-            any = true;
-            if (c == '\n') break;
-            out.push_back((char)c);
+    // The next line without its '\n' (and a trailing '\r'), valid until the
+    // next call; false at the end of the input.
+    bool line(const char *&s, size_t &n) {
+        for (;;) {
+            const char *p = (const char *)memchr(buf_.data() + pos_, '\n', end_ - pos_);
+            if (p || eof_) {
+                if (!p && pos_ == end_) return false;
+                const size_t e = p ? (size_t)(p - buf_.data()) : end_;
+                s = buf_.data() + pos_;
+                n = e - pos_;
+                pos_ = p ? e + 1 : end_;
+                if (n && s[n - 1] == '\r') n--;
+                return true;
+            }
+            // no complete line left: keep the tail, grow if one line fills the buffer, read more
+            memmove(buf_.data(), buf_.data() + pos_, end_ - pos_);
+            end_ -= pos_;
+            pos_ = 0;
+            if (end_ == buf_.size()) buf_.resize(buf_.size() * 2);
+            const size_t got = fread(buf_.data() + end_, 1, buf_.size() - end_, f_);
+            end_ += got;
+            if (got == 0) eof_ = true;
         }
-        if (!any) return false;
-        if (!out.empty() && out.back() == '\r') out.pop_back();
+    }
+    template <typename NameF, typename SeqF>
+    bool record(NameF name, SeqF seq) {
+        const char *s;
+        size_t n;
+        if (have_pending_) {
+            have_pending_ = false;
+            s = pending_.data();
+            n = pending_.size();
+        } else {
+            do {
+                if (!line(s, n)) return false;
+            } while (n == 0 || (s[0] != '>' && s[0] != '@'));
+        }
+        const bool fastq = s[0] == '@';
+        size_t e = 1;
+        while (e < n && s[e] != ' ' && s[e] != '\t') e++;
+        name(s + 1, e - 1);
+        if (fastq) {
+            if (line(s, n)) seq(s, n);
+            line(s, n);  // '+'
+            line(s, n);  // qualities
+            return true;
+        }
+        while (line(s, n)) {
+            if (n && s[0] == '>') {
+                pending_.assign(s, n);
+                have_pending_ = true;
+                break;
+            }
+            seq(s, n);
+        }
         return true;
     }
     FILE *f_;
+    std::vector<char> buf_;
+    size_t pos_ = 0, end_ = 0;
+    bool eof_ = false;
     std::string pending_;
     bool have_pending_ = false;
+};
+
+// Batches read ahead on a thread of their own (two in flight), so parsing
+// overlaps mapping and output.
+class BatchReader {
+  public:
+    BatchReader(FastxReader &rd, long max_reads) : rd_(rd), max_(max_reads), th_([this] { run(); }) {}
+    ~BatchReader() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    // The next batch (nullptr at the end); give it back with recycle().
+    Batch *next() {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !full_.empty() || done_; });
+        if (full_.empty()) return nullptr;
+        Batch *b = full_.front();
+        full_.pop_front();
+        cv_.notify_all();
+        return b;
+    }
+    void recycle(Batch *b) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.push_back(b);
+        cv_.notify_all();
+    }
+
+  private:
+    void run() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int i = 0; i < 2; i++) free_.push_back(&pool_[i]);
+        }
+        for (;;) {
+            Batch *b;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !free_.empty() || stop_; });
+                if (stop_) return;
+                b = free_.front();
+                free_.pop_front();
+            }
+            const bool got = rd_.next_batch(*b, max_);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (!got) {
+                done_ = true;
+                cv_.notify_all();
+                return;
+            }
+            full_.push_back(b);
+            cv_.notify_all();
+        }
+    }
+    FastxReader &rd_;
+    const long max_;
+    Batch pool_[2];
+    std::deque<Batch *> free_, full_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false, done_ = false;
+    std::thread th_;
 };
 
 double now_s() {
@@ -386,11 +511,30 @@ int main(int argc, char **argv) {
         } else if (a == "--batch") {
             p.batch = atol(need("--batch <N>"));
             if (p.batch < 1) p.batch = 1;
+        } else if (a == "--dump-reads") {
+            p.dump_reads = true;
         } else {
             fprintf(stderr, "error: unexpected argument '%s' found\n\nFor more information, try '--help'.\n",
                     argv[i]);
             return 2;
         }
+    }
+    if (p.dump_reads && have_p) {
+        // the reads as parsed, one FASTA record per line pair (no device needed)
+        FastxReader rd(p.p_file);
+        if (!rd.ok()) {
+            fprintf(stderr, "Error: failed to open \"%s\"\n", p.p_file.c_str());
+            return 1;
+        }
+        Batch b;
+        while (rd.next_batch(b, p.batch))
+            for (size_t i = 0; i < b.size(); i++) {
+                const std::string nm = b.name(i);
+                fprintf(stdout, ">%s\n", nm.c_str());
+                fwrite(b.bases.data() + b.off[i], 1, (size_t)(b.off[i + 1] - b.off[i]), stdout);
+                fputc('\n', stdout);
+            }
+        return 0;
     }
     if (!have_p || !have_s) {
         fprintf(stderr, "error: the following required arguments were not provided:\n%s%s\nUsage: shmap -p <P_FILE> "
@@ -482,55 +626,57 @@ int main(int argc, char **argv) {
     }
     static char outbuf[1 << 20];
     setvbuf(stdout, outbuf, _IOFBF, sizeof outbuf);
-    std::vector<Record> batch;
     std::vector<shm_read> reads;
     std::vector<shm_read_result> res;
     std::string line(8192, '\0');
     unsigned long long idx_read = 0, n_reads = 0, n_mapped = 0;
     double map_s = 0;
-    bool more = true;
     int rc = 0;
-    while (more) {
-        batch.clear();
-        Record r;
-        while ((long)batch.size() < p.batch && (more = rd.next(r))) batch.push_back(std::move(r));
-        if (batch.empty()) break;
-        reads.resize(batch.size());
-        for (size_t i = 0; i < batch.size(); i++) reads[i] = {batch[i].seq.data(), (int32_t)batch[i].seq.size()};
-        res.resize(batch.size());
-        const double t0 = now_s();
-        if (shm_map_batch(mp, reads.data(), (int32_t)reads.size(), res.data())) {
-            fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
-                    shm_last_error());
-            rc = 1;
-            break;
-        }
-        const double dt = now_s() - t0;
-        map_s += dt;
-        const double per_read = dt / batch.size();
-        for (size_t i = 0; i < batch.size(); i++, idx_read++) {
-            const shm_read_result &x = res[i];
-            if (stats && idx_read % p.per_read_stats_sample == 0)
-                fprintf(stats, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n", batch[i].name.c_str(),
-                        x.read_len, x.m, x.S, x.possible_matches, x.total_matches, x.seed_matches, x.seeded_buckets,
-                        x.refined_buckets, x.final_buckets, x.mapped ? x.intersection : 0, x.mapped,
-                        x.mapped ? (x.mapq == 60 ? 60 : 0) : -1, per_read);
-            if (x.mapped) {
-                int n = shm_format_paf(idx, &mpp, batch[i].name.c_str(), &x, per_read, &line[0], line.size());
-                if (n >= (int)line.size()) {
-                    line.resize(n + 1);
-                    n = shm_format_paf(idx, &mpp, batch[i].name.c_str(), &x, per_read, &line[0], line.size());
-                }
-                fwrite(line.data(), 1, n, stdout);
-                fputc('\n', stdout);
-                n_mapped++;
-            } else if (unmapped) {
-                int n = shm_format_unmapped(batch[i].name.c_str(), x.read_len, &line[0], line.size());
-                fwrite(line.data(), 1, n, unmapped);
-                fputc('\n', unmapped);
+    {
+        BatchReader br(rd, p.batch);
+        while (Batch *batch = br.next()) {
+            const size_t nb = batch->size();
+            reads.resize(nb);
+            for (size_t i = 0; i < nb; i++)
+                reads[i] = {batch->bases.data() + batch->off[i], (int32_t)(batch->off[i + 1] - batch->off[i])};
+            res.resize(nb);
+            const double t0 = now_s();
+            if (shm_map_batch(mp, reads.data(), (int32_t)nb, res.data())) {
+                fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
+                        shm_last_error());
+                rc = 1;
+                br.recycle(batch);
+                break;
             }
+            const double dt = now_s() - t0;
+            map_s += dt;
+            const double per_read = dt / nb;
+            for (size_t i = 0; i < nb; i++, idx_read++) {
+                const shm_read_result &x = res[i];
+                const std::string name = batch->name(i);
+                if (stats && idx_read % p.per_read_stats_sample == 0)
+                    fprintf(stats, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n", name.c_str(),
+                            x.read_len, x.m, x.S, x.possible_matches, x.total_matches, x.seed_matches,
+                            x.seeded_buckets, x.refined_buckets, x.final_buckets, x.mapped ? x.intersection : 0,
+                            x.mapped, x.mapped ? (x.mapq == 60 ? 60 : 0) : -1, per_read);
+                if (x.mapped) {
+                    int n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
+                    if (n >= (int)line.size()) {
+                        line.resize(n + 1);
+                        n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
+                    }
+                    fwrite(line.data(), 1, n, stdout);
+                    fputc('\n', stdout);
+                    n_mapped++;
+                } else if (unmapped) {
+                    int n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
+                    fwrite(line.data(), 1, n, unmapped);
+                    fputc('\n', unmapped);
+                }
+            }
+            n_reads += nb;
+            br.recycle(batch);
         }
-        n_reads += batch.size();
     }
     fflush(stdout);
     if (unmapped) fclose(unmapped);

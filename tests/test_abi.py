@@ -7,6 +7,8 @@ import subprocess
 
 import pytest
 
+import fastx
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "shmap_hip.h")
 LIB = os.path.join(ROOT, "shmap-rs_amd", "lib", "libshmap_hip.so")
@@ -65,3 +67,34 @@ def test_cli_validation_messages(args, needle):
 def test_cli_rejects_an_unknown_metric():
     r = subprocess.run([CLI, "-s", "ref.fa", "-p", "reads.fa", "-m", "Hamming"], capture_output=True, text=True)
     assert r.returncode == 2 and "possible values: Containment, Jaccard, bucket_SH, bucket_LCS" in r.stderr
+
+
+def _dump(path):
+    r = subprocess.run([CLI, "-p", str(path), "--dump-reads"], capture_output=True)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split(b"\n")
+    return [(lines[i][1:].decode(), lines[i + 1]) for i in range(0, len(lines) - 1, 2)]
+
+
+def test_cli_reader_matches_fasta_semantics(tmp_path):
+    """The CLI's block reader against tests/fastx.py (needletail's rules:
+    id cut at the first space or tab, line breaks and '\\r' removed): CRLF,
+    wrapped and blank lines, a record with no bases, a line longer than the
+    read block, no final newline."""
+    import random
+    rng = random.Random(5)
+    long_seq = bytes(rng.choice(b"ACGTNacgt") for _ in range(20_000_000))
+    body = (b">r1 first read\r\nACGTACGT\r\nacgtN\r\n\r\n>r2\tsecond\nAC\nGT\n\n\nTT\n>empty\n>r4 x y\n"
+            + long_seq + b"\n>last\nGATTACA")
+    f = tmp_path / "reads.fa"
+    f.write_bytes(body)
+    want = fastx.read_fasta(f)
+    got = _dump(f)
+    assert [n for n, _ in got] == [n for n, _ in want]
+    assert [s for _, s in got] == [s for _, s in want]
+
+
+def test_cli_reader_fastq(tmp_path):
+    f = tmp_path / "reads.fq"
+    f.write_bytes(b"@q1 desc\nACGTN\n+\nIIIII\n@q2\nGGCC\r\n+q2\r\n!!!!\r\n")
+    assert _dump(f) == [("q1", b"ACGTN"), ("q2", b"GGCC")]
