@@ -16,6 +16,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/shmap_hip.h"
@@ -564,7 +565,7 @@ int main(int argc, char **argv) {
             return 1;
         }
         Record r;
-        while (rd.next(r)) ref.push_back(r);
+        while (rd.next(r)) ref.push_back(std::move(r));  // next() refills the moved-from record
     }
     shm_sketcher *sk = nullptr;
     if (shm_sketcher_create(p.k, p.h_frac, &sk)) {
