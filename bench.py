@@ -141,7 +141,7 @@ def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
     E = res["seed_matches"].astype(np.int64).clip(min=0).sum()
     return {
         "sketch": bases_total + 13 * m,                    # every base read once, (h, r, strand) per kept k-mer
-        "sketch_compact": 2 * 13 * m,                      # staged k-mers moved to their final positions
+        "sketch_compact": (16 + 13) * m,                   # one 16 B staged record in, (h, r, strand) out
         "seed_wave": 13 * m + 16 * u + 37 * u,             # sketch in, one table slot per probe, seed rows out
         "contrib": 12 * E,                                 # one packed hit record per examined hit
     }
