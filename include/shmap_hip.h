@@ -181,8 +181,9 @@ const char *shm_mapper_stage_name(int32_t stage);
  * of what the match_rest passes read), [4] read groups (table-budget batches),
  * [5] first-pass candidates (bound at least theta after the full-extension
  * prune), [6] buckets only a second pass can reach, [7] examined hits (the
- * index hits of every consumed seed).  Writes up to n values and their number
- * to *n_written. */
+ * index hits of every consumed seed), [8] reads whose buckets were counted in
+ * device-memory hash tables rather than on chip (many hits or buckets).
+ * Writes up to n values and their number to *n_written. */
 int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */

@@ -739,7 +739,7 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
         const int32_t hl = A.abs_pos ? len : (int32_t)m;
         const bool ok = hl >= MIN_HALFLEN;
-        uint32_t occ_carry = 0, cnt_carry = 0, c = 0, seeds = 0, mx = 0;
+        uint32_t occ_carry = 0, cnt_carry = 0, c = 0, z = 0, seeds = 0, mx = 0;
         for (uint32_t b0 = 0; ok && b0 < u; b0 += WAVE) {
             const uint32_t q = b0 + lane;
             uint32_t occ = 0, cnt = 0, sid = 0;
@@ -757,6 +757,7 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             if (consumed) A.sd_epre[o + q] = cnt_carry + cinc - cv;
             const uint64_t cb = __ballot(consumed);
             c += __popcll(cb);
+            z += __popcll(__ballot(consumed && cnt == 0));  // hitless seeds sort first
             uint32_t so_ = consumed ? occ : 0u, mm = consumed ? cnt : 0u;
             for (int d = WAVE / 2; d > 0; d >>= 1) {
                 so_ += __shfl_xor(so_, d);
@@ -778,6 +779,7 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             A.rd_hl[rd] = hl;
             A.rd_ok[rd] = ok ? 1 : 0;
             A.rd_c[rd] = (int32_t)c;
+            A.rd_z[rd] = (int32_t)z;
             A.rd_bseeds[rd] = (int32_t)seeds;
             A.rd_max_seed[rd] = (int32_t)mx;
             A.rd_seed_matches[rd] = (int32_t)E;
@@ -921,6 +923,13 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
             A.rd_hl[rd] = hl;
             A.rd_ok[rd] = ok ? 1 : 0;
             A.rd_c[rd] = (int32_t)s_c;
+            // hitless seeds sort first: the first key with a nonzero count
+            uint32_t lo = 0, hi = u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((A.sd_key[o + mid] >> 32) == 0) lo = mid + 1; else hi = mid;
+            }
+            A.rd_z[rd] = (int32_t)(lo < s_c ? lo : s_c);
             A.rd_bseeds[rd] = (int32_t)s_seeds;
             A.rd_max_seed[rd] = (int32_t)s_max;
             A.rd_seed_matches[rd] = (int32_t)E;
@@ -941,13 +950,6 @@ __global__ void __launch_bounds__(WG) k_tab_init(MapArgs A) {
         A.grp_nb[i] = 0;
         A.rd_first[A.r0 + i] = INT32_MAX;
         A.rd_flag[A.r0 + i] = INT32_MAX;
-    }
-    const uint64_t n = A.tab_base[A.RG];
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
-        A.tb_key[i] = EMPTY_KEY;
-        A.tb_mc[i] = 0;
-        A.tb_min[i] = INT32_MAX;
-        A.tb_max[i] = -1;
     }
 }
 
@@ -989,10 +991,11 @@ __device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t
     return lo;
 }
 
-// Adds one (seed, bucket) contribution to read lr's bucket table; the insert
-// that claims an empty slot counts the read's new bucket and lists its slot.
+// Adds a bucket's partial sums (mc = sum of (matches << 32) + codirection,
+// and the r range) to read lr's global bucket table; the insert that claims an
+// empty slot counts the read's new bucket and lists its slot.
 __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t base, uint64_t cap, uint32_t g,
-                                        uint32_t matches, int32_t codir, int32_t rmin, int32_t rmax) {
+                                        uint64_t mcv, int32_t rmin, int32_t rmax) {
     const uint32_t lg = 63 - __clzll(cap);
     const uint32_t mask = (uint32_t)(cap - 1);
     uint32_t pos = ((g * 0x9E3779B1u) >> (32 - lg)) & mask;
@@ -1006,93 +1009,397 @@ __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t 
         if (old == g) break;
         pos = (pos + 1u) & mask;
     }
-    atomicAdd((unsigned long long *)&A.tb_mc[base + pos],
-              (unsigned long long)(((uint64_t)matches << 32) + (uint64_t)(int64_t)codir));
+    atomicAdd((unsigned long long *)&A.tb_mc[base + pos], (unsigned long long)mcv);
     atomicMin(&A.tb_min[base + pos], rmin);
     atomicMax(&A.tb_max[base + pos], rmax);
 }
 
-// match_seeds (src/shmap/seeding.rs:98-212) as one flat pass over every hit of
-// every consumed seed of the group.  A seed's hits are in (segm_id, pos) order,
-// so the first hit of each (segment, half-bucket) run emits the clamped
-// per-(seed, bucket) contribution of bucket b (runs b and b+1) and -- unless the
-// previous run, at b-1, already did -- of bucket b-1 (run b alone).  Counts come
-// from binary searches, strand agreement from the strand prefix.  This is the
-// work of one examined hit f of group read lr.
-__device__ __forceinline__ void contrib_hit(const MapArgs &A, const DevIndex &I, uint64_t f, uint32_t lr, int ap) {
-    {
-        const int32_t rd = A.r0 + (int32_t)lr;
-        const uint32_t local = (uint32_t)(f - A.hit_base[lr]);
-        const uint64_t o = A.koff[rd];
-        uint32_t lo = 0, hi = (uint32_t)A.rd_c[rd];
+// A wave's bucket table in LDS: LTAB slots, of which at most LTAB_MAX are
+// filled before the table is emptied into the read's global table (or the
+// read is handed to that path).
+constexpr uint32_t LTAB_BITS = 8;
+constexpr uint32_t LTAB = 1u << LTAB_BITS;
+constexpr uint32_t LTAB_MAX = LTAB * 7 / 8;
+// One wave takes a whole read up to CONTRIB_WAVE_E examined hits.  Larger
+// reads, and reads with more buckets than one LDS table holds, are split into
+// chunks, one wave per chunk, whose LDS tables are merged in the read's global
+// table (k_contrib_chunks).  The chunk length is chosen per group: the largest
+// power of two from CONTRIB_CHUNK_MIN to CONTRIB_CHUNK_MAX hits that still
+// gives CONTRIB_CHUNKS_WANT chunks, so a few listed reads are spread over the
+// device and many are taken in long chunks (fewer merges).
+constexpr uint32_t CONTRIB_WAVE_E = 4096;
+constexpr uint32_t CONTRIB_CHUNK_MIN = 512, CONTRIB_CHUNK_MAX = 4096;
+constexpr uint64_t CONTRIB_CHUNKS_WANT = 8192;
+
+enum ContribMode { CM_LDS, CM_LDS_FLUSH };
+
+struct LTab {
+    uint32_t *key, *mark, *used;
+    uint64_t *mc;
+    int32_t *mn, *mx;
+};
+
+__device__ __forceinline__ void ltab_clear(const LTab &T, uint32_t lane) {
+    for (uint32_t i = lane; i < LTAB; i += WAVE) {
+        T.key[i] = EMPTY_KEY;
+        T.mc[i] = 0;
+        T.mn[i] = INT32_MAX;
+        T.mx[i] = -1;
+    }
+}
+
+__device__ __forceinline__ void ltab_add(const LTab &T, uint32_t g, uint64_t mcv, int32_t rmin, int32_t rmax) {
+    uint32_t pos = (g * 0x9E3779B1u) >> (32 - LTAB_BITS);
+    for (;;) {
+        const uint32_t old = atomicCAS(&T.key[pos], EMPTY_KEY, g);
+        if (old == EMPTY_KEY) {
+            atomicAdd(T.used, 1u);
+            break;
+        }
+        if (old == g) break;
+        pos = (pos + 1u) & (LTAB - 1);
+    }
+    atomicAdd((unsigned long long *)&T.mc[pos], (unsigned long long)mcv);
+    atomicMin(&T.mn[pos], rmin);
+    atomicMax(&T.mx[pos], rmax);
+}
+
+// Empties the LDS table into read lr's global table.
+__device__ __forceinline__ void ltab_flush(const MapArgs &A, const LTab &T, uint32_t lr, uint64_t tb, uint64_t cap,
+                                           uint32_t lane) {
+    for (uint32_t i = lane; i < LTAB; i += WAVE) {
+        const uint32_t k = T.key[i];
+        if (k != EMPTY_KEY) {
+            tab_add(A, lr, tb, cap, k, T.mc[i], T.mn[i], T.mx[i]);
+            T.key[i] = EMPTY_KEY;
+            T.mc[i] = 0;
+            T.mn[i] = INT32_MAX;
+            T.mx[i] = -1;
+        }
+    }
+    if (lane == 0) *T.used = 0;
+    wave_sync();
+}
+
+// match_seeds (src/shmap/seeding.rs:98-212) over positions [pa, pb) of read
+// lr's examined hits, by one wave into its LDS table.  The examined hits are
+// each consumed seed's hit run in seed order, flattened: seed q covers
+// positions [sd_epre[q], sd_epre[q] + cnt).  They are taken 64 at a time, lane
+// j at position p0 + j.  From the first seed with hits on (rd_z) the seeds'
+// first positions strictly increase, so a window's seeds are the current one
+// and those whose first position falls inside it: each lane marks its seed's
+// start, and a lane's seed is the current one plus the marks at or below it.
+// A seed's hits are in (segm_id, pos) order; the first hit of each (segment,
+// half-bucket) run emits the clamped per-(seed, bucket) contribution of bucket
+// b (the hits up to the first at half-bucket b+2 or in another segment or
+// seed) and -- unless the previous run, at b-1, already did -- of bucket b-1
+// (up to the first at half-bucket b+1), with strand agreement from the strand
+// prefix.  An end inside the window is the next run head; an end past it
+// leaves the contribution pending (at most three: the last run's two and the
+// run before it), settled by one ballot in the window where its run ends, or
+// by binary search at the end of the range.  CM_LDS returns false, the table
+// partly filled, when the table would pass LTAB_MAX; CM_LDS_FLUSH then
+// empties it into the global table and goes on.
+template <int MODE>
+__device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab &T, int32_t lr, uint32_t pa,
+                                uint32_t pb, uint32_t E, uint32_t lane, uint32_t &stamp) {
+    const int ap = A.abs_pos;
+    const uint64_t upto = (((1ull << lane) - 1) << 1) | 1ull;
+    const int32_t rd = A.r0 + lr;
+    const uint64_t o = A.koff[rd];
+    const uint32_t c = (uint32_t)A.rd_c[rd], z = (uint32_t)A.rd_z[rd];
+    const int32_t hl = A.rd_hl[rd];
+    uint32_t qc = z;  // the seed holding position p0
+    if (pa > 0) {
+        uint32_t lo = z, hi = c;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (A.sd_epre[o + mid] <= local) lo = mid + 1; else hi = mid;
+            if (A.sd_epre[o + mid] <= pa) lo = mid + 1; else hi = mid;
         }
-        const uint32_t q = lo - 1;
-        const uint32_t sid = A.sd_ord[o + q];
-        const uint32_t j = local - A.sd_epre[o + q];
-        const uint32_t hoff = A.sd_hoff[o + sid], cnt = A.sd_cnt[o + sid];
-        const uint32_t occs = A.sd_occs[o + sid];
-        const uint32_t sstrand = A.sd_strand[o + sid];
-        const int32_t hl = A.rd_hl[rd];
-        const uint32_t idx = hoff + j;
-        const uint32_t seg = I.hit_seg[idx];
-        const int32_t b = hit_pos(I, idx, seg, ap) / hl;
-        bool prev_bm1 = false;
-        if (j > 0) {
-            const uint32_t seg0 = I.hit_seg[idx - 1];
-            if (seg0 == seg) {
-                const int32_t b0 = hit_pos(I, idx - 1, seg0, ap) / hl;
-                if (b0 == b) return;  // inside a run
-                prev_bm1 = b0 == b - 1;
+        qc = lo - 1;
+    }
+    // one contribution: hits [idx0, e) of a seed (occs | strand << 31) to bucket bb
+    auto add = [&](uint32_t e, uint32_t idx0, uint32_t ocv, int32_t rmin0, uint32_t sg, int32_t bb) {
+        const uint32_t n = e - idx0, occs = ocv & 0x7fffffffu;
+        const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx0];
+        const uint32_t same = (ocv >> 31) ? ones : n - ones;
+        const uint32_t matches = n < occs ? n : occs;
+        const int32_t codir = (int32_t)(2 * same) - (int32_t)n;
+        ltab_add(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir, rmin0,
+                 (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
+    };
+    // pending contributions: lane i < 3 holds item i (pmask, uniform): its
+    // seed, segment and first position/hit, the half-bucket that ends it, the
+    // bucket, the seed's end, occs | strand and the first hit's r
+    uint32_t pmask = 0;
+    uint32_t pq = 0, psg = 0, ppos = 0, pidx = 0, pee = 0, poc = 0;
+    int32_t ptb = 0, pbb = 0, prmin = 0;
+    for (uint32_t p0 = pa; p0 < pb; p0 += WAVE) {
+        ++stamp;
+        const uint32_t qi = qc + lane;
+        const bool qv = qi < c;
+        const uint32_t st = qv ? A.sd_epre[o + qi] : 0xffffffffu;
+        const uint32_t sid = qv ? A.sd_ord[o + qi] : 0u;
+        if (lane > 0 && st > p0 && st < p0 + WAVE) T.mark[st - p0] = stamp;
+        wave_sync();
+        const uint64_t M = __ballot(T.mark[lane] == stamp);
+        const uint32_t ns = (uint32_t)__popcll(M), d = (uint32_t)__popcll(M & upto);
+        // lanes 0..ns hold the window's seeds
+        uint32_t sa = 0, send = 0, socc = 0;
+        if (lane <= ns && qv) {
+            const uint32_t hoff = A.sd_hoff[o + sid];
+            sa = hoff - st;  // hit index = sa + position
+            send = hoff + A.sd_cnt[o + sid];
+            socc = A.sd_occs[o + sid] | ((uint32_t)A.sd_strand[o + sid] << 31);
+        }
+        // every hit of the window is read (run heads past pb end runs before
+        // it); only those before pb emit
+        const uint32_t p = p0 + lane, q = qc + d;
+        const bool live = p < E, act = p < pb;
+        const uint32_t a = __shfl(sa, (int)d), ee = __shfl(send, (int)d), oc = __shfl(socc, (int)d);
+        const uint32_t s0 = __shfl(st, (int)d);
+        const uint32_t idx = a + p;
+        uint32_t seg = 0xffffffffu;
+        int32_t bk = -1;
+        if (live) {
+            seg = I.hit_seg[idx];
+            bk = hit_pos(I, idx, seg, ap) / hl;
+        }
+        const uint32_t pseg = __shfl_up(seg, 1);
+        const int32_t pbk = __shfl_up(bk, 1);
+        bool head = false, prev_bm1 = false;
+        if (live) {
+            if (p == s0) {
+                head = true;
+            } else {
+                uint32_t sg0 = pseg;
+                int32_t b0 = pbk;
+                if (lane == 0) {
+                    sg0 = I.hit_seg[idx - 1];
+                    b0 = hit_pos(I, idx - 1, sg0, ap) / hl;
+                }
+                head = sg0 != seg || b0 != bk;
+                prev_bm1 = sg0 == seg && b0 == bk - 1;
             }
         }
-        const uint64_t tbase = A.tab_base[lr], tcap = A.rd_cap[rd];
-        const int32_t rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
-        for (int side = 0; side < 2; side++) {
-            if (side == 1 && (b == 0 || prev_bm1)) break;
-            const int32_t bb = side == 0 ? b : b - 1;
-            const uint32_t e = hits_lower_bound(I, idx, hoff + cnt, seg, (int64_t)(bb + 2) * hl, ap);
-            const uint32_t n = e - idx;
-            const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx];
-            const uint32_t same = sstrand ? ones : n - ones;
-            tab_add(A, lr, tbase, tcap, (uint32_t)bucket_id(A, seg, bb, hl), n < occs ? n : occs,
-                    (int32_t)(2 * same) - (int32_t)n, rmin, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
+        // pending contributions whose run ends in this window
+        bool settle = false;
+        uint32_t e_set = 0;
+        for (uint32_t i = 0; i < 3; i++) {
+            if (!((pmask >> i) & 1u)) continue;
+            const uint32_t fq = __builtin_amdgcn_readlane(pq, i), fsg = __builtin_amdgcn_readlane(psg, i);
+            const int32_t ftb = __builtin_amdgcn_readlane(ptb, i);
+            const uint64_t m = __ballot(!live || q != fq || seg != fsg || bk >= ftb);
+            if (m) {
+                const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                if (lane == i) {
+                    settle = true;
+                    e_set = pidx + (p0 + l - ppos);
+                }
+                pmask &= ~(1u << i);
+            }
+        }
+        const bool emit = head && act;
+        const uint32_t nh = (uint32_t)__popcll(__ballot(emit));
+        if (*T.used + 2 * nh + 3 > LTAB_MAX) {
+            if (MODE == CM_LDS) return false;
+            ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[rd], lane);
+        }
+        if (settle) add(e_set, pidx, poc, prmin, psg, pbb);
+        // next run heads (positions past the read's end count as heads)
+        const uint64_t H = __ballot(head || !live);
+        const uint64_t h1 = H & ~upto;
+        const uint32_t nh1 = h1 ? (uint32_t)__ffsll((unsigned long long)h1) - 1 : WAVE;
+        const int src1 = (int)(nh1 < WAVE ? nh1 : WAVE - 1);
+        const uint32_t q1 = __shfl(q, src1), sg1 = __shfl(seg, src1);
+        const int32_t b1 = __shfl(bk, src1);
+        bool pend0 = false, pend1 = false;
+        int32_t rmin = 0;
+        if (emit) {
+            const bool next_b1 = nh1 < WAVE && p0 + nh1 < E && q1 == q && sg1 == seg && b1 == bk + 1;
+            const uint64_t h2 = nh1 + 1 < WAVE ? (H >> (nh1 + 1)) : 0ull;
+            const uint32_t nh2 = h2 ? nh1 + (uint32_t)__ffsll((unsigned long long)h2) : WAVE;
+            const uint32_t nb = next_b1 ? nh2 : nh1;  // the run head ending bucket bk
+            rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
+            if (nb < WAVE) add(idx + (nb - lane), idx, oc, rmin, seg, bk);
+            else pend0 = true;
+            if (bk > 0 && !prev_bm1) {
+                if (nh1 < WAVE) add(idx + (nh1 - lane), idx, oc, rmin, seg, bk - 1);
+                else pend1 = true;
+            }
+        }
+        // this window's open contributions join the pending ones
+        uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
+        while (m0 | m1) {
+            const bool s1 = m0 == 0;
+            const uint32_t L = (uint32_t)__ffsll((unsigned long long)(s1 ? m1 : m0)) - 1;
+            if (s1) m1 &= m1 - 1; else m0 &= m0 - 1;
+            const uint32_t slot = (uint32_t)__ffs(~pmask & 7u) - 1;
+            pmask |= 1u << slot;
+            const uint32_t vq = __builtin_amdgcn_readlane(q, L), vsg = __builtin_amdgcn_readlane(seg, L);
+            const uint32_t vpos = __builtin_amdgcn_readlane(p, L), vidx = __builtin_amdgcn_readlane(idx, L);
+            const uint32_t vee = __builtin_amdgcn_readlane(ee, L), voc = __builtin_amdgcn_readlane(oc, L);
+            const int32_t vbk = __builtin_amdgcn_readlane(bk, L), vrmin = __builtin_amdgcn_readlane(rmin, L);
+            if (lane == slot) {
+                pq = vq;
+                psg = vsg;
+                ppos = vpos;
+                pidx = vidx;
+                pee = vee;
+                poc = voc;
+                ptb = vbk + (s1 ? 1 : 2);
+                pbb = vbk - (s1 ? 1 : 0);
+                prmin = vrmin;
+            }
+        }
+        // the range ends: what is still open ends past it
+        if (p0 + WAVE >= pb && pmask) {
+            if (lane < 3 && ((pmask >> lane) & 1u)) {
+                const uint32_t from = pidx + (p0 + WAVE - ppos);
+                add(hits_lower_bound(I, from, pee, psg, (int64_t)ptb * hl, ap), pidx, poc, prmin, psg, pbb);
+            }
+            pmask = 0;
+        }
+        // the seed holding p0 + 64: every seed starting at or before it
+        uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= p0 + WAVE));
+        if (adv == WAVE - 1 && qc + WAVE < c && A.sd_epre[o + qc + WAVE] <= p0 + WAVE) adv++;
+        qc += adv;
+        wave_sync();
+    }
+    return true;
+}
+
+#define LTAB_DECLARE()                                                                         \
+    __shared__ uint32_t s_key[NWAVES][LTAB];                                                   \
+    __shared__ uint64_t s_mc[NWAVES][LTAB];                                                    \
+    __shared__ int32_t s_mn[NWAVES][LTAB], s_mx[NWAVES][LTAB];                                 \
+    __shared__ uint32_t s_mark[NWAVES][WAVE];                                                  \
+    __shared__ uint32_t s_used[NWAVES];                                                        \
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); \
+    const LTab T{s_key[w], s_mark[w], &s_used[w], s_mc[w], s_mn[w], s_mx[w]};                  \
+    ltab_clear(T, lane);                                                                       \
+    T.mark[lane] = 0;                                                                          \
+    if (lane == 0) *T.used = 0;                                                                \
+    wave_sync();                                                                               \
+    uint32_t stamp = 0
+
+// One wave per read with at most CONTRIB_WAVE_E examined hits: its buckets end
+// as the first grp_nb rows of its table region, listed in order.  Larger
+// reads, and reads with more than LTAB_MAX buckets, are listed for
+// k_contrib_chunks.
+__global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
+    LTAB_DECLARE();
+    const uint64_t below = (1ull << lane) - 1;
+    for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
+        const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
+        if (E == 0) continue;
+        if (E > CONTRIB_WAVE_E) {
+            if (lane == 0) A.ov_list[atomicAdd(A.ov_count, 1u)] = (uint32_t)lr;
+            continue;
+        }
+        if (!contrib_windows<CM_LDS>(A, I, T, lr, 0, E, E, lane, stamp)) {  // too many buckets
+            ltab_clear(T, lane);
+            if (lane == 0) {
+                *T.used = 0;
+                A.ov_list[atomicAdd(A.ov_count, 1u)] = (uint32_t)lr;
+            }
+            wave_sync();
+            continue;
+        }
+        // the used slots become rows 0.. of the read's table region
+        const uint64_t tb = A.tab_base[lr];
+        uint32_t nb = 0;
+        for (uint32_t s0 = 0; s0 < LTAB; s0 += WAVE) {
+            const uint32_t sl = s0 + lane;
+            const uint32_t k = T.key[sl];
+            const bool used = k != EMPTY_KEY;
+            const uint64_t bm = __ballot(used);
+            if (used) {
+                const uint32_t at = nb + (uint32_t)__popcll(bm & below);
+                A.tb_key[tb + at] = k;
+                A.tb_mc[tb + at] = T.mc[sl];
+                A.tb_min[tb + at] = T.mn[sl];
+                A.tb_max[tb + at] = T.mx[sl];
+                A.tb_list[tb + at] = at;
+                T.key[sl] = EMPTY_KEY;
+                T.mc[sl] = 0;
+                T.mn[sl] = INT32_MAX;
+                T.mx[sl] = -1;
+            }
+            nb += (uint32_t)__popcll(bm);
+        }
+        if (lane == 0) {
+            A.grp_nb[lr] = nb;
+            *T.used = 0;
+        }
+        wave_sync();
+    }
+}
+
+// The chunk length for the listed reads (kept at ov_kpre[n + 1]) and their
+// chunk prefix (one workgroup).
+__global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
+    __shared__ uint64_t tmp[NWAVES + 1];
+    const uint32_t n = *A.ov_count;
+    uint64_t part = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += WG) {
+        const uint32_t lr = A.ov_list[i];
+        part += A.hit_base[lr + 1] - A.hit_base[lr];
+    }
+    uint64_t tot_e;
+    (void)wg_excl_scan64(part, tmp, &tot_e);
+    uint32_t cs = CONTRIB_CHUNK_MAX;
+    while (cs > CONTRIB_CHUNK_MIN && tot_e / cs < CONTRIB_CHUNKS_WANT) cs >>= 1;
+    uint64_t carry = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += WG) {
+        const uint32_t i = b0 + threadIdx.x;
+        uint64_t k = 0;
+        if (i < n) {
+            const uint32_t lr = A.ov_list[i];
+            k = (A.hit_base[lr + 1] - A.hit_base[lr] + cs - 1) / cs;
+        }
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan64(k, tmp, &tot);
+        if (i < n) A.ov_kpre[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        A.ov_kpre[n] = carry;
+        A.ov_kpre[n + 1] = cs;
+    }
+}
+
+// Empty global tables for the listed reads, one workgroup per read.
+__global__ void __launch_bounds__(WG) k_contrib_over_clear(MapArgs A) {
+    const uint32_t n = *A.ov_count;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+        const uint32_t lr = A.ov_list[t];
+        const uint64_t tb = A.tab_base[lr], cap = A.rd_cap[A.r0 + (int32_t)lr];
+        for (uint64_t i = threadIdx.x; i < cap; i += WG) {
+            A.tb_key[tb + i] = EMPTY_KEY;
+            A.tb_mc[tb + i] = 0;
+            A.tb_min[tb + i] = INT32_MAX;
+            A.tb_max[tb + i] = -1;
         }
     }
 }
 
-__global__ void __launch_bounds__(WG) k_contrib(MapArgs A, DevIndex I) {
-    // A block's WG consecutive hits lie in a run of reads [r0, r0 + nr): one
-    // binary search over the group's hit offsets finds the run, and when it is
-    // short (reads without hits can lengthen it) its offsets are staged in LDS
-    // for every thread's search.
-    __shared__ uint64_t s_hb[WG + 1];
-    __shared__ uint32_t s_r0, s_nr;
-    const uint64_t E = A.hit_base[A.RG];
-    const int ap = A.abs_pos;
-    for (uint64_t f0 = (uint64_t)blockIdx.x * WG; f0 < E; f0 += (uint64_t)gridDim.x * WG) {
-        if (threadIdx.x == 0) {
-            const uint64_t fl = f0 + WG - 1 < E ? f0 + WG - 1 : E - 1;
-            const uint32_t a = upper_bound_u64(A.hit_base, (uint32_t)A.RG + 1, f0) - 1;
-            const uint32_t b = upper_bound_u64(A.hit_base + a, (uint32_t)A.RG + 1 - a, fl) - 1 + a;
-            s_r0 = a;
-            s_nr = b - a + 1;
-        }
-        __syncthreads();
-        const uint32_t r0 = s_r0, nr = s_nr;
-        const bool staged = nr + 1 <= (uint32_t)WG + 1;
-        if (staged)
-            for (uint32_t i = threadIdx.x; i <= nr; i += WG) s_hb[i] = A.hit_base[r0 + i];
-        __syncthreads();
-        const uint64_t f = f0 + threadIdx.x;
-        if (f < E) {
-            const uint32_t lr = r0 - 1 + (staged ? upper_bound_u64(s_hb, nr + 1, f)
-                                                 : upper_bound_u64(A.hit_base + r0, nr + 1, f));
-            contrib_hit(A, I, f, lr, ap);
-        }
-        __syncthreads();
+// The listed reads in chunks (the length at ov_kpre[n + 1]), one wave per
+// chunk: each chunk is counted in LDS and merged into the read's global table
+// (when its table fills, and at the end).
+__global__ void __launch_bounds__(WG) k_contrib_chunks(MapArgs A, DevIndex I) {
+    LTAB_DECLARE();
+    const uint32_t nl = *A.ov_count;
+    const uint64_t nk = A.ov_kpre[nl];
+    const uint32_t cs = (uint32_t)A.ov_kpre[nl + 1];
+    for (uint64_t k = (uint64_t)blockIdx.x * NWAVES + w; k < nk; k += (uint64_t)gridDim.x * NWAVES) {
+        const uint32_t i = upper_bound_u64(A.ov_kpre, nl + 1, k) - 1;
+        const int32_t lr = (int32_t)A.ov_list[i];
+        const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
+        const uint32_t pa = (uint32_t)(k - A.ov_kpre[i]) * cs;
+        const uint32_t pb = pa + cs < E ? pa + cs : E;
+        contrib_windows<CM_LDS_FLUSH>(A, I, T, lr, pa, pb, E, lane, stamp);
+        ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[A.r0 + lr], lane);
     }
 }
 
@@ -2155,8 +2462,15 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
         k_seed_wave<<<PGRID(k_seed_wave, SEED_WAVE_LDS, (A.R + NWAVES - 1) / NWAVES), WG, SEED_WAVE_LDS, st>>>(A, I);
         break;
     case ST_SEED_WG: k_seed_wg<<<read_grid(A.R, max_grid), WG, SORT_LDS, st>>>(A, I); break;
-    case ST_TAB_INIT: k_tab_init<<<flat_grid(flat_n, 8192), WG, 0, st>>>(A); break;
-    case ST_CONTRIB: k_contrib<<<flat_grid(flat_n, 16384), WG, 0, st>>>(A, I); break;
+    case ST_TAB_INIT: k_tab_init<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A); break;
+    case ST_CONTRIB:
+        // one wave per read (the grid covers every read: reads differ widely
+        // in cost), then the listed reads in chunks through global tables
+        k_contrib_wave<<<(uint32_t)((A.RG + NWAVES - 1) / NWAVES), WG, 0, st>>>(A, I);
+        k_contrib_over_scan<<<1, WG, 0, st>>>(A);
+        k_contrib_over_clear<<<max_grid, WG, 0, st>>>(A);
+        k_contrib_chunks<<<PGRID(k_contrib_chunks, 0, max_grid), WG, 0, st>>>(A, I);
+        break;
     // per-read workgroup kernels with data-dependent cost: a grid above the
     // resident count lets the hardware backfill as reads finish
     case ST_EXTRACT:

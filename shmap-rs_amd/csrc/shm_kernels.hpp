@@ -20,7 +20,8 @@ constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
 constexpr uint32_t PRUNE_SEQ_SEEDS = 128;   // remaining seeds a thread extends a bucket with alone
-constexpr uint32_t TASK_COUNTERS = 6;       // per group: first-pass, second band, 3 refine rounds, prune list
+constexpr uint32_t TASK_COUNTERS = 7;       // per group: first-pass, second band, 3 refine rounds, prune list,
+                                            // reads on the global-table bucket path
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
@@ -74,6 +75,7 @@ struct MapArgs {
     // per read (whole batch)
     int32_t *rd_u, *rd_possible, *rd_S, *rd_hl, *rd_ok, *rd_c, *rd_bseeds, *rd_max_seed,
         *rd_seed_matches;
+    int32_t *rd_z;  // consumed seeds without hits (they sort first): rd_z is the first with hits
     uint64_t *rd_E, *rd_cap;
     // per group read (local index), RG+1 entries
     const uint64_t *hit_base, *tab_base, *bk_base;
@@ -111,6 +113,11 @@ struct MapArgs {
     uint32_t *big_list, *big_count;
     // group reads whose bucket tables exceed one wave's slice, for k_extract
     uint32_t *ext_list, *ext_count;
+    // group reads whose buckets go through their global tables in chunks
+    // (k_contrib_chunks), listed by k_contrib_wave; ov_kpre holds their chunk
+    // prefix and then the chunk length
+    uint32_t *ov_list, *ov_count;
+    uint64_t *ov_kpre;
     // k_final's batch totals: the mapped reads (for the host's finalization)
     // and the sum of distinct k-mers
     uint32_t *mapped_list, *mapped_count;

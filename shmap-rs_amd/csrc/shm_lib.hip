@@ -555,7 +555,7 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
 static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "seed_wg", "tab_init",
                                             "contrib", "extract",        "prune",     "refine",  "final"};
 
-constexpr int SHM_N_COUNTS = 8;  // see shm_mapper_last_counts
+constexpr int SHM_N_COUNTS = 9;  // see shm_mapper_last_counts
 
 struct shm_mapper {
     const shm_index *idx = nullptr;
@@ -579,7 +579,7 @@ struct shm_mapper {
     DevBuf<int32_t> k_r;
     DevBuf<uint8_t> k_s, sd_strand;
     DevBuf<uint32_t> ps_v, sd_first, sd_occs, sd_cnt, sd_hoff, sd_ord, sd_epre;
-    DevBuf<int32_t> rd_i32;  // 10 per-read int arrays
+    DevBuf<int32_t> rd_i32;  // 12 per-read int arrays
     DevBuf<uint64_t> rd_u64; // 2 per-read u64 arrays
     DevBuf<uint64_t> grp_base; // 3 x (RG+1): hit, table and bucket offsets of a group's reads
     DevBuf<uint64_t> grp_nb, part, stat;
@@ -596,6 +596,8 @@ struct shm_mapper {
     DevBuf<uint2> tasks, sel;
     DevBuf<uint32_t> big;  // [0] count, then the reads seeded by a workgroup
     DevBuf<uint32_t> ext;  // [0] count, then the group reads extracted by a workgroup
+    DevBuf<uint32_t> ov;       // the group reads bucketed through their global tables
+    DevBuf<uint64_t> ov_kpre;  // their chunk prefix, then the chunk length
     DevBuf<uint32_t> mapped;   // [0] count, then the mapped reads
     DevBuf<uint64_t> u_total;  // sum of distinct k-mers
     PinnedBuf<uint32_t> h_mapped;
@@ -814,6 +816,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_bseeds = ri + 6 * (size_t)R;
     A.rd_max_seed = ri + 7 * (size_t)R;
     A.rd_seed_matches = ri + 8 * (size_t)R;
+    A.rd_z = ri + 9 * (size_t)R;
     A.rd_flag = ri + 10 * (size_t)R;
     A.rd_first = ri + 11 * (size_t)R;
     A.rd_E = mp->rd_u64.p;
@@ -915,8 +918,17 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.tb_mc = mp->tb_mc.p;
         A.tb_min = mp->tb_min.p;
         A.tb_max = mp->tb_max.p;
+        // the group's task counters ([6] counts the reads k_contrib_wave hands
+        // to the global tables)
+        uint32_t *task_count = mp->task_count.p + TASK_COUNTERS * (size_t)ngroups;
+        HIPCHK(hipMemsetAsync(task_count, 0, TASK_COUNTERS * 4, st));
+        RCHK(mp->ov.ensure((size_t)RG + 1));
+        RCHK(mp->ov_kpre.ensure((size_t)RG + 2));
+        A.ov_count = task_count + 6;
+        A.ov_list = mp->ov.p;
+        A.ov_kpre = mp->ov_kpre.p;
         TIME_BEGIN(ST_TAB_INIT);
-        launch_stage(ST_TAB_INIT, A, I, nullptr, T, mp->max_grid, st);
+        launch_stage(ST_TAB_INIT, A, I, nullptr, RG, mp->max_grid, st);
         TIME_END(ST_TAB_INIT);
         TIME_BEGIN(ST_CONTRIB);
         launch_stage(ST_CONTRIB, A, I, nullptr, E_grp, mp->max_grid, st);
@@ -925,11 +937,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(mp->h_sum.p + 3, mp->grp_base.p + 2 * (size_t)(RG + 1) + RG, 8, hipMemcpyDeviceToHost,
                               st));
-        // the group's list and task counters, cleared while the host waits
+        // the group's extract list, cleared while the host waits
         RCHK(mp->ext.ensure((size_t)RG + 2));
-        uint32_t *task_count = mp->task_count.p + TASK_COUNTERS * (size_t)ngroups;
         HIPCHK(hipMemsetAsync(mp->ext.p, 0, 4, st));
-        HIPCHK(hipMemsetAsync(task_count, 0, TASK_COUNTERS * 4, st));
         HIPCHK(hipStreamSynchronize(st));
         RCHK(flush_timing(mp, pending));
         const uint64_t B = mp->h_sum.p[3];
@@ -1011,12 +1021,13 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     // only mapped reads need the host's double-precision finalization
     const uint32_t n_mapped = mp->h_mapped.p[0];
     for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
-    int64_t cand = 0, deferred = 0, refined = 0, seeds = 0;
+    int64_t cand = 0, deferred = 0, refined = 0, seeds = 0, global_reads = 0;
     for (int32_t g = 0; g < ngroups; g++) {
         const uint32_t *tc = &ntask[TASK_COUNTERS * (size_t)g];
         cand += tc[0];
         deferred += tc[1];
         refined += tc[2] + tc[3] + tc[4];
+        global_reads += tc[6];
     }
     seeds = (int64_t)mp->h_sum.p[4];
     mp->counts[0] = (int64_t)M;
@@ -1027,6 +1038,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     mp->counts[5] = cand;
     mp->counts[6] = deferred;
     mp->counts[7] = (int64_t)E_all;
+    mp->counts[8] = global_reads;
     return SHM_OK;
 }
 

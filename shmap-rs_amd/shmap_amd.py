@@ -303,9 +303,9 @@ class Mapper:
 
     def last_counts(self) -> dict:
         """Work counts of the last batch (k-mers, seeds, buckets, refined buckets, groups)."""
-        arr = (C.c_int64 * 8)()
+        arr = (C.c_int64 * len(COUNT_NAMES))()
         nw = C.c_int32()
-        _check(lib().shm_mapper_last_counts(self._p, arr, 8, C.byref(nw)))
+        _check(lib().shm_mapper_last_counts(self._p, arr, len(COUNT_NAMES), C.byref(nw)))
         return {COUNT_NAMES[i]: int(arr[i]) for i in range(min(nw.value, len(COUNT_NAMES)))}
 
 
@@ -382,7 +382,7 @@ def host_array(n: int, dtype) -> np.ndarray:
 
 
 COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band",
-               "examined_hits")
+               "examined_hits", "global_table_reads")
 
 
 def format_paf(index: SketchIndex, params: MapParams, qid: str, rec, map_time: float = 0.0) -> str:
