@@ -23,7 +23,8 @@ enum shm_status {
     SHM_ERR_ARG = 1,     /* invalid argument (the reference's Params::validate) */
     SHM_ERR_HIP = 2,     /* HIP runtime failure */
     SHM_ERR_NOMEM = 3,   /* device or host allocation failed */
-    SHM_ERR_CAPACITY = 4 /* caller buffer too small; the needed size is returned */
+    SHM_ERR_CAPACITY = 4, /* caller buffer too small; the needed size is returned */
+    SHM_ERR_IO = 5        /* a file could not be opened, read or written */
 };
 
 /* Metric -- src/types.rs:300-322 */
@@ -138,6 +139,22 @@ int shm_index_segment(const shm_index *idx, int32_t segm_id, const char **name, 
 /* get_kmer_stats counters -- src/index.rs:343-376 */
 int shm_index_stats(const shm_index *idx, int64_t *indexed_kmers, int64_t *indexed_hits,
                     int64_t *highest_freq, int64_t *blacklisted_kmers, int64_t *blacklisted_hits);
+/* The parameters the index was built with: k and h_frac of its sketcher and
+ * max_matches (0: -M unset).  No reference counterpart (the caller's own
+ * Params hold them there). */
+int shm_index_params(const shm_index *idx, int32_t *k, double *h_frac, int32_t *max_matches);
+
+/* Persistent index.  shm_index_save writes the whole device index (parameters,
+ * segment table, sketch, hit arrays and hash table) to a file;
+ * shm_index_load reads it onto device_id, and the loaded index maps exactly as
+ * the built one.  No reference counterpart: shmap-rs rebuilds its index from
+ * the FASTA on every run, and names an on-disk index as its next step
+ * (RESULTS.md:1646-1650).  shm_index_load checks the file's magic, version,
+ * header and size before it touches the device (SHM_ERR_ARG with a message
+ * for a file that is not an index, of another version, or truncated;
+ * SHM_ERR_IO when it cannot be opened). */
+int shm_index_save(const shm_index *idx, const char *path);
+int shm_index_load(const char *path, int32_t device_id, shm_index **out);
 
 /* create_mapper -- src/mapper.rs:25-39: a mapper over a built index, holding
  * the device scratch for batches of reads. */

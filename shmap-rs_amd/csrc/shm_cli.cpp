@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/time.h>
+#include <zlib.h>
 
 #include <condition_variable>
 #include <deque>
@@ -38,6 +39,7 @@ struct Params {
     int device = 0;
     long batch = 16384;
     bool dump_reads = false;
+    std::string save_index;
 };
 
 const char *metric_name(int32_t m) {
@@ -85,12 +87,22 @@ std::string rust_f64(double v) {
     return out;
 }
 
+// True when path holds an index written by --save-index (its magic bytes).
+static bool is_index_file(const std::string &path) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char m[8];
+    const bool yes = fread(m, 1, 8, f) == 8 && memcmp(m, "SHMIDX01", 8) == 0;
+    fclose(f);
+    return yes;
+}
+
 void usage(FILE *f) {
     fprintf(f,
             "Map-SHmap: fast and accurate sketch-based read mapper (MI355X)\n\n"
             "Usage: shmap -p <P_FILE> -s <T_FILE> [OPTIONS]\n\n"
             "  -p, --pattern <P_FILE>        Reads file (FASTA format)\n"
-            "  -s, --text <T_FILE>           Reference file (FASTA format)\n"
+            "  -s, --text <T_FILE>           Reference file (FASTA format), or an index written by --save-index\n"
             "  -k, --ksize <K>               K-mer length [default: 15]\n"
             "  -r, --hashratio <H_FRAC>      FracMinHash ratio in (0,1] [default: 0.05]\n"
             "  -S, --max_seeds <MAX_SEEDS>   Maximum seeds in a sketch (accepted, unused)\n"
@@ -112,6 +124,7 @@ void usage(FILE *f) {
             "      --per-read-stats-sample <N>  Keep every Nth read [default: 1]\n"
             "      --device <ID>             GPU device [default: 0]\n"
             "      --batch <N>               Reads per GPU batch [default: 16384]\n"
+            "      --save-index <FILE>       Write the index built from -s to FILE (reused as -s FILE)\n"
             "      --dump-reads              Print the reads of -p as parsed (FASTA) and exit\n");
 }
 
@@ -140,14 +153,49 @@ struct Batch {
 // FASTA/FASTQ records as needletail delivers them (src/io/mod.rs:383-427): the
 // id cut at the first space or tab, line breaks and '\r' removed.  The input
 // is read in large blocks and split into lines with memchr; a record's bases
-// are copied straight into the caller's storage.
+// are copied straight into the caller's storage.  Plain and gzip input (by
+// its magic bytes, any number of members) are read; bzip2, xz and zstd input,
+// which needletail also reads, is refused with a message.
 class FastxReader {
   public:
-    explicit FastxReader(const std::string &path) : f_(fopen(path.c_str(), "rb")), buf_(1u << 24) {}
+    explicit FastxReader(const std::string &path) : buf_(1u << 24) {
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) {
+            err_ = "failed to open \"" + path + "\"";
+            return;
+        }
+        unsigned char m[6] = {0, 0, 0, 0, 0, 0};
+        const size_t got = fread(m, 1, sizeof m, f);
+        const char *fmt = nullptr;
+        if (got >= 3 && m[0] == 'B' && m[1] == 'Z' && m[2] == 'h') fmt = "bzip2";
+        else if (got >= 6 && m[0] == 0xfd && m[1] == '7' && m[2] == 'z' && m[3] == 'X' && m[4] == 'Z' && m[5] == 0) fmt = "xz";
+        else if (got >= 4 && m[0] == 0x28 && m[1] == 0xb5 && m[2] == 0x2f && m[3] == 0xfd) fmt = "zstd";
+        if (fmt) {
+            fclose(f);
+            err_ = std::string("\"") + path + "\" is " + fmt +
+                   "-compressed; this build reads plain or gzip FASTA/FASTQ (decompress it first)";
+            return;
+        }
+        if (got >= 2 && m[0] == 0x1f && m[1] == 0x8b) {
+            fclose(f);
+            gz_ = gzopen(path.c_str(), "rb");
+            if (!gz_) {
+                err_ = "failed to open \"" + path + "\"";
+                return;
+            }
+            gzbuffer(gz_, 1u << 20);
+        } else {
+            rewind(f);
+            f_ = f;
+        }
+    }
     ~FastxReader() {
         if (f_) fclose(f_);
+        if (gz_) gzclose(gz_);
     }
-    bool ok() const { return f_ != nullptr; }
+    bool ok() const { return f_ != nullptr || gz_ != nullptr; }
+    // why the input could not be opened or read ("" if it was read whole)
+    const std::string &error() const { return err_; }
     bool next(Record &r) {
         r.name.clear();
         r.seq.clear();
@@ -189,7 +237,7 @@ class FastxReader {
             end_ -= pos_;
             pos_ = 0;
             if (end_ == buf_.size()) buf_.resize(buf_.size() * 2);
-            const size_t got = fread(buf_.data() + end_, 1, buf_.size() - end_, f_);
+            const size_t got = fill(buf_.data() + end_, buf_.size() - end_);
             end_ += got;
             if (got == 0) eof_ = true;
         }
@@ -227,7 +275,21 @@ class FastxReader {
         }
         return true;
     }
-    FILE *f_;
+    size_t fill(char *dst, size_t n) {
+        if (f_) return fread(dst, 1, n, f_);
+        if (!gz_ || !err_.empty()) return 0;
+        const int got = gzread(gz_, dst, (unsigned)(n < (1u << 30) ? n : (1u << 30)));
+        if (got < 0) {
+            int code = 0;
+            const char *msg = gzerror(gz_, &code);
+            err_ = std::string("failed to decompress the input: ") + (msg ? msg : "gzip error");
+            return 0;
+        }
+        return (size_t)got;
+    }
+    FILE *f_ = nullptr;
+    gzFile gz_ = nullptr;
+    std::string err_;
     std::vector<char> buf_;
     size_t pos_ = 0, end_ = 0;
     bool eof_ = false;
@@ -514,6 +576,8 @@ int main(int argc, char **argv) {
             if (p.batch < 1) p.batch = 1;
         } else if (a == "--dump-reads") {
             p.dump_reads = true;
+        } else if (a == "--save-index") {
+            p.save_index = need("--save-index <FILE>");
         } else {
             fprintf(stderr, "error: unexpected argument '%s' found\n\nFor more information, try '--help'.\n",
                     argv[i]);
@@ -524,7 +588,7 @@ int main(int argc, char **argv) {
         // the reads as parsed, one FASTA record per line pair (no device needed)
         FastxReader rd(p.p_file);
         if (!rd.ok()) {
-            fprintf(stderr, "Error: failed to open \"%s\"\n", p.p_file.c_str());
+            fprintf(stderr, "Error: %s\n", rd.error().c_str());
             return 1;
         }
         Batch b;
@@ -535,6 +599,10 @@ int main(int argc, char **argv) {
                 fwrite(b.bases.data() + b.off[i], 1, (size_t)(b.off[i + 1] - b.off[i]), stdout);
                 fputc('\n', stdout);
             }
+        if (!rd.error().empty()) {
+            fprintf(stderr, "Error: %s\n", rd.error().c_str());
+            return 1;
+        }
         return 0;
     }
     if (!have_p || !have_s) {
@@ -556,45 +624,88 @@ int main(int argc, char **argv) {
 
     // ---- index (src/main.rs:45-56)
     fprintf(stderr, "Indexing %s...\n", p.t_file.c_str());
-    std::vector<Record> ref;
-    {
-        FastxReader rd(p.t_file);
-        if (!rd.ok()) {
-            fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    failed to open \"%s\"\n",
-                    p.t_file.c_str(), p.t_file.c_str());
-            return 1;
-        }
-        Record r;
-        while (rd.next(r)) ref.push_back(std::move(r));  // next() refills the moved-from record
-    }
     shm_sketcher *sk = nullptr;
-    if (shm_sketcher_create(p.k, p.h_frac, &sk)) {
-        fprintf(stderr, "ERROR: %s\n", shm_last_error());
-        return 1;
-    }
-    std::vector<shm_segment> segs(ref.size());
-    for (size_t i = 0; i < ref.size(); i++) segs[i] = {ref[i].name.c_str(), ref[i].seq.data(), (int64_t)ref[i].seq.size()};
     shm_index *idx = nullptr;
     const double t_idx = now_s();
-    if (shm_index_build(sk, segs.data(), (int32_t)segs.size(), p.have_max_matches ? p.max_matches : 0, p.device,
-                        &idx)) {
-        fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
-                shm_last_error());
-        return 1;
+    long long nucl = 0;
+    size_t nsegs = 0;
+    if (is_index_file(p.t_file)) {
+        // an index written by --save-index: the build parameters must match
+        if (shm_index_load(p.t_file.c_str(), p.device, &idx)) {
+            fprintf(stderr, "Error: failed to load index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
+                    shm_last_error());
+            return 1;
+        }
+        int32_t ik = 0, im = 0;
+        double ir = 0;
+        shm_index_params(idx, &ik, &ir, &im);
+        const int32_t want_m = p.have_max_matches ? p.max_matches : 0;
+        if (ik != p.k || ir != p.h_frac || im != want_m) {
+            fprintf(stderr, "ERROR: the index in %s was built with -k %d -r %g -M %d; this run asks for -k %d -r %g -M %d\n",
+                    p.t_file.c_str(), ik, ir, im, p.k, p.h_frac, want_m);
+            shm_index_free(idx);
+            return 1;
+        }
+        int32_t n = 0;
+        shm_index_n_segments(idx, &n);
+        nsegs = (size_t)n;
+        for (int32_t s = 0; s < n; s++) {
+            const char *nm;
+            int32_t sz;
+            int64_t nk;
+            shm_index_segment(idx, s, &nm, &sz, &nk);
+            nucl += sz;
+        }
+    } else {
+        std::vector<Record> ref;
+        {
+            FastxReader rd(p.t_file);
+            if (!rd.ok()) {
+                fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
+                        rd.error().c_str());
+                return 1;
+            }
+            Record r;
+            while (rd.next(r)) ref.push_back(std::move(r));  // next() refills the moved-from record
+            if (!rd.error().empty()) {
+                fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
+                        rd.error().c_str());
+                return 1;
+            }
+        }
+        if (shm_sketcher_create(p.k, p.h_frac, &sk)) {
+            fprintf(stderr, "ERROR: %s\n", shm_last_error());
+            return 1;
+        }
+        std::vector<shm_segment> segs(ref.size());
+        for (size_t i = 0; i < ref.size(); i++)
+            segs[i] = {ref[i].name.c_str(), ref[i].seq.data(), (int64_t)ref[i].seq.size()};
+        if (shm_index_build(sk, segs.data(), (int32_t)segs.size(), p.have_max_matches ? p.max_matches : 0, p.device,
+                            &idx)) {
+            fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
+                    shm_last_error());
+            return 1;
+        }
+        for (auto &r : ref) nucl += (long long)r.seq.size();
+        nsegs = ref.size();
     }
     {
         int64_t ik, ih, hf, bk, bh;
         shm_index_stats(idx, &ik, &ih, &hf, &bk, &bh);
-        long long nucl = 0;
-        for (auto &r : ref) nucl += (long long)r.seq.size();
-        fprintf(stderr, " | total nucleotides:     %lld\n | index segments:        %zu\n", nucl, ref.size());
+        fprintf(stderr, " | total nucleotides:     %lld\n | index segments:        %zu\n", nucl, nsegs);
         fprintf(stderr, " | indexed kmers:         %lld\n | indexed hits:          %lld\n", (long long)ik, (long long)ih);
         fprintf(stderr, " | | most frequent kmer:      %lld times.\n | | blacklisted kmers:       %lld\n",
                 (long long)hf, (long long)bk);
         fprintf(stderr, " | indexing time:        %.1fs\n", now_s() - t_idx);
     }
-    ref.clear();
-    ref.shrink_to_fit();
+    if (!p.save_index.empty()) {
+        if (shm_index_save(idx, p.save_index.c_str())) {
+            fprintf(stderr, "Error: failed to write the index to %s\n\nCaused by:\n    %s\n", p.save_index.c_str(),
+                    shm_last_error());
+            return 1;
+        }
+        fprintf(stderr, "Index written to %s\n", p.save_index.c_str());
+    }
 
     // ---- mapping (src/shmap/mod.rs:374-650)
     shm_map_params mpp{p.k, p.h_frac, p.theta, p.min_diff, p.max_overlap, p.metric, (int32_t)p.nbp,
@@ -621,8 +732,8 @@ int main(int argc, char **argv) {
     }
     FastxReader rd(p.p_file);
     if (!rd.ok()) {
-        fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    failed to open \"%s\"\n",
-                p.p_file.c_str(), p.p_file.c_str());
+        fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
+                rd.error().c_str());
         return 1;
     }
     static char outbuf[1 << 20];
@@ -678,6 +789,11 @@ int main(int argc, char **argv) {
             n_reads += nb;
             br.recycle(batch);
         }
+    }
+    if (rc == 0 && !rd.error().empty()) {  // the reader thread has ended with the BatchReader
+        fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
+                rd.error().c_str());
+        rc = 1;
     }
     fflush(stdout);
     if (unmapped) fclose(unmapped);

@@ -901,11 +901,20 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
                 const bool consumed = q < (int32_t)u && (int64_t)pre < (int64_t)S;
                 uint32_t ctot;
                 const uint32_t cex = wg_excl_scan(consumed ? cnt : 0u, tmp, &ctot);
-                if (consumed) {
-                    A.sd_epre[o + q] = s_cnt_carry + cex;
-                    atomicAdd(&s_c, 1u);
-                    atomicAdd(&s_seeds, occs);
-                    atomicMax(&s_max, cnt);
+                if (consumed) A.sd_epre[o + q] = s_cnt_carry + cex;
+                // the consumed seeds' count, occurrences and largest hit count:
+                // one LDS atomic per wave
+                const uint64_t cb = __ballot(consumed);
+                uint32_t so_ = consumed ? occs : 0u, mm = consumed ? cnt : 0u;
+                for (int d = WAVE / 2; d > 0; d >>= 1) {
+                    so_ += __shfl_xor(so_, d);
+                    const uint32_t om = __shfl_xor(mm, d);
+                    mm = om > mm ? om : mm;
+                }
+                if ((threadIdx.x & (WAVE - 1)) == 0 && cb) {
+                    atomicAdd(&s_c, (uint32_t)__popcll(cb));
+                    atomicAdd(&s_seeds, so_);
+                    atomicMax(&s_max, mm);
                 }
                 __syncthreads();
                 if (threadIdx.x == 0) {
@@ -1048,12 +1057,14 @@ __device__ __forceinline__ void ltab_clear(const LTab &T, uint32_t lane) {
     }
 }
 
-__device__ __forceinline__ void ltab_add(const LTab &T, uint32_t g, uint64_t mcv, int32_t rmin, int32_t rmax) {
-    uint32_t pos = (g * 0x9E3779B1u) >> (32 - LTAB_BITS);
+// Adds a contribution to the LDS table; returns 1 when it claimed a new slot
+// (the caller counts claims into *T.used).
+__device__ __forceinline__ uint32_t ltab_add(const LTab &T, uint32_t g, uint64_t mcv, int32_t rmin, int32_t rmax) {
+    uint32_t pos = (g * 0x9E3779B1u) >> (32 - LTAB_BITS), claimed = 0;
     for (;;) {
         const uint32_t old = atomicCAS(&T.key[pos], EMPTY_KEY, g);
         if (old == EMPTY_KEY) {
-            atomicAdd(T.used, 1u);
+            claimed = 1;
             break;
         }
         if (old == g) break;
@@ -1062,6 +1073,7 @@ __device__ __forceinline__ void ltab_add(const LTab &T, uint32_t g, uint64_t mcv
     atomicAdd((unsigned long long *)&T.mc[pos], (unsigned long long)mcv);
     atomicMin(&T.mn[pos], rmin);
     atomicMax(&T.mx[pos], rmax);
+    return claimed;
 }
 
 // Empties the LDS table into read lr's global table.
@@ -1119,14 +1131,15 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         qc = lo - 1;
     }
     // one contribution: hits [idx0, e) of a seed (occs | strand << 31) to bucket bb
+    uint32_t claims = 0;  // this lane's new slots in the window (at most six)
     auto add = [&](uint32_t e, uint32_t idx0, uint32_t ocv, int32_t rmin0, uint32_t sg, int32_t bb) {
         const uint32_t n = e - idx0, occs = ocv & 0x7fffffffu;
         const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx0];
         const uint32_t same = (ocv >> 31) ? ones : n - ones;
         const uint32_t matches = n < occs ? n : occs;
         const int32_t codir = (int32_t)(2 * same) - (int32_t)n;
-        ltab_add(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir, rmin0,
-                 (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
+        claims += ltab_add(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
+                           rmin0, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
     };
     // pending contributions: lane i < 3 holds item i (pmask, uniform): its
     // seed, segment and first position/hit, the half-bucket that ends it, the
@@ -1201,7 +1214,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         }
         const bool emit = head && act;
         const uint32_t nh = (uint32_t)__popcll(__ballot(emit));
-        if (*T.used + 2 * nh + 3 > LTAB_MAX) {
+        if (*T.used + 2 * nh + 6 > LTAB_MAX) {  // heads, settled and closing pending contributions
             if (MODE == CM_LDS) return false;
             ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[rd], lane);
         }
@@ -1260,6 +1273,12 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
             }
             pmask = 0;
         }
+        // the window's new slots
+        const uint32_t nclaim = (uint32_t)__popcll(__ballot(claims & 1u)) + 2u * (uint32_t)__popcll(__ballot(claims & 2u)) +
+                                4u * (uint32_t)__popcll(__ballot(claims & 4u));
+        claims = 0;
+        wave_sync();
+        if (lane == 0) *T.used += nclaim;
         // the seed holding p0 + 64: every seed starting at or before it
         uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= p0 + WAVE));
         if (adv == WAVE - 1 && qc + WAVE < c && A.sd_epre[o + qc + WAVE] <= p0 + WAVE) adv++;

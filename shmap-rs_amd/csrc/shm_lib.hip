@@ -6,6 +6,7 @@
 // PAF text follows src/mapping.rs.
 #include <hip/hip_runtime.h>
 
+#include <errno.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -13,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -305,6 +307,7 @@ struct shm_index {
     int device = 0;
     int32_t k = 0;
     double h_frac = 0;
+    int32_t max_matches = 0;  // -M (<= 0: none)
     int32_t nseg = 0;
     std::vector<std::string> names;
     std::vector<int32_t> sz;
@@ -350,6 +353,7 @@ extern "C" int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, 
     idx->device = device_id;
     idx->k = sk->k;
     idx->h_frac = sk->h_frac;
+    idx->max_matches = max_matches > 0 ? max_matches : 0;
     idx->nseg = n_segments;
     hipStream_t st;
     HIPCHK(hipStreamCreate(&st));
@@ -545,6 +549,215 @@ extern "C" int shm_index_stats(const shm_index *idx, int64_t *ik, int64_t *ih, i
     if (hf) *hf = idx->highest;
     if (bk) *bk = idx->black_kmers;
     if (bh) *bh = idx->black_hits;
+    return SHM_OK;
+}
+
+extern "C" int shm_index_params(const shm_index *idx, int32_t *k, double *h_frac, int32_t *max_matches) {
+    if (!idx) return fail(SHM_ERR_ARG, "invalid argument");
+    if (k) *k = idx->k;
+    if (h_frac) *h_frac = idx->h_frac;
+    if (max_matches) *max_matches = idx->max_matches;
+    return SHM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// index file: the build parameters, the segment table and every device array,
+// little-endian, between a magic and an end marker
+// ---------------------------------------------------------------------------
+
+static const char IDX_MAGIC[8] = {'S', 'H', 'M', 'I', 'D', 'X', '0', '1'};
+static const char IDX_END[8] = {'S', 'H', 'M', 'I', 'D', 'X', 'E', 'N'};
+static const uint32_t IDX_VERSION = 1;
+static const size_t IDX_STAGE = 64ull << 20;  // bytes per staged copy
+
+struct IdxHeader {
+    char magic[8];
+    uint32_t version, nseg;
+    int32_t k, max_matches;
+    double h_frac;
+    int64_t n_kmers, indexed_kmers, indexed_hits, highest, black_kmers, black_hits;
+    uint64_t shard_base[8];
+    uint32_t shard_bits[8];
+    uint64_t table_slots, name_bytes;
+};
+static_assert(sizeof(IdxHeader) == 192, "index file header layout");
+
+static int idx_write(FILE *f, const void *p, size_t n) {
+    if (n && fwrite(p, 1, n, f) != n) return fail(SHM_ERR_IO, "write failed: %s", strerror(errno));
+    return SHM_OK;
+}
+
+static int idx_read(FILE *f, void *p, size_t n, const char *path) {
+    if (n && fread(p, 1, n, f) != n) return fail(SHM_ERR_ARG, "\"%s\" is truncated", path);
+    return SHM_OK;
+}
+
+// device array -> file, through a page-locked stage
+template <typename T>
+static int idx_put(FILE *f, const T *d, size_t n, PinnedBuf<uint8_t> &stage) {
+    size_t bytes = n * sizeof(T), done = 0;
+    while (done < bytes) {
+        const size_t c = bytes - done < IDX_STAGE ? bytes - done : IDX_STAGE;
+        HIPCHK(hipMemcpy(stage.p, (const uint8_t *)d + done, c, hipMemcpyDeviceToHost));
+        RCHK(idx_write(f, stage.p, c));
+        done += c;
+    }
+    return SHM_OK;
+}
+
+// file -> device array (allocated with n + extra entries)
+template <typename T>
+static int idx_get(FILE *f, DevBuf<T> &d, size_t n, size_t extra, PinnedBuf<uint8_t> &stage, const char *path) {
+    RCHK(d.ensure(n + extra));
+    size_t bytes = n * sizeof(T), done = 0;
+    while (done < bytes) {
+        const size_t c = bytes - done < IDX_STAGE ? bytes - done : IDX_STAGE;
+        RCHK(idx_read(f, stage.p, c, path));
+        HIPCHK(hipMemcpy((uint8_t *)d.p + done, stage.p, c, hipMemcpyHostToDevice));
+        done += c;
+    }
+    return SHM_OK;
+}
+
+extern "C" int shm_index_save(const shm_index *idx, const char *path) {
+    if (!idx || !path) return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(idx->device));
+    IdxHeader h{};
+    memcpy(h.magic, IDX_MAGIC, 8);
+    h.version = IDX_VERSION;
+    h.nseg = (uint32_t)idx->nseg;
+    h.k = idx->k;
+    h.max_matches = idx->max_matches;
+    h.h_frac = idx->h_frac;
+    h.n_kmers = idx->n_kmers;
+    h.indexed_kmers = idx->indexed_kmers;
+    h.indexed_hits = idx->indexed_hits;
+    h.highest = idx->highest;
+    h.black_kmers = idx->black_kmers;
+    h.black_hits = idx->black_hits;
+    for (int i = 0; i < 8; i++) {
+        h.shard_base[i] = idx->geom.base[i];
+        h.shard_bits[i] = idx->geom.bits[i];
+        h.table_slots += 1ull << idx->geom.bits[i];
+    }
+    for (const std::string &nm : idx->names) h.name_bytes += 4 + nm.size();
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(SHM_ERR_IO, "cannot create \"%s\": %s", path, strerror(errno));
+    PinnedBuf<uint8_t> stage;
+    int rc = stage.ensure(IDX_STAGE);
+    const size_t N = (size_t)idx->n_kmers;
+    if (rc == SHM_OK) rc = idx_write(f, &h, sizeof h);
+    for (size_t i = 0; rc == SHM_OK && i < idx->names.size(); i++) {
+        const uint32_t len = (uint32_t)idx->names[i].size();
+        rc = idx_write(f, &len, 4);
+        if (rc == SHM_OK) rc = idx_write(f, idx->names[i].data(), len);
+    }
+    if (rc == SHM_OK) rc = idx_write(f, idx->sz.data(), idx->sz.size() * 4);
+    if (rc == SHM_OK) rc = idx_write(f, idx->h_kstart.data(), idx->h_kstart.size() * 8);
+    if (rc == SHM_OK) rc = idx_write(f, idx->h_segcum.data(), idx->h_segcum.size() * 8);
+    if (rc == SHM_OK) rc = idx_put(f, idx->t_h.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->t_r.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->t_s.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->hit_g.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->hit_r.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->hit_seg.p, N, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->strand_pre.p, N + 1, stage);
+    if (rc == SHM_OK) rc = idx_put(f, idx->table.p, (size_t)h.table_slots, stage);
+    if (rc == SHM_OK) rc = idx_write(f, IDX_END, 8);
+    if (fclose(f) != 0 && rc == SHM_OK) rc = fail(SHM_ERR_IO, "write failed: %s", strerror(errno));
+    if (rc != SHM_OK) remove(path);
+    return rc;
+}
+
+extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **out) {
+    if (!path || !out) return fail(SHM_ERR_ARG, "invalid argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(SHM_ERR_IO, "cannot open \"%s\": %s", path, strerror(errno));
+    struct Closer {
+        FILE *f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    // the header is checked before the device is touched
+    IdxHeader h{};
+    if (fread(&h, 1, sizeof h, f) != sizeof h || memcmp(h.magic, IDX_MAGIC, 8) != 0)
+        return fail(SHM_ERR_ARG, "\"%s\" is not a shmap index file", path);
+    if (h.version != IDX_VERSION)
+        return fail(SHM_ERR_ARG, "\"%s\" is an index file of version %u; this build reads version %u", path,
+                    h.version, IDX_VERSION);
+    if (h.n_kmers < 0 || (uint64_t)h.n_kmers >= 0xffffffffull || h.nseg > (1u << 30))
+        return fail(SHM_ERR_ARG, "\"%s\" has an invalid header", path);
+    uint64_t slots = 0;
+    for (int i = 0; i < 8; i++) {
+        if (h.shard_bits[i] < 6 || h.shard_bits[i] > 40 || h.shard_base[i] != slots)
+            return fail(SHM_ERR_ARG, "\"%s\" has an invalid header", path);
+        slots += 1ull << h.shard_bits[i];
+    }
+    if (slots != h.table_slots || h.name_bytes > (1ull << 40))
+        return fail(SHM_ERR_ARG, "\"%s\" has an invalid header", path);
+    // the whole file must be there: header, names, tables, arrays, end marker
+    const uint64_t N = (uint64_t)h.n_kmers, S = h.nseg;
+    const uint64_t want = sizeof h + h.name_bytes + S * 4 + 2 * (S + 1) * 8 + N * (8 + 4 + 1 + 4 * 3) + (N + 1) * 4 +
+                          h.table_slots * 16 + 8;
+    if (fseeko(f, 0, SEEK_END) != 0) return fail(SHM_ERR_IO, "cannot read \"%s\"", path);
+    const uint64_t have = (uint64_t)ftello(f);
+    if (have != want)
+        return fail(SHM_ERR_ARG, "\"%s\" is %s (%llu bytes, the header describes %llu)", path,
+                    have < want ? "truncated" : "longer than its header describes", (unsigned long long)have,
+                    (unsigned long long)want);
+    char end[8];
+    if (fseeko(f, (off_t)(want - 8), SEEK_SET) != 0 || fread(end, 1, 8, f) != 8 || memcmp(end, IDX_END, 8) != 0)
+        return fail(SHM_ERR_ARG, "\"%s\" has no end marker", path);
+    if (fseeko(f, (off_t)sizeof h, SEEK_SET) != 0) return fail(SHM_ERR_IO, "cannot read \"%s\"", path);
+    std::unique_ptr<shm_index> idx(new shm_index);
+    idx->device = device_id;
+    idx->k = h.k;
+    idx->h_frac = h.h_frac;
+    idx->max_matches = h.max_matches;
+    idx->nseg = (int32_t)S;
+    idx->n_kmers = h.n_kmers;
+    idx->indexed_kmers = h.indexed_kmers;
+    idx->indexed_hits = h.indexed_hits;
+    idx->highest = h.highest;
+    idx->black_kmers = h.black_kmers;
+    idx->black_hits = h.black_hits;
+    for (int i = 0; i < 8; i++) {
+        idx->geom.base[i] = h.shard_base[i];
+        idx->geom.bits[i] = h.shard_bits[i];
+    }
+    uint64_t name_left = h.name_bytes;
+    for (uint64_t i = 0; i < S; i++) {
+        uint32_t len = 0;
+        RCHK(idx_read(f, &len, 4, path));
+        if ((uint64_t)len + 4 > name_left) return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+        name_left -= 4 + (uint64_t)len;
+        std::string nm(len, '\0');
+        RCHK(idx_read(f, &nm[0], len, path));
+        idx->names.push_back(std::move(nm));
+    }
+    if (name_left != 0) return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+    idx->sz.resize(S);
+    idx->h_kstart.resize(S + 1);
+    idx->h_segcum.resize(S + 1);
+    RCHK(idx_read(f, idx->sz.data(), S * 4, path));
+    RCHK(idx_read(f, idx->h_kstart.data(), (S + 1) * 8, path));
+    RCHK(idx_read(f, idx->h_segcum.data(), (S + 1) * 8, path));
+    if (idx->h_kstart[S] != h.n_kmers) return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+    HIPCHK(hipSetDevice(device_id));
+    PinnedBuf<uint8_t> stage;
+    RCHK(stage.ensure(IDX_STAGE));
+    RCHK(idx->kstart.ensure(S + 1));
+    RCHK(idx->segcum.ensure(S + 1));
+    HIPCHK(hipMemcpy(idx->kstart.p, idx->h_kstart.data(), (S + 1) * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(idx->segcum.p, idx->h_segcum.data(), (S + 1) * 8, hipMemcpyHostToDevice));
+    RCHK(idx_get(f, idx->t_h, N, 1, stage, path));
+    RCHK(idx_get(f, idx->t_r, N, 1, stage, path));
+    RCHK(idx_get(f, idx->t_s, N, 1, stage, path));
+    RCHK(idx_get(f, idx->hit_g, N, 1, stage, path));
+    RCHK(idx_get(f, idx->hit_r, N, 1, stage, path));
+    RCHK(idx_get(f, idx->hit_seg, N, 1, stage, path));
+    RCHK(idx_get(f, idx->strand_pre, N + 1, 0, stage, path));
+    RCHK(idx_get(f, idx->table, (size_t)h.table_slots, 0, stage, path));
+    *out = idx.release();
     return SHM_OK;
 }
 

@@ -110,6 +110,10 @@ def lib():
         L.shm_index_segment.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int64)]
         L.shm_index_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 5
+        L.shm_index_params.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int32)]
+        L.shm_index_save.argtypes = [C.c_void_p, C.c_char_p]
+        L.shm_index_load.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
         L.shm_mapper_create.argtypes = [C.c_void_p, C.POINTER(MapParams), C.POINTER(C.c_void_p)]
         L.shm_mapper_free.argtypes = [C.c_void_p]
         L.shm_mapper_set_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -182,7 +186,8 @@ class FracMinHash:
 
 
 class SketchIndex:
-    """src/index.rs:112-701 -- the reference index, built and kept on the GPU."""
+    """src/index.rs:112-701 -- the reference index, built and kept on the GPU
+    (or read back from a file written by save(), with load())."""
 
     def __init__(self, sketcher: FracMinHash, segments, max_matches: int | None = None, device: int = 0):
         self.sketcher = sketcher
@@ -225,6 +230,27 @@ class SketchIndex:
             _check(lib().shm_index_segment(self._p, s, C.byref(nm), C.byref(sz), C.byref(nk)))
             res.append((nm.value.decode(), sz.value, nk.value))
         return res
+
+    def save(self, path) -> None:
+        """Writes the index to a file (shm_index_save)."""
+        _check(lib().shm_index_save(self._p, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, device: int = 0) -> "SketchIndex":
+        """An index read from a file written by save() (shm_index_load)."""
+        p = C.c_void_p()
+        _check(lib().shm_index_load(os.fsencode(path), device, C.byref(p)))
+        self = cls.__new__(cls)
+        self._p, self._keep = p, None
+        k, r, mm = self.params()
+        self.sketcher = FracMinHash(k, r)
+        return self
+
+    def params(self) -> tuple[int, float, int]:
+        """(k, h_frac, max_matches) the index was built with (0: -M unset)."""
+        k, r, mm = C.c_int32(), C.c_double(), C.c_int32()
+        _check(lib().shm_index_params(self._p, C.byref(k), C.byref(r), C.byref(mm)))
+        return k.value, r.value, mm.value
 
     def stats(self) -> dict:
         v = [C.c_int64() for _ in range(5)]

@@ -46,6 +46,48 @@ def test_library_reports_argument_errors_without_a_device():
     assert b"hash ratio" in lib.shm_last_error()
 
 
+def _index_header(version=1, n_kmers=0, nseg=0, name_bytes=0, bits=6):
+    """An index file header (shm_index_save's layout: 192 bytes)."""
+    import struct
+    base = [i << bits for i in range(8)]
+    return (b"SHMIDX01" + struct.pack("<IIiid", version, nseg, 15, 0, 0.1) +
+            struct.pack("<6q", n_kmers, 0, n_kmers, 0, 0, 0) + struct.pack("<8Q", *base) +
+            struct.pack("<8I", *([bits] * 8)) + struct.pack("<QQ", 8 << bits, name_bytes))
+
+
+def test_index_load_checks_the_file_before_the_device(tmp_path):
+    """shm_index_load refuses what is not a complete index file of this
+    version with a message, before it touches the device."""
+    lib = C.CDLL(LIB)
+    lib.shm_last_error.restype = C.c_char_p
+    lib.shm_index_load.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
+    out = C.c_void_p()
+
+    def load(data):
+        f = tmp_path / "x.idx"
+        f.write_bytes(data)
+        rc = lib.shm_index_load(str(f).encode(), 0, C.byref(out))
+        return rc, lib.shm_last_error().decode()
+
+    assert len(_index_header()) == 192
+    rc = lib.shm_index_load(str(tmp_path / "none.idx").encode(), 0, C.byref(out))
+    assert rc == 5 and "cannot open" in lib.shm_last_error().decode()
+    rc, msg = load(b">r1\nACGT\n")
+    assert rc == 1 and "not a shmap index file" in msg
+    rc, msg = load(_index_header(version=2))
+    assert rc == 1 and "version 2" in msg
+    rc, msg = load(_index_header(bits=3))
+    assert rc == 1 and "invalid header" in msg
+    # an empty index needs 4 + 8 + 8 bytes of tables, strand prefix and slots, and the end marker
+    full = _index_header() + (0).to_bytes(8, "little") * 2 + (0).to_bytes(4, "little") + bytes(16 * (8 << 6))
+    rc, msg = load(full)
+    assert rc == 1 and "truncated" in msg
+    rc, msg = load(full + b"SHMIDXXX")
+    assert rc == 1 and "end marker" in msg
+    rc, msg = load(full + b"SHMIDXEN" + b"extra")
+    assert rc == 1 and "longer than" in msg
+
+
 @pytest.mark.parametrize("args,needle", [
     (["-k", "0"], "K-mer length (-k) should be positive. You provided 0."),
     (["-r", "0"], "Given hash ratio (-r) should be between 0 and 1. You provided 0."),
@@ -98,3 +140,47 @@ def test_cli_reader_fastq(tmp_path):
     f = tmp_path / "reads.fq"
     f.write_bytes(b"@q1 desc\nACGTN\n+\nIIIII\n@q2\nGGCC\r\n+q2\r\n!!!!\r\n")
     assert _dump(f) == [("q1", b"ACGTN"), ("q2", b"GGCC")]
+
+
+def test_cli_reader_gzip(tmp_path):
+    """gzip input (needletail's compressed FASTA/FASTQ, src/io/mod.rs:1, 357)
+    reads as the plain file does: one member, several members (concatenated
+    gzip streams), FASTQ, and a file whose blocks straddle the member ends."""
+    import gzip
+    import random
+    rng = random.Random(11)
+    recs = [(f"r{i} d", bytes(rng.choice(b"ACGTN") for _ in range(rng.randint(0, 5000)))) for i in range(300)]
+    body = b"".join(b">" + n.encode() + b"\n" + sq[:3000] + b"\r\n" + sq[3000:] + b"\n" for n, sq in recs)
+    plain = tmp_path / "reads.fa"
+    plain.write_bytes(body)
+    want = _dump(plain)
+    assert [n for n, _ in want] == [n.split()[0] for n, _ in recs]
+    one = tmp_path / "one.fa.gz"
+    one.write_bytes(gzip.compress(body))
+    assert _dump(one) == want
+    cut = [0, 1, len(body) // 3, len(body) // 3 + 7, len(body)]
+    multi = tmp_path / "multi.fa.gz"
+    multi.write_bytes(b"".join(gzip.compress(body[a:b]) for a, b in zip(cut, cut[1:])))
+    assert _dump(multi) == want
+    fq = tmp_path / "reads.fq.gz"
+    fq.write_bytes(gzip.compress(b"@q1 desc\nACGTN\n+\nIIIII\n@q2\nGGCC\r\n+q2\r\n!!!!\r\n"))
+    assert _dump(fq) == [("q1", b"ACGTN"), ("q2", b"GGCC")]
+
+
+def test_cli_reader_refuses_other_compression_and_corrupt_gzip(tmp_path):
+    import bz2
+    import gzip
+    import lzma
+    body = b">r1\nACGT\n"
+    for name, data, fmt in (("r.fa.bz2", bz2.compress(body), "bzip2"), ("r.fa.xz", lzma.compress(body), "xz"),
+                            ("r.fa.zst", bytes([0x28, 0xb5, 0x2f, 0xfd]) + b"\0" * 8, "zstd")):
+        f = tmp_path / name
+        f.write_bytes(data)
+        r = subprocess.run([CLI, "-p", str(f), "--dump-reads"], capture_output=True)
+        assert r.returncode == 1 and fmt.encode() + b"-compressed" in r.stderr, r.stderr
+    bad = bytearray(gzip.compress(body * 1000))
+    bad[len(bad) // 2] ^= 0xff
+    f = tmp_path / "bad.fa.gz"
+    f.write_bytes(bytes(bad))
+    r = subprocess.run([CLI, "-p", str(f), "--dump-reads"], capture_output=True)
+    assert r.returncode == 1 and b"decompress" in r.stderr, r.stderr

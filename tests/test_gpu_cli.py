@@ -1,6 +1,7 @@
 """The shmap command line end to end on the GPU: the reference's golden
-fixture (tests/golden_paf.rs:26-53), and a multi-batch run whose PAF lines,
-unmapped list and per-read rows equal the library's records."""
+fixture (tests/golden_paf.rs:26-53), plain and gzip-compressed, and a
+multi-batch run whose PAF lines, unmapped list and per-read rows equal the
+library's records."""
 import os
 import subprocess
 
@@ -26,6 +27,16 @@ def _run(args, cwd):
 def test_cli_golden_paf(golden_dir, tmp_path):
     out = _run(["-s", os.path.join(golden_dir, "tiny_ref.fa"), "-p", os.path.join(golden_dir, "tiny_reads.fa"),
                 "-k", "8", "-r", "1.0", "-t", "0.1"], tmp_path)
+    want = fastx.strip_time(open(os.path.join(golden_dir, "tiny.golden.paf")).read())
+    assert fastx.strip_time(out) == want
+
+
+def test_cli_golden_paf_from_gzip(golden_dir, tmp_path):
+    """The same fixture with both inputs gzip-compressed."""
+    import gzip
+    for nm in ("tiny_ref.fa", "tiny_reads.fa"):
+        (tmp_path / (nm + ".gz")).write_bytes(gzip.compress(open(os.path.join(golden_dir, nm), "rb").read()))
+    out = _run(["-s", "tiny_ref.fa.gz", "-p", "tiny_reads.fa.gz", "-k", "8", "-r", "1.0", "-t", "0.1"], tmp_path)
     want = fastx.strip_time(open(os.path.join(golden_dir, "tiny.golden.paf")).read())
     assert fastx.strip_time(out) == want
 
