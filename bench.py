@@ -7,7 +7,11 @@ Workload (default ``genome``): a 3.1 Gbp synthetic reference of 24 contigs
 40/30/30 sub/ins/del, both strands), k=25 r=0.01 t=0.4 -d 0.075 -o 0.3,
 Containment.  The reference index is built on the GPU and stays resident; the
 reads are uploaded to HBM before the timed region.  One step maps every read
-(batched, persistent kernels) and copies the per-read records back.
+(batched) and leaves the per-read records in page-locked host memory.  The
+timed steps are taken by --mappers mappers (default 2) in turn, each on its
+own HIP stream and host thread, so one batch's host waits and kernel tails
+overlap another's kernels (the serving loop's workers run the same way); the
+single-mapper rate is reported beside it (one_mapper).
 
 Also reported, outside the timed region: the plain one-read-per-launch form on
 a sample, per-stage kernel times with the dominant kernel's roofline, and the
@@ -24,6 +28,7 @@ import importlib.util
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -262,12 +267,15 @@ def run_serving(idx, prm, lens, ref, args) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="genome", choices=sorted(WORKLOADS))
     ap.add_argument("--metric", default="Containment")
     ap.add_argument("--reads", type=int, default=None, help="override the read count")
     ap.add_argument("--plain-reads", type=int, default=200)
+    ap.add_argument("--mappers", type=int, default=2,
+                    help="mappers mapping whole batches concurrently in the timed steps (each its own HIP stream, "
+                         "host thread and page-locked output), as the serving loop's workers do")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
@@ -328,16 +336,45 @@ def main():
         res = mp.map_device(d_bases, d_off, n_reads, out=out)
         log(f"[bench] warmup step {i} done")
     hip.sync()
+    # one mapper, one step at a time
+    t_one = time.time()
+    for _ in range(2):
+        mp.map_device(d_bases, d_off, n_reads, out=out)
+    one_ms = (time.time() - t_one) / 2 * 1e3
+    log(f"[bench] one mapper: {one_ms:.2f} ms per step")
+    # the timed steps: M mappers (threads, streams) take whole batches in turn,
+    # so one batch's host waits and kernel tails overlap another's kernels
+    M = max(1, args.mappers)
+    mps = [mp] + [G.Mapper(idx, prm) for _ in range(M - 1)]
+    outs = [out] + [G.host_array(n_reads, G.RESULT_DTYPE) for _ in range(M - 1)]
+    for m_, o_ in zip(mps[1:], outs[1:]):
+        m_.map_device(d_bases, d_off, n_reads, out=o_)
+    hip.sync()
     times = []
+    lock = threading.Lock()
+
+    def worker(wi):
+        for i in range(wi, args.steps, M):
+            t1 = time.time()
+            mps[wi].map_device(d_bases, d_off, n_reads, out=outs[wi])
+            dt = time.time() - t1
+            with lock:
+                times.append(dt)
+            log(f"[bench] step {i} (mapper {wi}): {dt * 1e3:.1f} ms")
+
     t_start = time.time()
-    for i in range(args.steps):
-        t1 = time.time()
-        res = mp.map_device(d_bases, d_off, n_reads, out=out)
-        times.append(time.time() - t1)
-        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms")
+    ths = [threading.Thread(target=worker, args=(wi,)) for wi in range(M)]
+    for t_ in ths:
+        t_.start()
+    for t_ in ths:
+        t_.join()
     hip.sync()
     elapsed = time.time() - t_start
     value = n_reads * args.steps / elapsed
+    res = out
+    for o_ in outs[1:]:
+        assert np.array_equal(o_, out), "concurrent mappers disagree"
+    del mps[1:], outs[1:]
     n_mapped = int(res["mapped"].sum())
     gbp = float(off[-1]) / 1e9
 
@@ -372,7 +409,8 @@ def main():
     log("[bench] stage ms: " + ", ".join(f"{s}={v:.2f}" for s, v in stage_ms.items()))
 
     # the same batch under each of the other scoring metrics (two timed steps each)
-    per_metric = {args.metric: {"reads_per_s": round(value, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+    # (one mapper, one step at a time)
+    per_metric = {args.metric: {"reads_per_s": round(n_reads / one_ms * 1e3, 2), "ms_per_step": round(one_ms, 3),
                                 "mapped_reads": n_mapped}}
     # (the genome workload's configuration names all four; elsewhere on request)
     if not args.no_metrics and (args.workload == "genome" or args.all_metrics):
@@ -495,7 +533,9 @@ def main():
         "data": "synthetic (seeded generator, shmap-rs_amd/csrc/synth.c)",
         "config": {"workload": args.workload, "reference_gbp": round(float(lens.sum()) / 1e9, 3),
                    "contigs": int(len(lens)), "reads": n_reads, "read_gbp": round(gbp, 3), "k": k, "r": h_frac,
-                   "t": theta, "d": d, "o": o, "metric": args.metric, "mode": "batched, device-resident"},
+                   "t": theta, "d": d, "o": o, "metric": args.metric, "mode": "batched, device-resident",
+                   "mappers": M},
+        "one_mapper": {"reads_per_s": round(n_reads / one_ms * 1e3, 2), "ms_per_step": round(one_ms, 3)},
         "gbp_per_s": round(gbp * args.steps / elapsed, 3),
         "mapped_reads": n_mapped,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),

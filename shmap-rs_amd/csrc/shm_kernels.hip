@@ -1029,14 +1029,15 @@ __device__ __forceinline__ void tab_add(const MapArgs &A, uint32_t lr, uint64_t 
 constexpr uint32_t LTAB_BITS = 8;
 constexpr uint32_t LTAB = 1u << LTAB_BITS;
 constexpr uint32_t LTAB_MAX = LTAB * 7 / 8;
-// One wave takes a whole read up to CONTRIB_WAVE_E examined hits.  Larger
+// One wave takes a whole read up to A.contrib_wave_e examined hits (the host
+// picks it per batch: long for big batches, which fill the device anyway,
+// short for small ones, whose latency a long serial read would set).  Larger
 // reads, and reads with more buckets than one LDS table holds, are split into
 // chunks, one wave per chunk, whose LDS tables are merged in the read's global
 // table (k_contrib_chunks).  The chunk length is chosen per group: the largest
 // power of two from CONTRIB_CHUNK_MIN to CONTRIB_CHUNK_MAX hits that still
 // gives CONTRIB_CHUNKS_WANT chunks, so a few listed reads are spread over the
 // device and many are taken in long chunks (fewer merges).
-constexpr uint32_t CONTRIB_WAVE_E = 4096;
 constexpr uint32_t CONTRIB_CHUNK_MIN = 512, CONTRIB_CHUNK_MAX = 4096;
 constexpr uint64_t CONTRIB_CHUNKS_WANT = 8192;
 
@@ -1302,7 +1303,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
     wave_sync();                                                                               \
     uint32_t stamp = 0
 
-// One wave per read with at most CONTRIB_WAVE_E examined hits: its buckets end
+// One wave per read with at most A.contrib_wave_e examined hits: its buckets end
 // as the first grp_nb rows of its table region, listed in order.  Larger
 // reads, and reads with more than LTAB_MAX buckets, are listed for
 // k_contrib_chunks.
@@ -1312,7 +1313,7 @@ __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
     for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
         const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
         if (E == 0) continue;
-        if (E > CONTRIB_WAVE_E) {
+        if (E > A.contrib_wave_e) {
             if (lane == 0) A.ov_list[atomicAdd(A.ov_count, 1u)] = (uint32_t)lr;
             continue;
         }
@@ -1497,12 +1498,12 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
 // value, and a bucket the reference refines is always fully extended.  Every
 // bucket of the group is tested against theta*(1-min_diff) (the lowest
 // threshold either match_rest pass can use).  k_prune takes one thread per
-// bucket: with few remaining seeds the thread extends its bucket seed by seed
-// (the two binary searches of matches_in_bucket for the bucket's span in the
-// seed's hit run), stopping as soon as sh falls below the threshold; with many
-// it only tests the starting sh and lists the buckets still alive for
-// k_prune_ext, where one wave extends each, 64 seeds at a time, stopping at
-// the first chunk that kills it.  Survivors are listed for refinement, and
+// bucket: the thread extends its bucket seed by seed (the two binary searches
+// of matches_in_bucket for the bucket's span in the seed's hit run), stopping
+// as soon as sh falls below the threshold, for up to PRUNE_SEQ_SEEDS seeds
+// (most buckets die within a few); a bucket still alive with seeds left is
+// listed, with its partial sums, for k_prune_ext, where one wave extends it 64
+// seeds at a time, stopping at the first chunk that kills it.  Survivors are listed for refinement, and
 // each read's first first-pass candidate is kept in rd_first.
 
 // A surviving bucket's refine task: first-pass list when sh reaches theta,
@@ -1554,9 +1555,11 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 const uint32_t t = atomicAdd(&A.task_count[0], 1u);
                 A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
                 atomicMin(&A.rd_first[rd], p);
-            } else if (u - c <= (int32_t)PRUNE_SEQ_SEEDS) {
-                // few seeds to extend with: this thread extends its bucket
-                // seed by seed, stopping as soon as it dies
+            } else {
+                // this thread extends its bucket seed by seed, stopping as soon
+                // as it dies, for up to PRUNE_SEQ_SEEDS seeds; a bucket still
+                // alive with seeds left goes to k_prune_ext with its partial
+                // sums, its next seed (sc_i) and its seeds so far (sc_same)
                 int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
                 int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
                 int32_t seeds = seeds0;
@@ -1566,13 +1569,15 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
                 uint8_t alive = 1;
                 double sh;
-                for (int32_t q = c;; q++) {
+                const int32_t qend = u - c <= (int32_t)PRUNE_SEQ_SEEDS ? u : c + (int32_t)PRUNE_SEQ_SEEDS;
+                int32_t q = c;
+                for (;; q++) {
                     sh = 1.0 - (double)(seeds - matches) / (double)m;
                     if (sh < A.thr_min) {
                         alive = 0;
                         break;
                     }
-                    if (q >= u) break;
+                    if (q >= qend) break;
                     const uint32_t sid = A.sd_ord[o + q];
                     const uint32_t occs = A.sd_occs[o + sid];
                     const uint32_t cnt = A.sd_cnt[o + sid];
@@ -1596,16 +1601,14 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 A.bs_c[base + p] = codir;
                 A.bs_min[base + p] = rmin;
                 A.bs_max[base + p] = rmax;
-                A.bs_alive[base + p] = alive;
-                A.bs_sh[base + p] = sh;
-                if (alive) prune_keep(A, lr, p, sh);
-            } else {
-                const double sh = 1.0 - (double)(seeds0 - A.bs_m[base + p]) / (double)m;
-                if (sh < A.thr_min) {
-                    A.bs_alive[base + p] = 0;
-                    A.bs_sh[base + p] = sh;
-                } else {
+                if (alive && q < u) {
+                    A.sc_i[base + p] = q;
+                    A.sc_same[base + p] = seeds;
                     A.sel[atomicAdd(&A.task_count[5], 1u)] = make_uint2((uint32_t)lr, (uint32_t)p);
+                } else {
+                    A.bs_alive[base + p] = alive;
+                    A.bs_sh[base + p] = sh;
+                    if (alive) prune_keep(A, lr, p, sh);
                 }
             }
         }
@@ -1623,17 +1626,17 @@ __global__ void __launch_bounds__(WG) k_prune_ext(MapArgs A, DevIndex I) {
         const int32_t rd = A.r0 + lr;
         const uint64_t base = A.bk_base[lr], o = A.koff[rd];
         const int32_t m = (int32_t)(A.koff[rd + 1] - o);
-        const int32_t u = A.rd_u[rd], c = A.rd_c[rd], hl = A.rd_hl[rd];
+        const int32_t u = A.rd_u[rd], hl = A.rd_hl[rd];
         int32_t matches = A.bs_m[base + pb], codir = A.bs_c[base + pb];
         int32_t rmin = A.bs_min[base + pb], rmax = A.bs_max[base + pb];
-        int32_t seeds = A.rd_bseeds[rd];
+        int32_t seeds = A.sc_same[base + pb];  // k_prune's hand-over
         uint32_t seg;
         int32_t b;
         decode_bucket(A, A.bs_g[base + pb], hl, seg, b);
         const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
         double sh = 1.0 - (double)(seeds - matches) / (double)m;
         bool alive = true;
-        for (int32_t q0 = c; alive && q0 < u; q0 += WAVE) {
+        for (int32_t q0 = A.sc_i[base + pb]; alive && q0 < u; q0 += WAVE) {
             const int32_t q = q0 + (int32_t)lane;
             int32_t d_seeds = 0, d_matches = 0, d_codir = 0, d_min = INT32_MAX, d_max = -1;
             if (q < u) {

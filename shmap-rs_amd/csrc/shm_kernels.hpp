@@ -19,7 +19,7 @@ constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
-constexpr uint32_t PRUNE_SEQ_SEEDS = 128;   // remaining seeds a thread extends a bucket with alone
+constexpr uint32_t PRUNE_SEQ_SEEDS = 16;    // seeds a thread extends a bucket with before a wave takes over
 constexpr uint32_t TASK_COUNTERS = 7;       // per group: first-pass, second band, 3 refine rounds, prune list,
                                             // reads on the global-table bucket path
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
@@ -103,7 +103,8 @@ struct MapArgs {
     // (refine_pass 0-2) that refine a superset of what the two match_rest
     // passes read; each round's buckets are selected into sel, counted in
     // task_count[2 + round].  Before that, sel lists the buckets k_prune_ext
-    // extends (task_count[5]).
+    // extends (task_count[5]), with k_prune's next seed and seeds so far in
+    // sc_i and sc_same (refinement writes those later for what it refines).
     uint2 *tasks, *sel;
     uint32_t *task_count;
     uint32_t task_cap;
@@ -118,6 +119,7 @@ struct MapArgs {
     // prefix and then the chunk length
     uint32_t *ov_list, *ov_count;
     uint64_t *ov_kpre;
+    uint32_t contrib_wave_e;  // examined hits one k_contrib_wave wave takes whole
     // k_final's batch totals: the mapped reads (for the host's finalization)
     // and the sum of distinct k-mers
     uint32_t *mapped_list, *mapped_count;

@@ -776,6 +776,7 @@ struct shm_mapper {
     shm_sketcher sk{};
     hipStream_t st = nullptr;
     uint32_t max_grid = 1024;
+    uint32_t contrib_wave_e = 0;  // 0: chosen per batch
     uint64_t tab_budget = 1ull << 30;  // bucket-table slots per group (24 B each, ~26 GB)
     bool timing = false;
     hipEvent_t ev[ST_COUNT][2];
@@ -838,6 +839,7 @@ extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, 
     mp->sk.h_frac = p->h_frac;
     mp->sk.thres = hash_threshold(p->h_frac);
     mp->max_grid = device_grid(idx->device);
+    if (const char *e = getenv("SHM_CONTRIB_WAVE_E")) mp->contrib_wave_e = (uint32_t)strtoul(e, nullptr, 10);
     if (const char *e = getenv("SHM_TAB_BUDGET")) mp->tab_budget = strtoull(e, nullptr, 10);
     if (hipStreamCreate(&mp->st) != hipSuccess) {
         delete mp;
@@ -1140,6 +1142,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.ov_count = task_count + 6;
         A.ov_list = mp->ov.p;
         A.ov_kpre = mp->ov_kpre.p;
+        // a big batch fills the device with whole reads; a small one splits
+        // long reads so no read's serial walk sets the batch's time
+        A.contrib_wave_e = mp->contrib_wave_e ? mp->contrib_wave_e : (E_grp >= (1ull << 24) ? 4096u : 1024u);
         TIME_BEGIN(ST_TAB_INIT);
         launch_stage(ST_TAB_INIT, A, I, nullptr, RG, mp->max_grid, st);
         TIME_END(ST_TAB_INIT);
