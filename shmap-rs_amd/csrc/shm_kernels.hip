@@ -725,13 +725,31 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         wave_sync();
-        // rarest-first order: keys (hits << 32 | seed id), distinct
-        if (!wave_sort_n<false>(
-                u, [&](uint32_t x, uint64_t &key, uint32_t &) { key = ((uint64_t)kv[x] << 32) | x; },
-                [&](uint32_t x, uint64_t key, uint32_t) { kh[x] = key; })) {
-            for (uint32_t q = lane; q < u; q += WAVE) kh[q] = ((uint64_t)kv[q] << 32) | q;
-            wave_sync();
-            wave_bitonic_keys(kh, u);
+        // rarest-first order: keys (hits << 32 | seed id), distinct.  Seeds
+        // without hits lead, in seed-id order, so they are placed by ballot
+        // and only the others are sorted (after them, in kh[zs, u)).
+        uint32_t zs = 0;
+        for (uint32_t b0 = 0; b0 < u; b0 += WAVE) zs += __popcll(__ballot(b0 + lane < u && kv[b0 + lane] == 0));
+        {
+            uint32_t zi = 0, ni = zs;
+            for (uint32_t b0 = 0; b0 < u; b0 += WAVE) {
+                const uint32_t q = b0 + lane;
+                const uint32_t cq = q < u ? kv[q] : 0u;
+                const uint64_t bz = __ballot(q < u && cq == 0), bn = __ballot(q < u && cq != 0);
+                if (q < u) {
+                    const uint32_t at = cq == 0 ? zi + __popcll(bz & lt) : ni + __popcll(bn & lt);
+                    kh[at] = ((uint64_t)cq << 32) | q;
+                }
+                zi += __popcll(bz);
+                ni += __popcll(bn);
+            }
+        }
+        wave_sync();
+        if (u - zs > 1 &&
+            !wave_sort_n<false>(
+                u - zs, [&](uint32_t x, uint64_t &key, uint32_t &) { key = kh[zs + x]; },
+                [&](uint32_t x, uint64_t key, uint32_t) { kh[zs + x] = key; })) {
+            wave_bitonic_keys(kh + zs, u - zs);
         }
         wave_sync();
         // budget: consume seeds while fewer than S occurrences are consumed
@@ -871,16 +889,40 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
                 A.sd_strand[o + sid] = A.k_s[o + A.ps_v[o + first]];
                 A.sd_cnt[o + sid] = cnt[t];
                 A.sd_hoff[o + sid] = hoff[t];
-                A.sd_key[o + sid] = ((uint64_t)cnt[t] << 32) | sid;
-                A.sd_ord[o + sid] = sid;
                 pm += cnt[t];
             }
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) pm += __shfl_xor(pm, d);
         if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&s_pm, pm);
         __syncthreads();
-        // 3. rarest-first order
-        wg_sort_pairs(A.sd_key + o, A.sd_ord + o, u, sk, sv, SORT_CAP);
+        // 3. rarest-first order: seeds without hits lead, in seed-id order,
+        //    placed by a scan; only the others are sorted, after them
+        uint32_t zs = 0;
+        for (uint32_t base = 0; base < u; base += WG) {
+            const uint32_t q = base + threadIdx.x;
+            uint32_t tot;
+            (void)wg_excl_scan(q < u && A.sd_cnt[o + q] == 0 ? 1u : 0u, tmp, &tot);
+            zs += tot;
+        }
+        {
+            uint32_t zi = 0, ni = zs;
+            for (uint32_t base = 0; base < u; base += WG) {
+                const uint32_t q = base + threadIdx.x;
+                const uint32_t cq = q < u ? A.sd_cnt[o + q] : 1u;
+                uint32_t tz, tn;
+                const uint32_t ez = wg_excl_scan(q < u && cq == 0 ? 1u : 0u, tmp, &tz);
+                const uint32_t en = wg_excl_scan(q < u && cq != 0 ? 1u : 0u, tmp, &tn);
+                if (q < u) {
+                    const uint32_t at = cq == 0 ? zi + ez : ni + en;
+                    A.sd_key[o + at] = ((uint64_t)cq << 32) | q;
+                    A.sd_ord[o + at] = q;
+                }
+                zi += tz;
+                ni += tn;
+            }
+        }
+        __syncthreads();
+        if (u - zs > 1) wg_sort_pairs(A.sd_key + o + zs, A.sd_ord + o + zs, u - zs, sk, sv, SORT_CAP);
         // 4. budget: consume seeds while fewer than S occurrences are consumed
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
@@ -932,13 +974,7 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
             A.rd_hl[rd] = hl;
             A.rd_ok[rd] = ok ? 1 : 0;
             A.rd_c[rd] = (int32_t)s_c;
-            // hitless seeds sort first: the first key with a nonzero count
-            uint32_t lo = 0, hi = u;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if ((A.sd_key[o + mid] >> 32) == 0) lo = mid + 1; else hi = mid;
-            }
-            A.rd_z[rd] = (int32_t)(lo < s_c ? lo : s_c);
+            A.rd_z[rd] = (int32_t)(zs < s_c ? zs : s_c);  // hitless seeds lead the order
             A.rd_bseeds[rd] = (int32_t)s_seeds;
             A.rd_max_seed[rd] = (int32_t)s_max;
             A.rd_seed_matches[rd] = (int32_t)E;
