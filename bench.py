@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1_000_000.0, help="offered reads/s (Poisson arrivals)")
     ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--serve-batch", type=int, default=16384, help="admission budget: reads per batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in batching choices instead of tuning them on the batch first")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM traffic")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
@@ -331,6 +333,12 @@ def main():
     d_bases = hip.upload(bases)
     d_off = hip.upload(off)
 
+    # batching choices tuned on this batch (shm_mapper_autotune), untimed
+    t_tune = time.time()
+    tuning = mp.tuning() if args.no_autotune else mp.autotune((bases, off))
+    tune_s = time.time() - t_tune
+    log(f"[bench] tuning {tuning} ({'built-in' if args.no_autotune else f'autotuned in {tune_s:.1f}s'})")
+
     out = G.host_array(n_reads, G.RESULT_DTYPE)  # page-locked: results come back at link speed
     for i in range(args.warmup):
         res = mp.map_device(d_bases, d_off, n_reads, out=out)
@@ -346,6 +354,8 @@ def main():
     # so one batch's host waits and kernel tails overlap another's kernels
     M = max(1, args.mappers)
     mps = [mp] + [G.Mapper(idx, prm) for _ in range(M - 1)]
+    for m_ in mps[1:]:
+        m_.set_tuning(**tuning)
     outs = [out] + [G.host_array(n_reads, G.RESULT_DTYPE) for _ in range(M - 1)]
     for m_, o_ in zip(mps[1:], outs[1:]):
         m_.map_device(d_bases, d_off, n_reads, out=o_)
@@ -419,6 +429,7 @@ def main():
                 continue
             pm = G.Mapper(idx, G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o,
                                              metric=mname))
+            pm.set_tuning(**tuning)
             rm = pm.map_device(d_bases, d_off, n_reads, out=out)
             hip.sync()
             log(f"[bench] {mname}: warmup step done")
@@ -534,7 +545,8 @@ def main():
         "config": {"workload": args.workload, "reference_gbp": round(float(lens.sum()) / 1e9, 3),
                    "contigs": int(len(lens)), "reads": n_reads, "read_gbp": round(gbp, 3), "k": k, "r": h_frac,
                    "t": theta, "d": d, "o": o, "metric": args.metric, "mode": "batched, device-resident",
-                   "mappers": M},
+                   "mappers": M, "tuning": tuning,
+                   "tuning_by": "built-in rule" if args.no_autotune else "shm_mapper_autotune on this batch"},
         "one_mapper": {"reads_per_s": round(n_reads / one_ms * 1e3, 2), "ms_per_step": round(one_ms, 3)},
         "gbp_per_s": round(gbp * args.steps / elapsed, 3),
         "mapped_reads": n_mapped,

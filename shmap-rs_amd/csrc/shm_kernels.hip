@@ -1070,10 +1070,11 @@ constexpr uint32_t LTAB_MAX = LTAB * 7 / 8;
 // short for small ones, whose latency a long serial read would set).  Larger
 // reads, and reads with more buckets than one LDS table holds, are split into
 // chunks, one wave per chunk, whose LDS tables are merged in the read's global
-// table (k_contrib_chunks).  The chunk length is chosen per group: the largest
-// power of two from CONTRIB_CHUNK_MIN to CONTRIB_CHUNK_MAX hits that still
-// gives CONTRIB_CHUNKS_WANT chunks, so a few listed reads are spread over the
-// device and many are taken in long chunks (fewer merges).
+// table (k_contrib_chunks).  The chunk length is A.contrib_chunk when set,
+// else chosen per group: the largest power of two from CONTRIB_CHUNK_MIN to
+// CONTRIB_CHUNK_MAX hits that still gives CONTRIB_CHUNKS_WANT chunks, so a few
+// listed reads are spread over the device and many are taken in long chunks
+// (fewer merges).
 constexpr uint32_t CONTRIB_CHUNK_MIN = 512, CONTRIB_CHUNK_MAX = 4096;
 constexpr uint64_t CONTRIB_CHUNKS_WANT = 8192;
 
@@ -1404,8 +1405,11 @@ __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
     }
     uint64_t tot_e;
     (void)wg_excl_scan64(part, tmp, &tot_e);
-    uint32_t cs = CONTRIB_CHUNK_MAX;
-    while (cs > CONTRIB_CHUNK_MIN && tot_e / cs < CONTRIB_CHUNKS_WANT) cs >>= 1;
+    uint32_t cs = A.contrib_chunk;
+    if (cs == 0) {
+        cs = CONTRIB_CHUNK_MAX;
+        while (cs > CONTRIB_CHUNK_MIN && tot_e / cs < CONTRIB_CHUNKS_WANT) cs >>= 1;
+    }
     uint64_t carry = 0;
     for (uint32_t b0 = 0; b0 < n; b0 += WG) {
         const uint32_t i = b0 + threadIdx.x;
@@ -1536,7 +1540,7 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
 // threshold either match_rest pass can use).  k_prune takes one thread per
 // bucket: the thread extends its bucket seed by seed (the two binary searches
 // of matches_in_bucket for the bucket's span in the seed's hit run), stopping
-// as soon as sh falls below the threshold, for up to PRUNE_SEQ_SEEDS seeds
+// as soon as sh falls below the threshold, for up to A.prune_seq seeds
 // (most buckets die within a few); a bucket still alive with seeds left is
 // listed, with its partial sums, for k_prune_ext, where one wave extends it 64
 // seeds at a time, stopping at the first chunk that kills it.  Survivors are listed for refinement, and
@@ -1593,7 +1597,7 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 atomicMin(&A.rd_first[rd], p);
             } else {
                 // this thread extends its bucket seed by seed, stopping as soon
-                // as it dies, for up to PRUNE_SEQ_SEEDS seeds; a bucket still
+                // as it dies, for up to A.prune_seq seeds; a bucket still
                 // alive with seeds left goes to k_prune_ext with its partial
                 // sums, its next seed (sc_i) and its seeds so far (sc_same)
                 int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
@@ -1605,7 +1609,7 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 const int64_t begin = (int64_t)b * hl, end = (int64_t)(b + 2) * hl;
                 uint8_t alive = 1;
                 double sh;
-                const int32_t qend = u - c <= (int32_t)PRUNE_SEQ_SEEDS ? u : c + (int32_t)PRUNE_SEQ_SEEDS;
+                const int32_t qend = u - c <= (int32_t)A.prune_seq ? u : c + (int32_t)A.prune_seq;
                 int32_t q = c;
                 for (;; q++) {
                     sh = 1.0 - (double)(seeds - matches) / (double)m;

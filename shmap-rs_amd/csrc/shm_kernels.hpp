@@ -19,7 +19,6 @@ constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
-constexpr uint32_t PRUNE_SEQ_SEEDS = 16;    // seeds a thread extends a bucket with before a wave takes over
 constexpr uint32_t TASK_COUNTERS = 7;       // per group: first-pass, second band, 3 refine rounds, prune list,
                                             // reads on the global-table bucket path
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
@@ -120,6 +119,8 @@ struct MapArgs {
     uint32_t *ov_list, *ov_count;
     uint64_t *ov_kpre;
     uint32_t contrib_wave_e;  // examined hits one k_contrib_wave wave takes whole
+    uint32_t contrib_chunk;   // chunk length of the listed reads (0: chosen by k_contrib_over_scan)
+    uint32_t prune_seq;       // seeds one k_prune thread extends a bucket with
     // k_final's batch totals: the mapped reads (for the host's finalization)
     // and the sum of distinct k-mers
     uint32_t *mapped_list, *mapped_count;

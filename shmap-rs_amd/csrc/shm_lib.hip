@@ -776,7 +776,7 @@ struct shm_mapper {
     shm_sketcher sk{};
     hipStream_t st = nullptr;
     uint32_t max_grid = 1024;
-    uint32_t contrib_wave_e = 0;  // 0: chosen per batch
+    shm_tuning tune{};  // batching choices (0: the built-in rule)
     uint64_t tab_budget = 1ull << 30;  // bucket-table slots per group (24 B each, ~26 GB)
     bool timing = false;
     hipEvent_t ev[ST_COUNT][2];
@@ -839,7 +839,7 @@ extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, 
     mp->sk.h_frac = p->h_frac;
     mp->sk.thres = hash_threshold(p->h_frac);
     mp->max_grid = device_grid(idx->device);
-    if (const char *e = getenv("SHM_CONTRIB_WAVE_E")) mp->contrib_wave_e = (uint32_t)strtoul(e, nullptr, 10);
+
     if (const char *e = getenv("SHM_TAB_BUDGET")) mp->tab_budget = strtoull(e, nullptr, 10);
     if (hipStreamCreate(&mp->st) != hipSuccess) {
         delete mp;
@@ -862,6 +862,73 @@ extern "C" void shm_mapper_free(shm_mapper *mp) {
 extern "C" int shm_mapper_set_timing(shm_mapper *mp, int32_t on) {
     if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
     mp->timing = on != 0;
+    return SHM_OK;
+}
+
+extern "C" int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t) {
+    if (!mp || !t) return fail(SHM_ERR_ARG, "invalid argument");
+    if (t->contrib_wave_hits < 0 || t->prune_thread_seeds < 0 || t->contrib_chunk_hits < 0 ||
+        (t->contrib_chunk_hits &&
+         (t->contrib_chunk_hits < 64 || t->contrib_chunk_hits > 4096 || (t->contrib_chunk_hits & (t->contrib_chunk_hits - 1)))))
+        return fail(SHM_ERR_ARG, "tuning out of range (contrib_chunk_hits: 0 or a power of two from 64 to 4096; "
+                                 "the others >= 0)");
+    mp->tune = *t;
+    return SHM_OK;
+}
+
+extern "C" int shm_mapper_get_tuning(const shm_mapper *mp, shm_tuning *t) {
+    if (!mp || !t) return fail(SHM_ERR_ARG, "invalid argument");
+    *t = mp->tune;
+    return SHM_OK;
+}
+
+extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out);
+
+// Coordinate search: each field in turn over its candidates, the others held
+// at the best so far; a candidate's cost is the better of two timed batches
+// (GPU time from the first kernel to the last record).
+extern "C" int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_t n, shm_tuning *out) {
+    if (!mp || n < 0 || (n > 0 && !reads)) return fail(SHM_ERR_ARG, "invalid argument");
+    std::vector<shm_read_result> res((size_t)(n > 0 ? n : 1));
+    shm_tuning best = mp->tune;
+    auto cost = [&](const shm_tuning &t, float *ms) -> int {
+        mp->tune = t;
+        float b = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            RCHK(shm_map_batch(mp, reads, n, res.data()));
+            b = mp->total_ms < b ? mp->total_ms : b;
+        }
+        *ms = b;
+        return SHM_OK;
+    };
+    RCHK(shm_map_batch(mp, reads, n, res.data()));  // buffers sized before timing
+    float best_ms;
+    RCHK(cost(best, &best_ms));
+    static const int32_t cand_prune[] = {4, 8, 16, 32, 64, 128};
+    static const int32_t cand_wave[] = {512, 1024, 2048, 4096, 8192};
+    static const int32_t cand_chunk[] = {0, 512, 1024, 2048, 4096};
+    for (int field = 0; field < 3; field++) {
+        const int32_t *cand = field == 0 ? cand_prune : field == 1 ? cand_wave : cand_chunk;
+        const int nc = field == 0 ? 6 : 5;
+        for (int i = 0; i < nc; i++) {
+            shm_tuning t = best;
+            int32_t &f = field == 0 ? t.prune_thread_seeds : field == 1 ? t.contrib_wave_hits : t.contrib_chunk_hits;
+            if (f == cand[i]) continue;
+            f = cand[i];
+            float ms;
+            const int rc = cost(t, &ms);
+            if (rc != SHM_OK) {
+                mp->tune = best;
+                return rc;
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    }
+    mp->tune = best;
+    if (out) *out = best;
     return SHM_OK;
 }
 
@@ -1144,7 +1211,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         A.ov_kpre = mp->ov_kpre.p;
         // a big batch fills the device with whole reads; a small one splits
         // long reads so no read's serial walk sets the batch's time
-        A.contrib_wave_e = mp->contrib_wave_e ? mp->contrib_wave_e : (E_grp >= (1ull << 24) ? 4096u : 1024u);
+        A.contrib_wave_e = mp->tune.contrib_wave_hits > 0 ? (uint32_t)mp->tune.contrib_wave_hits
+                                                          : (E_grp >= (1ull << 24) ? 4096u : 1024u);
+        A.contrib_chunk = (uint32_t)mp->tune.contrib_chunk_hits;
+        A.prune_seq = mp->tune.prune_thread_seeds > 0 ? (uint32_t)mp->tune.prune_thread_seeds : 16u;
         TIME_BEGIN(ST_TAB_INIT);
         launch_stage(ST_TAB_INIT, A, I, nullptr, RG, mp->max_grid, st);
         TIME_END(ST_TAB_INIT);

@@ -117,6 +117,9 @@ def lib():
         L.shm_mapper_create.argtypes = [C.c_void_p, C.POINTER(MapParams), C.POINTER(C.c_void_p)]
         L.shm_mapper_free.argtypes = [C.c_void_p]
         L.shm_mapper_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.shm_mapper_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.shm_mapper_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.shm_mapper_autotune.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.POINTER(Tuning)]
         L.shm_map_batch.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.c_void_p]
         L.shm_map_batch_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.shm_format_paf.argtypes = [C.c_void_p, C.POINTER(MapParams), C.c_char_p, C.POINTER(ReadResult),
@@ -142,6 +145,15 @@ def lib():
         L.shm_host_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+class Tuning(C.Structure):
+    """shm_tuning: a mapper's batching choices (0: the built-in rule)."""
+    _fields_ = [("contrib_wave_hits", C.c_int32), ("contrib_chunk_hits", C.c_int32),
+                ("prune_thread_seeds", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def _check(rc: int) -> None:
@@ -268,6 +280,27 @@ def make_params(k=15, h_frac=0.05, theta=0.9, min_diff=0.02, max_overlap=0.5, me
                      int(abs_pos))
 
 
+def _reads(reads):
+    """(shm_read array, count, objects to keep alive) for a list of bytes or
+    a (bases uint8 array, offsets int64 array) pair."""
+    if isinstance(reads, tuple):
+        bases, off = reads
+        n = len(off) - 1
+        arr = (Read * max(n, 1))()
+        base_ptr = bases.ctypes.data
+        for i in range(n):
+            arr[i].seq, arr[i].len = base_ptr + int(off[i]), int(off[i + 1] - off[i])
+        return arr, n, bases
+    n = len(reads)
+    arr = (Read * max(n, 1))()
+    keep = []
+    for i, s in enumerate(reads):
+        a, ptr = _buf(s)
+        keep.append(a)
+        arr[i].seq, arr[i].len = ptr, len(a)
+    return arr, n, keep
+
+
 class Mapper:
     """create_mapper + Mapper::map_reads (src/mapper.rs:15-39) over a GPU index."""
 
@@ -285,25 +318,33 @@ class Mapper:
     def set_timing(self, on: bool) -> None:
         _check(lib().shm_mapper_set_timing(self._p, int(on)))
 
+    def set_tuning(self, **fields) -> None:
+        """shm_mapper_set_tuning: contrib_wave_hits, contrib_chunk_hits,
+        prune_thread_seeds (0: the built-in rule); unnamed fields keep their values."""
+        t = Tuning()
+        _check(lib().shm_mapper_get_tuning(self._p, C.byref(t)))
+        for k, v in fields.items():
+            setattr(t, k, int(v))
+        _check(lib().shm_mapper_set_tuning(self._p, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().shm_mapper_get_tuning(self._p, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, reads) -> dict:
+        """shm_mapper_autotune on a sample batch (as map_batch takes it);
+        returns the tuning kept."""
+        arr, n, keep = _reads(reads)
+        t = Tuning()
+        _check(lib().shm_mapper_autotune(self._p, arr, n, C.byref(t)))
+        del keep
+        return t.as_dict()
+
     def map_batch(self, reads) -> np.ndarray:
         """reads: list of bytes (or (bases uint8 array, offsets int64 array)).
         Returns RESULT_DTYPE records in input order."""
-        if isinstance(reads, tuple):
-            bases, off = reads
-            n = len(off) - 1
-            arr = (Read * max(n, 1))()
-            base_ptr = bases.ctypes.data
-            for i in range(n):
-                arr[i].seq, arr[i].len = base_ptr + int(off[i]), int(off[i + 1] - off[i])
-            keep = bases
-        else:
-            n = len(reads)
-            arr = (Read * max(n, 1))()
-            keep = []
-            for i, s in enumerate(reads):
-                a, ptr = _buf(s)
-                keep.append(a)
-                arr[i].seq, arr[i].len = ptr, len(a)
+        arr, n, keep = _reads(reads)
         out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
         _check(lib().shm_map_batch(self._p, arr, n, out.ctypes.data))
         del keep
