@@ -8,7 +8,7 @@ Workload (default ``genome``): a 3.1 Gbp synthetic reference of 24 contigs
 Containment.  The reference index is built on the GPU and stays resident; the
 reads are uploaded to HBM before the timed region.  One step maps every read
 (batched) and leaves the per-read records in page-locked host memory.  The
-timed steps are taken by --mappers mappers (default 2) in turn, each on its
+timed steps are taken by --mappers mappers (default 3) in turn, each on its
 own HIP stream and host thread, so one batch's host waits and kernel tails
 overlap another's kernels (the serving loop's workers run the same way); the
 single-mapper rate is reported beside it (one_mapper).
@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--metric", default="Containment")
     ap.add_argument("--reads", type=int, default=None, help="override the read count")
     ap.add_argument("--plain-reads", type=int, default=200)
-    ap.add_argument("--mappers", type=int, default=2,
+    ap.add_argument("--mappers", type=int, default=3,
                     help="mappers mapping whole batches concurrently in the timed steps (each its own HIP stream, "
                          "host thread and page-locked output), as the serving loop's workers do")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -284,7 +284,7 @@ def main():
                     help="time the other scoring metrics on any workload (default: genome only)")
     ap.add_argument("--serve-reads", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=1_000_000.0, help="offered reads/s (Poisson arrivals)")
-    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=16384, help="admission budget: reads per batch")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in batching choices instead of tuning them on the batch first")

@@ -244,7 +244,7 @@ int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_t n_reads, 
 typedef struct shm_server shm_server;
 
 typedef struct {
-    int32_t workers;         /* concurrent batches (mappers/streams); 0 = 2 */
+    int32_t workers;         /* concurrent batches (mappers/streams); 0 = 3 */
     int32_t max_batch_reads; /* admission budget per batch; 0 = 65536 */
     int64_t max_batch_bases; /* admission budget per batch; 0 = 2^31 */
 } shm_server_config;

@@ -102,7 +102,7 @@ extern "C" int shm_server_create(const shm_index *idx, const shm_map_params *par
     *out = nullptr;
     shm_server *s = new shm_server;
     if (cfg) s->cfg = *cfg;
-    if (s->cfg.workers <= 0) s->cfg.workers = 2;
+    if (s->cfg.workers <= 0) s->cfg.workers = 3;  // three streams overlap best on one device
     if (s->cfg.max_batch_reads <= 0) s->cfg.max_batch_reads = 65536;
     if (s->cfg.max_batch_bases <= 0) s->cfg.max_batch_bases = (int64_t)1 << 31;
     for (int i = 0; i < s->cfg.workers; i++) {

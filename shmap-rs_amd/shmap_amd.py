@@ -380,7 +380,7 @@ class Server:
     """Serving loop over a resident index (shm_server_*): reads are submitted as
     they arrive and mapped in batches by `workers` concurrent mappers."""
 
-    def __init__(self, index: SketchIndex, params: MapParams, workers: int = 2, max_batch_reads: int = 0,
+    def __init__(self, index: SketchIndex, params: MapParams, workers: int = 3, max_batch_reads: int = 0,
                  max_batch_bases: int = 0):
         self._index = index
         self._params = params
