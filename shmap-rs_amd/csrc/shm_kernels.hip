@@ -1815,15 +1815,25 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // occurrences, latest first -- and of the reversed sequence, by patience
 // sorting (each value replaces the first tail at or above it, or is appended).
 // The wave walks 64 positions at a time: a scan of their occurrence counts
-// places each of the next 64 values on a lane, which loads it, and the values
-// are then taken one by one.  Up to LCS_TAILS tails live across the wave's
-// registers (tail i in register i / 64 of lane i % 64, and each chunk's first
-// tail in one more register, lane = chunk); a value's place is the number of
-// smaller tails: one ballot over the chunk heads picks the chunk, one more
-// counts inside it.  Returns -1 when the read's sketch (which bounds the
-// tails) is larger.
+// places each of the next 64 values on a lane, which loads it.  Up to
+// LCS_TAILS tails live across the wave's registers (tail i in register i / 64
+// of lane i % 64, and each chunk's first tail in one more register, lane =
+// chunk); a value's place is the number of smaller tails: one ballot over the
+// chunk heads picks the chunk, one more counts inside it.
+//
+// A position's values form a run, strictly decreasing (forward) or increasing
+// (reversed).  Short runs are taken value by value.  A run of LCS_RUN_MIN or
+// more values in a round is applied at once from the tails in LDS (Ts):
+// every value's lower bound p among the tails before the run, by binary
+// search.  Decreasing: processing them one by one would give each value that
+// same p (earlier, larger values only lower tails at or after it), so tail p
+// becomes the smallest value with that p, and the run appends at most once.
+// Increasing: value j's place is max over i <= j of p_i + (j - i), a running
+// maximum, and the places are distinct.  Returns -1 when the read's sketch
+// (which bounds the tails) is larger.
 constexpr int32_t LCS_TAILS = 32 * WAVE;
-__device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m) {
+constexpr uint32_t LCS_RUN_MIN = 16;
+__device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m, int32_t *Ts) {
     if (m > LCS_TAILS) return -1;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     int32_t best = 0;
@@ -1833,6 +1843,7 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
         for (int r = 0; r < LCS_TAILS / WAVE; r++) t[r] = 0;
         int32_t head = INT32_MAX;  // lane r: the first tail of chunk r (tails 64r ..), INT32_MAX if none
         int32_t nt = 0;
+        bool in_lds = false;  // the tails are in Ts rather than the registers
         for (int64_t g0 = 0; g0 < n; g0 += WAVE) {
             const int64_t gi = g0 + (int64_t)lane;
             const int64_t x = dir == 0 ? gi : n - 1 - gi;
@@ -1862,20 +1873,74 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
                 int32_t val = 0;
                 if (vi < T) val = A.k_r[o + A.ps_v[o + (dir == 0 ? fa + ca - 1 - q : fa + q)]];
                 const uint32_t cnt = T - v0 < (uint32_t)WAVE ? T - v0 : (uint32_t)WAVE;
-                for (uint32_t j = 0; j < cnt; j++) {
-                    const int32_t v = __builtin_amdgcn_readlane(val, (int)j);
-                    // chunks whose first tail is below v, then the tails below v
-                    // in the last of them
-                    const uint32_t k = (uint32_t)__popcll(__ballot(head < v));
-                    uint32_t pos = 0;
-                    if (k > 0) {
-                        const int32_t ck = (int32_t)k - 1;
-                        pos = (uint32_t)ck * WAVE +
-                              (uint32_t)__popcll(__ballot(ck * WAVE + (int32_t)lane < nt && t[ck] < v));
+                // the round's runs: lanes whose value starts a position's run
+                const uint32_t ap = __shfl_up(a, 1);
+                uint64_t rest = __ballot(lane < cnt && (lane == 0 || a != ap));
+                while (rest) {
+                    const uint32_t s0 = (uint32_t)__ffsll((unsigned long long)rest) - 1;
+                    rest &= rest - 1;
+                    const uint32_t e0 = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : cnt;
+                    if (e0 - s0 < LCS_RUN_MIN) {
+                        if (in_lds) {  // back to the registers
+                            wave_sync();
+#pragma unroll
+                            for (int r = 0; r < LCS_TAILS / WAVE; r++)
+                                t[r] = r * WAVE + (int32_t)lane < nt ? Ts[r * WAVE + lane] : 0;
+                            head = (int32_t)lane < LCS_TAILS / WAVE && (int32_t)lane * WAVE < nt ? Ts[lane * WAVE]
+                                                                                                : INT32_MAX;
+                            in_lds = false;
+                        }
+                        for (uint32_t j = s0; j < e0; j++) {
+                            const int32_t v = __builtin_amdgcn_readlane(val, (int)j);
+                            // chunks whose first tail is below v, then the tails
+                            // below v in the last of them
+                            const uint32_t k = (uint32_t)__popcll(__ballot(head < v));
+                            uint32_t pos = 0;
+                            if (k > 0) {
+                                const int32_t ck = (int32_t)k - 1;
+                                pos = (uint32_t)ck * WAVE +
+                                      (uint32_t)__popcll(__ballot(ck * WAVE + (int32_t)lane < nt && t[ck] < v));
+                            }
+                            if (pos == (uint32_t)nt) nt++;
+                            if (lane == pos % WAVE) t[pos / WAVE] = v;
+                            if (pos % WAVE == 0 && lane == pos / WAVE) head = v;
+                        }
+                    } else {
+                        if (!in_lds) {
+#pragma unroll
+                            for (int r = 0; r < LCS_TAILS / WAVE; r++) Ts[r * WAVE + lane] = t[r];
+                            in_lds = true;
+                            wave_sync();
+                        }
+                        if (lane == 0) Ts[nt] = INT32_MAX;  // an append slot (Ts holds LCS_TAILS + 1)
+                        wave_sync();
+                        const bool mine = lane >= s0 && lane < e0;
+                        uint32_t lo = 0, hi = (uint32_t)nt;
+                        if (mine)
+                            while (lo < hi) {
+                                const uint32_t mid = (lo + hi) >> 1;
+                                if (Ts[mid] < val) lo = mid + 1; else hi = mid;
+                            }
+                        if (dir == 0) {
+                            const bool grew = __ballot(mine && lo == (uint32_t)nt) != 0;
+                            wave_sync();  // every search is done before a tail changes
+                            if (mine) atomicMin(&Ts[lo], val);
+                            if (grew) nt++;
+                        } else {
+                            int32_t pm = mine ? (int32_t)lo - (int32_t)(lane - s0) : INT32_MIN;
+#pragma unroll
+                            for (int d = 1; d < WAVE; d <<= 1) {
+                                const int32_t o2 = __shfl_up(pm, d);
+                                if ((int32_t)lane >= d && o2 > pm) pm = o2;
+                            }
+                            const int32_t place = mine ? pm + (int32_t)(lane - s0) : 0;
+                            const int32_t last = __builtin_amdgcn_readlane(place, (int)(e0 - 1));
+                            wave_sync();  // every search is done before a tail changes
+                            if (mine) Ts[place] = val;
+                            if (last + 1 > nt) nt = last + 1;
+                        }
+                        wave_sync();
                     }
-                    if (pos == (uint32_t)nt) nt++;
-                    if (lane == pos % WAVE) t[pos / WAVE] = v;
-                    if (pos % WAVE == 0 && lane == pos / WAVE) head = v;
                 }
             }
         }
@@ -1904,6 +1969,9 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
 // fits and in the workgroup's global scratch otherwise.  With abs_pos the
 // window edge depends on positions, so that mode keeps the sequential sweep;
 // so does LCS (:114-170), whose patience LIS is sequential by nature.
+// LCS: the bucket_LCS instance (its patience sweep needs registers the other
+// metrics' instance is better off without).
+template <bool LCS>
 __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     extern __shared__ uint8_t lds[];
     int32_t *l_sid = (int32_t *)lds;           // REFINE_CAP
@@ -1961,7 +2029,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         const int64_t end = be < tlen ? be : tlen;
         const int64_t n = end > l0 ? end - l0 : 0;
         const bool fits = n <= REFINE_CAP;
-        const bool parallel = n > 0 && !ap && A.metric != METRIC_BUCKET_LCS;
+        const bool parallel = n > 0 && !ap && !LCS;
         // LDS when the span fits, else this workgroup's global scratch
         int32_t *const wsid = A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
         int32_t *const wtr = A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
@@ -2089,10 +2157,10 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         // sequential forms: LCS, abs_pos windows, or spans too large for LDS
         int32_t *tr_arr = keys;
         int32_t *diff = u <= REFINE_CAP ? l_c : A.g_diff + (uint64_t)blockIdx.x * A.diff_stride;
-        if (A.metric == METRIC_BUCKET_LCS) {
+        if constexpr (LCS) {
             // one wave streams the LIS (s_lcs < 0: thread 0 sweeps it below)
             if (threadIdx.x < WAVE) {
-                const int32_t best = wave_lcs(A, sid, n, o, m);
+                const int32_t best = wave_lcs(A, sid, n, o, m, l_c);  // l_c: free in this form
                 if (threadIdx.x == 0) s_lcs = best;
             }
         } else {
@@ -2103,7 +2171,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         if (threadIdx.x == 0) {
             double J = -1.0;
             int32_t inter = -1, same = -1, tl = -1, tr = -1, ssz = -1, pend = -1, mseg = -1;
-            if (A.metric == METRIC_BUCKET_LCS) {
+            if (LCS) {
                 // Patience LIS of the pmatches sequence and of its reverse; the
                 // tails hold distinct read positions, so at most m of them.
                 int32_t *tails = A.g_tails + (uint64_t)blockIdx.x * A.diff_stride;
@@ -2549,7 +2617,10 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
         for (int32_t round = 0; round < 3; round++) {
             P.refine_pass = round;
             k_refine_select<<<max_grid, WG, 0, st>>>(P);
-            k_refine<<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+            if (A.metric == METRIC_BUCKET_LCS)
+                k_refine<true><<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+            else
+                k_refine<false><<<max_grid, WG, REFINE_LDS, st>>>(P, I);
         }
         break;
     }

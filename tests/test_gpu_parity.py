@@ -138,6 +138,28 @@ def test_lcs_long_sketches_match_oracle(h_frac):
     assert not bad, bad
 
 
+def test_lcs_repeat_runs_match_oracle():
+    """bucket_LCS inside tandem arrays: HiFi reads of 15 kb in arrays of one
+    171 bp unit (0.1-0.2% copy divergence), so a seed occurs 15-25 times in a
+    read on average and the pmatches sequence is made of long runs
+    (decreasing forward, increasing reversed): the runs the wave applies at
+    once from the tails' lower bounds."""
+    lens = [200_000, 200_000]
+    ref = G.synth_reference(51, lens, repeat_frac=0.7, unit_len=171, div_min=0.001, div_max=0.002, n_units=1,
+                            array_len=60_000)
+    bases, off, _ = G.synth_reads(52, ref, lens, 14, (15_000, 0.0), errors=((0.001, 0.6, 0.2, 0.2),))
+    segs, reads = P.contigs(ref, lens), P.split_reads(bases, off)
+    prm = dict(theta=0.4, min_diff=0.075, max_overlap=0.3, metric="bucket_LCS")
+    oi, op, ores = P.oracle_map(segs, reads, 15, 0.1, **prm)
+    gi, gp, mp, gres = P.gpu_map(segs, reads, 15, 0.1, **prm)
+    assert ores["mapped"].sum() > 0
+    # reads whose k-mers repeat at least 16 times on average, within one wave's tails
+    rep = (gres["u"] > 0) & (gres["m"] >= 16 * gres["u"]) & (gres["m"] <= 2048)
+    assert rep.sum() >= 3, (gres["m"], gres["u"])
+    bad = P.diff_results(ores, gres)
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("variant", [dict(no_bucket_pruning=True), dict(one_sweep=True),
                                      dict(no_bucket_pruning=True, one_sweep=True), dict(abs_pos=True)])
 def test_variants_match_oracle(variant):
