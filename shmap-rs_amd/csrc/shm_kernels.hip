@@ -1822,8 +1822,9 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // chunk heads picks the chunk, one more counts inside it.
 //
 // A position's values form a run, strictly decreasing (forward) or increasing
-// (reversed).  Short runs are taken value by value.  A run of LCS_RUN_MIN or
-// more values in a round is applied at once from the tails in LDS (Ts):
+// (reversed).  Single values (and runs shorter than LCS_RUN_MIN) are taken
+// one by one.  A run of LCS_RUN_MIN or more values in a round is applied at
+// once from the tails in LDS (Ts):
 // every value's lower bound p among the tails before the run, by binary
 // search.  Decreasing: processing them one by one would give each value that
 // same p (earlier, larger values only lower tails at or after it), so tail p
@@ -1832,7 +1833,7 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // maximum, and the places are distinct.  Returns -1 when the read's sketch
 // (which bounds the tails) is larger.
 constexpr int32_t LCS_TAILS = 32 * WAVE;
-constexpr uint32_t LCS_RUN_MIN = 16;
+constexpr uint32_t LCS_RUN_MIN = 4;
 __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m, int32_t *Ts) {
     if (m > LCS_TAILS) return -1;
     const uint32_t lane = threadIdx.x & (WAVE - 1);

@@ -154,7 +154,7 @@ def test_lcs_repeat_runs_match_oracle():
     gi, gp, mp, gres = P.gpu_map(segs, reads, 15, 0.1, **prm)
     assert ores["mapped"].sum() > 0
     # reads whose k-mers repeat at least 16 times on average, within one wave's tails
-    rep = (gres["u"] > 0) & (gres["m"] >= 16 * gres["u"]) & (gres["m"] <= 2048)
+    rep = (gres["u"] > 0) & (gres["m"] >= 16 * gres["u"]) & (gres["m"] <= 2048)  # noqa: E501
     assert rep.sum() >= 3, (gres["m"], gres["u"])
     bad = P.diff_results(ores, gres)
     assert not bad, bad
