@@ -384,7 +384,7 @@ def main():
     res = out
     for o_ in outs[1:]:
         assert np.array_equal(o_, out), "concurrent mappers disagree"
-    del mps[1:], outs[1:]
+    mps = outs = m_ = o_ = None  # the extra mappers (and their streams) go now
     n_mapped = int(res["mapped"].sum())
     gbp = float(off[-1]) / 1e9
 
@@ -491,6 +491,7 @@ def main():
 
     serving = None
     if not args.no_serve:
+        mp = None  # the serving workers get the device's hardware queues to themselves
         serving = run_serving(idx, prm, lens, ref, args)
         log(f"[bench] serving: {serving}")
 
