@@ -47,6 +47,7 @@ def _load(name, path):
 G = _load("shmap_amd", os.path.join(ROOT, "shmap-rs_amd", "shmap_amd.py"))
 
 HBM_PEAK_GBS = 8000.0
+ENGINE_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 PROBE_COPY_BYTES = 1 << 30  # calibration copy: reads and writes exactly this many bytes
 
 # main kernel of each timed stage (for the counter passes)
@@ -153,7 +154,8 @@ def stage_bytes(res: np.ndarray, bases_total: int) -> dict:
 
 
 def _pmc_pass(counter: str, args, out_dir: str) -> dict:
-    """One rocprofv3 --pmc pass over a probe run; {kernel name: [values per dispatch]} in KiB."""
+    """One rocprofv3 --pmc pass over a probe run; {kernel name: [values per dispatch]}
+    (KiB for the size counters, instructions for SQ_INSTS_VALU)."""
     import csv
     import subprocess
     cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "--output-format", "csv",
@@ -202,6 +204,25 @@ def measure_traffic(args, stage: str) -> dict:
             res[key + "_bytes"] = int(kern[0][-1] * 1024.0 * scale)
     res["bytes"] = res["fetch_bytes"] + res["write_bytes"]
     return res
+
+
+def measure_issue(args, stage: str, kernel_ms: float) -> dict:
+    """VALU issue of the stage's kernel in one mapping step: SQ_INSTS_VALU (wave
+    instructions; a wave64 VALU instruction takes 4 cycles of a 16-lane SIMD)
+    against what the device's SIMDs issue in the kernel's time at the peak
+    engine clock."""
+    import tempfile
+    want = STAGE_KERNEL[stage]
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        vals = _pmc_pass("SQ_INSTS_VALU", args, os.path.join(td, "valu"))
+    kern = [x for k, x in vals.items() if want in k]
+    if not kern:
+        raise RuntimeError("SQ_INSTS_VALU: dispatch not found")
+    insts = kern[0][-1]
+    simds = 4 * G.device_cus()
+    frac = insts * 4.0 / (kernel_ms * 1e-3 * ENGINE_CLOCK_GHZ * 1e9 * simds)
+    return {"valu_wave_insts": int(insts), "simds": simds, "engine_clock_ghz": ENGINE_CLOCK_GHZ,
+            "valu_issue_frac": round(frac, 4)}
 
 
 def run_serving(idx, prm, lens, ref, args) -> dict:
@@ -414,6 +435,12 @@ def main():
         except Exception as e:  # the counters are a report, not the measurement
             roof["traffic_error"] = str(e)[:300]
             log(f"[bench] traffic: {e}")
+        try:
+            roof["issue"] = measure_issue(args, dom, stage_ms[dom])
+            log(f"[bench] issue: {roof['issue']}")
+        except Exception as e:
+            roof["issue_error"] = str(e)[:300]
+            log(f"[bench] issue: {e}")
     stage_rates = {s: (round(sb[s] / (stage_ms[s] * 1e-3) / 1e9, 2) if s in sb and stage_ms[s] > 0 else None)
                    for s in stage_ms}
     log("[bench] stage ms: " + ", ".join(f"{s}={v:.2f}" for s, v in stage_ms.items()))

@@ -186,6 +186,10 @@ int shm_format_unmapped(const char *query_id, int32_t read_len, char *buf, size_
 int shm_host_alloc(size_t bytes, void **out);
 void shm_host_free(void *p);
 
+/* The number of compute units of device `device` (for occupancy and issue
+ * figures); no reference counterpart. */
+int shm_device_compute_units(int32_t device, int32_t *cus);
+
 /* Device kernel timing of the last batch on this mapper, in milliseconds:
  * total GPU time and the time of each stage (HIP events on the mapper's own
  * stream).  stage_ms receives up to n_stages values; names via

@@ -859,6 +859,14 @@ extern "C" void shm_mapper_free(shm_mapper *mp) {
     delete mp;
 }
 
+extern "C" int shm_device_compute_units(int32_t device, int32_t *cus) {
+    if (!cus) return fail(SHM_ERR_ARG, "invalid argument");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    *cus = prop.multiProcessorCount;
+    return SHM_OK;
+}
+
 extern "C" int shm_mapper_set_timing(shm_mapper *mp, int32_t on) {
     if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
     mp->timing = on != 0;

@@ -117,6 +117,7 @@ def lib():
         L.shm_mapper_create.argtypes = [C.c_void_p, C.POINTER(MapParams), C.POINTER(C.c_void_p)]
         L.shm_mapper_free.argtypes = [C.c_void_p]
         L.shm_mapper_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.shm_device_compute_units.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
         L.shm_mapper_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.shm_mapper_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.shm_mapper_autotune.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.POINTER(Tuning)]
@@ -145,6 +146,13 @@ def lib():
         L.shm_host_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+def device_cus(device: int = 0) -> int:
+    """Compute units of the device (shm_device_compute_units)."""
+    n = C.c_int32()
+    _check(lib().shm_device_compute_units(device, C.byref(n)))
+    return n.value
 
 
 class Tuning(C.Structure):
