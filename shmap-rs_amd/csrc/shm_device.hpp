@@ -234,14 +234,14 @@ __device__ void bitonic_core(K *k, V *v, uint32_t n) {
     for (uint32_t size = 2; size <= N; size <<= 1) {
         const uint32_t half = size >> 1;
         for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
-            const uint32_t blk = i / half, p = i % half;
+            const uint32_t blk = i >> (31 - __builtin_clz(half)), p = i & (half - 1);  // half: a power of two
             const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
             if (b < n) cswap(k, v, a, b);
         }
         __syncthreads();
         for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
             for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
-                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                const uint32_t a = ((i & ~(stride - 1)) << 1) | (i & (stride - 1)), b = a + stride;  // stride: a power of two
                 if (b < n) cswap(k, v, a, b);
             }
             __syncthreads();
@@ -266,14 +266,14 @@ __device__ void bitonic_keys(K *k, uint32_t n) {
     for (uint32_t size = 2; size <= N; size <<= 1) {
         const uint32_t half = size >> 1;
         for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
-            const uint32_t blk = i / half, p = i % half;
+            const uint32_t blk = i >> (31 - __builtin_clz(half)), p = i & (half - 1);  // half: a power of two
             const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
             if (b < n) cswap_k(k, a, b);
         }
         __syncthreads();
         for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
             for (uint32_t i = threadIdx.x; i < (N >> 1); i += WG) {
-                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                const uint32_t a = ((i & ~(stride - 1)) << 1) | (i & (stride - 1)), b = a + stride;  // stride: a power of two
                 if (b < n) cswap_k(k, a, b);
             }
             __syncthreads();
@@ -420,14 +420,14 @@ __device__ void wave_bitonic(K *k, V *v, uint32_t n) {
     for (uint32_t size = 2; size <= N; size <<= 1) {
         const uint32_t half = size >> 1;
         for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
-            const uint32_t blk = i / half, p = i % half;
+            const uint32_t blk = i >> (31 - __builtin_clz(half)), p = i & (half - 1);  // half: a power of two
             const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
             if (b < n) cswap(k, v, a, b);
         }
         wave_sync();
         for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
             for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
-                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                const uint32_t a = ((i & ~(stride - 1)) << 1) | (i & (stride - 1)), b = a + stride;  // stride: a power of two
                 if (b < n) cswap(k, v, a, b);
             }
             wave_sync();
@@ -444,14 +444,14 @@ __device__ void wave_bitonic_keys(K *k, uint32_t n) {
     for (uint32_t size = 2; size <= N; size <<= 1) {
         const uint32_t half = size >> 1;
         for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
-            const uint32_t blk = i / half, p = i % half;
+            const uint32_t blk = i >> (31 - __builtin_clz(half)), p = i & (half - 1);  // half: a power of two
             const uint32_t a = blk * size + p, b = blk * size + size - 1 - p;
             if (b < n) cswap_k(k, a, b);
         }
         wave_sync();
         for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
             for (uint32_t i = lane; i < (N >> 1); i += WAVE) {
-                const uint32_t a = (i / stride) * (stride << 1) + (i % stride), b = a + stride;
+                const uint32_t a = ((i & ~(stride - 1)) << 1) | (i & (stride - 1)), b = a + stride;  // stride: a power of two
                 if (b < n) cswap_k(k, a, b);
             }
             wave_sync();
