@@ -458,7 +458,7 @@ extern "C" int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, 
         uint64_t tot = 0;
         for (int s = 0; s < 8; s++) {
             uint32_t bits = 6;
-            while ((1ull << bits) < 2ull * hn[s] + 1) bits++;
+            while ((1ull << bits) < 4ull * hn[s] + 1) bits++;  // load <= 1/4: short probe chains
             idx->geom.bits[s] = bits;
             idx->geom.base[s] = tot;
             tot += 1ull << bits;
