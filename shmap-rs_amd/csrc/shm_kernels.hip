@@ -140,6 +140,30 @@ __device__ __forceinline__ uint32_t stage_codes8(uint32_t v0, uint32_t v1) {
     return __builtin_amdgcn_perm(t1, t0, 0x07050301u);  // bytes 1 and 3 of t0, then of t1
 }
 
+// Byte b of v times 16 (a 16-byte table entry's offset) as one byte-select
+// shift.  Written out because the compiler, knowing the bytes' high bits are
+// clear, otherwise emits a shift and a mask.  b is a constant after unrolling.
+template <int B> __device__ __forceinline__ uint32_t byte_x16_c(uint32_t v) {
+    uint32_t r;
+    if constexpr (B == 0)
+        asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(v));
+    else if constexpr (B == 1)
+        asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1" : "=v"(r) : "v"(v));
+    else if constexpr (B == 2)
+        asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(v));
+    else
+        asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ uint32_t byte_x16(uint32_t v, int b) {
+    switch (b) {
+        case 0: return byte_x16_c<0>(v);
+        case 1: return byte_x16_c<1>(v);
+        case 2: return byte_x16_c<2>(v);
+        default: return byte_x16_c<3>(v);
+    }
+}
+
 // One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
 // are staged in LDS as 4-bit codes, base p at nibble p & 7 of dword p >> 3, with
 // one dword of bank skew per 16 dwords (lanes rolling runs 128 windows = 16
@@ -230,7 +254,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
             const int64_t nwin = w1 - w0;
             const int64_t lw0 = (int64_t)lane * LANE_W;
             const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
-            uint32_t cnt = 0;
+            uint32_t cnt = 0;                        // PASS 1: k-mers written so far
+            uint32_t so = (uint32_t)lane * 16u;      // PASS 0: byte offset of the next staging slot
+            const uint32_t so_lim = lcap * (WAVE * 16u);
             uint64_t base_out = 0;
             if (PASS == 1) {
                 const uint32_t mine = lane_cnt[c * WAVE + lane];
@@ -268,7 +294,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 }
                 // A kept k-mer goes to the lane's next staging slot (PASS 0):
                 // one 16-byte record (hash, position, strand) at chunk c's
-                // uniform base plus a 32-bit lane offset, so one store.
+                // uniform base plus a running 32-bit byte offset (slots of one
+                // lane are WAVE * 16 bytes apart), so one add and one store;
+                // the offset keeps counting past lcap slots without storing.
                 uint4 *st_c = stage + (PASS == 0 ? (uint64_t)c * lcap * WAVE : 0);
                 const int32_t r0 = (int32_t)(w0 + k - 1);  // window wi ends at base w0 + wi + k - 1
                 auto put = [&](int64_t wi, uint64_t h, bool strand) {
@@ -277,10 +305,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                         out_h[o] = h;
                         out_r[o] = r0 + (int32_t)wi;
                         out_s[o] = strand;
+                        cnt++;
                     } else {
-                        if (cnt >= lcap) return;
-                        st_c[cnt * WAVE + (uint32_t)lane] =
-                            make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)(r0 + (int32_t)wi), strand ? 1u : 0u);
+                        if (so < so_lim)
+                            *(uint4 *)((uint8_t *)st_c + so) =
+                                make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)(r0 + (int32_t)wi), strand ? 1u : 0u);
+                        so += WAVE * 16u;
                     }
                 };
                 // windows lw0 .. lw1-1; the step into window lw0+j rolls out
@@ -313,8 +343,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 #pragma unroll
                         for (int t4 = 0; t4 < 4; t4++) {
                             const int t = half * 4 + t4;
-                            const uint4 tt =
-                                T4[__builtin_amdgcn_perm(0u, (t & 1) ? po : pe, 0x0c0c0c00u | (uint32_t)(t >> 1))];
+                            const uint32_t toff = byte_x16((t & 1) ? po : pe, t >> 1);
+                            const uint4 tt = *(const uint4 *)((const uint8_t *)T4 + toff);
                             const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 31) ^ tt.y;  // rotl1
                             const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 31) ^ tt.x;
                             const uint32_t nrh = __builtin_amdgcn_alignbit(rl, rh, 1) ^ tt.w;   // rotr1
@@ -330,10 +360,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                     const uint64_t f = ((uint64_t)sfh[t4] << 32) | sfl[t4];
                                     const uint64_t r = ((uint64_t)srh[t4] << 32) | srl[t4];
                                     const uint64_t h = f ^ r;
-                                    if (h <= thres) {
-                                        put(lw0 + j + half * 4 + t4, h, f > r);
-                                        cnt++;
-                                    }
+                                    if (h <= thres) put(lw0 + j + half * 4 + t4, h, f > r);
                                 }
                         }
                     }
@@ -346,13 +373,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                     hf = rotl1(hf) ^ tt.x;
                     hr = rotr1(hr) ^ tt.y;
                     const uint64_t h = hr ^ hf;
-                    if (h <= thres) {
-                        put(lw0 + j, h, hf > hr);
-                        cnt++;
-                    }
+                    if (h <= thres) put(lw0 + j, h, hf > hr);
                 }
             }
             if (PASS == 0) {
+                cnt = (so - (uint32_t)lane * 16u) / (WAVE * 16u);
                 lane_cnt[c * WAVE + lane] = cnt;
                 uint32_t s2 = cnt;
                 for (int d = WAVE / 2; d > 0; d >>= 1) s2 += __shfl_xor(s2, d);
