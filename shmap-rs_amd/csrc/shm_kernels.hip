@@ -222,28 +222,52 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
             w0 = (int64_t)(c - chunk_start[seq]) * CHUNK_W;
             w1 = w0 + CHUNK_W < nw ? w0 + CHUNK_W : nw;
             const int64_t nb = w1 - w0 + k - 1;
-            const uintptr_t src = (uintptr_t)(S.bases + S.off[seq] + w0);
-            const uintptr_t a = src & ~(uintptr_t)3;
-            const uint32_t sh = (uint32_t)(src - a) * 8;
+            // Pointer arithmetic (not integer casts) keeps the loads global,
+            // so they do not wait on the LDS writes; the aligned start is
+            // wave-uniform (a scalar base plus 32-bit lane offsets).
+            const uint8_t *srcp = S.bases + S.off[seq] + w0;
+            const uint32_t al = (uint32_t)((uintptr_t)srcp & 3u);
+            const uint32_t sh = al * 8;  // alignbit by 0 returns the low dword
+            // (readfirstlane returns a signed int: each half goes through
+            // uint32_t so the low half is not sign-extended past 2 GiB)
+            const uint64_t a_off = (uint64_t)(S.off[seq] + w0 - al);
+            const uint64_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a_off >> 32));
+            const uint64_t a_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a_off);
+            const uint32_t *s32 = (const uint32_t *)(S.bases + ((a_hi << 32) | a_lo));
             const uintptr_t lim = (uintptr_t)(S.bases + S.off[S.n]);  // never read past the set
-            const uint32_t *s32 = (const uint32_t *)a;
-            const int64_t ng = (nb + 7) >> 3;  // code dwords, 8 bases each
-            if (a + 4 * (uintptr_t)(2 * ng + 1) <= lim) {
-                // every aligned dword this chunk needs lies inside the set
-#pragma unroll 2
-                for (int64_t g = lane; g < ng; g += WAVE) {
-                    const uint32_t d0 = s32[2 * g], d1 = s32[2 * g + 1], d2 = s32[2 * g + 2];
-                    const uint32_t v0 = sh ? __builtin_amdgcn_alignbit(d1, d0, sh) : d0;
-                    const uint32_t v1 = sh ? __builtin_amdgcn_alignbit(d2, d1, sh) : d1;
-                    SKW(g) = stage_codes8(v0, v1);
+            const uint32_t ng = (uint32_t)((nb + 7) >> 3);  // code dwords, 8 bases each
+            if ((uintptr_t)s32 + 4 * (uintptr_t)(2 * ng + 1) <= lim) {
+                // every aligned dword this chunk needs lies inside the set.
+                // Four groups per lane at a time: their loads first (a group
+                // past the chunk's last reloads the last, whose codes are not
+                // stored), so four are in flight; q is SKW's skewed index of g
+                // (g steps by WAVE, so q steps by WAVE + WAVE / 16).
+                constexpr uint32_t QS = WAVE + WAVE / 16;
+                const uint8_t *s8 = (const uint8_t *)s32;
+                uint32_t q = (uint32_t)lane + ((uint32_t)lane >> 4);
+                for (uint32_t g0 = lane; g0 < ng; g0 += 4 * WAVE, q += 4 * QS) {
+                    uint32_t d[4][3];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t bo = 8u * min(g0 + i * WAVE, ng - 1);  // byte offset of the group
+                        d[i][0] = *(const uint32_t *)(s8 + bo);
+                        d[i][1] = *(const uint32_t *)(s8 + bo + 4);
+                        d[i][2] = *(const uint32_t *)(s8 + bo + 8);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const uint32_t v = stage_codes8(__builtin_amdgcn_alignbit(d[i][1], d[i][0], sh),
+                                                        __builtin_amdgcn_alignbit(d[i][2], d[i][1], sh));
+                        if (g0 + i * WAVE < ng) b32[q + i * QS] = v;
+                    }
                 }
             } else {
-                for (int64_t g = lane; g < ng; g += WAVE) {
-                    const uintptr_t at = src + 8 * (uintptr_t)g;  // first byte of this group
+                for (uint32_t g = lane; g < ng; g += WAVE) {
+                    const uint8_t *at = srcp + 8 * (size_t)g;  // first byte of this group
                     uint32_t v0 = 0, v1 = 0;
                     for (uint32_t j = 0; j < 4; j++) {
-                        if (at + j < lim) v0 |= (uint32_t)(*(const uint8_t *)(at + j)) << (8 * j);
-                        if (at + 4 + j < lim) v1 |= (uint32_t)(*(const uint8_t *)(at + 4 + j)) << (8 * j);
+                        if ((uintptr_t)(at + j) < lim) v0 |= (uint32_t)at[j] << (8 * j);
+                        if ((uintptr_t)(at + 4 + j) < lim) v1 |= (uint32_t)at[4 + j] << (8 * j);
                     }
                     SKW(g) = pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
                 }

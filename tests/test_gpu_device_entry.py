@@ -79,3 +79,26 @@ def test_device_entry_matches_batch_and_oracle(setup):
     _, _, ores = P.oracle_map(segs, reads, K, H_FRAC, **PRM)
     bad = P.diff_results(ores, got)
     assert not bad, bad
+
+
+def test_device_entry_reads_past_2gib(setup):
+    """Reads whose bases start beyond 2 GiB into the device buffer, at an
+    unaligned byte: every offset the sketch forms is then >= 2^31 (a 32-bit
+    sign extension of one would send its loads astray)."""
+    segs, bases, off, idx, mp = setup
+    n = len(off) - 1
+    want = mp.map_batch(P.split_reads(bases, off))
+    start = (1 << 31) + 12_345
+    dev = _Dev()
+    try:
+        p = C.c_void_p()
+        assert dev.L.hipMalloc(C.byref(p), start + bases.nbytes + 64) == 0
+        dev.ptrs.append(p)
+        assert dev.L.hipMemcpy(C.c_void_p(p.value + start), bases.ctypes.data, bases.nbytes, 1) == 0
+        d_off = dev.upload((off + start).astype(np.int64))
+        got = mp.map_device(p.value, d_off, n).copy()
+    finally:
+        dev.free()
+    bad = P.diff_results(want, got)
+    assert not bad, bad
+    assert want["mapped"].sum() > 0
