@@ -255,11 +255,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                         d[i][2] = *(const uint32_t *)(s8 + bo + 8);
                     }
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const uint32_t v = stage_codes8(__builtin_amdgcn_alignbit(d[i][1], d[i][0], sh),
-                                                        __builtin_amdgcn_alignbit(d[i][2], d[i][1], sh));
-                        if (g0 + i * WAVE < ng) b32[q + i * QS] = v;
-                    }
+                    for (int i = 0; i < 4; i++)
+                        if (g0 + i * WAVE < ng)  // false for the whole wave in a chunk's last round
+                            b32[q + i * QS] = stage_codes8(__builtin_amdgcn_alignbit(d[i][1], d[i][0], sh),
+                                                           __builtin_amdgcn_alignbit(d[i][2], d[i][1], sh));
                 }
             } else {
                 for (uint32_t g = lane; g < ng; g += WAVE) {
@@ -299,7 +298,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 #pragma unroll
                     for (int t = 0; t < 4; t++) {
                         if (p + t < npair) {
-                            const uint4 tt = ((const uint4 *)T2)[__builtin_amdgcn_perm(0u, e, 0x0c0c0c00u | (uint32_t)t)];
+                            const uint4 tt = *(const uint4 *)((const uint8_t *)T2 + byte_x16(e, t));
                             const uint32_t nfh = __builtin_amdgcn_alignbit(fh, fl, 30) ^ tt.y;  // rotl2
                             const uint32_t nfl = __builtin_amdgcn_alignbit(fl, fh, 30) ^ tt.x;
                             const uint32_t nrh = __builtin_amdgcn_alignbit(rl, rh, 2) ^ tt.w;   // rotr2
@@ -347,16 +346,21 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 // Kept windows are rare, so the steps only test the high word of
                 // the hash against the threshold's, and the exact test and the
                 // writes come after.
-                const int64_t nwin_l = lw1 - lw0;
+                const int32_t nwin_l = (int32_t)(lw1 - lw0);
                 const uint32_t ish = (uint32_t)((k - 1) & 7) * 4;
-                const uint32_t *obase = &SKW((uint32_t)lw0 >> 3);
+                // The lane's first code dword q0 = lw0 / 8 is a multiple of 16,
+                // so SKW(q0 + m) = lb[m + m / 16] with lb = &SKW(q0): a per-lane
+                // base plus an offset that is the same for every lane.
+                static_assert(LANE_W % 128 == 0, "a lane's code dwords must start on a skew step");
+                const uint32_t *lb = &SKW((uint32_t)lw0 >> 3);
                 const uint4 *T4 = (const uint4 *)T;  // (fw lo, fw hi, rc lo, rc hi)
-                int64_t j = 0;
+                int32_t j = 0;
                 for (; j + 8 <= nwin_l; j += 8) {
-                    const uint32_t oc = obase[j >> 3], op = j ? obase[(j >> 3) - 1] : 0x44444444u;
+                    // outgoing dwords: m = j / 8 < LANE_W / 8 <= 16, no skew step
+                    const uint32_t oc = lb[j >> 3], op = j ? lb[(j >> 3) - 1] : 0x44444444u;
                     const uint32_t od = __builtin_amdgcn_alignbit(oc, op, 28);  // bases lw0+j-1 .. lw0+j+6
-                    const uint32_t iq = (uint32_t)(lw0 + j + k - 1) >> 3;
-                    const uint32_t i0 = SKW(iq), i1 = SKW(iq + 1);
+                    const uint32_t mi = (uint32_t)(j + k - 1) >> 3;
+                    const uint32_t i0 = lb[mi + (mi >> 4)], i1 = lb[mi + 1 + ((mi + 1) >> 4)];
                     const uint32_t id = ish ? __builtin_amdgcn_alignbit(i1, i0, ish) : i0;
                     const uint32_t pe = ((od & 0x0f0f0f0fu) << 3) | (id & 0x0f0f0f0fu);            // windows 0, 2, 4, 6
                     const uint32_t po = (((od >> 4) & 0x0f0f0f0fu) << 3) | ((id >> 4) & 0x0f0f0f0fu); // windows 1, 3, 5, 7
