@@ -83,20 +83,22 @@ __device__ __forceinline__ uint32_t upper_bound_u64(const uint64_t *a, uint32_t 
     return lo;
 }
 
-__global__ void __launch_bounds__(WG) k_chunk_counts(SeqSet S, int32_t k, uint64_t *nch) {
+__global__ void __launch_bounds__(WG) k_unit_counts(SeqSet S, int32_t k, uint64_t *nunit) {
+    const uint32_t halo = sketch_halo(k);
     for (int32_t i = blockIdx.x * WG + threadIdx.x; i < S.n; i += gridDim.x * WG) {
         const int64_t len = S.off[i + 1] - S.off[i];
         const int64_t nw = (len >= k && len >= 1) ? len - k + 1 : 0;
-        nch[i] = (uint64_t)((nw + CHUNK_W - 1) / CHUNK_W);
+        nunit[i] = nw > 0 ? (uint64_t)((nw + LANE_W - 1) / LANE_W) + halo : 0;
     }
 }
 
-// Sequence of every chunk, looked up once per chunk in parallel so the sketch
-// waves start without a dependent search.
-__global__ void __launch_bounds__(WG) k_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq) {
-    const uint64_t total = chunk_start[S.n];
-    for (uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x; c < total; c += (uint64_t)gridDim.x * WG)
-        chunk_seq[c] = upper_bound_u64(chunk_start, (uint32_t)S.n + 1, c) - 1;
+// Sequence of every unit: one wave per sequence writes its run of unit_seq.
+__global__ void __launch_bounds__(WG) k_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int64_t s = ((int64_t)blockIdx.x * WG + threadIdx.x) / WAVE; s < n; s += (int64_t)gridDim.x * NWAVES) {
+        const uint64_t a = unit_start[s], e = unit_start[s + 1];
+        for (uint64_t u = a + (uint64_t)lane; u < e; u += WAVE) unit_seq[u] = (uint32_t)s;
+    }
 }
 
 // Codes of four packed bytes: ((c >> 1) ^ (c >> 2)) & 3 sends a/c/g/t in
@@ -164,16 +166,21 @@ __device__ __forceinline__ uint32_t byte_x16(uint32_t v, int b) {
     }
 }
 
-// One wave per chunk of CHUNK_W windows.  The chunk's bases plus the k-1 halo
-// are staged in LDS as 4-bit codes, base p at nibble p & 7 of dword p >> 3, with
-// one dword of bank skew per 16 dwords (lanes rolling runs 128 windows = 16
-// dwords apart hit different banks).  The
+// One wave per chunk of WAVE units, one unit per lane: unit u of sequence s is
+// its windows [LANE_W * (u - unit_start[s]), + LANE_W), and every sequence ends
+// with sketch_halo(k) halo units that hold bases only, so the lanes of a wave
+// may roll windows of several sequences.  The chunk's WAVE + halo unit slots
+// (LANE_W bases each, from the unit's first base on) are staged in LDS as 4-bit
+// codes, base p of slot q at nibble p & 7 of code dword 16q + p / 8, with one
+// dword of bank skew per 16 dwords (lanes 16 dwords apart hit different
+// banks); a lane reads its own slot and the next halo slots, which always hold
+// the continuation of its sequence.  The
 // rolling update (src/sketch.rs:90-97, 202-203) xors, per strand, the outgoing
 // base's rotated seed and the incoming base's; both come from one 64-entry LDS
 // table indexed by (outgoing code << 3 | incoming code), entry
 // (rotl(fw_out, k) ^ fw_in, rotr(rc_out, 1) ^ rotl(rc_in, k-1)), one 16-byte
 // load per window (codes 4-7 are non-ACGT bytes, seed 0).  Each lane rolls
-// LANE_W consecutive windows; its first window is rolled in from a zero hash
+// up to LANE_W consecutive windows; its first window is rolled in from a zero hash
 // two bases per step (a second 64-entry table of base pairs) and, for odd k,
 // one more through the outgoing-code-4 row, which equals the direct definition
 // of src/sketch.rs:180-184.
@@ -186,8 +193,9 @@ __device__ __forceinline__ uint32_t byte_x16(uint32_t v, int b) {
 // positions: the chunk's scanned offset plus the lane's exclusive prefix.
 template <int PASS>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_sketch(SeqSet S, int32_t k, uint64_t thres,
-                                               const uint64_t *chunk_start, const uint32_t *chunk_seq,
-                                               uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
+                                               const uint64_t *unit_start, const uint32_t *unit_seq,
+                                               uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
+                                               uint32_t *lane_pre, uint4 *stage,
                                                uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                                                uint32_t *ovf_list, uint32_t *ovf_count) {
     extern __shared__ uint8_t lds[];
@@ -206,77 +214,102 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
     }
     __syncthreads();
     const uint32_t thres_hi = (uint32_t)(thres >> 32);
-    const uint64_t total = PASS == 0 ? chunk_start[S.n] : (uint64_t)*ovf_count;
+    const uint32_t halo = sketch_halo(k), nslot = (uint32_t)WAVE + halo;
+    uint64_t *sbase = (uint64_t *)(buf + ((sketch_codes_bytes(k) + 7u) & ~7u));  // slot q's first base, or ~0
+    const uint64_t total = PASS == 0 ? (n_units + WAVE - 1) / WAVE : (uint64_t)*ovf_count;
     uint32_t *b32 = (uint32_t *)buf;
 #define SKW(q) b32[(uint32_t)(q) + ((uint32_t)(q) >> 4)]
 #define SKN(p) ((SKW((uint32_t)(p) >> 3) >> (((uint32_t)(p) & 7u) * 4u)) & 0xfu)
+    static_assert(LANE_W == 128, "a unit slot is 16 code dwords, one skew step");
     // Each wave stages and reads only its own slice, so the waves of a
     // workgroup run independently (wave-level ordering only).
     for (uint64_t ci = (uint64_t)blockIdx.x * NWAVES + w; ci < total; ci += (uint64_t)gridDim.x * NWAVES) {
         const uint64_t c = PASS == 0 ? ci : (uint64_t)ovf_list[ci];
-        int64_t w0 = 0, w1 = 0;
+        int64_t ws = 0;    // the lane's first window in its sequence
+        int32_t nwl = 0;   // its window count (0 for a halo unit or past the last unit)
         {
-            const uint32_t seq = chunk_seq[c];
-            const int64_t len = S.off[seq + 1] - S.off[seq];
-            const int64_t nw = len - k + 1;
-            w0 = (int64_t)(c - chunk_start[seq]) * CHUNK_W;
-            w1 = w0 + CHUNK_W < nw ? w0 + CHUNK_W : nw;
-            const int64_t nb = w1 - w0 + k - 1;
-            // Pointer arithmetic (not integer casts) keeps the loads global,
-            // so they do not wait on the LDS writes; the aligned start is
-            // wave-uniform (a scalar base plus 32-bit lane offsets).
-            const uint8_t *srcp = S.bases + S.off[seq] + w0;
-            const uint32_t al = (uint32_t)((uintptr_t)srcp & 3u);
-            const uint32_t sh = al * 8;  // alignbit by 0 returns the low dword
-            // (readfirstlane returns a signed int: each half goes through
-            // uint32_t so the low half is not sign-extended past 2 GiB)
-            const uint64_t a_off = (uint64_t)(S.off[seq] + w0 - al);
-            const uint64_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a_off >> 32));
-            const uint64_t a_lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a_off);
-            const uint32_t *s32 = (const uint32_t *)(S.bases + ((a_hi << 32) | a_lo));
-            const uintptr_t lim = (uintptr_t)(S.bases + S.off[S.n]);  // never read past the set
-            const uint32_t ng = (uint32_t)((nb + 7) >> 3);  // code dwords, 8 bases each
-            if ((uintptr_t)s32 + 4 * (uintptr_t)(2 * ng + 1) <= lim) {
-                // every aligned dword this chunk needs lies inside the set.
-                // Four groups per lane at a time: their loads first (a group
-                // past the chunk's last reloads the last, whose codes are not
-                // stored), so four are in flight; q is SKW's skewed index of g
-                // (g steps by WAVE, so q steps by WAVE + WAVE / 16).
-                constexpr uint32_t QS = WAVE + WAVE / 16;
-                const uint8_t *s8 = (const uint8_t *)s32;
-                uint32_t q = (uint32_t)lane + ((uint32_t)lane >> 4);
-                for (uint32_t g0 = lane; g0 < ng; g0 += 4 * WAVE, q += 4 * QS) {
-                    uint32_t d[4][3];
+            // slot table: the first base of each slot's unit, as an offset into
+            // S.bases; bit 63 marks a slot to stage byte by byte: one past the
+            // last unit (all ~0) or one whose aligned dwords may reach past
+            // the end of the set
+            const uint64_t set_end = (uint64_t)S.off[S.n];
+            auto slot_entry = [&](uint64_t u) -> uint64_t {
+                if (u >= n_units) return ~0ull;
+                const uint32_t sq = unit_seq[u];
+                const uint64_t b = (uint64_t)(S.off[sq] + (int64_t)(u - unit_start[sq]) * LANE_W);
+                return (b & ~3ull) + 4u * (2u * (LANE_W / 8) + 1u) <= set_end ? b : b | (1ull << 63);
+            };
+            const uint64_t u = c * WAVE + (uint64_t)lane;
+            if (u < n_units) {
+                const uint32_t sq = unit_seq[u];
+                const int64_t nw = S.off[sq + 1] - S.off[sq] - k + 1;
+                ws = (int64_t)(u - unit_start[sq]) * LANE_W;
+                const int64_t left = nw - ws;
+                nwl = left <= 0 ? 0 : (left < LANE_W ? (int32_t)left : LANE_W);
+            }
+            sbase[lane] = slot_entry(u);
+            if ((uint32_t)lane < halo) sbase[WAVE + lane] = slot_entry(c * WAVE + WAVE + (uint64_t)lane);
+        }
+        wave_sync();
+        {
+            // Lane l stages code dword l % 16 (eight bases from the slot's first
+            // base plus 8 * (l % 16)) of slots l / 16, l / 16 + 4, ...  Only the
+            // last chunks of a set have marked slots; every other chunk takes
+            // the fast path: the WAVE main slots in four rounds of four, each
+            // round's loads first so four are in flight, then the halo slots.
+            // Pointer arithmetic (not integer casts) keeps the loads global, so
+            // they do not wait on the LDS writes.
+            const uint32_t qi = (uint32_t)lane >> 4, di = (uint32_t)lane & 15u;
+            const uint8_t *pl = S.bases + 8u * di;
+            const bool mk = (sbase[lane] >> 63) || ((uint32_t)lane < halo && (sbase[WAVE + lane] >> 63));
+            if (__ballot(mk) == 0) {
+                auto code8 = [&](const uint32_t (&d)[3], uint32_t sh) {
+                    return stage_codes8(__builtin_amdgcn_alignbit(d[1], d[0], sh),
+                                        __builtin_amdgcn_alignbit(d[2], d[1], sh));
+                };
+#pragma unroll 1
+                for (uint32_t q0 = 0; q0 < (uint32_t)WAVE; q0 += 16) {
+                    uint32_t d[4][3], sh[4];
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        const uint32_t bo = 8u * min(g0 + i * WAVE, ng - 1);  // byte offset of the group
-                        d[i][0] = *(const uint32_t *)(s8 + bo);
-                        d[i][1] = *(const uint32_t *)(s8 + bo + 4);
-                        d[i][2] = *(const uint32_t *)(s8 + bo + 8);
+                        const uint64_t bq = sbase[q0 + 4 * i + qi];
+                        const uint32_t *a32 = (const uint32_t *)(pl + (bq & ~3ull));
+                        d[i][0] = a32[0];
+                        d[i][1] = a32[1];
+                        d[i][2] = a32[2];
+                        sh[i] = ((uint32_t)bq & 3u) * 8u;  // alignbit by 0 returns the low dword
                     }
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if (g0 + i * WAVE < ng)  // false for the whole wave in a chunk's last round
-                            b32[q + i * QS] = stage_codes8(__builtin_amdgcn_alignbit(d[i][1], d[i][0], sh),
-                                                           __builtin_amdgcn_alignbit(d[i][2], d[i][1], sh));
+                    for (int i = 0; i < 4; i++) b32[(q0 + 4 * i + qi) * 17u + di] = code8(d[i], sh[i]);
+                }
+                for (uint32_t q = WAVE + qi; q < nslot; q += 4) {
+                    const uint64_t bq = sbase[q];
+                    const uint32_t *a32 = (const uint32_t *)(pl + (bq & ~3ull));
+                    const uint32_t d[3] = {a32[0], a32[1], a32[2]};
+                    b32[q * 17u + di] = code8(d, ((uint32_t)bq & 3u) * 8u);
                 }
             } else {
-                for (uint32_t g = lane; g < ng; g += WAVE) {
-                    const uint8_t *at = srcp + 8 * (size_t)g;  // first byte of this group
+                // byte by byte; bytes past the set read as 0 (no window uses
+                // them), and a slot past the last unit is never read (a unit's
+                // halo slots belong to its own sequence)
+                const uintptr_t lim = (uintptr_t)(S.bases + S.off[S.n]);
+                for (uint32_t q = qi; q < nslot; q += 4) {
+                    const uint64_t bq = sbase[q];
+                    if (bq == ~0ull) continue;
+                    const uint8_t *p = pl + (bq & ~(1ull << 63));
                     uint32_t v0 = 0, v1 = 0;
                     for (uint32_t j = 0; j < 4; j++) {
-                        if ((uintptr_t)(at + j) < lim) v0 |= (uint32_t)at[j] << (8 * j);
-                        if ((uintptr_t)(at + 4 + j) < lim) v1 |= (uint32_t)at[4 + j] << (8 * j);
+                        if ((uintptr_t)(p + j) < lim) v0 |= (uint32_t)p[j] << (8 * j);
+                        if ((uintptr_t)(p + 4 + j) < lim) v1 |= (uint32_t)p[4 + j] << (8 * j);
                     }
-                    SKW(g) = pack_nibbles4(base_codes4(v0)) | (pack_nibbles4(base_codes4(v1)) << 16);
+                    b32[q * 17u + di] = stage_codes8(v0, v1);
                 }
             }
         }
         wave_sync();
         {
-            const int64_t nwin = w1 - w0;
-            const int64_t lw0 = (int64_t)lane * LANE_W;
-            const int64_t lw1 = lw0 + LANE_W < nwin ? lw0 + LANE_W : nwin;
+            const int64_t lw0 = (int64_t)lane * LANE_W;  // the lane's slot, in LDS base positions
+            const int64_t lw1 = lw0 + nwl;
             uint32_t cnt = 0;                        // PASS 1: k-mers written so far
             uint32_t so = (uint32_t)lane * 16u;      // PASS 0: byte offset of the next staging slot
             const uint32_t so_lim = lcap * (WAVE * 16u);
@@ -321,7 +354,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                 // lane are WAVE * 16 bytes apart), so one add and one store;
                 // the offset keeps counting past lcap slots without storing.
                 uint4 *st_c = stage + (PASS == 0 ? (uint64_t)c * lcap * WAVE : 0);
-                const int32_t r0 = (int32_t)(w0 + k - 1);  // window wi ends at base w0 + wi + k - 1
+                // window lw0 + j of the slot is window ws + j of the sequence,
+                // which ends at base ws + j + k - 1
+                const int32_t r0 = (int32_t)(ws - lw0 + k - 1);
                 auto put = [&](int64_t wi, uint64_t h, bool strand) {
                     if (PASS == 1) {
                         const uint64_t o = base_out + cnt;
@@ -407,6 +442,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
             if (PASS == 0) {
                 cnt = (so - (uint32_t)lane * 16u) / (WAVE * 16u);
                 lane_cnt[c * WAVE + lane] = cnt;
+                lane_pre[c * WAVE + lane] = wave_incl_scan(cnt) - cnt;
                 uint32_t s2 = cnt;
                 for (int d = WAVE / 2; d > 0; d >>= 1) s2 += __shfl_xor(s2, d);
                 if (lane == 0) chunk_val[c] = s2;
@@ -456,18 +492,25 @@ __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const 
     }
 }
 
-// Per-sequence k-mer offsets from the chunk scan: koff[i] = chunk_koff[chunk_start[i]],
-// and the largest per-sequence sketch into *max_m (zeroed by the caller).
-__global__ void __launch_bounds__(WG) k_seq_koff(int32_t n, const uint64_t *chunk_start,
-                                                 const uint64_t *chunk_koff, uint64_t *koff,
-                                                 unsigned long long *max_m) {
+// Per-sequence k-mer offsets: sequence i's first unit u = unit_start[i] is lane
+// u % WAVE of chunk u / WAVE, so koff[i] = chunk_koff[u / WAVE] plus that lane's
+// exclusive prefix of kept counts in its chunk (lane_pre; none for lane 0, which
+// also covers u = n_units at a chunk boundary).  The largest per-sequence
+// sketch goes into *max_m (zeroed by the caller).
+__device__ __forceinline__ uint64_t seq_koff_at(uint64_t u, const uint64_t *chunk_koff, const uint32_t *lane_pre) {
+    return chunk_koff[u / WAVE] + ((u & (WAVE - 1)) ? lane_pre[u] : 0u);
+}
+
+__global__ void __launch_bounds__(WG) k_seq_koff(int32_t n, const uint64_t *unit_start,
+                                                 const uint64_t *chunk_koff, const uint32_t *lane_pre,
+                                                 uint64_t *koff, unsigned long long *max_m) {
     for (int32_t i0 = blockIdx.x * WG; i0 <= n; i0 += gridDim.x * WG) {
         const int32_t i = i0 + (int32_t)threadIdx.x;
         uint64_t m = 0;
         if (i <= n) {
-            const uint64_t a = chunk_koff[chunk_start[i]];
+            const uint64_t a = seq_koff_at(unit_start[i], chunk_koff, lane_pre);
             koff[i] = a;
-            if (i < n) m = chunk_koff[chunk_start[i + 1]] - a;
+            if (i < n) m = seq_koff_at(unit_start[i + 1], chunk_koff, lane_pre) - a;
         }
         for (int d = WAVE / 2; d > 0; d >>= 1) {
             const uint64_t o = __shfl_xor(m, d);
@@ -2534,8 +2577,8 @@ void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint6
     k_scan_down<<<G, WG, 0, st>>>(in, out, n_dev, n_hint, part);
 }
 
-void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st) {
-    k_chunk_counts<<<flat_grid((uint64_t)S.n, 1024), WG, 0, st>>>(S, k, nch);
+void launch_unit_counts(SeqSet S, int32_t k, uint64_t *nunit, hipStream_t st) {
+    k_unit_counts<<<flat_grid((uint64_t)S.n, 1024), WG, 0, st>>>(S, k, nunit);
 }
 
 size_t sketch_lds(int32_t k) { return (size_t)NWAVES * sketch_span(k); }
@@ -2547,25 +2590,24 @@ uint32_t sketch_lane_cap(uint64_t thres) {
     return cap >= (double)LANE_W ? (uint32_t)LANE_W : (uint32_t)cap;
 }
 
-void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
-                      hipStream_t st) {
-    k_chunk_seq<<<flat_grid(n_chunks, 8192), WG, 0, st>>>(S, chunk_start, chunk_seq);
+void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st) {
+    if (n > 0) k_unit_seq<<<flat_grid((uint64_t)n * WAVE, 4096), WG, 0, st>>>(n, unit_start, unit_seq);
 }
 
-void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
-                   uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list,
-                   uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
+                   const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
+                   uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
+                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
     const size_t lds = sketch_lds(k);
     const uint64_t want = (n_chunks + NWAVES - 1) / NWAVES;
     if (pass == 0)
         k_sketch<0><<<persistent_grid((const void *)k_sketch<0>, lds, want, max_grid), WG, lds, st>>>(
-            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, stage, out_h, out_r, out_s, lcap, ovf_list,
-            ovf_count);
+            S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
+            lcap, ovf_list, ovf_count);
     else
         k_sketch<1><<<persistent_grid((const void *)k_sketch<1>, lds, want, max_grid), WG, lds, st>>>(
-            S, k, thres, chunk_start, chunk_seq, chunk_val, lane_cnt, stage, out_h, out_r, out_s, lcap, ovf_list,
-            ovf_count);
+            S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
+            lcap, ovf_list, ovf_count);
 }
 
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
@@ -2576,9 +2618,9 @@ void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const ui
     k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s);
 }
 
-void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
-                     uint64_t *max_m, hipStream_t st) {
-    k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, chunk_start, chunk_koff, koff,
+void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,
+                     uint64_t *koff, uint64_t *max_m, hipStream_t st) {
+    k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, unit_start, chunk_koff, lane_pre, koff,
                                                                 (unsigned long long *)max_m);
 }
 

@@ -10,7 +10,6 @@
 namespace shm {
 
 constexpr int LANE_W = 128;               // consecutive windows rolled by one lane
-constexpr int CHUNK_W = 64 * LANE_W;      // windows per wave-chunk when sketching
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
@@ -25,12 +24,20 @@ constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one w
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
 
-// LDS bytes staged per wave: the chunk's CHUNK_W + k - 1 bases as 4-bit codes,
-// eight per dword, with two dwords of read-ahead, plus one dword of bank skew
-// per 16 dwords, rounded up to 16.
+// Halo units per sequence: the lane rolling a unit's LANE_W windows also reads
+// the bases of the next ceil((k + 7) / LANE_W) units of its sequence (the k - 1
+// bases of its last window plus a dword of read-ahead), so every sequence with
+// windows gets that many extra units whose lanes only hold bases.
+__host__ __device__ inline uint32_t sketch_halo(int32_t k) {
+    return (uint32_t)((k + 7 + LANE_W - 1) / LANE_W);
+}
+// LDS bytes per wave: WAVE + halo unit slots of LANE_W / 8 code dwords, with one
+// dword of bank skew per 16, then one 8-byte base offset per slot.
+__host__ __device__ inline uint32_t sketch_codes_bytes(int32_t k) {
+    return 4u * ((uint32_t)WAVE + sketch_halo(k)) * (uint32_t)(LANE_W / 8 + LANE_W / 128);
+}
 __host__ __device__ inline uint32_t sketch_span(int32_t k) {
-    const uint32_t dw = (uint32_t)(CHUNK_W + k) / 8u + 2u;
-    return (4u * (dw + dw / 16u + 1u) + 15u) & ~15u;
+    return (((sketch_codes_bytes(k) + 7u) & ~7u) + 8u * ((uint32_t)WAVE + sketch_halo(k)) + 15u) & ~15u;
 }
 
 // A set of sequences: sequence i is bases[off[i], off[i+1]); bytes up to
@@ -155,25 +162,28 @@ enum Stage {
 // host launch wrappers (shm_kernels.hip)
 void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint, uint64_t *part,
                  hipStream_t st);
-void launch_chunk_counts(SeqSet S, int32_t k, uint64_t *nch, hipStream_t st);
+// Units of LANE_W windows per sequence (plus its halo units; none without windows).
+void launch_unit_counts(SeqSet S, int32_t k, uint64_t *nunit, hipStream_t st);
 size_t sketch_lds(int32_t k);
-void launch_chunk_seq(SeqSet S, const uint64_t *chunk_start, uint32_t *chunk_seq, uint64_t n_chunks,
-                      hipStream_t st);
-// pass 0: roll every chunk into lane-strided staging (lcap 16-byte records per lane),
-// listing chunks with a lane over lcap; pass 1: re-roll the listed chunks
-// straight to their final positions (chunk_val = scanned chunk offsets).
-void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *chunk_start,
-                   const uint32_t *chunk_seq, uint64_t *chunk_val, uint32_t *lane_cnt, uint4 *stage,
-                   uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap, uint32_t *ovf_list,
-                   uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+// unit_seq[u] = the sequence of unit u, from the scanned unit counts.
+void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st);
+// A chunk is WAVE consecutive units (one per lane).  pass 0: roll every chunk
+// into lane-strided staging (lcap 16-byte records per lane), listing chunks with
+// a lane over lcap, each lane's count and its exclusive prefix in the chunk;
+// pass 1: re-roll the listed chunks straight to their final positions
+// (chunk_val = scanned chunk offsets).
+void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
+                   const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
+                   uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
+                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
                            uint32_t max_grid, hipStream_t st);
 // Staging slots per lane for a keep threshold: the expected kept count of
 // LANE_W windows plus five standard deviations and a margin, at most LANE_W.
 uint32_t sketch_lane_cap(uint64_t thres);
-void launch_seq_koff(int32_t n, const uint64_t *chunk_start, const uint64_t *chunk_koff, uint64_t *koff,
-                     uint64_t *max_m, hipStream_t st);
+void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,
+                     uint64_t *koff, uint64_t *max_m, hipStream_t st);
 void launch_max_i32(const int32_t *a, int32_t n, uint64_t *out, hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);
 void launch_radix_pass(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t n,

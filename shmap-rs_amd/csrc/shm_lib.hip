@@ -179,8 +179,8 @@ extern "C" void shm_sketcher_free(shm_sketcher *sk) { delete sk; }
 // Sketches a set of sequences already on the device: chunk table on the host,
 // count pass, scan, write pass.  koff (n+1) receives per-sequence offsets.
 struct SketchOut {
-    DevBuf<uint64_t> nch, chunk_start, chunk_cnt, chunk_koff, koff, part, stat;  // stat: [0] largest sketch
-    DevBuf<uint32_t> lane_cnt, chunk_seq, ovf, ovf_n;  // overflow chunk list and its count
+    DevBuf<uint64_t> nunit, unit_start, chunk_cnt, chunk_koff, koff, part, stat;  // stat: [0] largest sketch
+    DevBuf<uint32_t> lane_cnt, lane_pre, unit_seq, ovf, ovf_n;  // overflow chunk list and its count
     DevBuf<uint4> stage;                        // lane-strided staging records of the rolling pass
     std::vector<uint64_t> h_koff;  // per-sequence offsets, when asked for
     PinnedBuf<uint64_t> h_sum;     // total, largest sketch, overflow chunks
@@ -192,25 +192,27 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
                          SketchOut &so, DevBuf<uint64_t> &out_h, DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s,
                          uint32_t max_grid, hipStream_t st, hipEvent_t *ev_count, hipEvent_t *ev_write,
                          bool full_koff) {
-    // chunk table: chunks per sequence, scanned on the device
+    // unit table: units of LANE_W windows per sequence, scanned on the device;
+    // a chunk is WAVE consecutive units
     SeqSet S{d_bases, d_off, n};
-    RCHK(so.chunk_start.ensure(n + 1));
-    RCHK(so.nch.ensure(n + 1));
+    RCHK(so.unit_start.ensure(n + 1));
+    RCHK(so.nunit.ensure(n + 1));
     RCHK(so.koff.ensure(n + 1));
     RCHK(so.part.ensure(SCAN_PARTS + 1));
     RCHK(so.h_sum.ensure(3));
     RCHK(so.ovf_n.ensure(1));
-    launch_chunk_counts(S, sk->k, so.nch.p, st);
-    launch_scan(so.nch.p, so.chunk_start.p, nullptr, n, so.part.p, st);
-    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.chunk_start.p + n, 8, hipMemcpyDeviceToHost, st));
+    launch_unit_counts(S, sk->k, so.nunit.p, st);
+    launch_scan(so.nunit.p, so.unit_start.p, nullptr, n, so.part.p, st);
+    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.unit_start.p + n, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemsetAsync(so.ovf_n.p, 0, 4, st));  // cleared while the host waits
     HIPCHK(hipStreamSynchronize(st));
-    const uint64_t C = so.h_sum.p[0];
+    const uint64_t U = so.h_sum.p[0], C = (U + WAVE - 1) / WAVE;
     RCHK(so.chunk_cnt.ensure(C + 1));
     RCHK(so.chunk_koff.ensure(C + 1));
     RCHK(so.lane_cnt.ensure(C * WAVE + 1));
-    RCHK(so.chunk_seq.ensure(C + 1));
-    if (C > 0) launch_chunk_seq(S, so.chunk_start.p, so.chunk_seq.p, C, st);
+    RCHK(so.lane_pre.ensure(C * WAVE + 1));
+    RCHK(so.unit_seq.ensure(U + 1));
+    if (C > 0) launch_unit_seq(n, so.unit_start.p, so.unit_seq.p, st);
     const uint32_t lcap = sketch_lane_cap(sk->thres);
     const uint64_t slots = C * WAVE * (uint64_t)lcap;
     RCHK(so.stage.ensure(slots + 1));
@@ -218,13 +220,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     uint32_t *ovf_count = so.ovf_n.p, *ovf_list = so.ovf.p;
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
-        launch_sketch(0, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_cnt.p, so.lane_cnt.p,
-                      so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st);
+        launch_sketch(0, S, sk->k, sk->thres, so.unit_start.p, so.unit_seq.p, U, so.chunk_cnt.p, so.lane_cnt.p,
+                      so.lane_pre.p, so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
     RCHK(so.stat.ensure(1));
     HIPCHK(hipMemsetAsync(so.stat.p, 0, 8, st));
-    launch_seq_koff(n, so.chunk_start.p, so.chunk_koff.p, so.koff.p, so.stat.p, st);
+    launch_seq_koff(n, so.unit_start.p, so.chunk_koff.p, so.lane_pre.p, so.koff.p, so.stat.p, st);
     if (full_koff) {
         so.h_koff.resize(n + 1);
         HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
@@ -247,8 +249,8 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
         launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.stage.p, out_h.p,
                               out_r.p, out_s.p, max_grid, st);
     if (so.n_overflow > 0)
-        launch_sketch(1, S, sk->k, sk->thres, so.chunk_start.p, so.chunk_seq.p, so.chunk_koff.p, so.lane_cnt.p,
-                      nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
+        launch_sketch(1, S, sk->k, sk->thres, so.unit_start.p, so.unit_seq.p, U, so.chunk_koff.p, so.lane_cnt.p,
+                      so.lane_pre.p, nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
