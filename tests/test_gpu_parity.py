@@ -89,6 +89,30 @@ def test_index_counts_and_hits_match_oracle(max_matches):
     assert [s[2] for s in g.segments()] == [len(O.sketch(k, hf, s)) for _, s in segs]
 
 
+@pytest.mark.parametrize("k,h_frac", [(25, 1.0), (25, 0.3), (130, 1.0), (130, 0.3)])
+def test_many_short_segments_sketch_matches_oracle(k, h_frac):
+    """One batch sketch over ~100 segments whose lengths sit at the unit edges
+    (under k, k, k + 127, k + 128, ... windows in units of 128), so each wave
+    packs lanes of several segments, each segment ends with its halo units
+    (one for k = 25, two for k = 130), and the last one ends at the end of the
+    set.  At h_frac 1.0 every lane keeps all of its windows, so the re-roll
+    pass writes them.  Every occurrence of every hash must match the oracle."""
+    edge = [0, 1, k - 1, k, k + 1, k + 126, k + 127, k + 128, k + 129, k + 255, k + 256, k + 300]
+    r = np.random.default_rng(77)
+    lens = edge * 6 + [int(x) for x in r.integers(1, 700, 30)] + [9000, k + 127]
+    segs = [(f"s{i}", _rng_seq(1000 + i, n, b"ACGTacgtN")) for i, n in enumerate(lens)]
+    g = G.SketchIndex(G.FracMinHash(k, h_frac), segs, None)
+    o = O.OracleIndex(k, h_frac, segs, None)
+    assert [s[2] for s in g.segments()] == [len(O.sketch(k, h_frac, s)) for _, s in segs]
+    hashes = set()
+    for _, s in segs:
+        hashes.update(int(h) for h in O.sketch(k, h_frac, s)["h"])
+    assert len(hashes) > 1000
+    for h in sorted(hashes):
+        assert g.count(h) == o.count(h), h
+        assert g.hits(h) == o.hits(h), h
+
+
 def _workload(seed, ref_len, n_reads, read_len, err, repeat=0.0, lognormal=False, sd=0.0, n_contigs=3):
     lens = [ref_len // n_contigs] * n_contigs
     ref = G.synth_reference(seed, lens, repeat_frac=repeat, unit_len=171, div_min=0.01, div_max=0.03,
