@@ -232,6 +232,12 @@ def test_empty_and_degenerate_batches():
     res = mp.map_batch([b"", b"ACGT", b"N" * 5000, reads[0]])
     assert list(res["mapped"][:3]) == [0, 0, 0]
     assert res["unmappable_short"][0] == 1
+    # every read shorter than k: the sketch has no units at all
+    res = mp.map_batch([b"", b"ACGT", b"A" * 14])
+    assert list(res["mapped"]) == [0, 0, 0] and list(res["m"]) == [0, 0, 0]
+    # and the mapper still works afterwards
+    res = mp.map_batch([reads[0]])
+    assert res["m"][0] > 0
 
 
 def test_table_budget_groups_give_the_same_results():
