@@ -62,55 +62,17 @@ def log(*a):
 
 
 # ---------------------------------------------------------------------------
-# workloads (synthetic and seeded)
+# workloads (shmap-rs_amd/workloads.py: the five configurations, seeded)
 # ---------------------------------------------------------------------------
 
-def genome_contigs(total=3_100_000_000, n=24, lo=50_000_000, hi=250_000_000):
-    f = lambda p: np.array([lo + (hi - lo) * (i / (n - 1)) ** p for i in range(n)])
-    a, b = 0.3, 5.0
-    for _ in range(80):
-        mid = (a + b) / 2
-        if f(mid).sum() > total:
-            a = mid
-        else:
-            b = mid
-    lens = np.round(f(mid)).astype(np.int64)
-    lens[n // 2] += total - lens.sum()
-    return lens
+sys.path.insert(0, os.path.join(ROOT, "shmap-rs_amd"))
+import workloads as W  # noqa: E402
 
-
-WORKLOADS = {
-    # 10 Mbp uniform, 2,000 x 10 kb reads, 0.5% substitutions
-    "plain_small": dict(lens=[10_000_000], repeat=0.0, n_reads=2000, length=(10_000, 0), lognormal=False,
-                        clip=(1, 10_000), errors=((0.005, 1.0, 0.0, 0.0),), t=0.4),
-    # 100 Mbp, 20% tandem arrays (171 bp, 2%), 20,000 HiFi-like reads
-    "batched_uneven": dict(lens=[100_000_000], repeat=0.20, div=(0.02, 0.02), n_units=16, n_reads=20_000,
-                           length=(20_000, 8_000), lognormal=True, clip=(1_000, 80_000),
-                           errors=((0.001, 0.6, 0.2, 0.2),), t=0.4),
-    # 3.1 Gbp, 24 contigs, 8% satellite 1-3%, 92,000 ONT-like reads, 7% errors
-    "genome": dict(lens=None, repeat=0.08, div=(0.01, 0.03), n_units=64, n_reads=92_000,
-                   length=(24_000, 12_000), lognormal=True, clip=(1_000, 200_000),
-                   errors=((0.07, 0.4, 0.3, 0.3),), t=0.4),
-    # the same reference sketched at r=0.10, 125,000 HiFi-like reads (mean 23 kb, 0.1%)
-    "dense": dict(lens=None, repeat=0.08, div=(0.01, 0.03), n_units=64, n_reads=125_000,
-                  length=(23_000, 10_000), lognormal=True, clip=(1_000, 200_000),
-                  errors=((0.001, 0.6, 0.2, 0.2),), t=0.4, r=0.10),
-}
+WORKLOADS = W.WORKLOADS
 
 
 def make_workload(name, seed=1, n_reads=None):
-    w = dict(WORKLOADS[name])
-    lens = genome_contigs() if w["lens"] is None else np.asarray(w["lens"], dtype=np.int64)
-    t0 = time.time()
-    div = w.get("div", (0.02, 0.02))
-    ref = G.synth_reference(seed, lens, repeat_frac=w["repeat"], unit_len=171, div_min=div[0], div_max=div[1],
-                            n_units=w.get("n_units", 16), array_len=200_000)
-    n = n_reads or w["n_reads"]
-    bases, off, truth = G.synth_reads(seed + 1, ref, lens, n, w["length"], lognormal=w["lognormal"], clip=w["clip"],
-                                      errors=w["errors"], both_strands=True)
-    log(f"[bench] workload {name}: ref {lens.sum() / 1e9:.3f} Gbp in {len(lens)} contigs, {n} reads "
-        f"({(off[-1]) / 1e9:.3f} Gbp), generated in {time.time() - t0:.1f}s")
-    return lens, ref, bases, off, truth, w
+    return W.make(name, seed, n_reads, log=lambda m: log("[bench] " + m))
 
 
 # ---------------------------------------------------------------------------
@@ -231,8 +193,7 @@ def run_serving(idx, prm, lens, ref, args) -> dict:
     resident index; per-read latency from scheduled arrival to result."""
     import threading
     n = args.serve_reads
-    bases, off, _ = G.synth_reads(5, ref, lens, n, (15_000, 10_000), lognormal=True, clip=(1_000, 100_000),
-                                  errors=((0.001, 0.6, 0.2, 0.2), (0.07, 0.4, 0.3, 0.3)), both_strands=True)
+    bases, off, _ = W.serving_reads(lens, ref, n)
     hb = G.host_array(len(bases), np.uint8)  # page-locked: a batch of consecutive arrivals is one upload
     hb[:] = bases
     rng = np.random.default_rng(11)
@@ -320,12 +281,8 @@ def main():
 
     lens, ref, bases, off, truth, w = make_workload(args.workload, n_reads=args.reads)
     n_reads = len(off) - 1
-    k, h_frac, theta, d, o = 25, w.get("r", 0.01), w["t"], 0.075, 0.3
-    names, at = [], 0
-    segs = []
-    for i, L in enumerate(lens):
-        segs.append((f"chr{i + 1}", ref[at:at + int(L)]))
-        at += int(L)
+    k, h_frac, theta, d, o = W.PARAMS["k"], w["r"], W.PARAMS["theta"], W.PARAMS["min_diff"], W.PARAMS["max_overlap"]
+    segs = W.segments(lens, ref)
 
     hip = Hip()
     t0 = time.time()

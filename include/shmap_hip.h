@@ -100,7 +100,7 @@ typedef struct {
     int32_t intersection, intersection2;
     int32_t same_strand;
     int32_t b_segm, b_b, b2_segm, b2_b;
-    int32_t status;           /* 0 = ok; nonzero = this read failed alone */
+    int32_t status;           /* 0 = ok; nonzero (a shm_status) = this read failed (see shm_map_batch) */
     int32_t has_second;       /* a second-best mapping was found */
     double J, J2, sh, sigmas, match_inefficiency;
 } shm_read_result;
@@ -163,8 +163,23 @@ void shm_mapper_free(shm_mapper *mp);
 
 /* Mapper::map_reads for a batch of host reads -- src/shmap/mod.rs:374-650,
  * with map_read (:653-971) for each.  out[i] is read i's result, in input
- * order.  A read that fails is reported in out[i].status alone. */
+ * order.  As in the reference, where a read that panics becomes one Err while
+ * every other read is still written (src/shmap/mod.rs:219-238, 597-612), a
+ * read can fail alone: a malformed read (negative length, no bases:
+ * SHM_ERR_ARG), a read over the mapper's length limit (SHM_ERR_CAPACITY, see
+ * shm_mapper_set_read_limit), or a read that does not fit device memory even
+ * in a batch of its own (SHM_ERR_NOMEM; a batch that runs out of memory is
+ * split and mapped again).  Such a read has out[i].status != 0 and
+ * mapped = 0; every other record is complete.  The call then returns the
+ * first failed read's status, with shm_last_error() naming it.  A failure of
+ * the whole batch (SHM_ERR_HIP) sets every read's status and is returned. */
 int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out);
+
+/* Reads longer than max_read_len bases fail alone in shm_map_batch (status
+ * SHM_ERR_CAPACITY) before any device work; 0 (the default) means no limit.
+ * No reference counterpart: an admission guard for callers (the serving loop)
+ * that must keep one oversized request from taking a batch's device memory. */
+int shm_mapper_set_read_limit(shm_mapper *mp, int32_t max_read_len);
 
 /* The same over reads already resident in device memory: bases d_seq with
  * n+1 offsets d_off (device pointers), results copied to host out. */
@@ -251,6 +266,8 @@ typedef struct {
     int32_t workers;         /* concurrent batches (mappers/streams); 0 = 3 */
     int32_t max_batch_reads; /* admission budget per batch; 0 = 65536 */
     int64_t max_batch_bases; /* admission budget per batch; 0 = 2^31 */
+    int32_t max_read_len;    /* longer reads fail alone (their status); 0 = no limit */
+    int32_t pad;
 } shm_server_config;
 
 typedef struct {
@@ -260,7 +277,7 @@ typedef struct {
     double done_s;         /* its result was ready */
     int32_t batch_reads;   /* reads in the batch it was mapped with */
     int32_t worker;
-    shm_read_result r;     /* r.status != 0: the batch failed (see shm_server_error) */
+    shm_read_result r;     /* r.status != 0: this read failed (see shm_map_batch, shm_server_error) */
 } shm_served;
 
 int shm_server_create(const shm_index *idx, const shm_map_params *params, const shm_server_config *cfg,

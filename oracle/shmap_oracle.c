@@ -256,34 +256,167 @@ static void *chunk_worker(void *arg) {
     return NULL;
 }
 
-/* Stable LSD radix sort of (key, val) pairs by key. */
-static void radix_sort_kv(uint64_t *key, uint32_t *val, int64_t n) {
-    if (n <= 1) return;
-    uint64_t *k2 = xmalloc((size_t)n * 8);
-    uint32_t *v2 = xmalloc((size_t)n * 4);
-    uint64_t all_or = 0;
-    for (int64_t i = 0; i < n; i++) all_or |= key[i];
-    for (int shift = 0; shift < 64; shift += 16) {
-        if (((all_or >> shift) & 0xffffull) == 0 && (all_or >> shift) == 0) break;
-        static __thread int64_t cnt[65536];
-        memset(cnt, 0, sizeof(cnt));
-        for (int64_t i = 0; i < n; i++) cnt[(key[i] >> shift) & 0xffff]++;
-        int64_t sum = 0;
-        for (int d = 0; d < 65536; d++) {
-            const int64_t c = cnt[d];
-            cnt[d] = sum;
-            sum += c;
-        }
-        for (int64_t i = 0; i < n; i++) {
-            const int64_t o = cnt[(key[i] >> shift) & 0xffff]++;
-            k2[o] = key[i];
-            v2[o] = val[i];
-        }
-        memcpy(key, k2, (size_t)n * 8);
-        memcpy(val, v2, (size_t)n * 4);
+/* A parallel loop: fn(ctx, lo, hi, t) over n_threads contiguous slices of
+ * [0, n).  Index construction only; the mapping path is not threaded here. */
+typedef void (*range_fn)(void *ctx, int64_t lo, int64_t hi, int t);
+typedef struct {
+    range_fn fn;
+    void *ctx;
+    int64_t lo, hi;
+    int t;
+} range_job_t;
+
+static void *range_worker(void *arg) {
+    range_job_t *j = (range_job_t *)arg;
+    j->fn(j->ctx, j->lo, j->hi, j->t);
+    return NULL;
+}
+
+static void par_for(int n_threads, int64_t n, range_fn fn, void *ctx) {
+    if (n_threads < 1) n_threads = 1;
+    if (n_threads > 256) n_threads = 256;
+    pthread_t th[256];
+    range_job_t jobs[256];
+    for (int t = 0; t < n_threads; t++) {
+        jobs[t] = (range_job_t){fn, ctx, n * t / n_threads, n * (t + 1) / n_threads, t};
+        pthread_create(&th[t], NULL, range_worker, &jobs[t]);
     }
-    free(k2);
-    free(v2);
+    for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+}
+
+/* Stable LSD radix sort of (key, val) pairs by key, 16-bit digits: each
+ * thread histograms its slice, the counts are scanned in (digit, thread)
+ * order, and each thread scatters its slice in order -- so equal keys keep
+ * their input order. */
+typedef struct {
+    uint64_t *key, *k2;
+    uint32_t *val, *v2;
+    int shift, nt;
+    int64_t *cnt; /* nt x 65536 */
+} radix_ctx;
+
+static void radix_count(void *c, int64_t lo, int64_t hi, int t) {
+    radix_ctx *x = (radix_ctx *)c;
+    int64_t *cnt = x->cnt + (size_t)t * 65536;
+    memset(cnt, 0, 65536 * sizeof(int64_t));
+    for (int64_t i = lo; i < hi; i++) cnt[(x->key[i] >> x->shift) & 0xffff]++;
+}
+
+static void radix_scatter(void *c, int64_t lo, int64_t hi, int t) {
+    radix_ctx *x = (radix_ctx *)c;
+    int64_t *cnt = x->cnt + (size_t)t * 65536;
+    for (int64_t i = lo; i < hi; i++) {
+        const int64_t o = cnt[(x->key[i] >> x->shift) & 0xffff]++;
+        x->k2[o] = x->key[i];
+        x->v2[o] = x->val[i];
+    }
+}
+
+/* Sorts (*key, *val) in place of the pointers: the sorted arrays may end up
+ * in the scratch buffers, which the caller then owns (the others are freed). */
+static void radix_sort_kv(uint64_t **keyp, uint32_t **valp, int64_t n, int n_threads) {
+    if (n <= 1) return;
+    if (n_threads < 1) n_threads = 1;
+    if (n < (1 << 20)) n_threads = 1;
+    radix_ctx x = {*keyp, xmalloc((size_t)n * 8), *valp, xmalloc((size_t)n * 4), 0, n_threads,
+                   xmalloc(sizeof(int64_t) * 65536 * (size_t)n_threads)};
+    uint64_t all_or = 0;
+    for (int64_t i = 0; i < n; i++) all_or |= x.key[i];
+    for (int shift = 0; shift < 64; shift += 16) {
+        if ((all_or >> shift) == 0) break;
+        x.shift = shift;
+        par_for(n_threads, n, radix_count, &x);
+        int64_t sum = 0;
+        for (int d = 0; d < 65536; d++)
+            for (int t = 0; t < n_threads; t++) {
+                int64_t *c = &x.cnt[(size_t)t * 65536 + d];
+                const int64_t v = *c;
+                *c = sum;
+                sum += v;
+            }
+        par_for(n_threads, n, radix_scatter, &x);
+        uint64_t *tk = x.key;
+        x.key = x.k2;
+        x.k2 = tk;
+        uint32_t *tv = x.val;
+        x.val = x.v2;
+        x.v2 = tv;
+    }
+    free(x.k2);
+    free(x.v2);
+    free(x.cnt);
+    *keyp = x.key;
+    *valp = x.val;
+}
+
+typedef struct {
+    ok_index *idx;
+    const uint32_t *gid;
+} hits_ctx;
+
+static void fill_hits(void *c, int64_t lo, int64_t hi, int t) {
+    hits_ctx *x = (hits_ctx *)c;
+    ok_index *idx = x->idx;
+    const int32_t n_segments = idx->n_segments;
+    for (int64_t i = lo; i < hi; i++) {
+        const int64_t g = x->gid[i];
+        /* segment of global k-mer g */
+        int32_t s = 0, top = n_segments - 1;
+        while (s < top) {
+            const int32_t mid = (s + top + 1) / 2;
+            if (idx->kstart[mid] <= g) s = mid; else top = mid - 1;
+        }
+        idx->hits[i].r = idx->t_r[g];
+        idx->hits[i].tpos = (int32_t)(g - idx->kstart[s]);
+        idx->hits[i].segm_id = s;
+        idx->hits[i].strand = idx->t_strand[g];
+    }
+}
+
+/* Inserts the runs of equal keys that start in [lo, hi) (a run belongs to the
+ * slice holding its first element); slots are claimed by compare-and-swap on
+ * the count, so a key's slot depends on timing but every lookup's answer
+ * does not (keys are distinct). */
+typedef struct {
+    ok_index *idx;
+    const uint64_t *key;
+    int64_t total;
+    int32_t max_matches;
+    int64_t kmers[256], hits[256], black[256];
+} insert_ctx;
+
+static void insert_runs(void *c, int64_t lo, int64_t hi, int t) {
+    insert_ctx *x = (insert_ctx *)c;
+    ok_index *idx = x->idx;
+    const uint64_t *key = x->key;
+    int64_t i = lo;
+    while (i > 0 && i < hi && key[i] == key[i - 1]) i++;
+    int64_t nk = 0, nh = 0, nb = 0;
+    while (i < hi) {
+        int64_t e = i + 1;
+        while (e < x->total && key[e] == key[i]) e++;
+        const int64_t cnt = e - i;
+        nk++;
+        nh += cnt;
+        if (x->max_matches > 0 && cnt > x->max_matches && cnt > 1) {
+            nb++;
+        } else {
+            uint64_t pos = slot_of(key[i], idx->mask);
+            for (;;) {
+                uint32_t zero = 0;
+                if (__atomic_compare_exchange_n(&idx->table[pos].count, &zero, (uint32_t)cnt, 0, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED))
+                    break;
+                pos = (pos + 1) & idx->mask;
+            }
+            idx->table[pos].key = key[i];
+            idx->table[pos].start = (uint32_t)i;
+        }
+        i = e;
+    }
+    x->kmers[t] = nk;
+    x->hits[t] = nh;
+    x->black[t] = nb;
 }
 
 /* src/index.rs:384-663 -- build_index.  Segments get ids in file order; the
@@ -368,24 +501,11 @@ ok_index *ok_index_build(int32_t k, double h_frac, int32_t max_matches, int32_t 
         key[i] = idx->t_h[i];
         gid[i] = (uint32_t)i;
     }
-    radix_sort_kv(key, gid, total);
+    radix_sort_kv(&key, &gid, total, n_threads);
     idx->hits = xmalloc(sizeof(hit_t) * (size_t)(total > 0 ? total : 1));
     {
-        int32_t s = 0;
-        for (int64_t i = 0; i < total; i++) {
-            const int64_t g = gid[i];
-            /* segment of global k-mer g */
-            int32_t lo = 0, hi = n_segments - 1;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi + 1) / 2;
-                if (idx->kstart[mid] <= g) lo = mid; else hi = mid - 1;
-            }
-            s = lo;
-            idx->hits[i].r = idx->t_r[g];
-            idx->hits[i].tpos = (int32_t)(g - idx->kstart[s]);
-            idx->hits[i].segm_id = s;
-            idx->hits[i].strand = idx->t_strand[g];
-        }
+        hits_ctx hc = {idx, gid};
+        par_for(total < (1 << 20) ? 1 : n_threads, total, fill_hits, &hc);
     }
     int64_t n_keys = 0;
     for (int64_t i = 0; i < total; i++)
@@ -394,22 +514,20 @@ ok_index *ok_index_build(int32_t k, double h_frac, int32_t max_matches, int32_t 
     while (cap < (uint64_t)n_keys * 2) cap <<= 1;
     idx->table = calloc(cap, sizeof(hslot_t));
     idx->mask = cap - 1;
-    for (int64_t i = 0; i < total;) {
-        int64_t e = i + 1;
-        while (e < total && key[e] == key[i]) e++;
-        const int64_t c = e - i;
-        idx->indexed_kmers++;
-        idx->indexed_hits += c;
-        if (max_matches > 0 && c > max_matches && c > 1) {
-            idx->blacklisted_kmers++;
-        } else {
-            uint64_t pos = slot_of(key[i], idx->mask);
-            while (idx->table[pos].count != 0) pos = (pos + 1) & idx->mask;
-            idx->table[pos].key = key[i];
-            idx->table[pos].start = (uint32_t)i;
-            idx->table[pos].count = (uint32_t)c;
+    {
+        const int nt = total < (1 << 20) ? 1 : (n_threads > 256 ? 256 : n_threads);
+        insert_ctx *ic = calloc(1, sizeof(insert_ctx));
+        ic->idx = idx;
+        ic->key = key;
+        ic->total = total;
+        ic->max_matches = max_matches;
+        par_for(nt, total, insert_runs, ic);
+        for (int t = 0; t < nt; t++) {
+            idx->indexed_kmers += ic->kmers[t];
+            idx->indexed_hits += ic->hits[t];
+            idx->blacklisted_kmers += ic->black[t];
         }
-        i = e;
+        free(ic);
     }
     free(key);
     free(gid);

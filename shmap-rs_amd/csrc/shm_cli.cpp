@@ -744,6 +744,7 @@ int main(int argc, char **argv) {
     unsigned long long idx_read = 0, n_reads = 0, n_mapped = 0;
     double map_s = 0;
     int rc = 0;
+    std::string first_error;  // the first read that failed
     {
         BatchReader br(rd, p.batch);
         while (Batch *batch = br.next()) {
@@ -753,18 +754,20 @@ int main(int argc, char **argv) {
                 reads[i] = {batch->bases.data() + batch->off[i], (int32_t)(batch->off[i + 1] - batch->off[i])};
             res.resize(nb);
             const double t0 = now_s();
-            if (shm_map_batch(mp, reads.data(), (int32_t)nb, res.data())) {
-                fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
-                        shm_last_error());
-                rc = 1;
-                br.recycle(batch);
-                break;
+            // A read that fails does so alone (its status); every other read is
+            // still written, and the run ends with the first failure, as the
+            // reference's collector does (src/shmap/mod.rs:597-612, 637-640).
+            if (shm_map_batch(mp, reads.data(), (int32_t)nb, res.data()) && first_error.empty()) {
+                size_t bad = 0;
+                while (bad < nb && res[bad].status == 0) bad++;
+                first_error = bad < nb ? "read " + batch->name(bad) + ": " + shm_last_error() : shm_last_error();
             }
             const double dt = now_s() - t0;
             map_s += dt;
             const double per_read = dt / nb;
             for (size_t i = 0; i < nb; i++, idx_read++) {
                 const shm_read_result &x = res[i];
+                if (x.status != 0) continue;  // a failed read writes nothing
                 const std::string name = batch->name(i);
                 if (stats && idx_read % p.per_read_stats_sample == 0)
                     fprintf(stats, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n", name.c_str(),
@@ -777,11 +780,19 @@ int main(int argc, char **argv) {
                         line.resize(n + 1);
                         n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
                     }
+                    if (n < 0) {  // not a well-formed mapped record: this read's failure
+                        if (first_error.empty()) first_error = "read " + name + ": " + shm_last_error();
+                        continue;
+                    }
                     fwrite(line.data(), 1, n, stdout);
                     fputc('\n', stdout);
                     n_mapped++;
                 } else if (unmapped) {
                     int n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
+                    if (n >= (int)line.size()) {
+                        line.resize(n + 1);
+                        n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
+                    }
                     fwrite(line.data(), 1, n, unmapped);
                     fputc('\n', unmapped);
                 }
@@ -789,6 +800,11 @@ int main(int argc, char **argv) {
             n_reads += nb;
             br.recycle(batch);
         }
+    }
+    if (!first_error.empty()) {
+        fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
+                first_error.c_str());
+        rc = 1;
     }
     if (rc == 0 && !rd.error().empty()) {  // the reader thread has ended with the BatchReader
         fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),

@@ -2736,4 +2736,37 @@ void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *
     k_lookup<<<flat_grid(n, 1024), WG, 0, st>>>(I, hs, n, cnt, off);
 }
 
+// Consistency of an index read from a file, after the host has checked that
+// every hit's k-mer id lies inside its segment and every slot's run inside the
+// hit arrays (run lengths summing to at most the hit count): each hit carries
+// its k-mer's position and strand, and each slot's run is strictly increasing
+// in (segment, k-mer id), which the lower-bound searches over runs rely on.
+// Violations are counted into *bad.
+__global__ void __launch_bounds__(WG) k_validate_hits(DevIndex I, unsigned long long *bad) {
+    uint32_t nb = 0;
+    const uint64_t N = (uint64_t)I.n_kmers;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < N; i += (uint64_t)gridDim.x * WG) {
+        const uint32_t g = I.hit_g[i];
+        nb += I.hit_r[i] != ((uint32_t)I.t_r[g] | ((uint32_t)I.t_s[g] << 31));
+    }
+    if (nb) atomicAdd(bad, (unsigned long long)nb);
+}
+
+__global__ void __launch_bounds__(WG) k_validate_runs(DevIndex I, uint64_t slots, unsigned long long *bad) {
+    uint32_t nb = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * WG) {
+        const uint4 s = I.table[i];
+        for (uint64_t j = (uint64_t)s.z + 1; s.w != 0 && j < (uint64_t)s.z + s.w; j++) {
+            const uint32_t sa = I.hit_seg[j - 1], sb = I.hit_seg[j];
+            nb += sa > sb || (sa == sb && I.hit_g[j - 1] >= I.hit_g[j]);
+        }
+    }
+    if (nb) atomicAdd(bad, (unsigned long long)nb);
+}
+
+void launch_validate_index(const DevIndex &I, uint64_t slots, unsigned long long *bad, hipStream_t st) {
+    if (I.n_kmers > 0) k_validate_hits<<<flat_grid((uint64_t)I.n_kmers, 4096), WG, 0, st>>>(I, bad);
+    if (slots > 0) k_validate_runs<<<flat_grid(slots, 4096), WG, 0, st>>>(I, slots, bad);
+}
+
 }  // namespace shm

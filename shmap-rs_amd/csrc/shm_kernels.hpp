@@ -203,5 +203,9 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
                   uint32_t max_grid, hipStream_t st);
 void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,
                    hipStream_t st);
+// Counts into *bad (zeroed by the caller) the hits whose position or strand is
+// not their k-mer's and the slot runs not strictly increasing in (segment,
+// k-mer id); for an index read from a file, after the host's bounds checks.
+void launch_validate_index(const DevIndex &I, uint64_t slots, unsigned long long *bad, hipStream_t st);
 
 }  // namespace shm

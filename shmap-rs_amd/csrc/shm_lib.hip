@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <errno.h>
+#include <unistd.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -256,11 +257,20 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     return SHM_OK;
 }
 
+// A stream owned by one call, destroyed on every return path.
+struct OwnedStream {
+    hipStream_t st = nullptr;
+    ~OwnedStream() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
 extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t len, int32_t offset,
                           shm_kmer *out, int64_t cap, int64_t *n) {
     if (!sk || (!seq && len > 0) || !n || len < 0) return fail(SHM_ERR_ARG, "invalid argument");
-    hipStream_t st;
-    HIPCHK(hipStreamCreate(&st));
+    OwnedStream os;
+    HIPCHK(hipStreamCreate(&os.st));
+    hipStream_t st = os.st;
     DevBuf<uint8_t> bases;
     DevBuf<int64_t> off;
     RCHK(bases.ensure(len + 64));
@@ -278,10 +288,8 @@ extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t le
         RCHK(sketch_device(sk, bases.p, off.p, 1, so, kh, kr, ks, 1024, st, nullptr, nullptr, false));
     const int64_t M = empty ? 0 : (int64_t)so.M;
     *n = M;
-    if (M > cap) {
-        (void)hipStreamDestroy(st);
+    if (M > cap)
         return fail(SHM_ERR_CAPACITY, "sketch has %lld k-mers, buffer holds %lld", (long long)M, (long long)cap);
-    }
     if (M > 0) {
         std::vector<uint64_t> h(M);
         std::vector<int32_t> r(M);
@@ -297,7 +305,6 @@ extern "C" int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t le
             out[i].pad[0] = out[i].pad[1] = out[i].pad[2] = 0;
         }
     }
-    HIPCHK(hipStreamDestroy(st));
     return SHM_OK;
 }
 
@@ -350,6 +357,12 @@ struct shm_index {
 extern "C" int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, int32_t n_segments,
                                int32_t max_matches, int32_t device_id, shm_index **out) {
     if (!sk || !out || n_segments < 0 || (n_segments > 0 && !segs)) return fail(SHM_ERR_ARG, "invalid argument");
+    // positions are i32 in the reference (RPos, src/types.rs) and 31 bits in
+    // the hit records here; the PAF's target length is an i32 too
+    for (int32_t s = 0; s < n_segments; s++)
+        if (segs[s].len < 0 || segs[s].len > INT32_MAX || (segs[s].len > 0 && !segs[s].seq))
+            return fail(SHM_ERR_ARG, "segment %d: length %lld is outside [0, 2^31-1] or its bases are missing", s,
+                        (long long)segs[s].len);
     HIPCHK(hipSetDevice(device_id));
     shm_index *idx = new shm_index;
     idx->device = device_id;
@@ -671,6 +684,108 @@ extern "C" int shm_index_save(const shm_index *idx, const char *path) {
     return rc;
 }
 
+// Reads n elements at file offset `at` in pieces of at most `step`, handing
+// each piece to fn(first element index, pointer, count); false from fn stops.
+template <typename T, typename F>
+static int idx_scan(int fd, uint64_t at, uint64_t n, std::vector<T> &buf, const char *path, F fn) {
+    const uint64_t step = buf.size();
+    for (uint64_t i = 0; i < n; i += step) {
+        const uint64_t c = n - i < step ? n - i : step;
+        const size_t bytes = (size_t)c * sizeof(T);
+        size_t got = 0;
+        while (got < bytes) {
+            const ssize_t r = pread(fd, (uint8_t *)buf.data() + got, bytes - got, (off_t)(at + i * sizeof(T) + got));
+            if (r <= 0) return fail(SHM_ERR_IO, "cannot read \"%s\"", path);
+            got += (size_t)r;
+        }
+        if (!fn(i, buf.data(), c)) return fail(SHM_ERR_ARG, "\"%s\" is inconsistent", path);
+    }
+    return SHM_OK;
+}
+
+// The host's checks of an index file's arrays (at file offset `at`: t_h, t_r,
+// t_s, hit_g, hit_r, hit_seg, strand_pre, table), before any of it reaches the
+// device: sketch positions strictly increasing inside each segment and within
+// its length, strands 0/1; every hit's segment in range and its k-mer id inside
+// that segment; the strand prefix counting exactly the reverse-strand hits;
+// every slot's run inside the hit arrays, the runs' lengths summing to at most
+// the hit count, and an empty slot in every shard, so that each probe ends.
+static int check_index_arrays(int fd, uint64_t at, const IdxHeader &h, const shm_index &ix, const char *path) {
+    const uint64_t N = (uint64_t)h.n_kmers, S = h.nseg;
+    const uint64_t o_tr = at + N * 8, o_ts = o_tr + N * 4, o_hg = o_ts + N, o_hr = o_hg + N * 4,
+                   o_hs = o_hr + N * 4, o_sp = o_hs + N * 4, o_tab = o_sp + (N + 1) * 4;
+    const uint64_t STEP = 1u << 20;
+    {
+        std::vector<int32_t> tr(STEP);
+        uint64_t seg = 0;
+        int32_t prev = -1;
+        RCHK(idx_scan(fd, o_tr, N, tr, path, [&](uint64_t i0, const int32_t *v, uint64_t c) {
+            for (uint64_t j = 0; j < c; j++) {
+                const uint64_t g = i0 + j;
+                while (seg < S && (uint64_t)ix.h_kstart[seg + 1] <= g) {
+                    seg++;
+                    prev = -1;
+                }
+                if (seg >= S || v[j] <= prev || v[j] >= ix.sz[seg]) return false;
+                prev = v[j];
+            }
+            return true;
+        }));
+        std::vector<uint8_t> ts(STEP);
+        RCHK(idx_scan(fd, o_ts, N, ts, path, [&](uint64_t, const uint8_t *v, uint64_t c) {
+            for (uint64_t j = 0; j < c; j++)
+                if (v[j] > 1) return false;
+            return true;
+        }));
+    }
+    {
+        std::vector<uint32_t> hg(STEP), hr(STEP), hs(STEP), sp(STEP);
+        uint32_t sp0 = 0;
+        if (pread(fd, &sp0, 4, (off_t)o_sp) != 4) return fail(SHM_ERR_IO, "cannot read \"%s\"", path);
+        if (sp0 != 0) return fail(SHM_ERR_ARG, "\"%s\" is inconsistent", path);
+        // hit_g, hit_r and hit_seg in step; strand_pre one ahead
+        RCHK(idx_scan(fd, o_hg, N, hg, path, [&](uint64_t i0, const uint32_t *g, uint64_t c) {
+            if (idx_scan(fd, o_hr + i0 * 4, c, hr, path, [](uint64_t, const uint32_t *, uint64_t) { return true; }) ||
+                idx_scan(fd, o_hs + i0 * 4, c, hs, path, [](uint64_t, const uint32_t *, uint64_t) { return true; }) ||
+                idx_scan(fd, o_sp + (i0 + 1) * 4, c, sp, path, [](uint64_t, const uint32_t *, uint64_t) { return true; }))
+                return false;
+            for (uint64_t j = 0; j < c; j++) {
+                const uint32_t sg = hs[j];
+                if (sg >= S || (int64_t)g[j] < ix.h_kstart[sg] || (int64_t)g[j] >= ix.h_kstart[sg + 1]) return false;
+                const uint32_t before = j ? sp[j - 1] : sp0;
+                if (sp[j] != before + (hr[j] >> 31)) return false;
+            }
+            sp0 = sp[c - 1];
+            return true;
+        }));
+    }
+    {
+        std::vector<uint4> tab(STEP);
+        uint64_t total = 0;
+        int shard = 0;
+        bool empty_seen = false;
+        RCHK(idx_scan(fd, o_tab, h.table_slots, tab, path, [&](uint64_t i0, const uint4 *t, uint64_t c) {
+            for (uint64_t j = 0; j < c; j++) {
+                const uint64_t slot = i0 + j;
+                while (shard < 7 && slot >= h.shard_base[shard + 1]) {
+                    if (!empty_seen) return false;
+                    shard++;
+                    empty_seen = false;
+                }
+                if (t[j].w == 0) {
+                    empty_seen = true;
+                    continue;
+                }
+                total += t[j].w;
+                if ((uint64_t)t[j].z + t[j].w > N || total > N) return false;
+            }
+            return true;
+        }));
+        if (!empty_seen) return fail(SHM_ERR_ARG, "\"%s\" is inconsistent", path);
+    }
+    return SHM_OK;
+}
+
 extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **out) {
     if (!path || !out) return fail(SHM_ERR_ARG, "invalid argument");
     FILE *f = fopen(path, "rb");
@@ -688,9 +803,16 @@ extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **o
                     h.version, IDX_VERSION);
     if (h.n_kmers < 0 || (uint64_t)h.n_kmers >= 0xffffffffull || h.nseg > (1u << 30))
         return fail(SHM_ERR_ARG, "\"%s\" has an invalid header", path);
+    // the parameters shm_sketcher_create accepts (the mapper's kernels take k
+    // and the threshold from here)
+    if (h.k <= 0 || h.k > 4096 || !(h.h_frac > 0.0) || h.h_frac > 1.0 || h.max_matches < 0)
+        return fail(SHM_ERR_ARG, "\"%s\" has invalid index parameters (k=%d r=%g M=%d)", path, h.k, h.h_frac,
+                    h.max_matches);
+    // the device probes use 32-bit slot positions and masks: at most 2^31 slots
+    // a shard (the build makes far fewer)
     uint64_t slots = 0;
     for (int i = 0; i < 8; i++) {
-        if (h.shard_bits[i] < 6 || h.shard_bits[i] > 40 || h.shard_base[i] != slots)
+        if (h.shard_bits[i] < 6 || h.shard_bits[i] > 31 || h.shard_base[i] != slots)
             return fail(SHM_ERR_ARG, "\"%s\" has an invalid header", path);
         slots += 1ull << h.shard_bits[i];
     }
@@ -744,6 +866,16 @@ extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **o
     RCHK(idx_read(f, idx->h_kstart.data(), (S + 1) * 8, path));
     RCHK(idx_read(f, idx->h_segcum.data(), (S + 1) * 8, path));
     if (idx->h_kstart[S] != h.n_kmers) return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+    // segment lengths and base offsets agree, k-mer starts never decrease
+    if (idx->h_kstart[0] != 0 || idx->h_segcum[0] != 0)
+        return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+    for (uint64_t s = 0; s < S; s++)
+        if (idx->sz[s] < 0 || idx->h_segcum[s + 1] - idx->h_segcum[s] != idx->sz[s] ||
+            idx->h_kstart[s + 1] < idx->h_kstart[s])
+            return fail(SHM_ERR_ARG, "\"%s\" has an invalid segment table", path);
+    // every array the device will index through, checked on the host before
+    // the device is touched
+    RCHK(check_index_arrays(fileno(f), (uint64_t)ftello(f), h, *idx, path));
     HIPCHK(hipSetDevice(device_id));
     PinnedBuf<uint8_t> stage;
     RCHK(stage.ensure(IDX_STAGE));
@@ -759,6 +891,19 @@ extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **o
     RCHK(idx_get(f, idx->hit_seg, N, 1, stage, path));
     RCHK(idx_get(f, idx->strand_pre, N + 1, 0, stage, path));
     RCHK(idx_get(f, idx->table, (size_t)h.table_slots, 0, stage, path));
+    // what the host cannot check in one pass (random access): each hit carries
+    // its k-mer's position and strand, and every slot's run is ordered
+    {
+        DevBuf<unsigned long long> bad;
+        RCHK(bad.ensure(1));
+        HIPCHK(hipMemset(bad.p, 0, 8));
+        launch_validate_index(idx->view(), h.table_slots, bad.p, 0);
+        HIPCHK(hipGetLastError());
+        unsigned long long nbad = 0;
+        HIPCHK(hipMemcpy(&nbad, bad.p, 8, hipMemcpyDeviceToHost));
+        if (nbad) return fail(SHM_ERR_ARG, "\"%s\" is inconsistent (%llu hit records disagree with the sketch "
+                                           "or are out of order)", path, nbad);
+    }
     *out = idx.release();
     return SHM_OK;
 }
@@ -782,6 +927,8 @@ struct shm_mapper {
     uint64_t tab_budget = 1ull << 30;  // bucket-table slots per group (24 B each, ~26 GB)
     bool timing = false;
     hipEvent_t ev[ST_COUNT][2];
+    hipEvent_t t0 = nullptr, t1 = nullptr;  // a batch's first and last GPU work
+    int32_t max_read_len = 0;  // reads longer than this fail alone (0: no limit)
     double stage_ms[ST_COUNT] = {};
     double total_ms = 0;
     int64_t counts[SHM_N_COUNTS] = {};
@@ -819,6 +966,24 @@ struct shm_mapper {
     PinnedBuf<uint32_t> h_mapped;
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
+
+    // Frees every batch buffer (after a failed allocation, so that a retry
+    // with fewer reads starts from the device memory a smaller batch needs).
+    void release_batch_buffers() {
+        bases.release(), roff.release(), h_bases.release();
+        so.nunit.release(), so.unit_start.release(), so.chunk_cnt.release(), so.chunk_koff.release();
+        so.koff.release(), so.lane_cnt.release(), so.lane_pre.release(), so.unit_seq.release();
+        so.ovf.release(), so.stage.release();
+        k_h.release(), ps_h.release(), sd_h.release(), sd_key.release(), k_r.release(), k_s.release();
+        sd_strand.release(), ps_v.release(), sd_first.release(), sd_occs.release(), sd_cnt.release();
+        sd_hoff.release(), sd_ord.release(), sd_epre.release(), rd_i32.release(), rd_u64.release();
+        grp_base.release(), grp_nb.release(), tb_key.release(), tb_list.release(), tb_mc.release();
+        tb_min.release(), tb_max.release(), bk_key.release(), bk_val.release(), bs_g.release();
+        bk_i32.release(), bs_alive.release(), bs_sh.release(), sc_j.release(), g_sid.release();
+        g_tr.release(), g_diff.release(), g_tails.release(), tasks.release(), sel.release(), big.release();
+        ext.release(), ov.release(), ov_kpre.release(), mapped.release(), h_mapped.release();
+        task_count.release(), d_out.release();
+    }
 };
 
 extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, shm_mapper **out) {
@@ -849,6 +1014,10 @@ extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, 
     }
     for (int s = 0; s < ST_COUNT; s++)
         for (int j = 0; j < 2; j++) (void)hipEventCreate(&mp->ev[s][j]);
+    if (hipEventCreate(&mp->t0) != hipSuccess || hipEventCreate(&mp->t1) != hipSuccess) {
+        shm_mapper_free(mp);
+        return fail(SHM_ERR_HIP, "hipEventCreate failed");
+    }
     *out = mp;
     return SHM_OK;
 }
@@ -857,6 +1026,8 @@ extern "C" void shm_mapper_free(shm_mapper *mp) {
     if (!mp) return;
     for (int s = 0; s < ST_COUNT; s++)
         for (int j = 0; j < 2; j++) (void)hipEventDestroy(mp->ev[s][j]);
+    if (mp->t0) (void)hipEventDestroy(mp->t0);
+    if (mp->t1) (void)hipEventDestroy(mp->t1);
     (void)hipStreamDestroy(mp->st);
     delete mp;
 }
@@ -1017,15 +1188,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     const DevIndex I = ix->view();
     for (int s = 0; s < ST_COUNT; s++) mp->stage_ms[s] = 0;
     std::vector<int> pending;
-    hipEvent_t t0, t1;
-    HIPCHK(hipEventCreate(&t0));
-    HIPCHK(hipEventCreate(&t1));
-    HIPCHK(hipEventRecord(t0, st));
+    HIPCHK(hipEventRecord(mp->t0, st));
     for (int i = 0; i < SHM_N_COUNTS; i++) mp->counts[i] = 0;
     if (R == 0) {
         mp->total_ms = 0;
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
         return SHM_OK;
     }
 
@@ -1308,14 +1474,12 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->h_sum.ensure(6));
     HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 4, A.u_total, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(t1, st));
+    HIPCHK(hipEventRecord(mp->t1, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    HIPCHK(hipEventElapsedTime(&ms, mp->t0, mp->t1));
     mp->total_ms = ms;
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     // only mapped reads need the host's double-precision finalization
     const uint32_t n_mapped = mp->h_mapped.p[0];
     for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
@@ -1340,14 +1504,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     return SHM_OK;
 }
 
-extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out) {
-    if (!mp || n < 0 || (n > 0 && (!reads || !out))) return fail(SHM_ERR_ARG, "invalid argument");
-    HIPCHK(hipSetDevice(mp->idx->device));
+// Uploads n valid host reads and maps them as one batch.
+static int map_host_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out) {
     std::vector<int64_t> off(n + 1, 0);
-    for (int32_t i = 0; i < n; i++) {
-        if (reads[i].len < 0 || (reads[i].len > 0 && !reads[i].seq)) return fail(SHM_ERR_ARG, "read %d invalid", i);
-        off[i + 1] = off[i] + reads[i].len;
-    }
+    for (int32_t i = 0; i < n; i++) off[i + 1] = off[i] + reads[i].len;
     RCHK(mp->bases.ensure(off[n] + 64));
     RCHK(mp->roff.ensure(n + 1));
     // Reads back to back in one caller buffer go up in one copy; otherwise they
@@ -1368,6 +1528,106 @@ extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, s
     }
     HIPCHK(hipMemcpyAsync(mp->roff.p, mp->h_roff.p, (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
     return map_device(mp, mp->bases.p, mp->roff.p, n, out);
+}
+
+// The record of a read that failed alone: its length and status, nothing mapped.
+static void failed_record(shm_read_result &r, int32_t len, int code) {
+    memset(&r, 0, sizeof r);
+    r.read_len = len > 0 ? len : 0;
+    r.b_segm = r.b_b = r.b2_segm = r.b2_b = -1;
+    r.status = code;
+}
+
+// The first read (in input order) that failed alone, with its message.
+struct ReadFailure {
+    int32_t read = -1;
+    int code = SHM_OK;
+    std::string msg;
+    void note(int32_t i, int c, const std::string &m) {
+        if (read < 0 || i < read) {
+            read = i;
+            code = c;
+            msg = m;
+        }
+    }
+};
+
+// Maps the reads ids[0, n) (ascending) into their records.  A batch that runs
+// out of device memory is halved and each half mapped again, down to single
+// reads: a read that cannot be mapped even alone fails alone (its status), as
+// a read that panics does in the reference (src/shmap/mod.rs:219-238).  Any
+// other failure is the whole batch's (every read's status) and is returned.
+static int map_isolating(shm_mapper *mp, const shm_read *reads, const int32_t *ids, int32_t n,
+                         shm_read_result *out, ReadFailure &rf) {
+    if (n == 0) return SHM_OK;
+    const bool dense = ids[n - 1] - ids[0] == n - 1;  // a contiguous run: the caller's arrays as they are
+    std::vector<shm_read> sub;
+    std::vector<shm_read_result> tmp;
+    if (!dense) {
+        sub.resize(n);
+        tmp.resize(n);
+        for (int32_t j = 0; j < n; j++) sub[j] = reads[ids[j]];
+    }
+    const int rc = dense ? map_host_batch(mp, reads + ids[0], n, out + ids[0]) : map_host_batch(mp, sub.data(), n, tmp.data());
+    if (rc == SHM_OK) {
+        if (!dense)
+            for (int32_t j = 0; j < n; j++) out[ids[j]] = tmp[j];
+        return SHM_OK;
+    }
+    const std::string msg = g_err;
+    if (rc == SHM_ERR_NOMEM) {
+        (void)hipStreamSynchronize(mp->st);
+        (void)hipGetLastError();
+        mp->release_batch_buffers();
+        if (n == 1) {
+            failed_record(out[ids[0]], reads[ids[0]].len, rc);
+            rf.note(ids[0], rc, msg);
+            return SHM_OK;
+        }
+        const int32_t h = n / 2;
+        const int r1 = map_isolating(mp, reads, ids, h, out, rf);
+        if (r1 != SHM_OK) {
+            for (int32_t j = h; j < n; j++) failed_record(out[ids[j]], reads[ids[j]].len, r1);
+            return r1;
+        }
+        return map_isolating(mp, reads, ids + h, n - h, out, rf);
+    }
+    for (int32_t j = 0; j < n; j++) failed_record(out[ids[j]], reads[ids[j]].len, rc);
+    g_err = msg;
+    return rc;
+}
+
+extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out) {
+    if (!mp || n < 0 || (n > 0 && (!reads || !out))) return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(mp->idx->device));
+    // reads that fail alone before the device sees them: malformed, or over
+    // the mapper's length limit
+    ReadFailure rf;
+    std::vector<int32_t> ids;
+    ids.reserve(n);
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t len = reads[i].len;
+        if (len < 0 || (len > 0 && !reads[i].seq)) {
+            failed_record(out[i], len, SHM_ERR_ARG);
+            rf.note(i, SHM_ERR_ARG, "invalid read (negative length or no bases)");
+        } else if (mp->max_read_len > 0 && len > mp->max_read_len) {
+            failed_record(out[i], len, SHM_ERR_CAPACITY);
+            char m[128];
+            snprintf(m, sizeof m, "read of %d bases is over the mapper's limit of %d", len, mp->max_read_len);
+            rf.note(i, SHM_ERR_CAPACITY, m);
+        } else {
+            ids.push_back(i);
+        }
+    }
+    RCHK(map_isolating(mp, reads, ids.data(), (int32_t)ids.size(), out, rf));
+    if (rf.read >= 0) return fail(rf.code, "read %d failed: %s", rf.read, rf.msg.c_str());
+    return SHM_OK;
+}
+
+extern "C" int shm_mapper_set_read_limit(shm_mapper *mp, int32_t max_read_len) {
+    if (!mp || max_read_len < 0) return fail(SHM_ERR_ARG, "invalid argument");
+    mp->max_read_len = max_read_len;
+    return SHM_OK;
 }
 
 extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,

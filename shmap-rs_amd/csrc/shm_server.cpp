@@ -72,9 +72,13 @@ void shm_server::worker(int wi) {
         reads.resize(n);
         res.assign(n, shm_read_result{});
         for (int32_t i = 0; i < n; i++) reads[i] = shm_read{batch[i].seq, batch[i].len};
+        // a read that fails does so alone (its record's status); the others
+        // of the batch are mapped and delivered as usual
         const int rc = shm_map_batch(mp, reads.data(), n, res.data());
         const std::string msg = rc ? shm_last_error() : "";
         const double end = now();
+        bool marked = false;
+        for (int32_t i = 0; i < n && !marked; i++) marked = res[i].status != 0;
         {
             std::lock_guard<std::mutex> lk(mu);
             if (rc && error.empty()) error = msg;
@@ -87,7 +91,7 @@ void shm_server::worker(int wi) {
                 d.batch_reads = n;
                 d.worker = wi;
                 d.r = res[i];
-                if (rc) d.r.status = rc;
+                if (rc && !marked) d.r.status = rc;  // a failure before any read was looked at
                 done.push_back(d);
             }
             running -= n;
@@ -114,6 +118,7 @@ extern "C" int shm_server_create(const shm_index *idx, const shm_map_params *par
             return rc;
         }
         s->mappers.push_back(mp);
+        if (s->cfg.max_read_len > 0) (void)shm_mapper_set_read_limit(mp, s->cfg.max_read_len);
     }
     s->t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < s->cfg.workers; i++) s->threads.emplace_back(&shm_server::worker, s, i);

@@ -67,7 +67,8 @@ assert RESULT_DTYPE.itemsize == C.sizeof(ReadResult)
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("workers", C.c_int32), ("max_batch_reads", C.c_int32), ("max_batch_bases", C.c_int64)]
+    _fields_ = [("workers", C.c_int32), ("max_batch_reads", C.c_int32), ("max_batch_bases", C.c_int64),
+                ("max_read_len", C.c_int32), ("pad", C.c_int32)]
 
 
 class Served(C.Structure):
@@ -122,6 +123,7 @@ def lib():
         L.shm_mapper_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.shm_mapper_autotune.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.POINTER(Tuning)]
         L.shm_map_batch.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.c_void_p]
+        L.shm_mapper_set_read_limit.argtypes = [C.c_void_p, C.c_int32]
         L.shm_map_batch_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.shm_format_paf.argtypes = [C.c_void_p, C.POINTER(MapParams), C.c_char_p, C.POINTER(ReadResult),
                                      C.c_double, C.c_char_p, C.c_size_t]
@@ -349,13 +351,21 @@ class Mapper:
         del keep
         return t.as_dict()
 
-    def map_batch(self, reads) -> np.ndarray:
+    def set_read_limit(self, max_read_len: int) -> None:
+        """shm_mapper_set_read_limit: longer reads fail alone (0: no limit)."""
+        _check(lib().shm_mapper_set_read_limit(self._p, int(max_read_len)))
+
+    def map_batch(self, reads, allow_failed_reads: bool = False) -> np.ndarray:
         """reads: list of bytes (or (bases uint8 array, offsets int64 array)).
-        Returns RESULT_DTYPE records in input order."""
+        Returns RESULT_DTYPE records in input order.  A read that fails alone
+        has a nonzero `status`; with allow_failed_reads the records are
+        returned then (the others are complete), otherwise it raises."""
         arr, n, keep = _reads(reads)
         out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
-        _check(lib().shm_map_batch(self._p, arr, n, out.ctypes.data))
+        rc = lib().shm_map_batch(self._p, arr, n, out.ctypes.data)
         del keep
+        if rc and not (allow_failed_reads and out[:n]["status"].any()):
+            _check(rc)
         return out[:n]
 
     def map_device(self, d_bases_ptr: int, d_off_ptr: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
@@ -389,10 +399,10 @@ class Server:
     they arrive and mapped in batches by `workers` concurrent mappers."""
 
     def __init__(self, index: SketchIndex, params: MapParams, workers: int = 3, max_batch_reads: int = 0,
-                 max_batch_bases: int = 0):
+                 max_batch_bases: int = 0, max_read_len: int = 0):
         self._index = index
         self._params = params
-        cfg = ServerConfig(workers, max_batch_reads, max_batch_bases)
+        cfg = ServerConfig(workers, max_batch_reads, max_batch_bases, max_read_len, 0)
         self._p = C.c_void_p()
         _check(lib().shm_server_create(index._p, C.byref(params), C.byref(cfg), C.byref(self._p)))
 

@@ -88,6 +88,77 @@ def test_index_load_checks_the_file_before_the_device(tmp_path):
     assert rc == 1 and "longer than" in msg
 
 
+def _index_file(k=15, h_frac=0.1, bits=6, t_r=(24, 30, 50), t_s=(0, 1, 0), hit_g=(0, 2, 1),
+                hit_seg=(0, 0, 0), strand_pre=(0, 0, 0, 1), kstart=(0, 3), slots=None, fill_shard=None):
+    """A well-framed index file of one 100-base segment with three k-mers
+    (hashes 5, 6, 5) in shm_index_save's layout; the keyword arguments
+    replace parts of it."""
+    import struct
+    n = len(t_r)
+    nslots = 8 << bits
+    tab = [(0, 0, 0, 0)] * nslots
+    for at, slot in (slots or {(5 << bits): (5, 0, 0, 2), (6 << bits): (6, 0, 2, 1)}).items():
+        tab[at] = slot
+    if fill_shard is not None:  # no empty slot left in that shard
+        for i in range(fill_shard << bits, (fill_shard + 1) << bits):
+            tab[i] = (100 + i, 0, 0, 1)
+    hit_r = [((t_r[g] | (t_s[g] << 31)) & 0xffffffff) if g < n else 0 for g in hit_g]
+    base = [i << bits for i in range(8)]
+    hdr = (b"SHMIDX01" + struct.pack("<IIiid", 1, 1, k, 0, h_frac) + struct.pack("<6q", n, 2, n, 2, 0, 0) +
+           struct.pack("<8Q", *base) + struct.pack("<8I", *([bits] * 8)) + struct.pack("<QQ", nslots, 4 + 1))
+    body = (struct.pack("<I", 1) + b"s" + struct.pack("<i", 100) + struct.pack("<2q", *kstart) +
+            struct.pack("<2q", 0, 100) + struct.pack(f"<{n}Q", 5, 6, 5) + struct.pack(f"<{n}i", *t_r) +
+            bytes(t_s) + struct.pack(f"<{n}I", *hit_g) + struct.pack(f"<{n}I", *hit_r) +
+            struct.pack(f"<{n}I", *hit_seg) + struct.pack(f"<{n + 1}I", *strand_pre) +
+            b"".join(struct.pack("<4I", *t) for t in tab))
+    return hdr + body + b"SHMIDXEN"
+
+
+def test_index_load_checks_the_contents_before_the_device(tmp_path):
+    """A well-framed file whose arrays would send a device lookup out of
+    bounds or into an endless probe is refused on the host, before any device
+    work: slots past the hits, a shard without an empty slot, hits outside
+    their segment, a wrong strand prefix, unordered sketch positions, header
+    parameters the sketcher refuses, shards beyond 2^31 slots."""
+    lib = C.CDLL(LIB)
+    lib.shm_last_error.restype = C.c_char_p
+    lib.shm_index_load.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]
+    lib.shm_index_free.argtypes = [C.c_void_p]
+
+    def load(data):
+        f = tmp_path / "x.idx"
+        f.write_bytes(data)
+        out = C.c_void_p()
+        rc = lib.shm_index_load(str(f).encode(), 0, C.byref(out))
+        if rc == 0:
+            lib.shm_index_free(out)
+        return rc, lib.shm_last_error().decode()
+
+    # the well-formed file passes every host check: it loads where a device
+    # exists, and fails only at the device (SHM_ERR_HIP) where none does
+    rc, msg = load(_index_file())
+    assert rc in (0, 2), msg
+    bad = {
+        "slot past the hits": _index_file(slots={5 << 6: (5, 0, 2, 2)}),
+        "runs longer than the hits": _index_file(slots={5 << 6: (5, 0, 0, 3), 6 << 6: (6, 0, 0, 3)}),
+        "no empty slot": _index_file(fill_shard=3),
+        "segment out of range": _index_file(hit_seg=(0, 1, 0)),
+        "hit past the k-mers": _index_file(hit_g=(0, 3, 1), kstart=(0, 3)),
+        "strand prefix": _index_file(strand_pre=(0, 1, 1, 1)),
+        "positions out of order": _index_file(t_r=(24, 24, 50)),
+        "position past the segment": _index_file(t_r=(24, 30, 100)),
+        "strand not 0/1": _index_file(t_s=(0, 2, 0)),
+    }
+    for what, data in bad.items():
+        rc, msg = load(data)
+        assert rc == 1 and "inconsistent" in msg, (what, rc, msg)
+    for kw in (dict(k=0), dict(k=5000), dict(h_frac=0.0), dict(h_frac=1.5)):
+        rc, msg = load(_index_file(**kw))
+        assert rc == 1 and "invalid index parameters" in msg, (kw, msg)
+    rc, msg = load(_index_header(bits=32))
+    assert rc == 1 and "invalid header" in msg
+
+
 @pytest.mark.parametrize("args,needle", [
     (["-k", "0"], "K-mer length (-k) should be positive. You provided 0."),
     (["-r", "0"], "Given hash ratio (-r) should be between 0 and 1. You provided 0."),
