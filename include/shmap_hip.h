@@ -121,6 +121,28 @@ void shm_sketcher_free(shm_sketcher *sk);
 int shm_sketch(const shm_sketcher *sk, const uint8_t *seq, int64_t len, int32_t offset,
                shm_kmer *out, int64_t cap, int64_t *n);
 
+/* No reference counterpart: shm_sketch for callers that cannot allocate (code
+ * under stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same k-mers as shm_sketch, with every device intermediate carved from
+ * one fixed array per GPU of SHM_NOALLOC_SCRATCH_BYTES (held until the process
+ * ends) instead of allocated: no hipMalloc, hipFree, stream creation or other
+ * allocating call.  seq and out are the caller's host memory; the work runs on
+ * `stream` (a hipStream_t of the current device; NULL for the null stream)
+ * and the call returns when out is written.  A sequence whose intermediates do
+ * not fit the array is refused with SHM_ERR_CAPACITY before any
+ * device work; *n and SHM_ERR_CAPACITY as shm_sketch when out is too small.
+ * Calls on one GPU share its array and would race: this library serialises
+ * them (one host mutex per device), so threads of one process may call it
+ * freely, each call waiting for the one before. */
+#define SHM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int shm_sketch_noalloc(const shm_sketcher *sk, const uint8_t *seq, int64_t len, int32_t offset,
+                       shm_kmer *out, int64_t cap, int64_t *n, void *stream);
+/* Bytes of the fixed array shm_sketch_noalloc needs for len bases and room for
+ * cap k-mers (-1 on a bad argument): 29 B per k-mer of room plus about 2 B per
+ * base, so 64 MiB holds a 23 Mbp sequence at k=25 r=0.01 with cap = 1.2 len r,
+ * and 2.1 Mbp with cap = len. */
+int64_t shm_sketch_noalloc_bytes(const shm_sketcher *sk, int64_t len, int64_t cap);
+
 /* SketchIndex::build_index -- src/index.rs:384-663.  Builds the whole index on
  * the GPU and keeps it resident.  max_matches <= 0 means None (-M unset). */
 int shm_index_build(const shm_sketcher *sk, const shm_segment *segs, int32_t n_segments,

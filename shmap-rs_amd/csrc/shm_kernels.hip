@@ -2537,18 +2537,22 @@ static uint32_t resident_wgs(const void *fn, size_t lds) {
         int dev;
         uint32_t n;
     };
+    // a fixed table (no allocation on any launch path); past its end the
+    // count is recomputed each time
+    constexpr int CACHE = 256;
     static std::mutex mu;
-    static std::vector<Entry> cache;
+    static Entry cache[CACHE];
+    static int used = 0;
     int dev = 0;
     (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(mu);
-    for (const Entry &e : cache)
-        if (e.fn == fn && e.lds == lds && e.dev == dev) return e.n;
+    for (int i = 0; i < used; i++)
+        if (cache[i].fn == fn && cache[i].lds == lds && cache[i].dev == dev) return cache[i].n;
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WG, lds) != hipSuccess || per_cu < 1) per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
     const uint32_t n = (uint32_t)(per_cu * cus);
-    cache.push_back(Entry{fn, lds, dev, n});
+    if (used < CACHE) cache[used++] = Entry{fn, lds, dev, n};
     return n;
 }
 

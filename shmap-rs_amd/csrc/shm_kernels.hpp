@@ -3,9 +3,18 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
 #include <stdint.h>
 
 #include "../../include/shmap_hip.h"
+
+// FracMinHash parameters (src/sketch.rs:58-132): k, the hash ratio and its
+// keep threshold.
+struct shm_sketcher {
+    int32_t k;
+    double h_frac;
+    uint64_t thres;
+};
 
 namespace shm {
 
@@ -158,6 +167,9 @@ enum Stage {
     ST_FINAL,
     ST_COUNT
 };
+
+// Sets the calling thread's shm_last_error() message (shm_lib.hip); allocates nothing.
+void set_error(const char *fmt, va_list ap);
 
 // host launch wrappers (shm_kernels.hip)
 void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint, uint64_t *part,

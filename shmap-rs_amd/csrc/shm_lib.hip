@@ -29,15 +29,17 @@ using namespace shm;
 // errors
 // ---------------------------------------------------------------------------
 
-static thread_local std::string g_err;
+// The calling thread's last failure message: a fixed buffer, so that setting
+// it allocates nothing (shm_sketch_noalloc reports through it too).
+static thread_local char g_err[1024];
+
+void shm::set_error(const char *fmt, va_list ap) { vsnprintf(g_err, sizeof g_err, fmt, ap); }
 
 static int fail(int code, const char *fmt, ...) {
-    char buf[1024];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    shm::set_error(fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -55,7 +57,7 @@ static int fail(int code, const char *fmt, ...) {
         if (rc_ != SHM_OK) return rc_; \
     } while (0)
 
-extern "C" const char *shm_last_error(void) { return g_err.c_str(); }
+extern "C" const char *shm_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -155,11 +157,6 @@ static uint32_t device_grid(int device) {
 // sketcher
 // ---------------------------------------------------------------------------
 
-struct shm_sketcher {
-    int32_t k;
-    double h_frac;
-    uint64_t thres;
-};
 
 extern "C" int shm_sketcher_create(int32_t k, double h_frac, shm_sketcher **out) {
     if (!out) return fail(SHM_ERR_ARG, "null output");
@@ -1574,7 +1571,7 @@ static int map_isolating(shm_mapper *mp, const shm_read *reads, const int32_t *i
             for (int32_t j = 0; j < n; j++) out[ids[j]] = tmp[j];
         return SHM_OK;
     }
-    const std::string msg = g_err;
+    const std::string msg = g_err;  // a failed batch: this path may allocate
     if (rc == SHM_ERR_NOMEM) {
         (void)hipStreamSynchronize(mp->st);
         (void)hipGetLastError();
@@ -1593,7 +1590,7 @@ static int map_isolating(shm_mapper *mp, const shm_read *reads, const int32_t *i
         return map_isolating(mp, reads, ids + h, n - h, out, rf);
     }
     for (int32_t j = 0; j < n; j++) failed_record(out[ids[j]], reads[ids[j]].len, rc);
-    g_err = msg;
+    snprintf(g_err, sizeof g_err, "%s", msg.c_str());
     return rc;
 }
 

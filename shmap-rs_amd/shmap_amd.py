@@ -102,6 +102,10 @@ def lib():
         L.shm_sketcher_free.argtypes = [C.c_void_p]
         L.shm_sketch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64,
                                  C.POINTER(C.c_int64)]
+        L.shm_sketch_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64,
+                                         C.POINTER(C.c_int64), C.c_void_p]
+        L.shm_sketch_noalloc_bytes.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
+        L.shm_sketch_noalloc_bytes.restype = C.c_int64
         L.shm_index_build.argtypes = [C.c_void_p, C.POINTER(Segment), C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_void_p)]
         L.shm_index_free.argtypes = [C.c_void_p]
