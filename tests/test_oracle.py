@@ -460,3 +460,137 @@ def test_seeded_buckets_and_seed_counters_against_brute_force():
         assert rec["seeded_buckets"] == len(buckets)
         assert rec["seed_matches"] == sum(idx.count(h) for h in consumed)
         assert rec["max_seed_matches"] == max([idx.count(h) for h in consumed] + [0])
+
+
+# ---- match_rest's decisions against brute force (tests/brute_map.py) ---------
+
+def _dup_reference():
+    """Two copies of a 12 kb block (0.2% substitutions apart), a tandem array
+    of a 171 bp unit, and unique sequence: reads from the block have a
+    competing second placement, reads from the array many equal buckets."""
+    rng = np.random.default_rng(123)
+    block = bytearray(xorshift_dna(31, 12_000))
+    copy = bytearray(block)
+    for j in rng.integers(0, len(copy), 24):
+        copy[j] = b"ACGT"[(b"ACGT".index(copy[j]) + 1 + int(rng.integers(0, 3))) % 4]
+    unit = xorshift_dna(32, 171)
+    array = bytearray(unit * 60)
+    for j in rng.integers(0, len(array), 150):
+        array[j] = b"ACGT"[(b"ACGT".index(array[j]) + 1) % 4]
+    chr1 = xorshift_dna(33, 9_000) + bytes(block) + xorshift_dna(34, 7_000) + bytes(array)
+    chr2 = xorshift_dna(35, 5_000) + bytes(copy) + xorshift_dna(36, 6_000)
+    return [("chr1", chr1), ("chr2", chr2)]
+
+
+def _dup_reads(segs, n=36, seed=9):
+    rng = np.random.default_rng(seed)
+    reads = []
+    for i in range(n):
+        sid = int(rng.integers(0, 2))
+        s = segs[sid][1]
+        ln = int(rng.integers(800, 3200))
+        at = int(rng.integers(0, len(s) - ln))
+        r = bytearray(s[at:at + ln])
+        for j in rng.integers(0, len(r), int(ln * 0.004)):
+            r[j] = b"ACGT"[(b"ACGT".index(r[j]) + 1) % 4]
+        if i % 3 == 0:  # reverse complement
+            r = bytearray(r[::-1].translate(bytes.maketrans(b"ACGT", b"TGCA")))
+        reads.append(bytes(r))
+    return reads
+
+
+_BRUTE_FIELDS = ["b_segm", "b_b", "J", "intersection", "same_strand", "t_l", "t_r", "sh", "b2_segm", "b2_b",
+                 "intersection2", "J2", "sigmas", "mapq"]
+
+
+@pytest.mark.parametrize("metric", ["Containment", "Jaccard", "bucket_SH", "bucket_LCS"])
+@pytest.mark.parametrize("variant", ["default", "P", "B"])
+def test_match_rest_decisions_against_brute_force(metric, variant):
+    """The best and second-best buckets, their scores, sigma, MAPQ, the
+    printed J2 and sh, final_buckets, refined_buckets and refine_memo_hits of
+    every read, against brute_map's closed forms (src/shmap/scoring.rs:367-489,
+    src/shmap/mod.rs:809-861, src/mapping.rs:331-361)."""
+    import brute_map as B
+    segs = _dup_reference()
+    reads = _dup_reads(segs)
+    k, hf = 15, 0.1
+    prm = dict(theta=0.3, min_diff=0.075, max_overlap=0.3)
+    bi = B.BruteIndex(k, hf, segs)
+    oi = O.OracleIndex(k, hf, segs)
+    p = O.params(k=k, h_frac=hf, metric=metric, no_bucket_pruning=variant == "P", one_sweep=variant == "B", **prm)
+    res = oi.map_reads(p, reads)
+    n_second = n_mapq0 = 0
+    for i, (read, rec) in enumerate(zip(reads, res)):
+        want = B.brute_map(bi, k, hf, read, metric=metric, nbp=variant == "P", one_sweep=variant == "B", **prm)
+        assert rec["mapped"] == want["mapped"], i
+        for f in ("final_buckets", "refined_buckets", "refine_memo_hits", "seeded_buckets", "S"):
+            assert rec[f] == want[f], (i, f, rec[f], want[f])
+        if not want["mapped"]:
+            continue
+        for f in _BRUTE_FIELDS:
+            assert rec[f] == want[f], (i, f, rec[f], want[f])
+        n_second += want["b2_segm"] >= 0
+        n_mapq0 += want["mapq"] == 0
+    assert res["mapped"].sum() >= 20
+    if metric != "bucket_SH" or variant != "P":  # under -P every bucket_SH score is 1.0
+        assert n_second >= 5, n_second  # the second pass is exercised
+    assert n_mapq0 >= 3, n_mapq0
+
+
+@pytest.mark.parametrize("metric", ["Containment", "Jaccard", "bucket_LCS"])
+def test_pruning_is_sound(metric):
+    """-P scores every seeded bucket; the default prunes by the seed
+    heuristic.  Pruning only drops buckets that cannot win, so both give the
+    same best and second-best mappings (src/shmap/pruning.rs:89-119), and the
+    heuristic bounds every score: sh >= J."""
+    segs = _dup_reference()
+    reads = _dup_reads(segs, n=60, seed=17)
+    k, hf = 15, 0.1
+    oi = O.OracleIndex(k, hf, segs)
+    prm = dict(k=k, h_frac=hf, theta=0.3, min_diff=0.075, max_overlap=0.3, metric=metric)
+    a = oi.map_reads(O.params(**prm), reads)
+    b = oi.map_reads(O.params(no_bucket_pruning=True, **prm), reads)
+    assert np.array_equal(a["mapped"], b["mapped"]) and a["mapped"].sum() >= 30
+    mk = a["mapped"] == 1
+    fields = ["b_segm", "b_b", "J", "b2_segm", "b2_b", "J2", "mapq"]
+    if metric != "bucket_LCS":  # LCS reports its bucket's content, which -P leaves unextended
+        fields += ["intersection", "intersection2", "t_l", "t_r", "same_strand", "sigmas"]
+    for f in fields:
+        assert np.array_equal(a[f][mk], b[f][mk]), f
+    # sh = 1 - (m - matches)/m and J = I/m with I <= matches: sh >= J exactly,
+    # up to the last bit of the two roundings when I == matches
+    assert np.all(a["sh"][mk] >= a["J"][mk] - 1e-12)
+    assert np.any(a["sh"][mk] > a["J"][mk] + 1e-3)
+    assert np.all(b["sh"][mk] == 1.0)
+    assert np.all(a["refined_buckets"] <= b["refined_buckets"])
+
+
+def test_sigma_and_mapq_at_hand_computed_points():
+    """src/mapping.rs:331-361: sigma = |I2 - I| / sqrt(I2 + I) with -1 read
+    as 0; J2 below theta2 is raised to theta2 and printed so; MAPQ is 60 (5
+    when |same_strand| < I/2, integer division) only when 1 - J2/J > d
+    strictly."""
+    q = O.lib().ok_calc_mapq
+    j2 = C.c_double()
+    # frac == d exactly: 1 - 0.5/1.0 == 0.5 in binary, not > 0.5
+    assert q(1.0, 0.5, 10, 10, 0.1, 0.5, C.byref(j2)) == 0 and j2.value == 0.5
+    assert q(1.0, 0.5, 10, 10, 0.1, 0.4999999, C.byref(j2)) == 60
+    # J2 below theta2: printed J2 is theta2, and frac uses it
+    assert q(0.8, 0.1, 10, 10, 0.4, 0.075, C.byref(j2)) == 60 and j2.value == 0.4
+    assert q(0.42, 0.1, 10, 10, 0.4, 0.075, C.byref(j2)) == 0 and j2.value == 0.4  # 1 - 0.4/0.42 < 0.075
+    # strand agreement: |same| < I/2 with I/2 truncated
+    assert q(0.9, -1.0, 2, 5, 0.3, 0.075, C.byref(j2)) == 60   # 2 < 5/2 == 2 is false
+    assert q(0.9, -1.0, -1, 5, 0.3, 0.075, C.byref(j2)) == 5
+    assert q(0.9, -1.0, -3, 7, 0.3, 0.075, C.byref(j2)) == 60
+    # sigma through whole reads: equal intersections give 0, a missing second best -1
+    import brute_map as B
+    segs = _dup_reference()
+    reads = _dup_reads(segs)
+    oi = O.OracleIndex(15, 0.1, segs)
+    res = oi.map_reads(O.params(k=15, h_frac=0.1, theta=0.3, min_diff=0.075, max_overlap=0.3), reads)
+    for rec in res[res["mapped"] == 1]:
+        I, I2 = int(rec["intersection"]), int(rec["intersection2"])
+        if rec["b2_segm"] < 0:
+            assert rec["sigmas"] == -1.0 and I2 == -1
+        else:
+            assert rec["sigmas"] == abs(I2 - I) / np.sqrt(I2 + I)
