@@ -2049,37 +2049,64 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
 // find_best_mapping (src/shmap/scoring.rs:296-359) for every bucket a
 // refinement round selected, one workgroup per bucket.
 //
-// The span's matched reference k-mers come from the read's own seeds: every
-// hit of every seed inside [l0, end) of the segment marks its position with
-// the seed (a position has one hash, so at most one seed).
+// The span [l0, end) of the segment's sketch is read in order, as
+// best_fixed_length and lcs_get_ppos_in_t do (:114-135, :189-291): each
+// reference k-mer is matched by looking its hash up among the read's seeds
+// (hash-ordered; binary search, the hashes staged in LDS when they fit), so a
+// task reads its span once, coalesced, whatever the read's seed count, and a
+// hash the index dropped under -M still matches, as p_ht.get does.
 //
-// Containment/Jaccard (best_fixed_length, :189-291) run in parallel: window l
-// covers [l, min(l+m, n)); a matched position x of seed s counts in window l
-// exactly when fewer than occs(s) occurrences of s lie in [l, x), i.e. when the
+// Containment/Jaccard (best_fixed_length) run in parallel: window l covers
+// [l, min(l+m, n)); a matched position x of seed s counts in window l exactly
+// when fewer than occs(s) occurrences of s lie in [l, x), i.e. when the
 // occs(s)-th previous occurrence P(x) is before l.  So x adds 1 to every l in
 // [max(P(x)+1, x-m+1, 0), x] and its strand agreement to every l in
-// [max(x-m+1, 0), x]; two difference arrays and scans give intersection(l) and
-// same_strand(l), and the first l of the greatest score wins, as the strict
-// `score > best` of the sequential sweep picks.  A seed's hits inside the span
-// are a position-ordered run of the index, so P(x) of its k-th hit there is its
-// (k - occs)-th: no sort is needed.  The two arrays live in LDS when the span
-// fits and in the workgroup's global scratch otherwise.  With abs_pos the
-// window edge depends on positions, so that mode keeps the sequential sweep;
-// so does LCS (:114-170), whose patience LIS is sequential by nature.
+// [max(x-m+1, 0), x].  Both difference arrays share one int32 per position
+// (intersection * 2^16 + strand agreement, |agreement| <= m < 2^15), so one
+// scan gives both, and the first l of the greatest score wins, as the strict
+// `score > best` of the sequential sweep picks.  P(x) comes from the index: the
+// seed's hits are a position-ordered run, so P(x) is the hit occs(s) before x's
+// own (found by binary search, and only for seeds with more hits than occs).
+// A seed the index dropped (-M) has no run: such a task, and abs_pos (whose
+// window edge depends on positions), LCS (whose patience LIS is sequential by
+// nature) and reads with m >= 2^15, take the sequential sweep.  Arrays live in
+// LDS while they fit, else in the workgroup's global scratch.
 // LCS: the bucket_LCS instance (its patience sweep needs registers the other
 // metrics' instance is better off without).
+constexpr uint32_t REFINE_LDS_BYTES = 60 * 1024;
+
+// Takes LDS bytes (8-aligned) while they last; nullptr once they do not fit.
+struct LdsCarve {
+    uint8_t *base;
+    uint32_t used;
+    __device__ void *take(uint64_t bytes) {
+        const uint64_t b = (bytes + 7) & ~7ull;
+        if ((uint64_t)used + b > REFINE_LDS_BYTES) return nullptr;
+        void *p = base + used;
+        used += (uint32_t)b;
+        return p;
+    }
+};
+
+// Index of hash h among the u hash-ordered seed hashes sh, or -1.
+__device__ __forceinline__ int32_t seed_find(const uint64_t *sh, int32_t u, uint64_t h) {
+    int32_t lo = 0, hi = u;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sh[mid] < h) lo = mid + 1; else hi = mid;
+    }
+    return (lo < u && sh[lo] == h) ? lo : -1;
+}
+
 template <bool LCS>
 __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
     extern __shared__ uint8_t lds[];
-    int32_t *l_sid = (int32_t *)lds;           // REFINE_CAP
-    int32_t *l_b = l_sid + REFINE_CAP;         // REFINE_CAP (+1)
-    int32_t *l_c = l_b + REFINE_CAP + 1;       // REFINE_CAP + 1
     __shared__ uint32_t tmp[NWAVES + 1];
-    __shared__ uint32_t s_pre[WG], s_lo[WG], s_occ[WG];
-    __shared__ uint8_t s_str[WG];
     __shared__ double s_bj[NWAVES];
     __shared__ int32_t s_bl[NWAVES];
-    __shared__ int32_t s_lcs;
+    __shared__ int32_t s_lcs, s_dropped;
+    __shared__ uint32_t s_pre[WG], s_lo[WG], s_occ[WG];
+    __shared__ uint8_t s_str[WG];
     const int ap = A.abs_pos;
     // Rounds (see the task-list comment of MapArgs):
     //   0  each read's first first-pass candidate in sorted order -- the first
@@ -2125,89 +2152,151 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         const int64_t l0 = bb > 0 ? bb : 0;
         const int64_t end = be < tlen ? be : tlen;
         const int64_t n = end > l0 ? end - l0 : 0;
-        const bool fits = n <= REFINE_CAP;
-        const bool parallel = n > 0 && !ap && !LCS;
-        // LDS when the span fits, else this workgroup's global scratch
         int32_t *const wsid = A.g_sid + (uint64_t)blockIdx.x * A.sid_stride;
         int32_t *const wtr = A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
-        int32_t *const cdf = fits ? l_c : wsid;   // parallel: intersection difference array (n+1)
-        int32_t *const sdf = fits ? l_b : wtr;    // parallel: strand difference array (n+1)
-        int32_t *const sid = fits ? l_sid : wsid; // sequential: seed of each position (n)
-        int32_t *const keys = fits ? l_b : wtr;   // sequential: t_r of each position (n)
-        // 1. every matched position of the span, WG seeds at a time: each thread
-        //    finds its seed's hit range inside the span, a scan lays the ranges
-        //    end to end, and the whole workgroup visits the hits, so a repeat
-        //    k-mer with many hits in the span is not left to one thread.
-        //    A seed's hits in the span are a position-ordered run of the index,
-        //    so the occs(s)-th previous occurrence P(x) of hit j is hit j - occs.
+        bool parallel = n > 0 && !ap && !LCS && m < 32768;
         if (parallel) {
-            for (int64_t x = threadIdx.x; x <= n; x += WG) {
-                cdf[x] = 0;
-                sdf[x] = 0;
-            }
-        } else {
-            for (int64_t x = threadIdx.x; x < n; x += WG) sid[x] = -1;
-        }
-        __syncthreads();
-        for (int32_t s0 = 0; s0 < u; s0 += WG) {
-            const int32_t sd = s0 + (int32_t)threadIdx.x;
-            uint32_t lo = 0, nin = 0, sstr = 0, occ = 0;
-            if (sd < u) {
-                const uint32_t cnt = A.sd_cnt[o + sd];
-                if (cnt) {
-                    const uint32_t hoff = A.sd_hoff[o + sd];
-                    lo = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0);
-                    nin = hits_lower_bound(I, lo, hoff + cnt, seg, l0 + n, 0) - lo;
-                    sstr = A.sd_strand[o + sd];
-                    occ = A.sd_occs[o + sd];
+            LdsCarve C{lds, 0};
+            int32_t *pk = (int32_t *)C.take((uint64_t)(n + 1) * 4);
+            if (!pk) pk = wsid;  // sid_stride >= 2 * hl + 2 > n
+            // seeds whose P(x) needs their hit run (more hits than occs), then
+            // the list of them; in global scratch when LDS is short
+            uint32_t *need = (uint32_t *)C.take((uint64_t)u * 4);
+            if (!need) need = (uint32_t *)(A.g_diff + (uint64_t)blockIdx.x * A.diff_stride);
+            // each seed's hash, and its kind: strand (bit 0), more hits than
+            // occs (bit 1), dropped from the index (bit 2)
+            uint64_t *s_h = (uint64_t *)C.take((uint64_t)u * 8);
+            uint8_t *s_kind = (uint8_t *)C.take((uint64_t)u);
+            for (int64_t x = threadIdx.x; x <= n; x += WG) pk[x] = 0;
+            for (int32_t i = threadIdx.x; i < u; i += WG) {
+                need[i] = 0;
+                if (s_h) s_h[i] = A.sd_h[o + i];
+                if (s_kind) {
+                    const uint32_t cnt = A.sd_cnt[o + i], occs = A.sd_occs[o + i];
+                    s_kind[i] = (uint8_t)(A.sd_strand[o + i] | (cnt > occs ? 2u : 0u) | (cnt == 0 ? 4u : 0u));
                 }
             }
-            uint32_t tot;
-            const uint32_t ex = wg_excl_scan(nin, tmp, &tot);  // ends with a barrier
-            s_pre[threadIdx.x] = ex;
-            s_lo[threadIdx.x] = lo;
-            s_occ[threadIdx.x] = occ;
-            s_str[threadIdx.x] = (uint8_t)sstr;
+            if (threadIdx.x == 0) s_dropped = 0;
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < tot; i += WG) {
-                // the last seed slot whose range starts at or before i
-                uint32_t a = 0, b2 = WG;
-                while (b2 - a > 1) {
-                    const uint32_t mid = (a + b2) >> 1;
-                    if (s_pre[mid] <= i) a = mid; else b2 = mid;
-                }
-                const uint32_t kk = i - s_pre[a];
-                const uint32_t j = s_lo[a] + kk;
-                const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
-                const bool same = s_str[a] == (I.hit_r[j] >> 31);
-                if (parallel) {
-                    // x counts in windows [max(P(x)+1, x-m+1, 0), x]; its strand
-                    // agreement in [max(x-m+1, 0), x]
-                    const uint32_t oc = s_occ[a];
-                    const int64_t P = kk >= oc ? (int64_t)I.hit_g[j - oc] - ks - l0 : -1;
-                    int64_t lw = x - m + 1;
-                    if (lw < 0) lw = 0;
-                    const int64_t lc = P + 1 > lw ? P + 1 : lw;
-                    atomicAdd(&cdf[lc], 1);
-                    atomicAdd(&cdf[x + 1], -1);
-                    const int32_t cd = same ? 1 : -1;
-                    atomicAdd(&sdf[lw], cd);
-                    atomicAdd(&sdf[x + 1], -cd);
+            const uint64_t *seeds = s_h ? s_h : A.sd_h + o;
+            // x counts in windows [max(P(x)+1, x-m+1, 0), x]; its strand
+            // agreement in [max(x-m+1, 0), x]
+            auto put = [&](int64_t x, int64_t P, int32_t cd) {
+                int64_t lw = x - m + 1;
+                if (lw < 0) lw = 0;
+                const int64_t lc = P + 1 > lw ? P + 1 : lw;
+                if (lc == lw) {
+                    atomicAdd(&pk[lw], 65536 + cd);
                 } else {
-                    sid[x] = (s0 + (int32_t)a) | (same ? 0 : 0x40000000);
+                    atomicAdd(&pk[lw], cd);
+                    atomicAdd(&pk[lc], 65536);
+                }
+                atomicAdd(&pk[x + 1], -65536 - cd);
+            };
+            // a. the span in order: positions of seeds with at most occs hits
+            //    (P(x) = -1) are placed at once, the others' seeds marked
+            // (four positions a thread at a time, their loads in flight together)
+            constexpr int PF = 4;
+            for (int64_t x0 = 0; x0 < n; x0 += (int64_t)PF * WG) {
+                uint64_t h[PF];
+                uint32_t ts[PF];
+#pragma unroll
+                for (int k = 0; k < PF; k++) {
+                    const int64_t x = x0 + k * WG + threadIdx.x;
+                    h[k] = x < n ? I.t_h[ks + l0 + x] : 0ull;
+                    ts[k] = x < n ? (uint32_t)I.t_s[ks + l0 + x] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < PF; k++) {
+                    const int64_t x = x0 + k * WG + threadIdx.x;
+                    if (x >= n) continue;
+                    const int32_t s = seed_find(seeds, u, h[k]);
+                    if (s < 0) continue;
+                    uint32_t kind;
+                    if (s_kind) {
+                        kind = s_kind[s];
+                    } else {
+                        const uint32_t cnt = A.sd_cnt[o + s], occs = A.sd_occs[o + s];
+                        kind = A.sd_strand[o + s] | (cnt > occs ? 2u : 0u) | (cnt == 0 ? 4u : 0u);
+                    }
+                    if (kind & 4u) {  // dropped from the index (-M): no hit run
+                        s_dropped = 1;
+                    } else if (kind & 2u) {
+                        atomicAdd(&need[s], 1u);  // its positions in the span
+                    } else {
+                        put(x, -1, ts[k] == (kind & 1u) ? 1 : -1);
+                    }
                 }
             }
             __syncthreads();
+            parallel = s_dropped == 0;
+            if (parallel) {
+                // b. the marked seeds, WG at a time: each thread finds where its
+                //    seed's hits enter the span (it knows how many lie there),
+                //    a scan lays the ranges end to end and the workgroup visits
+                //    the hits, so a repeat k-mer with many hits in the span is
+                //    not left to one thread.  A seed's hits are a
+                //    position-ordered run of the index, so P(x) of hit j is hit
+                //    j - occs.  The list holds (seed << 16 | positions): m and
+                //    so u are below 2^15, a span below 2^16.
+                uint32_t nneed = 0;
+                for (int32_t s0 = 0; s0 < u; s0 += WG) {
+                    const int32_t sd = s0 + (int32_t)threadIdx.x;
+                    const uint32_t c = sd < u ? need[sd] : 0u;
+                    uint32_t tot;
+                    const uint32_t ex = wg_excl_scan(c ? 1u : 0u, tmp, &tot);
+                    if (c) need[nneed + ex] = ((uint32_t)sd << 16) | c;
+                    nneed += tot;
+                    __syncthreads();
+                }
+                for (uint32_t s0 = 0; s0 < nneed; s0 += WG) {
+                    const uint32_t li = s0 + threadIdx.x;
+                    uint32_t lo = 0, nin = 0, sstr = 0, occ = 0;
+                    if (li < nneed) {
+                        const uint32_t e = need[li], sd = e >> 16;
+                        const uint32_t cnt = A.sd_cnt[o + sd], hoff = A.sd_hoff[o + sd];
+                        lo = hits_lower_bound(I, hoff, hoff + cnt, seg, l0, 0);
+                        nin = e & 0xffffu;
+                        sstr = A.sd_strand[o + sd];
+                        occ = A.sd_occs[o + sd];
+                    }
+                    uint32_t tot;
+                    const uint32_t ex = wg_excl_scan(nin, tmp, &tot);  // ends with a barrier
+                    s_pre[threadIdx.x] = ex;
+                    s_lo[threadIdx.x] = lo;
+                    s_occ[threadIdx.x] = occ;
+                    s_str[threadIdx.x] = (uint8_t)sstr;
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < tot; i += WG) {
+                        // the last list slot whose range starts at or before i
+                        uint32_t a = 0, b2 = WG;
+                        while (b2 - a > 1) {
+                            const uint32_t mid = (a + b2) >> 1;
+                            if (s_pre[mid] <= i) a = mid; else b2 = mid;
+                        }
+                        const uint32_t kk = i - s_pre[a];
+                        const uint32_t j = s_lo[a] + kk;
+                        const int64_t x = (int64_t)I.hit_g[j] - ks - l0;
+                        const uint32_t oc = s_occ[a];
+                        const int64_t P = kk >= oc ? (int64_t)I.hit_g[j - oc] - ks - l0 : -1;
+                        put(x, P, s_str[a] == (I.hit_r[j] >> 31) ? 1 : -1);
+                    }
+                    __syncthreads();
+                }
+            }
         }
         if (parallel) {
-            wg_scan_inplace(cdf, n, tmp);
-            wg_scan_inplace(sdf, n, tmp);
-            // first window with the greatest score
+            LdsCarve C{lds, 0};
+            int32_t *pk = (int32_t *)C.take((uint64_t)(n + 1) * 4);
+            if (!pk) pk = wsid;
+            wg_scan_inplace(pk, n, tmp);
+            // first window with the greatest score; pk[l] = I(l) * 2^16 + S(l)
             const bool jac = A.metric == METRIC_JACCARD;
             double bj = -1.0;
             int32_t bl = 0x7fffffff;
             for (int64_t l = threadIdx.x; l < n; l += WG) {
-                const int32_t I2 = cdf[l];
+                const int32_t v = pk[l];
+                const int32_t I2 = (v - (int32_t)(int16_t)(v & 0xffff)) >> 16;
                 const int64_t r = l + m < n ? l + m : n;
                 const double score = jac ? (double)I2 / (double)(m + (int32_t)(r - l) - I2) : (double)I2 / (double)m;
                 if (score > bj) {
@@ -2238,10 +2327,12 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
                     }
                 const int64_t r = L + m < n ? L + m : n;
                 const int32_t tl = I.t_r[ks + l0 + L], tr = I.t_r[ks + l0 + r - 1];
+                const int32_t v = pk[L];
+                const int32_t S = (int32_t)(int16_t)(v & 0xffff);
                 if (J > A.theta) atomicMin(&A.rd_flag[rd], p);
                 A.sc_j[base + p] = J;
-                A.sc_i[base + p] = cdf[L];
-                A.sc_same[base + p] = sdf[L];
+                A.sc_i[base + p] = (v - S) >> 16;
+                A.sc_same[base + p] = S;
                 A.sc_tl[base + p] = tl;
                 A.sc_tr[base + p] = tr;
                 A.sc_ssz[base + p] = tr - tl + 1;
@@ -2251,18 +2342,40 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
             __syncthreads();
             continue;
         }
-        // sequential forms: LCS, abs_pos windows, or spans too large for LDS
-        int32_t *tr_arr = keys;
-        int32_t *diff = u <= REFINE_CAP ? l_c : A.g_diff + (uint64_t)blockIdx.x * A.diff_stride;
+        // sequential forms: LCS, abs_pos windows, seeds dropped by -M, m >= 2^15.
+        // sid[x]: the seed matching span position x (| 0x40000000 when its
+        // strand disagrees), -1 if none.
+        LdsCarve C{lds, 0};
+        int32_t *Ts = LCS ? (int32_t *)C.take((uint64_t)(LCS_TAILS + 1) * 4) : nullptr;  // always fits
+        int32_t *sid = (int32_t *)C.take((uint64_t)n * 4);
+        if (!sid) sid = wsid;
+        int32_t *tr_arr = nullptr, *diff = nullptr;
+        if (!LCS) {
+            tr_arr = (int32_t *)C.take((uint64_t)n * 4);
+            if (!tr_arr) tr_arr = wtr;
+            diff = (int32_t *)C.take((uint64_t)u * 4);
+            if (!diff) diff = A.g_diff + (uint64_t)blockIdx.x * A.diff_stride;
+        }
+        uint64_t *s_h = (uint64_t *)C.take((uint64_t)u * 8);
+        if (s_h)
+            for (int32_t i = threadIdx.x; i < u; i += WG) s_h[i] = A.sd_h[o + i];
+        if (!LCS)
+            for (int32_t x = threadIdx.x; x < u; x += WG) diff[x] = (int32_t)A.sd_occs[o + x];
+        __syncthreads();
+        const uint64_t *seeds = s_h ? s_h : A.sd_h + o;
+        for (int64_t x = threadIdx.x; x < n; x += WG) {
+            const int64_t q = ks + l0 + x;
+            const int32_t s = seed_find(seeds, u, I.t_h[q]);
+            sid[x] = s < 0 ? -1 : (s | ((uint32_t)I.t_s[q] == (uint32_t)A.sd_strand[o + s] ? 0 : 0x40000000));
+            if (!LCS) tr_arr[x] = I.t_r[q];
+        }
+        __syncthreads();
         if constexpr (LCS) {
             // one wave streams the LIS (s_lcs < 0: thread 0 sweeps it below)
             if (threadIdx.x < WAVE) {
-                const int32_t best = wave_lcs(A, sid, n, o, m, l_c);  // l_c: free in this form
+                const int32_t best = wave_lcs(A, sid, n, o, m, Ts);
                 if (threadIdx.x == 0) s_lcs = best;
             }
-        } else {
-            for (int64_t x = threadIdx.x; x < n; x += WG) tr_arr[x] = I.t_r[ks + l0 + x];
-            for (int32_t x = threadIdx.x; x < u; x += WG) diff[x] = (int32_t)A.sd_occs[o + x];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -2680,7 +2793,7 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 }
 
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
-static const size_t REFINE_LDS = ((size_t)REFINE_CAP * 3 + 2) * 4;
+static const size_t REFINE_LDS = REFINE_LDS_BYTES;
 static const size_t SEED_WAVE_LDS = (size_t)NWAVES * WAVE_SEED_LDS;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
@@ -2733,6 +2846,42 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
 __global__ void __launch_bounds__(WG) k_lookup(DevIndex I, const uint64_t *hs, uint32_t n, uint32_t *cnt,
                                                uint32_t *off) {
     for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) cnt[i] = index_lookup(I, hs[i], off[i]);
+}
+
+// 2-bit packed bases (base i at bits 2*(i%4) of byte i/4; A C G T = 0 1 2 3)
+// to ASCII letters, sixteen a thread, then the runs of other bytes (sorted,
+// disjoint [start, end) pairs) written as 'N': every non-ACGT byte has the
+// ntHash seed 0 (src/sketch.rs:73-88), so 'N' stands for any of them.
+__global__ void __launch_bounds__(WG) k_unpack_bases(const uint32_t *packed, uint64_t nbases, uint8_t *out) {
+    const uint64_t nq = (nbases + 15) / 16;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * WG) {
+        const uint32_t v = packed[i];
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            // four 2-bit codes to four bytes, then the letter of each by a byte permute
+            const uint32_t c = (v >> (8 * j)) & 0xffu;
+            const uint32_t b = (c & 3u) | ((c & 0x0cu) << 6) | ((c & 0x30u) << 12) | ((c & 0xc0u) << 18);
+            w[j] = __builtin_amdgcn_perm(0u, 0x54474341u, b);  // "ACGT"[code]
+        }
+        if ((i + 1) * 16 <= nbases) {
+            *(uint4 *)(out + i * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            for (uint64_t q = i * 16; q < nbases; q++) out[q] = (uint8_t)(w[(q - i * 16) / 4] >> (8 * (q % 4)));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_apply_exceptions(const int64_t *exc, int64_t n_exc, uint8_t *out) {
+    for (int64_t r = blockIdx.x; r < n_exc; r += gridDim.x)
+        for (int64_t q = exc[2 * r] + threadIdx.x; q < exc[2 * r + 1]; q += WG) out[q] = 'N';
+}
+
+void launch_unpack_bases(const uint8_t *packed, uint64_t nbases, const int64_t *exc, int64_t n_exc, uint8_t *out,
+                         hipStream_t st) {
+    if (nbases > 0)
+        k_unpack_bases<<<flat_grid((nbases + 15) / 16, 8192), WG, 0, st>>>((const uint32_t *)packed, nbases, out);
+    if (n_exc > 0) k_apply_exceptions<<<(uint32_t)(n_exc < 4096 ? n_exc : 4096), WG, 0, st>>>(exc, n_exc, out);
 }
 
 void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,

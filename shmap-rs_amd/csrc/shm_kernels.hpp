@@ -22,7 +22,6 @@ constexpr int LANE_W = 128;               // consecutive windows rolled by one l
 constexpr int RS_ITEMS = 16;              // radix sort keys per thread per tile
 constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
-constexpr int32_t REFINE_CAP = 4096;      // int32 entries per LDS array in refine
 constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
@@ -215,6 +214,11 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
                   uint32_t max_grid, hipStream_t st);
 void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,
                    hipStream_t st);
+// 2-bit packed bases to ASCII letters in out[0, nbases), then the exception runs
+// (n_exc sorted [start, end) pairs of device int64) written as 'N'.  packed is
+// 4-byte aligned and readable up to a whole dword past the last base.
+void launch_unpack_bases(const uint8_t *packed, uint64_t nbases, const int64_t *exc, int64_t n_exc, uint8_t *out,
+                         hipStream_t st);
 // Counts into *bad (zeroed by the caller) the hits whose position or strand is
 // not their k-mer's and the slot runs not strictly increasing in (segment,
 // k-mer id); for an index read from a file, after the host's bounds checks.

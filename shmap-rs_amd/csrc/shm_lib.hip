@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/shmap_hip.h"
@@ -934,6 +935,8 @@ struct shm_mapper {
     DevBuf<int64_t> roff;
     PinnedBuf<uint8_t> h_bases;  // upload staging of shm_map_batch
     PinnedBuf<int64_t> h_roff;
+    DevBuf<uint32_t> packed;  // shm_map_batch_packed: the 2-bit bases,
+    DevBuf<int64_t> exc;      // and their runs of other bytes
     SketchOut so;
     DevBuf<uint64_t> k_h, ps_h, sd_h, sd_key;
     DevBuf<int32_t> k_r;
@@ -967,7 +970,7 @@ struct shm_mapper {
     // Frees every batch buffer (after a failed allocation, so that a retry
     // with fewer reads starts from the device memory a smaller batch needs).
     void release_batch_buffers() {
-        bases.release(), roff.release(), h_bases.release();
+        bases.release(), roff.release(), h_bases.release(), packed.release(), exc.release();
         so.nunit.release(), so.unit_start.release(), so.chunk_cnt.release(), so.chunk_koff.release();
         so.koff.release(), so.lane_cnt.release(), so.lane_pre.release(), so.unit_seq.release();
         so.ovf.release(), so.stage.release();
@@ -1411,14 +1414,14 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->bs_alive.ensure(B + 1));
         RCHK(mp->bs_sh.ensure(B + 1));
         RCHK(mp->sc_j.ensure(B + 1));
-        // per-workgroup scratch, used only by spans / reads too large for LDS
+        // per-workgroup scratch of refinement, for the arrays of a span or read
+        // that do not fit the workgroup's LDS (a span has at most 2 * hl
+        // positions; the grid is max_grid workgroups)
         const uint64_t stride = 2ull * (uint64_t)max_hl + 2;
         const uint64_t dstride = max_m + 1;
-        const bool big_span = 2ull * (uint64_t)max_hl > (uint64_t)REFINE_CAP;
-        const bool big_read = max_m > (uint64_t)REFINE_CAP;
-        RCHK(mp->g_sid.ensure(big_span ? stride * mp->max_grid + 1 : 1));
-        RCHK(mp->g_tr.ensure(big_span ? stride * mp->max_grid + 1 : 1));
-        RCHK(mp->g_diff.ensure(big_read ? dstride * mp->max_grid + 1 : 1));
+        RCHK(mp->g_sid.ensure(stride * mp->max_grid + 1));
+        RCHK(mp->g_tr.ensure(stride * mp->max_grid + 1));
+        RCHK(mp->g_diff.ensure(dstride * mp->max_grid + 1));
         RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
         RCHK(mp->tasks.ensure(B + 1));
         A.ext_count = mp->ext.p;
@@ -1625,6 +1628,141 @@ extern "C" int shm_mapper_set_read_limit(shm_mapper *mp, int32_t max_read_len) {
     if (!mp || max_read_len < 0) return fail(SHM_ERR_ARG, "invalid argument");
     mp->max_read_len = max_read_len;
     return SHM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// 2-bit packed reads
+// ---------------------------------------------------------------------------
+
+// Packs reads [r0, r1) of a batch (bases at offsets off[]) into `packed`, and
+// lists their runs of non-ACGT bytes.  Bytes shared with the neighbouring
+// ranges (a read boundary inside a byte) are or-ed atomically; `packed` starts
+// zeroed.
+static int64_t pack_range(const shm_read *reads, const int64_t *off, int32_t r0, int32_t r1, uint8_t *packed,
+                          std::vector<int64_t> &exc) {
+    static const int8_t code[256] = {
+#define X4(a) a, a, a, a
+#define X16(a) X4(a), X4(a), X4(a), X4(a)
+        X16(-1), X16(-1), X16(-1), X16(-1),
+        -1, 0, -1, 1, -1, -1, -1, 2, -1, -1, -1, -1, -1, -1, -1, -1,  // @ A B C D E F G
+        -1, -1, -1, -1, 3, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,  // T
+        -1, 0, -1, 1, -1, -1, -1, 2, -1, -1, -1, -1, -1, -1, -1, -1,  // ` a b c d e f g
+        -1, -1, -1, -1, 3, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,  // t
+        X16(-1), X16(-1), X16(-1), X16(-1), X16(-1), X16(-1), X16(-1), X16(-1)
+#undef X16
+#undef X4
+    };
+    const int64_t lo = off[r0], hi = off[r1];
+    int64_t run = -1;  // start of the open exception run
+    for (int32_t r = r0; r < r1; r++) {
+        const uint8_t *s = reads[r].seq;
+        const int64_t at = off[r];
+        for (int64_t i = 0; i < reads[r].len; i++) {
+            const int64_t q = at + i;
+            const int c = code[s[i]];
+            if (c < 0) {
+                if (run < 0) run = q;
+            } else {
+                if (run >= 0) {
+                    exc.push_back(run);
+                    exc.push_back(q);
+                    run = -1;
+                }
+                const uint8_t bits = (uint8_t)(c << (2 * (q & 3)));
+                if (bits == 0) continue;
+                uint8_t *b = packed + (q >> 2);
+                const bool shared = (q >> 2) == (lo >> 2) || (q >> 2) == ((hi - 1) >> 2);
+                if (shared) __atomic_fetch_or(b, bits, __ATOMIC_RELAXED);
+                else *b |= bits;
+            }
+        }
+        if (run >= 0) {  // runs end at read boundaries
+            exc.push_back(run);
+            exc.push_back(at + reads[r].len);
+            run = -1;
+        }
+    }
+    return (int64_t)exc.size() / 2;
+}
+
+extern "C" int shm_pack_reads(const shm_read *reads, int32_t n, uint8_t *packed, int64_t packed_cap, int64_t *off,
+                              int64_t *exc, int64_t exc_cap, int64_t *n_exc, int32_t threads) {
+    if (n < 0 || (n > 0 && (!reads || !off)) || !n_exc || exc_cap < 0 || (exc_cap > 0 && !exc))
+        return fail(SHM_ERR_ARG, "invalid argument");
+    off[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        if (reads[i].len < 0 || (reads[i].len > 0 && !reads[i].seq)) return fail(SHM_ERR_ARG, "read %d invalid", i);
+        off[i + 1] = off[i] + reads[i].len;
+    }
+    const int64_t nbytes = (off[n] + 3) / 4;
+    if (packed_cap < nbytes)
+        return fail(SHM_ERR_CAPACITY, "%lld bases need %lld packed bytes, the buffer holds %lld", (long long)off[n],
+                    (long long)nbytes, (long long)packed_cap);
+    if (nbytes > 0 && !packed) return fail(SHM_ERR_ARG, "invalid argument");
+    memset(packed, 0, (size_t)nbytes);
+    int32_t nt = threads > 0 ? threads : 1;
+    if (nt > 64) nt = 64;
+    if (nt > n) nt = n > 0 ? n : 1;
+    std::vector<std::vector<int64_t>> part(nt);
+    std::vector<std::thread> th;
+    for (int32_t t = 0; t < nt; t++) {
+        // read ranges of about equal bases
+        auto bound = [&](int32_t tt) {
+            if (tt == 0) return (int32_t)0;
+            if (tt == nt) return n;
+            return (int32_t)(std::upper_bound(off, off + n + 1, off[n] * tt / nt) - off - 1);
+        };
+        const int32_t a = bound(t), b = bound(t + 1);
+        th.emplace_back([&, a, b, t] { if (a < b) pack_range(reads, off, a, b, packed, part[t]); });
+    }
+    for (std::thread &x : th) x.join();
+    int64_t total = 0;
+    for (const auto &p : part) total += (int64_t)p.size() / 2;
+    *n_exc = total;
+    if (total > exc_cap)
+        return fail(SHM_ERR_CAPACITY, "%lld runs of non-ACGT bytes, the list holds %lld", (long long)total,
+                    (long long)exc_cap);
+    int64_t k = 0;
+    for (const auto &p : part)
+        for (int64_t v : p) exc[k++] = v;
+    return SHM_OK;
+}
+
+extern "C" int shm_map_batch_packed(shm_mapper *mp, const uint8_t *packed, const int64_t *off, int32_t n,
+                                    const int64_t *exc, int64_t n_exc, shm_read_result *out) {
+    if (!mp || n < 0 || (n > 0 && (!off || !out)) || n_exc < 0 || (n_exc > 0 && !exc))
+        return fail(SHM_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(mp->idx->device));
+    const int64_t nb = n > 0 ? off[n] : 0;
+    if (nb < 0 || (nb > 0 && !packed)) return fail(SHM_ERR_ARG, "invalid argument");
+    for (int32_t i = 0; i < n; i++)
+        if (off[i + 1] < off[i] || off[i + 1] - off[i] > INT32_MAX) return fail(SHM_ERR_ARG, "read %d invalid", i);
+    for (int64_t r = 0; r < n_exc; r++)
+        if (exc[2 * r] < (r ? exc[2 * r - 1] : 0) || exc[2 * r + 1] < exc[2 * r] || exc[2 * r + 1] > nb)
+            return fail(SHM_ERR_ARG, "exception runs must be sorted, disjoint and inside the bases");
+    // reads over the length limit fail alone: the unpacked bases go through
+    // shm_map_batch, which isolates them
+    bool over = false;
+    for (int32_t i = 0; i < n && !over; i++) over = mp->max_read_len > 0 && off[i + 1] - off[i] > mp->max_read_len;
+    const int64_t nbytes = (nb + 3) / 4;
+    RCHK(mp->packed.ensure((uint64_t)(nbytes + 3) / 4 + 1));
+    RCHK(mp->exc.ensure(2 * (uint64_t)n_exc + 2));
+    RCHK(mp->bases.ensure(nb + 64));
+    RCHK(mp->roff.ensure(n + 1));
+    if (nbytes > 0) HIPCHK(hipMemcpyAsync(mp->packed.p, packed, (size_t)nbytes, hipMemcpyHostToDevice, mp->st));
+    if (n_exc > 0) HIPCHK(hipMemcpyAsync(mp->exc.p, exc, (size_t)n_exc * 16, hipMemcpyHostToDevice, mp->st));
+    HIPCHK(hipMemcpyAsync(mp->roff.p, off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, mp->st));
+    launch_unpack_bases((const uint8_t *)mp->packed.p, (uint64_t)nb, mp->exc.p, n_exc, mp->bases.p, mp->st);
+    HIPCHK(hipGetLastError());
+    if (over) {
+        std::vector<uint8_t> h((size_t)nb + 1);
+        if (nb > 0) HIPCHK(hipMemcpyAsync(h.data(), mp->bases.p, (size_t)nb, hipMemcpyDeviceToHost, mp->st));
+        HIPCHK(hipStreamSynchronize(mp->st));
+        std::vector<shm_read> rs(n);
+        for (int32_t i = 0; i < n; i++) rs[i] = shm_read{h.data() + off[i], (int32_t)(off[i + 1] - off[i])};
+        return shm_map_batch(mp, rs.data(), n, out);
+    }
+    return map_device(mp, mp->bases.p, mp->roff.p, n, out);
 }
 
 extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,

@@ -196,6 +196,28 @@ def test_variants_match_oracle(variant):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("metric", ["Containment", "Jaccard", "bucket_SH", "bucket_LCS"])
+@pytest.mark.parametrize("max_matches", [1, 4])
+def test_max_matches_maps_as_the_oracle(metric, max_matches):
+    """-M erases frequent hashes from the index (src/index.rs:264-291,
+    360-376): their seeds find no hits, but refinement still matches them in
+    the reference sketch through the read's own hash table, as
+    best_fixed_length and lcs_get_ppos_in_t do (src/shmap/scoring.rs:114-135,
+    189-291).  Every field and the PAF text against the oracle."""
+    segs, reads = _workload(**WORKLOADS["repeats"])
+    prm = dict(theta=0.4, min_diff=0.075, max_overlap=0.3, metric=metric)
+    oi, op, ores = P.oracle_map(segs, reads, 15, 0.1, max_matches=max_matches, **prm)
+    gi, gp, mp, gres = P.gpu_map(segs, reads, 15, 0.1, max_matches=max_matches, **prm)
+    bad = P.diff_results(ores, gres)
+    assert not bad, bad
+    assert ores["mapped"].sum() > 0
+    for i in np.nonzero(ores["mapped"] == 1)[0]:
+        assert G.format_paf(gi, gp, f"r{i}", gres[i]) == oi.format_paf(op, f"r{i}", ores[i])
+    # the erasure changes the outcome somewhere, so the case is exercised
+    _, _, plain = P.oracle_map(segs, reads, 15, 0.1, **prm)
+    assert P.diff_results(plain, ores)
+
+
 def test_short_read_floor():
     """tests/short_read_sketch_floor.rs: sketches below MIN_HALFLEN never map."""
     x, seq = 0x2545F4914F6CDD1D, []
