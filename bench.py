@@ -57,6 +57,17 @@ STAGE_KERNEL = {"sketch": "k_sketch<0>(", "sketch_compact": "k_sketch_compact(",
                 "prune": "k_prune(", "refine": "k_refine(", "final": "k_final("}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -311,9 +322,15 @@ def main():
     d_bases = hip.upload(bases)
     d_off = hip.upload(off)
 
-    # batching choices tuned on this batch (shm_mapper_autotune), untimed
+    # batching choices tuned (shm_mapper_autotune, untimed) on a sample of the
+    # same workload drawn with another seed, not on the timed batch
     t_tune = time.time()
-    tuning = mp.tuning() if args.no_autotune else mp.autotune((bases, off))
+    if args.no_autotune:
+        tuning = mp.tuning()
+    else:
+        tb, to, _ = W.reads(args.workload, lens, ref, seed=7, n_reads=min(n_reads, 20_000))
+        tuning = mp.autotune((tb, to))
+        del tb, to
     tune_s = time.time() - t_tune
     log(f"[bench] tuning {tuning} ({'built-in' if args.no_autotune else f'autotuned in {tune_s:.1f}s'})")
 
@@ -446,7 +463,51 @@ def main():
     plain_same = bool(np.array_equal(plain[["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]],
                                      res[:n_plain][["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]]))
 
-    # PCIe-inclusive: reads start in (page-locked) host memory, uploaded inside the step
+    # End to end from host memory: the reads start in page-locked host memory,
+    # 2-bit packed as the reader delivers them (shm_pack_reads; its rate is
+    # reported beside), and every step uploads its batch and maps it; M mappers
+    # take the steps in turn, so one batch's upload overlaps another's kernels.
+    t_pack = time.time()
+    pk = G.PackedReads((bases, off), threads=min(len(os.sched_getaffinity(0)), 16))
+    pack_s = time.time() - t_pack
+    log(f"[bench] packed {pk.bases / 1e9:.3f} Gbp into {len(pk.packed) / 1e9:.3f} GB in {pack_s:.2f}s "
+        f"({pk.n_exc} runs of other bytes)")
+    mps = [mp] + [G.Mapper(idx, prm) for _ in range(M - 1)]
+    for m_ in mps[1:]:
+        m_.set_tuning(**tuning)
+    outs = [G.host_array(n_reads, G.RESULT_DTYPE) for _ in range(M)]
+    for m_, o_ in zip(mps, outs):
+        m_.map_packed(pk, out=o_)  # warm each mapper's buffers
+    assert np.array_equal(outs[0], res), "packed reads map differently"
+    e2e_steps = max(args.steps, M)
+
+    def e2e_worker(wi):
+        for i in range(wi, e2e_steps, M):
+            mps[wi].map_packed(pk, out=outs[wi])
+            log(f"[bench] end-to-end step {i} (mapper {wi}) done")
+
+    hip.sync()
+    tq = time.time()
+    ths = [threading.Thread(target=e2e_worker, args=(wi,)) for wi in range(M)]
+    for t_ in ths:
+        t_.start()
+    for t_ in ths:
+        t_.join()
+    hip.sync()
+    e2e_s = time.time() - tq
+    for o_ in outs:
+        assert np.array_equal(o_, res), "concurrent packed mappers disagree"
+    mps = outs = m_ = o_ = None
+    end_to_end = {"reads_per_s": round(n_reads * e2e_steps / e2e_s, 2),
+                  "ms_per_step": round(e2e_s / e2e_steps * 1e3, 3), "steps": e2e_steps, "mappers": M,
+                  "upload_gb_per_step": round(len(pk.packed) / 1e9, 3),
+                  "gbp_per_s": round(pk.bases * e2e_steps / e2e_s / 1e9, 3),
+                  "pack_gbp_per_s": round(pk.bases / pack_s / 1e9, 3),
+                  "mode": "reads in page-locked host memory, 2-bit packed, uploaded and unpacked inside each step, "
+                          "upload overlapped across mappers"}
+    log(f"[bench] end to end: {end_to_end}")
+    del pk
+    # the same with the reads as bytes (four times the upload), one mapper
     h_bases = G.host_array(len(bases), np.uint8)
     h_bases[:] = bases
     hip.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -459,8 +520,8 @@ def main():
     pcie_ms = (time.time() - tq) / 2 * 1e3
     del h_bases
     pcie = {"reads_per_s": round(n_reads / pcie_ms * 1e3, 2), "ms_per_step": round(pcie_ms, 3),
-            "upload_gb": round(len(bases) / 1e9, 3)}
-    log(f"[bench] PCIe-inclusive: {pcie}")
+            "upload_gb": round(len(bases) / 1e9, 3), "mode": "reads as bytes, one mapper, upload then map"}
+    log(f"[bench] bytes upload: {pcie}")
 
     # whole-path algorithmic bytes per step: bases, one 16 B probe per distinct
     # k-mer, 8 B per examined hit, 2m reference k-mers of 12 B per refined
@@ -513,6 +574,20 @@ def main():
                "sample": f"first {n_s} reads of the same workload, oracle -@1 ({spent:.1f}s)",
                "all_cores": {"value": round(cpun, 2), "cores": ncores}}
         log(f"[bench] CPU oracle: {cpu1:.1f} reads/s at -@1, {cpun:.1f} reads/s at -@{ncores}; parity {parity}")
+        cpu["cpu_model"] = cpu_model()
+        if args.workload == "genome":
+            # sanity anchor: the oracle's -@1 mean per-read time on a B02-like
+            # input (the 3.1 Gbp reference, 24 kb reads with 0.5% errors, r=0.01)
+            # against the reference's published 0.184 ms/read on a Xeon Gold 5218
+            # (RESULTS.md:345, per-read-B02-Containment.tsv)
+            ab, ao, _ = G.synth_reads(17, ref, lens, 2000, (24_000, 0), lognormal=False, clip=(24_000, 24_000),
+                                      errors=((0.005, 0.6, 0.2, 0.2),), both_strands=True)
+            ares, aspent = oidx.map_reads_timed(op, (ab, ao), threads=1)
+            ms = aspent / (len(ao) - 1) * 1e3
+            cpu["anchor"] = {"input": "B02-like: 2,000 reads of 24 kb, 0.5% errors (60/20/20), genome reference",
+                             "oracle_ms_per_read_at_1": round(ms, 4), "published_ms_per_read": 0.184,
+                             "ratio": round(ms / 0.184, 3), "mapped": int(ares["mapped"].sum())}
+            log(f"[bench] CPU anchor: {cpu['anchor']}")
 
     out = {
         "metric": "mapped reads/s, 3.1 Gbp ref r=0.01 (batched vs plain GPU, bit-exact PAF); p99 ms",
@@ -540,7 +615,8 @@ def main():
         "plain_gpu": {"reads_per_s": round(plain_rate, 2) if plain_rate else None, "reads": n_plain,
                       "same_records": plain_same},
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
-        "pcie_inclusive": pcie,
+        "end_to_end": end_to_end,
+        "pcie_inclusive_bytes": pcie,
         "per_metric": per_metric,
         "placement": placement,
         "serving": serving,

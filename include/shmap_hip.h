@@ -208,6 +208,28 @@ int shm_mapper_set_read_limit(shm_mapper *mp, int32_t max_read_len);
 int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,
                          shm_read_result *out);
 
+/* 2-bit packed reads: a quarter of the bytes to move to the device.  No
+ * reference counterpart (the reference maps reads from memory it parsed
+ * itself); for callers whose reads start in host memory, where the copy over
+ * the link, not the mapping, sets the rate.
+ *
+ * shm_pack_reads packs a batch on the host (threads > 1 splits it): base i of
+ * the batch (reads back to back) goes to bits 2*(i%4) of packed[i/4], A C G T
+ * (either case) as 0 1 2 3; off[0..n] receives the reads' base offsets, and
+ * exc[0..2*n_exc) the runs of every other byte as sorted [start, end) base
+ * positions (any non-ACGT byte has the ntHash seed 0, src/sketch.rs:73-88, so
+ * the mapping does not depend on which byte it was).  packed needs
+ * (bases + 3) / 4 bytes; SHM_ERR_CAPACITY when packed or exc is too small
+ * (*n_exc is still set).
+ *
+ * shm_map_batch_packed maps such a batch: the same records as shm_map_batch on
+ * the reads it was packed from.  packed, off and exc are host memory (from
+ * shm_host_alloc for full link speed); the bases are unpacked on the device. */
+int shm_pack_reads(const shm_read *reads, int32_t n, uint8_t *packed, int64_t packed_cap, int64_t *off,
+                   int64_t *exc, int64_t exc_cap, int64_t *n_exc, int32_t threads);
+int shm_map_batch_packed(shm_mapper *mp, const uint8_t *packed, const int64_t *off, int32_t n,
+                         const int64_t *exc, int64_t n_exc, shm_read_result *out);
+
 /* Per-read record formatting shared by every caller: the PAF line of a mapped
  * read (src/mapping.rs:87-107, 161-182, 228-247; no newline) with map_time as
  * t:f:, or the unmapped line (src/mapping.rs:82-84).  Returns the length

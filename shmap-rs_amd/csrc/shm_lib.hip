@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -926,6 +927,7 @@ struct shm_mapper {
     bool timing = false;
     hipEvent_t ev[ST_COUNT][2];
     hipEvent_t t0 = nullptr, t1 = nullptr;  // a batch's first and last GPU work
+    hipEvent_t ev_up = nullptr;              // the end of a batch's upload
     int32_t max_read_len = 0;  // reads longer than this fail alone (0: no limit)
     double stage_ms[ST_COUNT] = {};
     double total_ms = 0;
@@ -1014,7 +1016,8 @@ extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, 
     }
     for (int s = 0; s < ST_COUNT; s++)
         for (int j = 0; j < 2; j++) (void)hipEventCreate(&mp->ev[s][j]);
-    if (hipEventCreate(&mp->t0) != hipSuccess || hipEventCreate(&mp->t1) != hipSuccess) {
+    if (hipEventCreate(&mp->t0) != hipSuccess || hipEventCreate(&mp->t1) != hipSuccess ||
+        hipEventCreateWithFlags(&mp->ev_up, hipEventDisableTiming) != hipSuccess) {
         shm_mapper_free(mp);
         return fail(SHM_ERR_HIP, "hipEventCreate failed");
     }
@@ -1028,6 +1031,7 @@ extern "C" void shm_mapper_free(shm_mapper *mp) {
         for (int j = 0; j < 2; j++) (void)hipEventDestroy(mp->ev[s][j]);
     if (mp->t0) (void)hipEventDestroy(mp->t0);
     if (mp->t1) (void)hipEventDestroy(mp->t1);
+    if (mp->ev_up) (void)hipEventDestroy(mp->ev_up);
     (void)hipStreamDestroy(mp->st);
     delete mp;
 }
@@ -1504,6 +1508,21 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     return SHM_OK;
 }
 
+// Large host-to-device uploads of concurrent mappers on one device go one at a
+// time, in turn: copies that share the link at once all finish late together
+// and leave it idle while their mappers compute, whereas in turn one mapper's
+// kernels run while the next one's batch goes up.  The caller holds the turn
+// from its first copy until they are done (small uploads skip it).
+constexpr uint64_t UPLOAD_TURN_BYTES = 8ull << 20;
+static std::mutex g_upload_turn[64];
+
+struct UploadTurn {
+    std::unique_lock<std::mutex> lk;
+    shm_mapper *mp = nullptr;
+    UploadTurn(shm_mapper *m, uint64_t bytes);
+    int done();  // waits for the copies queued so far on the mapper's stream
+};
+
 // Uploads n valid host reads and maps them as one batch.
 static int map_host_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out) {
     std::vector<int64_t> off(n + 1, 0);
@@ -1518,16 +1537,35 @@ static int map_host_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_
     for (int32_t i = 1; i < n && contiguous; i++) contiguous = reads[i].seq == reads[i - 1].seq + reads[i - 1].len;
     RCHK(mp->h_roff.ensure(n + 1));
     memcpy(mp->h_roff.p, off.data(), (n + 1) * 8);
-    if (contiguous && off[n] > 0) {
-        HIPCHK(hipMemcpyAsync(mp->bases.p, reads[0].seq, off[n], hipMemcpyHostToDevice, mp->st));
-    } else if (off[n] > 0) {
+    const uint8_t *src = contiguous ? reads[0].seq : nullptr;
+    if (!contiguous && off[n] > 0) {
         RCHK(mp->h_bases.ensure(off[n] + 1));
         for (int32_t i = 0; i < n; i++)
             if (reads[i].len) memcpy(mp->h_bases.p + off[i], reads[i].seq, reads[i].len);
-        HIPCHK(hipMemcpyAsync(mp->bases.p, mp->h_bases.p, off[n], hipMemcpyHostToDevice, mp->st));
+        src = mp->h_bases.p;
     }
+    UploadTurn turn(mp, (uint64_t)off[n]);
+    if (off[n] > 0) HIPCHK(hipMemcpyAsync(mp->bases.p, src, off[n], hipMemcpyHostToDevice, mp->st));
     HIPCHK(hipMemcpyAsync(mp->roff.p, mp->h_roff.p, (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
+    RCHK(turn.done());
     return map_device(mp, mp->bases.p, mp->roff.p, n, out);
+}
+
+UploadTurn::UploadTurn(shm_mapper *m, uint64_t bytes) {
+    if (bytes < UPLOAD_TURN_BYTES) return;
+    const int dev = m->idx->device;
+    lk = std::unique_lock<std::mutex>(g_upload_turn[(dev >= 0 && dev < 64) ? dev : 0]);
+    mp = m;
+}
+
+int UploadTurn::done() {
+    if (!mp) return SHM_OK;
+    HIPCHK(hipEventRecord(mp->ev_up, mp->st));
+    const hipError_t e = hipEventSynchronize(mp->ev_up);
+    lk.unlock();
+    mp = nullptr;
+    HIPCHK(e);
+    return SHM_OK;
 }
 
 // The record of a read that failed alone: its length and status, nothing mapped.
@@ -1749,9 +1787,11 @@ extern "C" int shm_map_batch_packed(shm_mapper *mp, const uint8_t *packed, const
     RCHK(mp->exc.ensure(2 * (uint64_t)n_exc + 2));
     RCHK(mp->bases.ensure(nb + 64));
     RCHK(mp->roff.ensure(n + 1));
+    UploadTurn turn(mp, (uint64_t)nbytes);
     if (nbytes > 0) HIPCHK(hipMemcpyAsync(mp->packed.p, packed, (size_t)nbytes, hipMemcpyHostToDevice, mp->st));
     if (n_exc > 0) HIPCHK(hipMemcpyAsync(mp->exc.p, exc, (size_t)n_exc * 16, hipMemcpyHostToDevice, mp->st));
     HIPCHK(hipMemcpyAsync(mp->roff.p, off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, mp->st));
+    RCHK(turn.done());
     launch_unpack_bases((const uint8_t *)mp->packed.p, (uint64_t)nb, mp->exc.p, n_exc, mp->bases.p, mp->st);
     HIPCHK(hipGetLastError());
     if (over) {

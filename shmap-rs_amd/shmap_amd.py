@@ -128,6 +128,10 @@ def lib():
         L.shm_mapper_autotune.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.POINTER(Tuning)]
         L.shm_map_batch.argtypes = [C.c_void_p, C.POINTER(Read), C.c_int32, C.c_void_p]
         L.shm_mapper_set_read_limit.argtypes = [C.c_void_p, C.c_int32]
+        L.shm_pack_reads.argtypes = [C.POINTER(Read), C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                     C.c_int64, C.POINTER(C.c_int64), C.c_int32]
+        L.shm_map_batch_packed.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64,
+                                           C.c_void_p]
         L.shm_map_batch_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.shm_format_paf.argtypes = [C.c_void_p, C.POINTER(MapParams), C.c_char_p, C.POINTER(ReadResult),
                                      C.c_double, C.c_char_p, C.c_size_t]
@@ -372,6 +376,17 @@ class Mapper:
             _check(rc)
         return out[:n]
 
+    def map_packed(self, batch: "PackedReads", out: np.ndarray | None = None) -> np.ndarray:
+        """shm_map_batch_packed over a PackedReads batch."""
+        n = batch.n
+        if out is None:
+            out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        elif out.dtype != RESULT_DTYPE or len(out) < n or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous RESULT_DTYPE array of at least n records")
+        _check(lib().shm_map_batch_packed(self._p, batch.packed.ctypes.data, batch.off.ctypes.data, n,
+                                          batch.exc.ctypes.data, batch.n_exc, out.ctypes.data))
+        return out[:n]
+
     def map_device(self, d_bases_ptr: int, d_off_ptr: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
         """Maps n reads whose bases and offsets are already in device memory.
         `out` (RESULT_DTYPE, at least n records; page-locked from host_array for
@@ -472,6 +487,32 @@ def host_array(n: int, dtype) -> np.ndarray:
 
 COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band",
                "examined_hits", "global_table_reads")
+
+
+class PackedReads:
+    """A batch packed by shm_pack_reads: 2-bit bases, base offsets and the
+    runs of non-ACGT bytes, in page-locked memory when pinned (full link speed)."""
+
+    def __init__(self, reads, threads: int = 8, pinned: bool = True):
+        arr, n, keep = _reads(reads)
+        total = sum(arr[i].len for i in range(n))
+        nbytes = (total + 3) // 4
+        alloc = (lambda m, dt: host_array(m, dt)) if pinned else (lambda m, dt: np.zeros(m, dtype=dt))
+        self.packed = alloc(max(nbytes, 1) + 4, np.uint8)
+        self.off = alloc(n + 1, np.int64)
+        cap = 1024
+        while True:
+            self.exc = alloc(2 * cap, np.int64)
+            ne = C.c_int64()
+            rc = lib().shm_pack_reads(arr, n, self.packed.ctypes.data, len(self.packed), self.off.ctypes.data,
+                                      self.exc.ctypes.data, cap, C.byref(ne), threads)
+            if rc == 4 and ne.value > cap:
+                cap = ne.value
+                continue
+            _check(rc)
+            break
+        del keep
+        self.n, self.n_exc, self.bases = n, ne.value, total
 
 
 def format_paf(index: SketchIndex, params: MapParams, qid: str, rec, map_time: float = 0.0) -> str:

@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import fastx
@@ -255,3 +256,43 @@ def test_cli_reader_refuses_other_compression_and_corrupt_gzip(tmp_path):
     f.write_bytes(bytes(bad))
     r = subprocess.run([CLI, "-p", str(f), "--dump-reads"], capture_output=True)
     assert r.returncode == 1 and b"decompress" in r.stderr, r.stderr
+
+
+def test_pack_reads_round_trip():
+    """shm_pack_reads (host only): 2-bit codes of A C G T in either case, base i
+    at bits 2*(i%4) of byte i/4, reads back to back, and every other byte in
+    sorted [start, end) runs that end at read boundaries -- the same whatever
+    the thread count (bytes shared by two threads' reads)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "shmap-rs_amd"))
+    import shmap_amd as G
+    rng = np.random.default_rng(1)
+    reads = []
+    for i in range(300):
+        n = int(rng.integers(0, 2000)) if i % 50 else 0
+        alpha = b"ACGTacgtNRYn-" if i % 3 == 0 else b"ACGTacgt"
+        reads.append(np.frombuffer(alpha, dtype=np.uint8)[rng.integers(0, len(alpha), n)].tobytes())
+    reads[7] = b"N" * 33  # a whole read of other bytes
+    allb = np.frombuffer(b"".join(reads), dtype=np.uint8)
+    low = allb | np.uint8(0x20)
+    acgt = np.isin(low, np.frombuffer(b"acgt", dtype=np.uint8))
+    code = np.zeros(len(allb), dtype=np.uint8)
+    for j, ch in enumerate(b"acgt"):
+        code[low == ch] = j
+    want_bytes = np.zeros((len(allb) + 3) // 4, dtype=np.uint8)
+    for j in range(4):
+        want_bytes |= np.pad(code[j::4], (0, len(want_bytes) - len(code[j::4]))) << np.uint8(2 * j)
+    ends = np.cumsum([len(r) for r in reads])
+    for th in (1, 4, 16):
+        pr = G.PackedReads(reads, threads=th, pinned=False)
+        assert pr.n == len(reads) and pr.bases == len(allb)
+        assert np.array_equal(pr.off[1:], ends)
+        assert np.array_equal(pr.packed[:len(want_bytes)], want_bytes)
+        runs = pr.exc[:2 * pr.n_exc].reshape(-1, 2)
+        mask = np.zeros(len(allb), dtype=bool)
+        for s, e in runs:
+            assert s < e and not mask[s:e].any()
+            mask[s:e] = True
+        assert np.array_equal(mask, ~acgt)
+        assert np.all(np.diff(runs[:, 0]) > 0)
+        assert np.all(np.isin(runs[:, 1], ends) | acgt[np.minimum(runs[:, 1], len(allb) - 1)])
