@@ -257,6 +257,63 @@ def run_serving(idx, prm, lens, ref, args) -> dict:
             "workload": "lognormal 1-100 kb (mean 15 kb), 50/50 0.1% and 7% errors, Poisson arrivals"}
 
 
+def run_cli(args, segs, bases, off, res, prm, idx, h_frac) -> dict:
+    """The shmap command line end to end on the same workload: the reference
+    and the reads written as FASTA (to /dev/shm when present), then one run
+    that builds the index from the FASTA and saves it, and one that maps from
+    the saved index; wall time of each, and the CLI's own timings of reading,
+    indexing, mapping and output (its --profile-log).  The PAF must equal the
+    library's records formatted."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "shmap-rs_amd", "lib", "shmap")
+    base = "/dev/shm" if os.path.isdir("/dev/shm") else "/tmp"
+    td = tempfile.mkdtemp(dir=base, prefix="shmap_cli_")
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    try:
+        t0 = time.time()
+        with open(os.path.join(td, "ref.fa"), "wb") as f:
+            for name, seq in segs:
+                f.write(b">" + name.encode() + b"\n")
+                f.write(np.ascontiguousarray(seq).tobytes())
+                f.write(b"\n")
+        n = len(off) - 1
+        with open(os.path.join(td, "reads.fa"), "wb") as f:
+            for i0 in range(0, n, 4096):
+                parts = []
+                for i in range(i0, min(n, i0 + 4096)):
+                    parts.append(b">r%d\n" % i)
+                    parts.append(bases[off[i]:off[i + 1]].tobytes())
+                    parts.append(b"\n")
+                f.write(b"".join(parts))
+        log(f"[bench] CLI inputs written in {time.time() - t0:.1f}s")
+        opts = ["-k", "25", "-r", str(h_frac), "-t", str(W.PARAMS["theta"]), "-d", str(W.PARAMS["min_diff"]),
+                "-o", str(W.PARAMS["max_overlap"]), "-@", str(threads), "-x"]
+        runs = {}
+        for tag, src, extra in (("from_fasta", "ref.fa", ["--save-index", "ref.idx"]), ("from_index", "ref.idx", [])):
+            t0 = time.time()
+            r = subprocess.run([exe, "-s", src, "-p", "reads.fa", *opts, *extra, "--profile-log", tag + ".json"],
+                               cwd=td, capture_output=True, timeout=600)
+            wall = time.time() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"shmap {tag} exited {r.returncode}: {r.stderr.decode()[-500:]}")
+            prof = json.load(open(os.path.join(td, tag + ".json")))
+            runs[tag] = (wall, prof, r.stdout.decode())
+            log(f"[bench] CLI {tag}: {wall:.2f}s wall, {prof}")
+        strip = lambda t: ["\t".join(x for x in l.split("\t") if not x.startswith("t:f:")) for l in t.splitlines() if l]
+        want = [G.format_paf(idx, prm, f"r{i}", res[i]) for i in np.nonzero(res["mapped"] == 1)[0]]
+        wall, prof, out = runs["from_index"]
+        return {"reads_per_s": round(n / wall, 1), "wall_s": round(wall, 3), "threads": threads,
+                "read_s": prof["read_s"], "parse_gb_per_s": round(prof["input_bytes"] / max(prof["read_s"], 1e-9) / 1e9, 3),
+                "index_load_s": prof["index_s"], "mapping_s": prof["mapping_s"], "output_s": prof["output_s"],
+                "from_fasta": {"wall_s": round(runs["from_fasta"][0], 3), "index_s": runs["from_fasta"][1]["index_s"]},
+                "same_paf": strip(out) == strip("\n".join(want)) and strip(runs["from_fasta"][2]) == strip(out),
+                "mode": "shmap -s <saved index> -p reads.fa: read, map on the GPU, write PAF; files in " + base}
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +329,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--no-metrics", action="store_true", help="skip the other scoring metrics' timings")
     ap.add_argument("--all-metrics", action="store_true",
                     help="time the other scoring metrics on any workload (default: genome only)")
@@ -534,6 +592,14 @@ def main():
     path_roof = {"bytes_per_step": pb, "achieved": round(path_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(path_gbs / HBM_PEAK_GBS, 5)}
 
+    cli = None
+    if not args.no_cli:
+        try:
+            cli = run_cli(args, segs, bases, off, res, prm, idx, h_frac)
+        except Exception as e:  # a report beside the measurement
+            cli = {"error": str(e)[:300]}
+        log(f"[bench] CLI: {cli}")
+
     serving = None
     if not args.no_serve:
         mp = None  # the serving workers get the device's hardware queues to themselves
@@ -606,7 +672,8 @@ def main():
                    "contigs": int(len(lens)), "reads": n_reads, "read_gbp": round(gbp, 3), "k": k, "r": h_frac,
                    "t": theta, "d": d, "o": o, "metric": args.metric, "mode": "batched, device-resident",
                    "mappers": M, "tuning": tuning,
-                   "tuning_by": "built-in rule" if args.no_autotune else "shm_mapper_autotune on this batch"},
+                   "tuning_by": "built-in rule" if args.no_autotune else
+                   "shm_mapper_autotune on 20,000 reads of the same workload drawn with another seed"},
         "one_mapper": {"reads_per_s": round(n_reads / one_ms * 1e3, 2), "ms_per_step": round(one_ms, 3)},
         "gbp_per_s": round(gbp * args.steps / elapsed, 3),
         "mapped_reads": n_mapped,
@@ -620,6 +687,7 @@ def main():
         "per_metric": per_metric,
         "placement": placement,
         "serving": serving,
+        "cli": cli,
         "path_roofline": path_roof,
         "index_build_s": round(t_index, 3),
         "stage_ms": {s: round(v, 3) for s, v in stage_ms.items()},

@@ -6,14 +6,20 @@
 // (src/shmap/mod.rs:280-327, 374-650).  Reads are mapped on the GPU in batches;
 // records are written strictly in input order.
 #include <errno.h>
+#include <fcntl.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
 #include <sys/time.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <condition_variable>
 #include <deque>
+#include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -117,7 +123,7 @@ void usage(FILE *f) {
             "  -P, --no_bucket_pruning       Disables bucket pruning\n"
             "  -B, --one_sweep               One sweep on all matches\n"
             "  -F, --abs_pos                 Absolute positions instead of k-mer positions\n"
-            "  -@, --threads <THREADS>       Host threads (accepted; mapping runs on the GPU) [default: 1]\n"
+            "  -@, --threads <THREADS>       Host threads reading FASTA and writing records (mapping runs on the GPU) [default: 1]\n"
             "  -x, --profile                 Write a timing report (JSON) to --profile-log\n"
             "      --profile-log <FILE>\n"
             "      --per-read-stats <FILE>   One TSV row per read\n"
@@ -297,11 +303,181 @@ class FastxReader {
     bool have_pending_ = false;
 };
 
+double now_s() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// Plain FASTA read in parallel byte ranges, as the reference's two-pass reader
+// does (src/io/mod.rs:198-355): the file is memory-mapped and taken a window
+// at a time, each window cut at record starts (a '>' at the file's start or
+// right after a '\n'), and the window's bytes into one range per thread; a
+// thread parses the records that start in its range, reading past the range's
+// end to finish its last one.  Records come out in file order, one batch per
+// range, with FastxReader's semantics (id cut at the first space or tab,
+// '\r' before a line break dropped, text before the first record skipped).
+class ParallelFasta {
+  public:
+    ParallelFasta(const std::string &path, int threads, size_t window)
+        : threads_(threads < 1 ? 1 : threads), window_(window < (1u << 20) ? (1u << 20) : window) {
+        fd_ = open(path.c_str(), O_RDONLY);
+        if (fd_ < 0) {
+            err_ = "failed to open \"" + path + "\"";
+            return;
+        }
+        struct stat st;
+        if (fstat(fd_, &st) != 0) {
+            err_ = "failed to read \"" + path + "\"";
+            return;
+        }
+        size_ = (size_t)st.st_size;
+        if (size_ > 0) {
+            void *m = mmap(nullptr, size_, PROT_READ, MAP_PRIVATE, fd_, 0);
+            if (m == MAP_FAILED) {
+                err_ = "failed to map \"" + path + "\"";
+                return;
+            }
+            data_ = (const char *)m;
+            madvise(m, size_, MADV_SEQUENTIAL);
+        }
+        pos_ = start_at(0);
+        ok_ = true;
+    }
+    ~ParallelFasta() {
+        if (data_) munmap((void *)data_, size_);
+        if (fd_ >= 0) close(fd_);
+    }
+    ParallelFasta(const ParallelFasta &) = delete;
+    ParallelFasta &operator=(const ParallelFasta &) = delete;
+    bool ok() const { return ok_; }
+    const std::string &error() const { return err_; }
+
+    // Plain (uncompressed) input whose first record is FASTA; FASTQ and
+    // compressed input take FastxReader.
+    static bool usable(const std::string &path) {
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) return false;
+        char buf[1 << 16];
+        const size_t n = fread(buf, 1, sizeof buf, f);
+        fclose(f);
+        if (n >= 2 && (unsigned char)buf[0] == 0x1f && (unsigned char)buf[1] == 0x8b) return false;
+        if (n >= 3 && buf[0] == 'B' && buf[1] == 'Z' && buf[2] == 'h') return false;
+        if (n >= 4 && (unsigned char)buf[0] == 0x28 && (unsigned char)buf[1] == 0xb5) return false;
+        if (n >= 6 && (unsigned char)buf[0] == 0xfd && buf[1] == '7') return false;
+        // the first line starting a record decides (FastxReader::record)
+        for (size_t i = 0; i < n; i++)
+            if (i == 0 || buf[i - 1] == '\n') {
+                if (buf[i] == '>') return true;
+                if (buf[i] == '@') return false;
+            }
+        return false;
+    }
+
+    // The next window's records, one batch per thread range, in order; false
+    // when the file is done.
+    bool next_window(std::vector<Batch> &out) {
+        if (!ok_ || pos_ >= size_) return false;
+        const size_t we = start_at(pos_ + window_ < size_ ? pos_ + window_ : size_);
+        out.resize(threads_);
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads_; t++) {
+            const size_t a = pos_ + (we - pos_) * t / threads_, b = pos_ + (we - pos_) * (t + 1) / threads_;
+            th.emplace_back([this, a, b, t, &out] { parse(a, b, out[t]); });
+        }
+        for (std::thread &x : th) x.join();
+        pos_ = we;
+        return true;
+    }
+
+  private:
+    // The first record start at or after p (or the file's size).
+    size_t start_at(size_t p) const {
+        if (p == 0 && size_ > 0 && data_[0] == '>') return 0;
+        size_t q = p > 0 ? p - 1 : 0;
+        while (q < size_) {
+            const char *nl = (const char *)memchr(data_ + q, '\n', size_ - q);
+            if (!nl) return size_;
+            q = (size_t)(nl - data_) + 1;
+            if (q < size_ && data_[q] == '>') return q;
+        }
+        return size_;
+    }
+    // The line at p: [p, e) without its '\n' and a '\r' before it; the next
+    // line's start in *next.
+    void line_at(size_t p, size_t &e, size_t &next) const {
+        const char *nl = (const char *)memchr(data_ + p, '\n', size_ - p);
+        const size_t end = nl ? (size_t)(nl - data_) : size_;
+        next = nl ? end + 1 : size_;
+        e = (end > p && data_[end - 1] == '\r') ? end - 1 : end;
+    }
+    void parse(size_t a, size_t b, Batch &out) const {
+        out.clear();
+        size_t p = start_at(a);
+        while (p < b) {
+            size_t e, next;
+            line_at(p, e, next);
+            size_t ne = p + 1;
+            while (ne < e && data_[ne] != ' ' && data_[ne] != '\t') ne++;
+            out.names.append(data_ + p + 1, ne - p - 1);
+            p = next;
+            while (p < size_ && data_[p] != '>') {
+                line_at(p, e, next);
+                out.bases.insert(out.bases.end(), data_ + p, data_ + e);
+                p = next;
+            }
+            out.off.push_back((int64_t)out.bases.size());
+            out.name_off.push_back(out.names.size());
+        }
+    }
+    int threads_;
+    size_t window_;
+    int fd_ = -1;
+    const char *data_ = nullptr;
+    size_t size_ = 0, pos_ = 0;
+    bool ok_ = false;
+    std::string err_;
+};
+
+// A reads file as batches: ParallelFasta for plain FASTA when more than one
+// host thread is asked for, FastxReader otherwise.
+class ReadSource {
+  public:
+    ReadSource(const std::string &path, int threads, long max_reads) : max_(max_reads) {
+        if (threads > 1 && ParallelFasta::usable(path)) pf_.reset(new ParallelFasta(path, threads, 512u << 20));
+        else fx_.reset(new FastxReader(path));
+    }
+    bool ok() const { return pf_ ? pf_->ok() : fx_->ok(); }
+    const std::string &error() const { return pf_ ? pf_->error() : fx_->error(); }
+    // The next non-empty batch; false at the end.
+    bool next(Batch &b) {
+        if (fx_) return fx_->next_batch(b, max_);
+        for (;;) {
+            while (at_ < win_.size()) {
+                Batch &w = win_[at_++];
+                if (w.size()) {
+                    std::swap(b, w);
+                    return true;
+                }
+            }
+            if (!pf_->next_window(win_)) return false;
+            at_ = 0;
+        }
+    }
+
+  private:
+    long max_;
+    std::unique_ptr<FastxReader> fx_;
+    std::unique_ptr<ParallelFasta> pf_;
+    std::vector<Batch> win_;
+    size_t at_ = 0;
+};
+
 // Batches read ahead on a thread of their own (two in flight), so parsing
 // overlaps mapping and output.
 class BatchReader {
   public:
-    BatchReader(FastxReader &rd, long max_reads) : rd_(rd), max_(max_reads), th_([this] { run(); }) {}
+    explicit BatchReader(std::function<bool(Batch &)> src) : src_(std::move(src)), th_([this] { run(); }) {}
     ~BatchReader() {
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -319,6 +495,11 @@ class BatchReader {
         full_.pop_front();
         cv_.notify_all();
         return b;
+    }
+    // Seconds the reader has spent in the source so far.
+    double busy_s() {
+        std::lock_guard<std::mutex> lk(mu_);
+        return busy_s_;
     }
     void recycle(Batch *b) {
         std::lock_guard<std::mutex> lk(mu_);
@@ -341,8 +522,11 @@ class BatchReader {
                 b = free_.front();
                 free_.pop_front();
             }
-            const bool got = rd_.next_batch(*b, max_);
+            const double t0 = now_s();
+            const bool got = src_(*b);
+            const double dt = now_s() - t0;
             std::lock_guard<std::mutex> lk(mu_);
+            busy_s_ += dt;
             if (!got) {
                 done_ = true;
                 cv_.notify_all();
@@ -352,8 +536,8 @@ class BatchReader {
             cv_.notify_all();
         }
     }
-    FastxReader &rd_;
-    const long max_;
+    std::function<bool(Batch &)> src_;
+    double busy_s_ = 0;  // time spent reading and parsing
     Batch pool_[2];
     std::deque<Batch *> free_, full_;
     std::mutex mu_;
@@ -362,11 +546,6 @@ class BatchReader {
     std::thread th_;
 };
 
-double now_s() {
-    struct timeval tv;
-    gettimeofday(&tv, nullptr);
-    return tv.tv_sec + 1e-6 * tv.tv_usec;
-}
 
 bool parse_i32(const char *s, int32_t &v) {
     char *e;
@@ -586,13 +765,13 @@ int main(int argc, char **argv) {
     }
     if (p.dump_reads && have_p) {
         // the reads as parsed, one FASTA record per line pair (no device needed)
-        FastxReader rd(p.p_file);
+        ReadSource rd(p.p_file, (int)p.threads, p.batch);
         if (!rd.ok()) {
             fprintf(stderr, "Error: %s\n", rd.error().c_str());
             return 1;
         }
         Batch b;
-        while (rd.next_batch(b, p.batch))
+        while (rd.next(b))
             for (size_t i = 0; i < b.size(); i++) {
                 const std::string nm = b.name(i);
                 fprintf(stdout, ">%s\n", nm.c_str());
@@ -627,6 +806,7 @@ int main(int argc, char **argv) {
     shm_sketcher *sk = nullptr;
     shm_index *idx = nullptr;
     const double t_idx = now_s();
+    double index_s = 0;
     long long nucl = 0;
     size_t nsegs = 0;
     if (is_index_file(p.t_file)) {
@@ -657,16 +837,19 @@ int main(int argc, char **argv) {
             nucl += sz;
         }
     } else {
-        std::vector<Record> ref;
+        std::vector<Batch> ref;
         {
-            FastxReader rd(p.t_file);
+            ReadSource rd(p.t_file, (int)p.threads, 1 << 20);
             if (!rd.ok()) {
                 fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
                         rd.error().c_str());
                 return 1;
             }
-            Record r;
-            while (rd.next(r)) ref.push_back(std::move(r));  // next() refills the moved-from record
+            Batch b;
+            while (rd.next(b)) {
+                ref.push_back(std::move(b));
+                b = Batch();
+            }
             if (!rd.error().empty()) {
                 fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
                         rd.error().c_str());
@@ -677,17 +860,22 @@ int main(int argc, char **argv) {
             fprintf(stderr, "ERROR: %s\n", shm_last_error());
             return 1;
         }
-        std::vector<shm_segment> segs(ref.size());
-        for (size_t i = 0; i < ref.size(); i++)
-            segs[i] = {ref[i].name.c_str(), ref[i].seq.data(), (int64_t)ref[i].seq.size()};
+        std::vector<std::string> names;
+        std::vector<shm_segment> segs;
+        for (const Batch &bt : ref)
+            for (size_t i = 0; i < bt.size(); i++) names.push_back(bt.name(i));
+        size_t at = 0;
+        for (const Batch &bt : ref)
+            for (size_t i = 0; i < bt.size(); i++, at++)
+                segs.push_back({names[at].c_str(), bt.bases.data() + bt.off[i], (int64_t)(bt.off[i + 1] - bt.off[i])});
         if (shm_index_build(sk, segs.data(), (int32_t)segs.size(), p.have_max_matches ? p.max_matches : 0, p.device,
                             &idx)) {
             fprintf(stderr, "Error: failed to build index from %s\n\nCaused by:\n    %s\n", p.t_file.c_str(),
                     shm_last_error());
             return 1;
         }
-        for (auto &r : ref) nucl += (long long)r.seq.size();
-        nsegs = ref.size();
+        for (const shm_segment &sg : segs) nucl += (long long)sg.len;
+        nsegs = segs.size();
     }
     {
         int64_t ik, ih, hf, bk, bh;
@@ -696,7 +884,8 @@ int main(int argc, char **argv) {
         fprintf(stderr, " | indexed kmers:         %lld\n | indexed hits:          %lld\n", (long long)ik, (long long)ih);
         fprintf(stderr, " | | most frequent kmer:      %lld times.\n | | blacklisted kmers:       %lld\n",
                 (long long)hf, (long long)bk);
-        fprintf(stderr, " | indexing time:        %.1fs\n", now_s() - t_idx);
+        index_s = now_s() - t_idx;
+        fprintf(stderr, " | indexing time:        %.1fs\n", index_s);
     }
     if (!p.save_index.empty()) {
         if (shm_index_save(idx, p.save_index.c_str())) {
@@ -730,7 +919,7 @@ int main(int argc, char **argv) {
                            "seed_matches\tseeded_buckets\trefined_buckets\tfinal_buckets\tmatches_in_mapping\tmapped\t"
                            "mapq\tquery_mapping_s\n");
     }
-    FastxReader rd(p.p_file);
+    ReadSource rd(p.p_file, (int)p.threads, p.batch);
     if (!rd.ok()) {
         fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
                 rd.error().c_str());
@@ -740,13 +929,21 @@ int main(int argc, char **argv) {
     setvbuf(stdout, outbuf, _IOFBF, sizeof outbuf);
     std::vector<shm_read> reads;
     std::vector<shm_read_result> res;
-    std::string line(8192, '\0');
     unsigned long long idx_read = 0, n_reads = 0, n_mapped = 0;
     double map_s = 0;
     int rc = 0;
     std::string first_error;  // the first read that failed
+    // a batch's records are formatted by up to -@ threads, a slice each, and
+    // written in order
+    const int nfmt = (int)(p.threads < 1 ? 1 : p.threads > 16 ? 16 : p.threads);
+    struct Slice {
+        std::string paf, unm, st, err;
+        unsigned long long mapped = 0;
+    };
+    std::vector<Slice> slices(nfmt);
+    double read_s = 0, fmt_s = 0;
     {
-        BatchReader br(rd, p.batch);
+        BatchReader br([&rd](Batch &b) { return rd.next(b); });
         while (Batch *batch = br.next()) {
             const size_t nb = batch->size();
             reads.resize(nb);
@@ -765,41 +962,80 @@ int main(int argc, char **argv) {
             const double dt = now_s() - t0;
             map_s += dt;
             const double per_read = dt / nb;
-            for (size_t i = 0; i < nb; i++, idx_read++) {
-                const shm_read_result &x = res[i];
-                if (x.status != 0) continue;  // a failed read writes nothing
-                const std::string name = batch->name(i);
-                if (stats && idx_read % p.per_read_stats_sample == 0)
-                    fprintf(stats, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n", name.c_str(),
-                            x.read_len, x.m, x.S, x.possible_matches, x.total_matches, x.seed_matches,
-                            x.seeded_buckets, x.refined_buckets, x.final_buckets, x.mapped ? x.intersection : 0,
-                            x.mapped, x.mapped ? (x.mapq == 60 ? 60 : 0) : -1, per_read);
-                if (x.mapped) {
-                    int n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
-                    if (n >= (int)line.size()) {
-                        line.resize(n + 1);
-                        n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
+            auto format = [&](Slice &sl, size_t i0, size_t i1) {
+                sl.paf.clear();
+                sl.unm.clear();
+                sl.st.clear();
+                sl.err.clear();
+                sl.mapped = 0;
+                std::string line(8192, '\0');
+                for (size_t i = i0; i < i1; i++) {
+                    const shm_read_result &x = res[i];
+                    if (x.status != 0) continue;  // a failed read writes nothing
+                    const std::string name = batch->name(i);
+                    if (stats && (idx_read + i) % p.per_read_stats_sample == 0) {
+                        int n = snprintf(&line[0], line.size(), "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n",
+                                         name.c_str(), x.read_len, x.m, x.S, x.possible_matches, x.total_matches,
+                                         x.seed_matches, x.seeded_buckets, x.refined_buckets, x.final_buckets,
+                                         x.mapped ? x.intersection : 0, x.mapped, x.mapped ? (x.mapq == 60 ? 60 : 0) : -1,
+                                         per_read);
+                        if (n >= (int)line.size()) {
+                            line.resize(n + 1);
+                            n = snprintf(&line[0], line.size(), "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%d\t%.9f\n",
+                                         name.c_str(), x.read_len, x.m, x.S, x.possible_matches, x.total_matches,
+                                         x.seed_matches, x.seeded_buckets, x.refined_buckets, x.final_buckets,
+                                         x.mapped ? x.intersection : 0, x.mapped, x.mapped ? (x.mapq == 60 ? 60 : 0) : -1,
+                                         per_read);
+                        }
+                        sl.st.append(line.data(), (size_t)n);
                     }
-                    if (n < 0) {  // not a well-formed mapped record: this read's failure
-                        if (first_error.empty()) first_error = "read " + name + ": " + shm_last_error();
-                        continue;
+                    if (x.mapped) {
+                        int n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
+                        if (n >= (int)line.size()) {
+                            line.resize(n + 1);
+                            n = shm_format_paf(idx, &mpp, name.c_str(), &x, per_read, &line[0], line.size());
+                        }
+                        if (n < 0) {  // not a well-formed mapped record: this read's failure
+                            if (sl.err.empty()) sl.err = "read " + name + ": " + shm_last_error();
+                            continue;
+                        }
+                        sl.paf.append(line.data(), (size_t)n);
+                        sl.paf.push_back('\n');
+                        sl.mapped++;
+                    } else if (unmapped) {
+                        int n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
+                        if (n >= (int)line.size()) {
+                            line.resize(n + 1);
+                            n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
+                        }
+                        sl.unm.append(line.data(), (size_t)n);
+                        sl.unm.push_back('\n');
                     }
-                    fwrite(line.data(), 1, n, stdout);
-                    fputc('\n', stdout);
-                    n_mapped++;
-                } else if (unmapped) {
-                    int n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
-                    if (n >= (int)line.size()) {
-                        line.resize(n + 1);
-                        n = shm_format_unmapped(name.c_str(), x.read_len, &line[0], line.size());
-                    }
-                    fwrite(line.data(), 1, n, unmapped);
-                    fputc('\n', unmapped);
                 }
+            };
+            const int ns = nb >= 4096 ? nfmt : 1;
+            if (ns == 1) {
+                format(slices[0], 0, nb);
+            } else {
+                std::vector<std::thread> th;
+                for (int t = 0; t < ns; t++)
+                    th.emplace_back([&, t] { format(slices[t], nb * t / ns, nb * (t + 1) / ns); });
+                for (std::thread &x : th) x.join();
             }
+            for (int t = 0; t < ns; t++) {
+                const Slice &sl = slices[t];
+                fwrite(sl.paf.data(), 1, sl.paf.size(), stdout);
+                if (unmapped) fwrite(sl.unm.data(), 1, sl.unm.size(), unmapped);
+                if (stats) fwrite(sl.st.data(), 1, sl.st.size(), stats);
+                if (first_error.empty() && !sl.err.empty()) first_error = sl.err;
+                n_mapped += sl.mapped;
+            }
+            idx_read += nb;
             n_reads += nb;
+            fmt_s += now_s() - t0 - dt;
             br.recycle(batch);
         }
+        read_s = br.busy_s();
     }
     if (!first_error.empty()) {
         fprintf(stderr, "Error: failed while mapping reads from %s\n\nCaused by:\n    %s\n", p.p_file.c_str(),
@@ -821,8 +1057,13 @@ int main(int argc, char **argv) {
                                  : !p.params_file.empty() ? p.params_file + ".profile.json"
                                                           : "shmap.profile.json";
         if (FILE *f = fopen(path.c_str(), "w")) {
-            fprintf(f, "{\"reads\": %llu, \"mapped_reads\": %llu, \"mapping_s\": %.6f, \"total_s\": %.6f}\n", n_reads,
-                    n_mapped, map_s, now_s() - t_start);
+            struct stat st;
+            const long long in_bytes = stat(p.p_file.c_str(), &st) == 0 ? (long long)st.st_size : -1;
+            fprintf(f,
+                    "{\"reads\": %llu, \"mapped_reads\": %llu, \"index_s\": %.6f, \"read_s\": %.6f, "
+                    "\"input_bytes\": %lld, \"mapping_s\": %.6f, \"output_s\": %.6f, \"total_s\": %.6f, "
+                    "\"threads\": %lld}\n",
+                    n_reads, n_mapped, index_s, read_s, in_bytes, map_s, fmt_s, now_s() - t_start, p.threads);
             fclose(f);
         }
     }

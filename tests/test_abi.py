@@ -183,8 +183,8 @@ def test_cli_rejects_an_unknown_metric():
     assert r.returncode == 2 and "possible values: Containment, Jaccard, bucket_SH, bucket_LCS" in r.stderr
 
 
-def _dump(path):
-    r = subprocess.run([CLI, "-p", str(path), "--dump-reads"], capture_output=True)
+def _dump(path, threads=1):
+    r = subprocess.run([CLI, "-p", str(path), "--dump-reads", "-@", str(threads)], capture_output=True)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.split(b"\n")
     return [(lines[i][1:].decode(), lines[i + 1]) for i in range(0, len(lines) - 1, 2)]
@@ -206,6 +206,40 @@ def test_cli_reader_matches_fasta_semantics(tmp_path):
     got = _dump(f)
     assert [n for n, _ in got] == [n for n, _ in want]
     assert [s for _, s in got] == [s for _, s in want]
+
+
+def test_cli_parallel_reader_matches_fasta_semantics(tmp_path):
+    """With -@ N plain FASTA is read in parallel byte ranges (windows of 1 MiB
+    and up, cut at record starts, one range per thread): the same records as
+    the sequential reader and tests/fastx.py, for records that straddle range
+    and window edges, CRLF, blank lines, '>' and '@' inside sequence lines of
+    the wrong column, a header-only record, text before the first record and
+    no final newline."""
+    import random
+    rng = random.Random(11)
+    parts = [b"junk before the first record\n\n"]
+    for i in range(700):
+        n = rng.choice([0, 1, 59, 60, 61, 700, 5_000, 40_000])
+        seq = bytes(rng.choice(b"ACGTNacgtRY") for _ in range(n))
+        width = rng.choice([60, 80, 1000])
+        eol = b"\r\n" if i % 7 == 0 else b"\n"
+        lines = [seq[j:j + width] for j in range(0, len(seq), width)]
+        if i % 11 == 0:
+            lines.insert(len(lines) // 2, b"")
+        if i % 13 == 0 and lines:
+            lines[-1] = lines[-1] + b">notheader@"
+        parts.append(b">read%d %s\t%s" % (i, b"desc" * (i % 3), eol) + eol.join(lines) + eol)
+    parts.append(b">last\nGATTACA")
+    body = b"".join(parts)
+    assert len(body) > 3_000_000
+    f = tmp_path / "reads.fa"
+    f.write_bytes(body)
+    want = fastx.read_fasta(f)
+    one = _dump(f, 1)
+    for threads in (2, 8, 16):
+        got = _dump(f, threads)
+        assert [n for n, _ in got] == [n for n, _ in want] == [n for n, _ in one]
+        assert [s for _, s in got] == [s for _, s in want] == [s for _, s in one]
 
 
 def test_cli_reader_fastq(tmp_path):
