@@ -1926,12 +1926,14 @@ __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
 // same p (earlier, larger values only lower tails at or after it), so tail p
 // becomes the smallest value with that p, and the run appends at most once.
 // Increasing: value j's place is max over i <= j of p_i + (j - i), a running
-// maximum, and the places are distinct.  Returns -1 when the read's sketch
-// (which bounds the tails) is larger.
+// maximum, and the places are distinct.  When the tails outnumber the
+// registers (a sketch above LCS_TAILS k-mers can), they stay in Ts for good and
+// a single value's place comes from ballots over 64 evenly spaced tails, a
+// few rounds each; Ts holds max(m, LCS_TAILS) + 2 entries (the tails are
+// distinct read positions, so at most m).
 constexpr int32_t LCS_TAILS = 32 * WAVE;
 constexpr uint32_t LCS_RUN_MIN = 4;
 __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uint64_t o, int32_t m, int32_t *Ts) {
-    if (m > LCS_TAILS) return -1;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     int32_t best = 0;
     for (int dir = 0; dir < 2; dir++) {
@@ -1940,7 +1942,8 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
         for (int r = 0; r < LCS_TAILS / WAVE; r++) t[r] = 0;
         int32_t head = INT32_MAX;  // lane r: the first tail of chunk r (tails 64r ..), INT32_MAX if none
         int32_t nt = 0;
-        bool in_lds = false;  // the tails are in Ts rather than the registers
+        bool in_lds = false;    // the tails are in Ts rather than the registers
+        bool lds_only = false;  // more tails than the registers hold: they stay in Ts
         for (int64_t g0 = 0; g0 < n; g0 += WAVE) {
             const int64_t gi = g0 + (int64_t)lane;
             const int64_t x = dir == 0 ? gi : n - 1 - gi;
@@ -1978,7 +1981,7 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
                     rest &= rest - 1;
                     const uint32_t e0 = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : cnt;
                     if (e0 - s0 < LCS_RUN_MIN) {
-                        if (in_lds) {  // back to the registers
+                        if (in_lds && !lds_only) {  // back to the registers
                             wave_sync();
 #pragma unroll
                             for (int r = 0; r < LCS_TAILS / WAVE; r++)
@@ -1989,18 +1992,48 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
                         }
                         for (uint32_t j = s0; j < e0; j++) {
                             const int32_t v = __builtin_amdgcn_readlane(val, (int)j);
-                            // chunks whose first tail is below v, then the tails
-                            // below v in the last of them
-                            const uint32_t k = (uint32_t)__popcll(__ballot(head < v));
-                            uint32_t pos = 0;
-                            if (k > 0) {
-                                const int32_t ck = (int32_t)k - 1;
-                                pos = (uint32_t)ck * WAVE +
-                                      (uint32_t)__popcll(__ballot(ck * WAVE + (int32_t)lane < nt && t[ck] < v));
+                            if (!in_lds) {
+                                // chunks whose first tail is below v, then the tails
+                                // below v in the last of them
+                                const uint32_t k = (uint32_t)__popcll(__ballot(head < v));
+                                uint32_t pos = 0;
+                                if (k > 0) {
+                                    const int32_t ck = (int32_t)k - 1;
+                                    pos = (uint32_t)ck * WAVE +
+                                          (uint32_t)__popcll(__ballot(ck * WAVE + (int32_t)lane < nt && t[ck] < v));
+                                }
+                                if (pos < (uint32_t)LCS_TAILS) {
+                                    if (pos == (uint32_t)nt) nt++;
+                                    if (lane == pos % WAVE) t[pos / WAVE] = v;
+                                    if (pos % WAVE == 0 && lane == pos / WAVE) head = v;
+                                    continue;
+                                }
+                                // the registers are full: the tails move to Ts for good
+#pragma unroll
+                                for (int r = 0; r < LCS_TAILS / WAVE; r++) Ts[r * WAVE + lane] = t[r];
+                                in_lds = lds_only = true;
+                                wave_sync();
                             }
+                            // the tails in Ts: v's lower bound among them, by
+                            // ballots over 64 evenly spaced tails at a time
+                            uint32_t lo = 0, hi = (uint32_t)nt;
+                            while (hi - lo > (uint32_t)WAVE) {
+                                const uint32_t step = (hi - lo + WAVE - 1) / WAVE;
+                                const uint32_t at = lo + lane * step;
+                                const uint32_t c = (uint32_t)__popcll(__ballot(at < hi && Ts[at] < v));
+                                if (c == 0) {
+                                    hi = lo;
+                                    break;
+                                }
+                                const uint32_t nh = lo + c * step;
+                                lo += (c - 1) * step + 1;
+                                if (nh < hi) hi = nh;
+                            }
+                            const uint32_t pos = lo + (uint32_t)__popcll(__ballot(lo + lane < hi && Ts[lo + lane] < v));
+                            wave_sync();  // every lane has read before the tail changes
+                            if (lane == 0) Ts[pos] = v;
                             if (pos == (uint32_t)nt) nt++;
-                            if (lane == pos % WAVE) t[pos / WAVE] = v;
-                            if (pos % WAVE == 0 && lane == pos / WAVE) head = v;
+                            wave_sync();
                         }
                     } else {
                         if (!in_lds) {
@@ -2346,7 +2379,9 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         // sid[x]: the seed matching span position x (| 0x40000000 when its
         // strand disagrees), -1 if none.
         LdsCarve C{lds, 0};
-        int32_t *Ts = LCS ? (int32_t *)C.take((uint64_t)(LCS_TAILS + 1) * 4) : nullptr;  // always fits
+        // LCS: the tails (at most m, the read's distinct positions) and the
+        // append slot; a sketch too large for them takes thread 0's sweep
+        int32_t *Ts = LCS ? (int32_t *)C.take((uint64_t)((m > LCS_TAILS ? m : LCS_TAILS) + 2) * 4) : nullptr;
         int32_t *sid = (int32_t *)C.take((uint64_t)n * 4);
         if (!sid) sid = wsid;
         int32_t *tr_arr = nullptr, *diff = nullptr;
@@ -2373,7 +2408,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         if constexpr (LCS) {
             // one wave streams the LIS (s_lcs < 0: thread 0 sweeps it below)
             if (threadIdx.x < WAVE) {
-                const int32_t best = wave_lcs(A, sid, n, o, m, Ts);
+                const int32_t best = Ts ? wave_lcs(A, sid, n, o, m, Ts) : -1;
                 if (threadIdx.x == 0) s_lcs = best;
             }
         }

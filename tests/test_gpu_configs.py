@@ -183,3 +183,13 @@ def test_dense_all_reads(genome):
     say(f"dense: {int(mk.sum())} mapped, {int(on.sum())} on their true origin")
     assert mk.sum() > 120_000 and on.sum() >= 0.99 * mk.sum()
     del g_idx, o_idx
+
+
+def test_dense_bucket_lcs_sample(genome):
+    """bucket_LCS at r=0.10: sketches of m ~ 2,300 (most above the wave's
+    register tails, so the patience tails continue in LDS) on the first 2,000
+    dense reads."""
+    lens, ref, _, _ = genome
+    gres, truth, nm, g_idx, o_idx = run_workload("dense", lens, ref, metric="bucket_LCS", n_reads=2000)
+    assert nm > 1500 and (gres["m"] > 2048).sum() > 500
+    del g_idx, o_idx
