@@ -262,12 +262,23 @@ const char *shm_mapper_stage_name(int32_t stage);
  * [5] first-pass candidates (bound at least theta after the full-extension
  * prune), [6] buckets only a second pass can reach, [7] examined hits (the
  * index hits of every consumed seed), [8] reads whose buckets were counted in
- * device-memory hash tables rather than on chip (many hits or buckets).
+ * device-memory hash tables rather than on chip (many hits or buckets), [9]
+ * host synchronisations of the batch's stream, [10] 1 when a speculative
+ * batch outgrew its buffers and was mapped again (see
+ * shm_mapper_set_speculation).
  * Writes up to n values and their number to *n_written. */
 int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */
 int shm_mapper_set_timing(shm_mapper *mp, int32_t on);
+/* Turns speculative batches on (the default) or off.  A speculative batch of
+ * shm_map_batch / shm_map_batch_packed sizes its device buffers from the
+ * batches the mapper has seen, runs every kernel without reading a size back,
+ * and synchronises with the host once; device-side checks stop it when a size
+ * outgrows its buffer, and it is then mapped again with a read-back of each
+ * size (four synchronisations, as shm_map_batch_device always does).  The
+ * records are the same either way.  No reference counterpart. */
+int shm_mapper_set_speculation(shm_mapper *mp, int32_t on);
 
 /* Batching choices of a mapper: where per-read work moves from one execution
  * shape to another.  No reference counterpart.  Any values give the same

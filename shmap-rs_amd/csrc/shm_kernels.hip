@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                                uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                                                uint32_t *lane_pre, uint4 *stage,
                                                uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
-                                               uint32_t *ovf_list, uint32_t *ovf_count) {
+                                               uint32_t *ovf_list, uint32_t *ovf_count, const uint32_t *abort) {
+    if (aborted(abort)) return;
     extern __shared__ uint8_t lds[];
     // T[(out code << 3) | in code]: the xor terms of both strands for one step
     // (row 4, an N rolling out, adds nothing: the lane starts use it).
@@ -463,7 +464,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
                                                        const uint64_t *chunk_koff, uint32_t lcap,
                                                        const uint4 *stage, uint64_t *out_h, int32_t *out_r,
-                                                       uint8_t *out_s) {
+                                                       uint8_t *out_s, const uint32_t *abort) {
+    if (aborted(abort)) return;
     __shared__ uint32_t s_pre[NWAVES][WAVE];
     const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     uint32_t *pre_w = s_pre[w];
@@ -730,6 +732,7 @@ __global__ void __launch_bounds__(WG) k_insert(const uint64_t *uk_h, const uint6
 //     (src/shmap/mod.rs:726-742, src/shmap/seeding.rs:101-106, 207-209).
 // Everything stays in the wave's LDS slice; seeds are written out once.
 __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     extern __shared__ uint8_t lds[];
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     // Per-wave LDS, 14 B per k-mer slot:
@@ -921,6 +924,7 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
 // the consumed prefix of match_seeds (src/shmap/mod.rs:726-742,
 // src/shmap/seeding.rs:101-106, 207-209).
 __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
@@ -1087,6 +1091,7 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
 }
 
 __global__ void __launch_bounds__(WG) k_tab_init(MapArgs A) {
+    if (aborted(A.abort)) return;
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < (uint64_t)A.RG; i += (uint64_t)gridDim.x * WG) {
         A.grp_nb[i] = 0;
         A.rd_first[A.r0 + i] = INT32_MAX;
@@ -1441,6 +1446,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
 // reads, and reads with more than LTAB_MAX buckets, are listed for
 // k_contrib_chunks.
 __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     LTAB_DECLARE();
     const uint64_t below = (1ull << lane) - 1;
     for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
@@ -1492,6 +1498,7 @@ __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
 // The chunk length for the listed reads (kept at ov_kpre[n + 1]) and their
 // chunk prefix (one workgroup).
 __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
+    if (aborted(A.abort)) return;
     __shared__ uint64_t tmp[NWAVES + 1];
     const uint32_t n = *A.ov_count;
     uint64_t part = 0;
@@ -1527,6 +1534,7 @@ __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
 
 // Empty global tables for the listed reads, one workgroup per read.
 __global__ void __launch_bounds__(WG) k_contrib_over_clear(MapArgs A) {
+    if (aborted(A.abort)) return;
     const uint32_t n = *A.ov_count;
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
         const uint32_t lr = A.ov_list[t];
@@ -1544,6 +1552,7 @@ __global__ void __launch_bounds__(WG) k_contrib_over_clear(MapArgs A) {
 // chunk: each chunk is counted in LDS and merged into the read's global table
 // (when its table fills, and at the end).
 __global__ void __launch_bounds__(WG) k_contrib_chunks(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     LTAB_DECLARE();
     const uint32_t nl = *A.ov_count;
     const uint64_t nk = A.ov_kpre[nl];
@@ -1583,6 +1592,7 @@ __device__ __forceinline__ void emit_bucket(const MapArgs &A, uint64_t base, uin
 // keyed, sorted in registers across the wave and emitted straight from them.
 // Reads with more buckets are listed for k_extract.
 __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
+    if (aborted(A.abort)) return;
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     for (int32_t lr = blockIdx.x * NWAVES + w; lr < A.RG; lr += gridDim.x * NWAVES) {
         const uint32_t n = (uint32_t)A.grp_nb[lr];
@@ -1606,6 +1616,7 @@ __global__ void __launch_bounds__(WG) k_extract_wave(MapArgs A) {
 // The same for the listed reads with more buckets, one workgroup each, through
 // global memory with the sort staged in LDS when it fits.
 __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
+    if (aborted(A.abort)) return;
     extern __shared__ uint8_t lds[];
     uint64_t *sk = (uint64_t *)lds;
     uint32_t *sv = (uint32_t *)(lds + (size_t)SORT_CAP * 8);
@@ -1656,6 +1667,7 @@ __device__ __forceinline__ void prune_keep(const MapArgs &A, int32_t lr, int32_t
 }
 
 __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     __shared__ uint64_t s_bb[WG + 1];
     __shared__ uint32_t s_r0, s_nr;
     const uint64_t B = A.bk_base[A.RG];
@@ -1753,6 +1765,7 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
 }
 
 __global__ void __launch_bounds__(WG) k_prune_ext(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ap = A.abs_pos;
     const uint32_t nl = A.task_count[5];
@@ -1861,6 +1874,7 @@ __device__ void wg_scan_inplace(int32_t *a, int64_t n, uint32_t *tmp) {
 //      L2 = J(first bucket above theta) * (1 - min_diff), which is at most the
 //      second pass's starting threshold best * (1 - min_diff).
 __global__ void __launch_bounds__(WG) k_refine_select(MapArgs A) {
+    if (aborted(A.abort)) return;
     const int32_t round = A.refine_pass;
     const uint32_t c0 = A.task_count[0], c1 = A.task_count[1];
     const uint64_t n = round == 0 ? (uint64_t)A.RG : round == 1 ? c0 : (uint64_t)c0 + c1;
@@ -2133,6 +2147,7 @@ __device__ __forceinline__ int32_t seed_find(const uint64_t *sh, int32_t u, uint
 
 template <bool LCS>
 __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
     extern __shared__ uint8_t lds[];
     __shared__ uint32_t tmp[NWAVES + 1];
     __shared__ double s_bj[NWAVES];
@@ -2524,6 +2539,7 @@ __device__ __forceinline__ double overlap_of(int32_t sa, int32_t la, int32_t ra,
 // settles every lane up to it, and the lanes after it are tested again with
 // the raised threshold.  The result is staged in LDS and written by the wave.
 __global__ void __launch_bounds__(WG) k_final(MapArgs A, shm_read_result *out) {
+    if (aborted(A.abort)) return;
     static_assert(sizeof(shm_read_result) % 4 == 0 && sizeof(shm_read_result) / 4 <= WAVE, "result layout");
     constexpr uint32_t RES_DW = sizeof(shm_read_result) / 4;
     __shared__ shm_read_result s_res[NWAVES];
@@ -2749,25 +2765,52 @@ void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
                    const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                    uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
-                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st) {
+                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st,
+                   const uint32_t *abort) {
     const size_t lds = sketch_lds(k);
     const uint64_t want = (n_chunks + NWAVES - 1) / NWAVES;
     if (pass == 0)
         k_sketch<0><<<persistent_grid((const void *)k_sketch<0>, lds, want, max_grid), WG, lds, st>>>(
             S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
-            lcap, ovf_list, ovf_count);
+            lcap, ovf_list, ovf_count, abort);
     else
         k_sketch<1><<<persistent_grid((const void *)k_sketch<1>, lds, want, max_grid), WG, lds, st>>>(
             S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
-            lcap, ovf_list, ovf_count);
+            lcap, ovf_list, ovf_count, abort);
 }
 
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                           uint32_t max_grid, hipStream_t st) {
+                           uint32_t max_grid, hipStream_t st, const uint32_t *abort) {
     const uint32_t g =
         persistent_grid((const void *)k_sketch_compact, 0, (n_chunks + NWAVES - 1) / NWAVES, max_grid);
-    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s);
+    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s, abort);
+}
+
+__global__ void __launch_bounds__(WG) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+__global__ void __launch_bounds__(WG) k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
+    if (n) k_fill_u32<<<flat_grid(n, 1024), WG, 0, st>>>(p, n, v);
+}
+void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
+    if (n) k_fill_u64<<<flat_grid(n, 1024), WG, 0, st>>>(p, n, v);
+}
+
+__global__ void k_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c1, const uint64_t *v2,
+                       uint64_t c2, const uint64_t *v3, uint64_t c3, uint32_t *abort) {
+    if (threadIdx.x != 0) return;
+    const bool over = (v0 && *v0 > c0) || (v1 && *v1 > c1) || (v2 && *v2 > c2) || (v3 && *v3 > c3);
+    if (over) *abort = 1u;
+}
+
+void launch_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c1, const uint64_t *v2, uint64_t c2,
+                 const uint64_t *v3, uint64_t c3, uint32_t *abort, hipStream_t st) {
+    k_gate<<<1, 64, 0, st>>>(v0, c0, v1, c1, v2, c2, v3, c3, abort);
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,

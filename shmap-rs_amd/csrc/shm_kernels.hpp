@@ -145,7 +145,15 @@ struct MapArgs {
     uint64_t sid_stride;
     int32_t *g_diff, *g_tails; // diff_stride entries each
     uint64_t diff_stride;
+    // set (nonzero) when a size computed on the device exceeds what the
+    // batch's buffers were sized for: every later stage then returns at once
+    // and the host maps the batch again with exact sizes (null: never set)
+    const uint32_t *abort;
 };
+
+__device__ __forceinline__ bool aborted(const uint32_t *abort) {
+    return abort && *(const volatile uint32_t *)abort;
+}
 
 }  // namespace shm
 
@@ -171,6 +179,13 @@ enum Stage {
 void set_error(const char *fmt, va_list ap);
 
 // host launch wrappers (shm_kernels.hip)
+// p[0, n) = v, as a kernel (no memset node when a caller captures the stream).
+void launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
+// *abort = 1 when any *val[i] > cap[i] (val[i] null: unused), for the batch's
+// capacity checks.
+void launch_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c1, const uint64_t *v2, uint64_t c2,
+                 const uint64_t *v3, uint64_t c3, uint32_t *abort, hipStream_t st);
 void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint, uint64_t *part,
                  hipStream_t st);
 // Units of LANE_W windows per sequence (plus its halo units; none without windows).
@@ -186,10 +201,11 @@ void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
                    const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                    uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
-                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st);
+                   uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st,
+                   const uint32_t *abort = nullptr);
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                           uint32_t max_grid, hipStream_t st);
+                           uint32_t max_grid, hipStream_t st, const uint32_t *abort = nullptr);
 // Staging slots per lane for a keep threshold: the expected kept count of
 // LANE_W windows plus five standard deviations and a margin, at most LANE_W.
 uint32_t sketch_lane_cap(uint64_t thres);

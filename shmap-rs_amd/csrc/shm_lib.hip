@@ -184,14 +184,37 @@ struct SketchOut {
     DevBuf<uint4> stage;                        // lane-strided staging records of the rolling pass
     std::vector<uint64_t> h_koff;  // per-sequence offsets, when asked for
     PinnedBuf<uint64_t> h_sum;     // total, largest sketch, overflow chunks
+    int syncs = 0;                 // host synchronisations so far
     uint64_t M = 0, max_m = 0;
     uint32_t n_overflow = 0;
 };
 
+// Units of a set of sequences, as k_unit_counts counts them (its halo units
+// included), from host offsets.
+static uint64_t count_units(int32_t k, const int64_t *h_off, int32_t n) {
+    const uint64_t halo = sketch_halo(k);
+    uint64_t U = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const int64_t len = h_off[i + 1] - h_off[i];
+        const int64_t nw = (len >= k && len >= 1) ? len - k + 1 : 0;
+        if (nw > 0) U += (uint64_t)((nw + LANE_W - 1) / LANE_W) + halo;
+    }
+    return U;
+}
+
+// Sketches a set of sequences already on the device: count pass, scan, write
+// pass.  koff (n+1) receives per-sequence offsets.  With the host's offsets
+// (h_off) the unit table is sized without a read-back; with an abort flag the
+// k-mer count is not read back either: the outputs hold m_cap + 1 entries,
+// and a larger count, or a sequence of more than mm_cap k-mers, sets *abort
+// and skips the writes (the caller reads so.koff[n] and so.stat later).
+// Otherwise so.M, so.max_m and so.n_overflow are set.  so.syncs counts the
+// host synchronisations.
 static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off, int32_t n,
                          SketchOut &so, DevBuf<uint64_t> &out_h, DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s,
                          uint32_t max_grid, hipStream_t st, hipEvent_t *ev_count, hipEvent_t *ev_write,
-                         bool full_koff) {
+                         bool full_koff, const int64_t *h_off = nullptr, uint32_t *abort = nullptr,
+                         uint64_t m_cap = 0, uint64_t mm_cap = 0) {
     // unit table: units of LANE_W windows per sequence, scanned on the device;
     // a chunk is WAVE consecutive units
     SeqSet S{d_bases, d_off, n};
@@ -201,12 +224,21 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.part.ensure(SCAN_PARTS + 1));
     RCHK(so.h_sum.ensure(3));
     RCHK(so.ovf_n.ensure(1));
+    RCHK(so.stat.ensure(1));
     launch_unit_counts(S, sk->k, so.nunit.p, st);
     launch_scan(so.nunit.p, so.unit_start.p, nullptr, n, so.part.p, st);
-    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.unit_start.p + n, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemsetAsync(so.ovf_n.p, 0, 4, st));  // cleared while the host waits
-    HIPCHK(hipStreamSynchronize(st));
-    const uint64_t U = so.h_sum.p[0], C = (U + WAVE - 1) / WAVE;
+    launch_fill_u32(so.ovf_n.p, 1, 0, st);
+    launch_fill_u64(so.stat.p, 1, 0, st);
+    uint64_t U;
+    if (h_off) {
+        U = count_units(sk->k, h_off, n);
+    } else {
+        HIPCHK(hipMemcpyAsync(so.h_sum.p, so.unit_start.p + n, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        so.syncs++;
+        U = so.h_sum.p[0];
+    }
+    const uint64_t C = (U + WAVE - 1) / WAVE;
     RCHK(so.chunk_cnt.ensure(C + 1));
     RCHK(so.chunk_koff.ensure(C + 1));
     RCHK(so.lane_cnt.ensure(C * WAVE + 1));
@@ -224,33 +256,41 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
                       so.lane_pre.p, so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
-    RCHK(so.stat.ensure(1));
-    HIPCHK(hipMemsetAsync(so.stat.p, 0, 8, st));
     launch_seq_koff(n, so.unit_start.p, so.chunk_koff.p, so.lane_pre.p, so.koff.p, so.stat.p, st);
     if (full_koff) {
         so.h_koff.resize(n + 1);
         HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
     }
-    RCHK(so.h_sum.ensure(3));
-    so.h_sum.p[2] = 0;
-    HIPCHK(hipMemcpyAsync(so.h_sum.p, so.koff.p + n, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(so.h_sum.p + 1, so.stat.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(so.h_sum.p + 2, ovf_count, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    so.M = so.h_sum.p[0];
-    so.max_m = so.h_sum.p[1];
-    so.n_overflow = (uint32_t)so.h_sum.p[2];
-    const uint64_t M = so.M;
-    RCHK(out_h.ensure(M + 1));
-    RCHK(out_r.ensure(M + 1));
-    RCHK(out_s.ensure(M + 1));
+    uint64_t M = m_cap;
+    if (abort) {
+        RCHK(out_h.ensure(m_cap + 1));
+        RCHK(out_r.ensure(m_cap + 1));
+        RCHK(out_s.ensure(m_cap + 1));
+        launch_gate(so.koff.p + n, m_cap, so.stat.p, mm_cap, nullptr, 0, nullptr, 0, abort, st);
+    } else {
+        so.h_sum.p[2] = 0;
+        HIPCHK(hipMemcpyAsync(so.h_sum.p, so.koff.p + n, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(so.h_sum.p + 1, so.stat.p, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(so.h_sum.p + 2, ovf_count, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        so.syncs++;
+        so.M = M = so.h_sum.p[0];
+        so.max_m = so.h_sum.p[1];
+        so.n_overflow = (uint32_t)so.h_sum.p[2];
+        RCHK(out_h.ensure(M + 1));
+        RCHK(out_r.ensure(M + 1));
+        RCHK(out_s.ensure(M + 1));
+    }
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
-        launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.stage.p, out_h.p,
-                              out_r.p, out_s.p, max_grid, st);
-    if (so.n_overflow > 0)
+        launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.stage.p, out_h.p, out_r.p, out_s.p,
+                              max_grid, st, abort);
+    // the re-roll of overflowing chunks: their count is on the device when
+    // the host has not read it back (the kernel takes it from there)
+    if (C > 0 && (abort || so.n_overflow > 0))
         launch_sketch(1, S, sk->k, sk->thres, so.unit_start.p, so.unit_seq.p, U, so.chunk_koff.p, so.lane_cnt.p,
-                      so.lane_pre.p, nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count, so.n_overflow, max_grid, st);
+                      so.lane_pre.p, nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count,
+                      abort ? C : so.n_overflow, max_grid, st, abort);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
@@ -914,7 +954,27 @@ extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **o
 static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "seed_wg", "tab_init",
                                             "contrib", "extract",        "prune",     "refine",  "final"};
 
-constexpr int SHM_N_COUNTS = 9;  // see shm_mapper_last_counts
+constexpr int SHM_N_COUNTS = 11;  // see shm_mapper_last_counts
+
+// The sizes batches have had so far, from which a batch's buffers are sized
+// before its own sizes are known (a speculative batch): the largest k-mers
+// per base, table slots and buckets per k-mer, and the capacities (in items)
+// the speculative buffers were last sized to.
+struct BatchShape {
+    bool seen = false;
+    double m_per_base = 0, t_per_m = 0, b_per_m = 0, mm_per_base = 0, e_per_m = 0;
+    uint64_t M = 0, T = 0, B = 0, max_m = 0;
+    void learn(uint64_t bases, uint64_t M_, uint64_t max_m_, int64_t max_len, uint64_t T_, uint64_t B_,
+               uint64_t E_) {
+        if (bases == 0 || M_ == 0) return;
+        seen = true;
+        e_per_m = fmax(e_per_m, (double)E_ / (double)M_);
+        m_per_base = fmax(m_per_base, (double)M_ / (double)bases);
+        if (max_len > 0) mm_per_base = fmax(mm_per_base, (double)max_m_ / (double)max_len);
+        t_per_m = fmax(t_per_m, (double)T_ / (double)M_);
+        b_per_m = fmax(b_per_m, (double)B_ / (double)M_);
+    }
+};
 
 struct shm_mapper {
     const shm_index *idx = nullptr;
@@ -929,6 +989,8 @@ struct shm_mapper {
     hipEvent_t t0 = nullptr, t1 = nullptr;  // a batch's first and last GPU work
     hipEvent_t ev_up = nullptr;              // the end of a batch's upload
     int32_t max_read_len = 0;  // reads longer than this fail alone (0: no limit)
+    bool speculate = true;     // size batches from earlier ones and synchronise once
+    BatchShape shape;
     double stage_ms[ST_COUNT] = {};
     double total_ms = 0;
     int64_t counts[SHM_N_COUNTS] = {};
@@ -968,6 +1030,8 @@ struct shm_mapper {
     PinnedBuf<uint32_t> h_mapped;
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
+    DevBuf<uint32_t> abort_flag;  // set by the capacity checks of a speculative batch
+    PinnedBuf<uint64_t> h_spec;   // its sizes and flag, read back with its records
 
     // Frees every batch buffer (after a failed allocation, so that a retry
     // with fewer reads starts from the device memory a smaller batch needs).
@@ -984,7 +1048,8 @@ struct shm_mapper {
         bk_i32.release(), bs_alive.release(), bs_sh.release(), sc_j.release(), g_sid.release();
         g_tr.release(), g_diff.release(), g_tails.release(), tasks.release(), sel.release(), big.release();
         ext.release(), ov.release(), ov_kpre.release(), mapped.release(), h_mapped.release();
-        task_count.release(), d_out.release();
+        task_count.release(), d_out.release(), abort_flag.release();
+        shape.M = shape.T = shape.B = shape.max_m = 0;  // the speculative capacities went with them
     }
 };
 
@@ -1047,6 +1112,12 @@ extern "C" int shm_device_compute_units(int32_t device, int32_t *cus) {
 extern "C" int shm_mapper_set_timing(shm_mapper *mp, int32_t on) {
     if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
     mp->timing = on != 0;
+    return SHM_OK;
+}
+
+extern "C" int shm_mapper_set_speculation(shm_mapper *mp, int32_t on) {
+    if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
+    mp->speculate = on != 0;
     return SHM_OK;
 }
 
@@ -1185,29 +1256,25 @@ static int flush_timing(shm_mapper *mp, std::vector<int> &pending) {
     return SHM_OK;
 }
 
-// Maps R reads whose bases and offsets are on the device.
-static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R, shm_read_result *out) {
-    hipStream_t st = mp->st;
-    const shm_index *ix = mp->idx;
-    const DevIndex I = ix->view();
-    for (int s = 0; s < ST_COUNT; s++) mp->stage_ms[s] = 0;
-    std::vector<int> pending;
-    HIPCHK(hipEventRecord(mp->t0, st));
-    for (int i = 0; i < SHM_N_COUNTS; i++) mp->counts[i] = 0;
-    if (R == 0) {
-        mp->total_ms = 0;
+// Results go straight into the caller's buffer when it is page-locked memory
+// the device can address (k_final's stores cross the link while it runs),
+// else into d_out and one copy at the end.
+static int result_target(shm_mapper *mp, shm_read_result *out, int32_t R, shm_read_result **res_dev, bool *direct) {
+    hipPointerAttribute_t pa;
+    if (hipPointerGetAttributes(&pa, out) == hipSuccess && pa.type == hipMemoryTypeHost && pa.devicePointer) {
+        *res_dev = (shm_read_result *)pa.devicePointer;
+        *direct = true;
         return SHM_OK;
     }
+    (void)hipGetLastError();  // pageable memory: the query's error is not ours
+    RCHK(mp->d_out.ensure(R));
+    *res_dev = mp->d_out.p;
+    *direct = false;
+    return SHM_OK;
+}
 
-    // sketch
-    RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
-                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
-                       false));
-    if (mp->timing) {
-        pending.push_back(ST_SKETCH);
-        pending.push_back(ST_SKETCH_COMPACT);
-    }
-    const uint64_t M = mp->so.M;
+// Per-read and per-k-mer buffers of a batch of R reads and up to M k-mers.
+static int ensure_read_buffers(shm_mapper *mp, int32_t R, uint64_t M) {
     RCHK(mp->ps_h.ensure(M + 1));
     RCHK(mp->ps_v.ensure(M + 1));
     RCHK(mp->sd_h.ensure(M + 1));
@@ -1221,23 +1288,24 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     RCHK(mp->sd_epre.ensure(M + 1));
     RCHK(mp->rd_i32.ensure((size_t)R * 12 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
-    // Results go straight into the caller's buffer when it is page-locked
-    // memory the device can address (k_final's stores cross the link while it
-    // runs), else into d_out and one copy at the end.
-    shm_read_result *res_dev = nullptr;
-    {
-        hipPointerAttribute_t pa;
-        if (hipPointerGetAttributes(&pa, out) == hipSuccess && pa.type == hipMemoryTypeHost && pa.devicePointer)
-            res_dev = (shm_read_result *)pa.devicePointer;
-        else
-            (void)hipGetLastError();  // pageable memory: the query's error is not ours
-    }
-    const bool direct = res_dev != nullptr;
-    if (!direct) {
-        RCHK(mp->d_out.ensure(R));
-        res_dev = mp->d_out.p;
-    }
+    RCHK(mp->big.ensure((size_t)R + 2));
+    RCHK(mp->grp_base.ensure(3 * (size_t)(R + 1)));
+    RCHK(mp->grp_nb.ensure((size_t)R + 1));
+    RCHK(mp->part.ensure(SCAN_PARTS + 1));
+    RCHK(mp->stat.ensure(1));
+    RCHK(mp->mapped.ensure((size_t)R + 2));
+    RCHK(mp->u_total.ensure(1));
+    RCHK(mp->h_mapped.ensure((size_t)R + 1));
+    RCHK(mp->h_sum.ensure(6));
+    RCHK(mp->ov.ensure((size_t)R + 1));
+    RCHK(mp->ov_kpre.ensure((size_t)R + 2));
+    RCHK(mp->ext.ensure((size_t)R + 2));
+    return SHM_OK;
+}
 
+// The read-level arguments of the mapping kernels.
+static MapArgs read_args(shm_mapper *mp, int32_t R, const int64_t *d_off) {
+    const shm_index *ix = mp->idx;
     MapArgs A{};
     A.R = R;
     A.k = mp->prm.k;
@@ -1285,12 +1353,288 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     A.rd_cap = mp->rd_u64.p + R;
     A.r0 = 0;
     A.RG = R;
-
-    const bool any_large = mp->so.max_m > WAVE_SEED_CAP;  // a read whose sketch exceeds one wave's LDS slice
-    RCHK(mp->big.ensure((size_t)R + 2));
     A.big_count = mp->big.p;
     A.big_list = mp->big.p + 1;
-    HIPCHK(hipMemsetAsync(A.big_count, 0, 4, st));
+    A.mapped_count = mp->mapped.p;
+    A.mapped_list = mp->mapped.p + 1;
+    A.u_total = mp->u_total.p;
+    return A;
+}
+
+// Bucket and refinement buffers of a group of RG reads with up to B buckets
+// and T table slots, whose reads have halflen at most max_hl and at most max_m
+// k-mers.
+static int ensure_group_buffers(shm_mapper *mp, uint64_t T, uint64_t B, int32_t RG, int32_t max_hl, uint64_t max_m) {
+    RCHK(mp->tb_key.ensure(T + 1));
+    RCHK(mp->tb_list.ensure(T + 1));
+    RCHK(mp->tb_mc.ensure(T + 1));
+    RCHK(mp->tb_min.ensure(T + 1));
+    RCHK(mp->tb_max.ensure(T + 1));
+    RCHK(mp->bk_key.ensure(B + 1));
+    RCHK(mp->bk_val.ensure(B + 1));
+    RCHK(mp->bs_g.ensure(B + 1));
+    RCHK(mp->bk_i32.ensure(11 * (B + 1)));
+    RCHK(mp->bs_alive.ensure(B + 1));
+    RCHK(mp->bs_sh.ensure(B + 1));
+    RCHK(mp->sc_j.ensure(B + 1));
+    RCHK(mp->tasks.ensure(B + 1));
+    RCHK(mp->sel.ensure(B + (uint64_t)RG + 1));
+    // per-workgroup scratch of refinement, for the arrays of a span or read
+    // that do not fit the workgroup's LDS (a span has at most 2 * hl
+    // positions; the grid is max_grid workgroups)
+    const uint64_t stride = 2ull * (uint64_t)max_hl + 2;
+    const uint64_t dstride = max_m + 1;
+    RCHK(mp->g_sid.ensure(stride * mp->max_grid + 1));
+    RCHK(mp->g_tr.ensure(stride * mp->max_grid + 1));
+    RCHK(mp->g_diff.ensure(dstride * mp->max_grid + 1));
+    RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
+    return SHM_OK;
+}
+
+// The table arguments of group [r0, r0 + RG), its offsets at grp_base.
+static void table_args(shm_mapper *mp, MapArgs &A, int32_t r0, int32_t RG, uint32_t *task_count, uint64_t E_grp) {
+    A.r0 = r0;
+    A.RG = RG;
+    A.hit_base = mp->grp_base.p;
+    A.tab_base = mp->grp_base.p + (RG + 1);
+    A.bk_base = mp->grp_base.p + 2 * (size_t)(RG + 1);
+    A.grp_nb = mp->grp_nb.p;
+    A.tb_key = mp->tb_key.p;
+    A.tb_list = mp->tb_list.p;
+    A.tb_mc = mp->tb_mc.p;
+    A.tb_min = mp->tb_min.p;
+    A.tb_max = mp->tb_max.p;
+    // the group's task counters ([6] counts the reads k_contrib_wave hands
+    // to the global tables)
+    A.task_count = task_count;
+    A.ov_count = task_count + 6;
+    A.ov_list = mp->ov.p;
+    A.ov_kpre = mp->ov_kpre.p;
+    // a big batch fills the device with whole reads; a small one splits
+    // long reads so no read's serial walk sets the batch's time
+    A.contrib_wave_e = mp->tune.contrib_wave_hits > 0 ? (uint32_t)mp->tune.contrib_wave_hits
+                                                      : (E_grp >= (1ull << 24) ? 4096u : 1024u);
+    A.contrib_chunk = (uint32_t)mp->tune.contrib_chunk_hits;
+    A.prune_seq = mp->tune.prune_thread_seeds > 0 ? (uint32_t)mp->tune.prune_thread_seeds : 16u;
+}
+
+// The bucket arguments of a group whose buffers hold B buckets.
+static void bucket_args(shm_mapper *mp, MapArgs &A, uint64_t B, int32_t max_hl, uint64_t max_m) {
+    A.ext_count = mp->ext.p;
+    A.ext_list = mp->ext.p + 1;
+    A.bk_key = mp->bk_key.p;
+    A.bk_val = mp->bk_val.p;
+    A.bs_g = mp->bs_g.p;
+    int32_t *bi = mp->bk_i32.p;
+    const size_t Bs = B + 1;
+    A.bs_m = bi + 0 * Bs;
+    A.bs_c = bi + 1 * Bs;
+    A.bs_min = bi + 2 * Bs;
+    A.bs_max = bi + 3 * Bs;
+    A.sc_i = bi + 4 * Bs;
+    A.sc_same = bi + 5 * Bs;
+    A.sc_tl = bi + 6 * Bs;
+    A.sc_tr = bi + 7 * Bs;
+    A.sc_ssz = bi + 8 * Bs;
+    A.sc_pend = bi + 9 * Bs;
+    A.sc_seg = bi + 10 * Bs;
+    A.bs_alive = mp->bs_alive.p;
+    A.bs_sh = mp->bs_sh.p;
+    A.sc_j = mp->sc_j.p;
+    A.g_sid = mp->g_sid.p;
+    A.g_tr = mp->g_tr.p;
+    A.sid_stride = 2ull * (uint64_t)max_hl + 2;
+    A.g_diff = mp->g_diff.p;
+    A.g_tails = mp->g_tails.p;
+    A.diff_stride = max_m + 1;
+    A.tasks = mp->tasks.p;
+    A.sel = mp->sel.p;
+    A.task_cap = (uint32_t)(B + 1);
+}
+
+// What every batch ends with once its records and counters are on the host:
+// the host's double-precision finalisation of the mapped reads and the work
+// counts.
+static void finish_batch(shm_mapper *mp, shm_read_result *out, const std::vector<uint32_t> &ntask, int32_t ngroups,
+                         uint64_t M, uint64_t seeds, int64_t nbuckets, uint64_t E_all) {
+    const uint32_t n_mapped = mp->h_mapped.p[0];
+    for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
+    int64_t cand = 0, deferred = 0, refined = 0, global_reads = 0;
+    for (int32_t g = 0; g < ngroups; g++) {
+        const uint32_t *tc = &ntask[TASK_COUNTERS * (size_t)g];
+        cand += tc[0];
+        deferred += tc[1];
+        refined += tc[2] + tc[3] + tc[4];
+        global_reads += tc[6];
+    }
+    mp->counts[0] = (int64_t)M;
+    mp->counts[1] = (int64_t)seeds;
+    mp->counts[2] = nbuckets;
+    mp->counts[3] = refined;
+    mp->counts[4] = ngroups;
+    mp->counts[5] = cand;
+    mp->counts[6] = deferred;
+    mp->counts[7] = (int64_t)E_all;
+    mp->counts[8] = global_reads;
+}
+
+// Bases and longest read of a batch, from host offsets.
+static void batch_extent(const int64_t *h_off, int32_t R, uint64_t *bases, int64_t *max_len) {
+    int64_t ml = 0;
+    for (int32_t i = 0; i < R; i++) ml = std::max(ml, h_off[i + 1] - h_off[i]);
+    *bases = (uint64_t)(h_off[R] - h_off[0]);
+    *max_len = ml;
+}
+
+// Maps R reads with one host synchronisation, at the end.  Every buffer is
+// sized before the first launch from the shapes of earlier batches, with a
+// margin; device checks after the sketch (k-mers, the largest sketch), after
+// seeding (table slots) and after counting (buckets) set an abort flag when a
+// size exceeds its buffer, and every later kernel then returns at once.
+// *done is false when the batch stopped so (or would need table groups): the
+// caller maps it again by map_synced.
+static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R,
+                           shm_read_result *out, const int64_t *h_off, bool *done) {
+    *done = false;
+    hipStream_t st = mp->st;
+    const DevIndex I = mp->idx->view();
+    BatchShape &sh = mp->shape;
+    uint64_t bases;
+    int64_t max_len;
+    batch_extent(h_off, R, &bases, &max_len);
+    // capacities: this batch's estimates with a margin, never below what the
+    // buffers were last sized to (buffers only grow)
+    auto grow = [](uint64_t have, double want) {
+        const uint64_t w = (uint64_t)(want * 1.15) + 1024;
+        return w > have ? w : have;
+    };
+    const double M_est = (double)bases * sh.m_per_base;
+    if (M_est * sh.t_per_m * 1.15 + 1024 > (double)mp->tab_budget) return SHM_OK;  // table groups
+    sh.M = grow(sh.M, M_est);
+    sh.max_m = grow(sh.max_m, (double)max_len * 2.0 * fmax(sh.mm_per_base, sh.m_per_base));
+    if (sh.max_m > sh.M) sh.max_m = sh.M;
+    sh.T = std::min<uint64_t>(grow(sh.T, (double)sh.M * sh.t_per_m), mp->tab_budget);
+    sh.B = std::min<uint64_t>(grow(sh.B, (double)sh.M * sh.b_per_m), (uint64_t)UINT32_MAX - 1);
+    // a read's halflen is its k-mer count, or its length with absolute positions
+    const int32_t max_hl = mp->prm.abs_pos ? (int32_t)max_len : (int32_t)sh.max_m;
+
+    shm_read_result *res_dev;
+    bool direct;
+    RCHK(result_target(mp, out, R, &res_dev, &direct));
+    RCHK(ensure_read_buffers(mp, R, sh.M));
+    RCHK(ensure_group_buffers(mp, sh.T, sh.B, R, max_hl, sh.max_m));
+    RCHK(mp->task_count.ensure(TASK_COUNTERS + 1));
+    RCHK(mp->abort_flag.ensure(1));
+    RCHK(mp->h_spec.ensure(8));
+    uint32_t *abort = mp->abort_flag.p;
+
+    std::vector<int> pending;
+    HIPCHK(hipEventRecord(mp->t0, st));
+    launch_fill_u32(abort, 1, 0, st);
+    mp->so.syncs = 0;
+    RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
+                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
+                       false, h_off, abort, sh.M, sh.max_m));
+    if (mp->timing) {
+        pending.push_back(ST_SKETCH);
+        pending.push_back(ST_SKETCH_COMPACT);
+    }
+    // the sketch kernels grew the k-mer buffers to sh.M + 1 items already
+    MapArgs A = read_args(mp, R, d_off);
+    A.abort = abort;
+    launch_fill_u32(A.big_count, 1, 0, st);
+    for (Stage s : {ST_SEED_WAVE, ST_SEED_WG}) {
+        TIME_BEGIN(s);
+        launch_stage(s, A, I, nullptr, 0, mp->max_grid, st);
+        TIME_END(s);
+    }
+    launch_fill_u64(mp->stat.p, 1, 0, st);
+    launch_max_i32(A.rd_hl, R, mp->stat.p, st);
+    launch_scan(A.rd_E, mp->grp_base.p, nullptr, R, mp->part.p, st);
+    launch_scan(A.rd_cap, mp->grp_base.p + (R + 1), nullptr, R, mp->part.p, st);
+    launch_gate(mp->grp_base.p + (R + 1) + R, sh.T, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
+    launch_fill_u32(mp->mapped.p, 1, 0, st);
+    launch_fill_u64(mp->u_total.p, 1, 0, st);
+    uint32_t *task_count = mp->task_count.p;
+    launch_fill_u32(task_count, TASK_COUNTERS, 0, st);
+    // the hit count only picks the contribution split, which records never
+    // depend on: the batch's estimate stands in for it
+    table_args(mp, A, 0, R, task_count, (uint64_t)(M_est * sh.e_per_m));
+    TIME_BEGIN(ST_TAB_INIT);
+    launch_stage(ST_TAB_INIT, A, I, nullptr, R, mp->max_grid, st);
+    TIME_END(ST_TAB_INIT);
+    TIME_BEGIN(ST_CONTRIB);
+    launch_stage(ST_CONTRIB, A, I, nullptr, 0, mp->max_grid, st);
+    TIME_END(ST_CONTRIB);
+    launch_scan(mp->grp_nb.p, mp->grp_base.p + 2 * (size_t)(R + 1), nullptr, R, mp->part.p, st);
+    launch_gate(mp->grp_base.p + 2 * (size_t)(R + 1) + R, sh.B, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
+    launch_fill_u32(mp->ext.p, 1, 0, st);
+    bucket_args(mp, A, sh.B, max_hl, sh.max_m);
+    for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
+        TIME_BEGIN(s);
+        launch_stage(s, A, I, res_dev, sh.B, mp->max_grid, st);
+        TIME_END(s);
+    }
+    HIPCHK(hipGetLastError());
+    // everything the host needs, then the batch's one synchronisation
+    std::vector<uint32_t> ntask(TASK_COUNTERS);
+    uint64_t *hs = mp->h_spec.p;
+    hs[7] = 0;
+    HIPCHK(hipMemcpyAsync(&hs[7], abort, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[0], mp->so.koff.p + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[1], mp->so.stat.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[2], mp->grp_base.p + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[3], mp->grp_base.p + (R + 1) + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[4], mp->grp_base.p + 2 * (size_t)(R + 1) + R, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hs[5], A.u_total, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ntask.data(), task_count, TASK_COUNTERS * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (!direct)
+        HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(mp->t1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    RCHK(flush_timing(mp, pending));
+    if ((uint32_t)hs[7] != 0) return SHM_OK;  // a size outgrew its buffer: mapped again, synchronised
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, mp->t0, mp->t1));
+    mp->total_ms = ms;
+    const uint64_t M = hs[0], E_all = hs[2], T_all = hs[3], B = hs[4];
+    finish_batch(mp, out, ntask, 1, M, hs[5], (int64_t)B, E_all);
+    mp->counts[9] = 1;
+    sh.learn(bases, M, hs[1], max_len, T_all, B, E_all);
+    *done = true;
+    return SHM_OK;
+}
+
+// Maps R reads whose bases and offsets are on the device, reading each size
+// back before sizing what depends on it: the sketch's k-mer count (and, with
+// no host offsets, its unit count), the table and hit totals after seeding,
+// and each table group's bucket count.  Tables of more slots than the table
+// budget are split into groups of reads here.
+static int map_synced(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R, shm_read_result *out,
+                      const int64_t *h_off) {
+    hipStream_t st = mp->st;
+    const DevIndex I = mp->idx->view();
+    std::vector<int> pending;
+    HIPCHK(hipEventRecord(mp->t0, st));
+    mp->so.syncs = 0;
+    RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
+                       mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
+                       false, h_off));
+    int syncs = mp->so.syncs;
+    if (mp->timing) {
+        pending.push_back(ST_SKETCH);
+        pending.push_back(ST_SKETCH_COMPACT);
+    }
+    const uint64_t M = mp->so.M;
+    shm_read_result *res_dev;
+    bool direct;
+    RCHK(result_target(mp, out, R, &res_dev, &direct));
+    RCHK(ensure_read_buffers(mp, R, M));
+    MapArgs A = read_args(mp, R, d_off);
+
+    const bool any_large = mp->so.max_m > WAVE_SEED_CAP;  // a read whose sketch exceeds one wave's LDS slice
+    launch_fill_u32(A.big_count, 1, 0, st);
     for (Stage s : {ST_SEED_WAVE, ST_SEED_WG}) {
         if (s == ST_SEED_WG && !any_large) continue;
         TIME_BEGIN(s);
@@ -1300,12 +1644,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     HIPCHK(hipGetLastError());
     // hit and table offsets of every read by device scans; only the totals and
     // the largest halflen come back
-    RCHK(mp->grp_base.ensure(3 * (size_t)(R + 1)));
-    RCHK(mp->grp_nb.ensure((size_t)R + 1));
-    RCHK(mp->part.ensure(SCAN_PARTS + 1));
-    RCHK(mp->stat.ensure(1));
-    RCHK(mp->h_sum.ensure(4));
-    HIPCHK(hipMemsetAsync(mp->stat.p, 0, 8, st));
+    launch_fill_u64(mp->stat.p, 1, 0, st);
     launch_max_i32(A.rd_hl, R, mp->stat.p, st);
     launch_scan(A.rd_E, mp->grp_base.p, nullptr, R, mp->part.p, st);
     launch_scan(A.rd_cap, mp->grp_base.p + (R + 1), nullptr, R, mp->part.p, st);
@@ -1313,11 +1652,10 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 1, mp->grp_base.p + (R + 1) + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 2, mp->stat.p, 8, hipMemcpyDeviceToHost, st));
     // batch counters of k_final, cleared while the host waits
-    RCHK(mp->mapped.ensure((size_t)R + 2));
-    RCHK(mp->u_total.ensure(1));
-    HIPCHK(hipMemsetAsync(mp->mapped.p, 0, 4, st));
-    HIPCHK(hipMemsetAsync(mp->u_total.p, 0, 8, st));
+    launch_fill_u32(mp->mapped.p, 1, 0, st);
+    launch_fill_u64(mp->u_total.p, 1, 0, st);
     HIPCHK(hipStreamSynchronize(st));
+    syncs++;
     RCHK(flush_timing(mp, pending));
     const uint64_t E_all = mp->h_sum.p[0], T_all = mp->h_sum.p[1];
     const int32_t hl_all = (int32_t)mp->h_sum.p[2];
@@ -1330,11 +1668,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipMemcpyAsync(E.data(), A.rd_E, R * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(cap.data(), A.rd_cap, R * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        syncs++;
     }
 
-    A.mapped_count = mp->mapped.p;
-    A.mapped_list = mp->mapped.p + 1;
-    A.u_total = mp->u_total.p;
     // bucketing, in groups of reads whose tables fit the budget
     RCHK(mp->task_count.ensure(TASK_COUNTERS * ((size_t)R + 1)));  // refine-task counters per group
     int32_t ngroups = 0;
@@ -1343,9 +1679,6 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     while (r0 < R) {
         int32_t r1 = R;
         uint64_t T = T_all, E_grp = E_all;
-        // scratch sizing bounds: the batch's largest halflen and sketch
-        const int32_t max_hl = hl_all;
-        const uint64_t max_m = mp->so.max_m;
         if (!one_group) {
             r1 = r0;
             uint64_t tsum = 0;
@@ -1362,6 +1695,7 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
             T = tb[RG];
             E_grp = hb[RG];
             HIPCHK(hipStreamSynchronize(st));  // hb and tb leave scope
+            syncs++;
         }
         const int32_t RG = r1 - r0;
         RCHK(mp->tb_key.ensure(T + 1));
@@ -1369,32 +1703,9 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         RCHK(mp->tb_mc.ensure(T + 1));
         RCHK(mp->tb_min.ensure(T + 1));
         RCHK(mp->tb_max.ensure(T + 1));
-        A.r0 = r0;
-        A.RG = RG;
-        A.hit_base = mp->grp_base.p;
-        A.tab_base = mp->grp_base.p + (RG + 1);
-        A.bk_base = mp->grp_base.p + 2 * (size_t)(RG + 1);
-        A.grp_nb = mp->grp_nb.p;
-        A.tb_key = mp->tb_key.p;
-        A.tb_list = mp->tb_list.p;
-        A.tb_mc = mp->tb_mc.p;
-        A.tb_min = mp->tb_min.p;
-        A.tb_max = mp->tb_max.p;
-        // the group's task counters ([6] counts the reads k_contrib_wave hands
-        // to the global tables)
         uint32_t *task_count = mp->task_count.p + TASK_COUNTERS * (size_t)ngroups;
-        HIPCHK(hipMemsetAsync(task_count, 0, TASK_COUNTERS * 4, st));
-        RCHK(mp->ov.ensure((size_t)RG + 1));
-        RCHK(mp->ov_kpre.ensure((size_t)RG + 2));
-        A.ov_count = task_count + 6;
-        A.ov_list = mp->ov.p;
-        A.ov_kpre = mp->ov_kpre.p;
-        // a big batch fills the device with whole reads; a small one splits
-        // long reads so no read's serial walk sets the batch's time
-        A.contrib_wave_e = mp->tune.contrib_wave_hits > 0 ? (uint32_t)mp->tune.contrib_wave_hits
-                                                          : (E_grp >= (1ull << 24) ? 4096u : 1024u);
-        A.contrib_chunk = (uint32_t)mp->tune.contrib_chunk_hits;
-        A.prune_seq = mp->tune.prune_thread_seeds > 0 ? (uint32_t)mp->tune.prune_thread_seeds : 16u;
+        launch_fill_u32(task_count, TASK_COUNTERS, 0, st);
+        table_args(mp, A, r0, RG, task_count, E_grp);
         TIME_BEGIN(ST_TAB_INIT);
         launch_stage(ST_TAB_INIT, A, I, nullptr, RG, mp->max_grid, st);
         TIME_END(ST_TAB_INIT);
@@ -1406,60 +1717,13 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         HIPCHK(hipMemcpyAsync(mp->h_sum.p + 3, mp->grp_base.p + 2 * (size_t)(RG + 1) + RG, 8, hipMemcpyDeviceToHost,
                               st));
         // the group's extract list, cleared while the host waits
-        RCHK(mp->ext.ensure((size_t)RG + 2));
-        HIPCHK(hipMemsetAsync(mp->ext.p, 0, 4, st));
+        launch_fill_u32(mp->ext.p, 1, 0, st);
         HIPCHK(hipStreamSynchronize(st));
+        syncs++;
         RCHK(flush_timing(mp, pending));
         const uint64_t B = mp->h_sum.p[3];
-        RCHK(mp->bk_key.ensure(B + 1));
-        RCHK(mp->bk_val.ensure(B + 1));
-        RCHK(mp->bs_g.ensure(B + 1));
-        RCHK(mp->bk_i32.ensure(11 * (B + 1)));
-        RCHK(mp->bs_alive.ensure(B + 1));
-        RCHK(mp->bs_sh.ensure(B + 1));
-        RCHK(mp->sc_j.ensure(B + 1));
-        // per-workgroup scratch of refinement, for the arrays of a span or read
-        // that do not fit the workgroup's LDS (a span has at most 2 * hl
-        // positions; the grid is max_grid workgroups)
-        const uint64_t stride = 2ull * (uint64_t)max_hl + 2;
-        const uint64_t dstride = max_m + 1;
-        RCHK(mp->g_sid.ensure(stride * mp->max_grid + 1));
-        RCHK(mp->g_tr.ensure(stride * mp->max_grid + 1));
-        RCHK(mp->g_diff.ensure(dstride * mp->max_grid + 1));
-        RCHK(mp->g_tails.ensure(mp->prm.metric == SHM_BUCKET_LCS ? dstride * mp->max_grid + 1 : 1));
-        RCHK(mp->tasks.ensure(B + 1));
-        A.ext_count = mp->ext.p;
-        A.ext_list = mp->ext.p + 1;
-        RCHK(mp->sel.ensure(B + (uint64_t)RG + 1));
-        A.bk_key = mp->bk_key.p;
-        A.bk_val = mp->bk_val.p;
-        A.bs_g = mp->bs_g.p;
-        int32_t *bi = mp->bk_i32.p;
-        const size_t Bs = B + 1;
-        A.bs_m = bi + 0 * Bs;
-        A.bs_c = bi + 1 * Bs;
-        A.bs_min = bi + 2 * Bs;
-        A.bs_max = bi + 3 * Bs;
-        A.sc_i = bi + 4 * Bs;
-        A.sc_same = bi + 5 * Bs;
-        A.sc_tl = bi + 6 * Bs;
-        A.sc_tr = bi + 7 * Bs;
-        A.sc_ssz = bi + 8 * Bs;
-        A.sc_pend = bi + 9 * Bs;
-        A.sc_seg = bi + 10 * Bs;
-        A.bs_alive = mp->bs_alive.p;
-        A.bs_sh = mp->bs_sh.p;
-        A.sc_j = mp->sc_j.p;
-        A.g_sid = mp->g_sid.p;
-        A.g_tr = mp->g_tr.p;
-        A.sid_stride = stride;
-        A.g_diff = mp->g_diff.p;
-        A.g_tails = mp->g_tails.p;
-        A.diff_stride = dstride;
-        A.tasks = mp->tasks.p;
-        A.sel = mp->sel.p;
-        A.task_count = task_count;
-        A.task_cap = (uint32_t)(B + 1);
+        RCHK(ensure_group_buffers(mp, T, B, RG, hl_all, mp->so.max_m));
+        bucket_args(mp, A, B, hl_all, mp->so.max_m);
         for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
             TIME_BEGIN(s);
             launch_stage(s, A, I, res_dev, B, mp->max_grid, st);
@@ -1474,37 +1738,49 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
     HIPCHK(hipMemcpyAsync(ntask.data(), mp->task_count.p, ntask.size() * 4, hipMemcpyDeviceToHost, st));
     if (!direct)
         HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
-    RCHK(mp->h_mapped.ensure((size_t)R + 1));
-    RCHK(mp->h_sum.ensure(6));
     HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_sum.p + 4, A.u_total, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(mp->t1, st));
     HIPCHK(hipStreamSynchronize(st));
+    syncs++;
     RCHK(flush_timing(mp, pending));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, mp->t0, mp->t1));
     mp->total_ms = ms;
-    // only mapped reads need the host's double-precision finalization
-    const uint32_t n_mapped = mp->h_mapped.p[0];
-    for (uint32_t i = 0; i < n_mapped; i++) finalize_result(mp->prm, out[mp->h_mapped.p[1 + i]]);
-    int64_t cand = 0, deferred = 0, refined = 0, seeds = 0, global_reads = 0;
-    for (int32_t g = 0; g < ngroups; g++) {
-        const uint32_t *tc = &ntask[TASK_COUNTERS * (size_t)g];
-        cand += tc[0];
-        deferred += tc[1];
-        refined += tc[2] + tc[3] + tc[4];
-        global_reads += tc[6];
+    finish_batch(mp, out, ntask, ngroups, M, mp->h_sum.p[4], nbuckets, E_all);
+    mp->counts[9] = syncs;
+    if (h_off && one_group) {
+        uint64_t bases;
+        int64_t max_len;
+        batch_extent(h_off, R, &bases, &max_len);
+        mp->shape.learn(bases, M, mp->so.max_m, max_len, T_all, (uint64_t)nbuckets, E_all);
     }
-    seeds = (int64_t)mp->h_sum.p[4];
-    mp->counts[0] = (int64_t)M;
-    mp->counts[1] = seeds;
-    mp->counts[2] = nbuckets;
-    mp->counts[3] = refined;
-    mp->counts[4] = ngroups;
-    mp->counts[5] = cand;
-    mp->counts[6] = deferred;
-    mp->counts[7] = (int64_t)E_all;
-    mp->counts[8] = global_reads;
+    return SHM_OK;
+}
+
+// Maps R reads whose bases and offsets are on the device.  With the host's
+// copy of the offsets (h_off) a batch is speculative (map_speculative), once
+// an earlier batch has shown the mapper its shape; otherwise, and when a
+// speculative batch stops, it is mapped by map_synced.
+static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R, shm_read_result *out,
+                      const int64_t *h_off = nullptr) {
+    for (int s = 0; s < ST_COUNT; s++) mp->stage_ms[s] = 0;
+    for (int i = 0; i < SHM_N_COUNTS; i++) mp->counts[i] = 0;
+    if (R == 0) {
+        mp->total_ms = 0;
+        return SHM_OK;
+    }
+    int64_t redone = 0;
+    if (h_off && mp->speculate && mp->shape.seen) {
+        bool done = false;
+        RCHK(map_speculative(mp, d_seq, d_off, R, out, h_off, &done));
+        if (done) return SHM_OK;
+        for (int s = 0; s < ST_COUNT; s++) mp->stage_ms[s] = 0;
+        redone = 1;
+    }
+    RCHK(map_synced(mp, d_seq, d_off, R, out, h_off));
+    mp->counts[9] += redone;  // the speculative attempt's synchronisation
+    mp->counts[10] = redone;
     return SHM_OK;
 }
 
@@ -1548,7 +1824,7 @@ static int map_host_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_
     if (off[n] > 0) HIPCHK(hipMemcpyAsync(mp->bases.p, src, off[n], hipMemcpyHostToDevice, mp->st));
     HIPCHK(hipMemcpyAsync(mp->roff.p, mp->h_roff.p, (n + 1) * 8, hipMemcpyHostToDevice, mp->st));
     RCHK(turn.done());
-    return map_device(mp, mp->bases.p, mp->roff.p, n, out);
+    return map_device(mp, mp->bases.p, mp->roff.p, n, out, mp->h_roff.p);
 }
 
 UploadTurn::UploadTurn(shm_mapper *m, uint64_t bytes) {
@@ -1802,7 +2078,7 @@ extern "C" int shm_map_batch_packed(shm_mapper *mp, const uint8_t *packed, const
         for (int32_t i = 0; i < n; i++) rs[i] = shm_read{h.data() + off[i], (int32_t)(off[i + 1] - off[i])};
         return shm_map_batch(mp, rs.data(), n, out);
     }
-    return map_device(mp, mp->bases.p, mp->roff.p, n, out);
+    return map_device(mp, mp->bases.p, mp->roff.p, n, out, off);
 }
 
 extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t n,

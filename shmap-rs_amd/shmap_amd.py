@@ -122,6 +122,7 @@ def lib():
         L.shm_mapper_create.argtypes = [C.c_void_p, C.POINTER(MapParams), C.POINTER(C.c_void_p)]
         L.shm_mapper_free.argtypes = [C.c_void_p]
         L.shm_mapper_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.shm_mapper_set_speculation.argtypes = [C.c_void_p, C.c_int32]
         L.shm_device_compute_units.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
         L.shm_mapper_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.shm_mapper_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
@@ -336,6 +337,10 @@ class Mapper:
     def set_timing(self, on: bool) -> None:
         _check(lib().shm_mapper_set_timing(self._p, int(on)))
 
+    def set_speculation(self, on: bool) -> None:
+        """Speculative batches (one host synchronisation each) on or off."""
+        _check(lib().shm_mapper_set_speculation(self._p, int(on)))
+
     def set_tuning(self, **fields) -> None:
         """shm_mapper_set_tuning: contrib_wave_hits, contrib_chunk_hits,
         prune_thread_seeds (0: the built-in rule); unnamed fields keep their values."""
@@ -486,7 +491,7 @@ def host_array(n: int, dtype) -> np.ndarray:
 
 
 COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band",
-               "examined_hits", "global_table_reads")
+               "examined_hits", "global_table_reads", "host_syncs", "redone")
 
 
 class PackedReads:
