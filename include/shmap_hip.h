@@ -265,20 +265,35 @@ const char *shm_mapper_stage_name(int32_t stage);
  * device-memory hash tables rather than on chip (many hits or buckets), [9]
  * host synchronisations of the batch's stream, [10] 1 when a speculative
  * batch outgrew its buffers and was mapped again (see
- * shm_mapper_set_speculation).
+ * shm_mapper_set_speculation), [11] device and page-locked buffer
+ * allocations the batch made (0 once the mapper's buffers fit its batches),
+ * [12] 1 when the batch ran as a graph launch, [13] 1 when that graph was
+ * captured by this batch (see shm_mapper_set_graphs).
  * Writes up to n values and their number to *n_written. */
 int shm_mapper_last_counts(const shm_mapper *mp, int64_t *counts, int32_t n, int32_t *n_written);
 /* Turns per-stage event timing on or off (off by default: it adds a stream
  * synchronisation per stage group). */
 int shm_mapper_set_timing(shm_mapper *mp, int32_t on);
-/* Turns speculative batches on (the default) or off.  A speculative batch of
- * shm_map_batch / shm_map_batch_packed sizes its device buffers from the
+/* Turns speculative batches of shm_map_batch and shm_map_batch_packed on (the
+ * default) or off.  A speculative batch sizes its device buffers from the
  * batches the mapper has seen, runs every kernel without reading a size back,
  * and synchronises with the host once; device-side checks stop it when a size
  * outgrows its buffer, and it is then mapped again with a read-back of each
- * size (four synchronisations, as shm_map_batch_device always does).  The
- * records are the same either way.  No reference counterpart. */
+ * size before what depends on it (three to four synchronisations, as every
+ * batch takes with speculation off, and shm_map_batch_device always: it has
+ * no host copy of the offsets).  The records are the same either way.  No
+ * reference counterpart. */
 int shm_mapper_set_speculation(shm_mapper *mp, int32_t on);
+/* Turns graph capture of speculative batches on (the default) or off.  The
+ * stream work of a speculative batch whose reads are in the mapper's own
+ * upload buffers (shm_map_batch, shm_map_batch_packed) depends only on its
+ * read count and the mapper's buffer capacities; the batch is padded with
+ * empty reads to its read class (1, 2, 3, 4, 6, 8, 12, ...: powers of two and
+ * their halfway points), captured into a HIP graph once per class and
+ * replayed for every later batch of the class, until a buffer grows.  Records
+ * are the same either way; per-stage timing (shm_mapper_set_timing) runs
+ * without graphs.  No reference counterpart. */
+int shm_mapper_set_graphs(shm_mapper *mp, int32_t on);
 
 /* Batching choices of a mapper: where per-read work moves from one execution
  * shape to another.  No reference counterpart.  Any values give the same

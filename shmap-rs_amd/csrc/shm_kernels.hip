@@ -93,7 +93,9 @@ __global__ void __launch_bounds__(WG) k_unit_counts(SeqSet S, int32_t k, uint64_
 }
 
 // Sequence of every unit: one wave per sequence writes its run of unit_seq.
-__global__ void __launch_bounds__(WG) k_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq) {
+__global__ void __launch_bounds__(WG) k_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq,
+                                                 const uint32_t *abort) {
+    if (aborted(abort)) return;
     const int lane = threadIdx.x & (WAVE - 1);
     for (int64_t s = ((int64_t)blockIdx.x * WG + threadIdx.x) / WAVE; s < n; s += (int64_t)gridDim.x * NWAVES) {
         const uint64_t a = unit_start[s], e = unit_start[s + 1];
@@ -197,8 +199,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
                                                uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                                                uint32_t *lane_pre, uint4 *stage,
                                                uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
-                                               uint32_t *ovf_list, uint32_t *ovf_count, const uint32_t *abort) {
+                                               uint32_t *ovf_list, uint32_t *ovf_count, const uint32_t *abort,
+                                               const uint64_t *n_units_dev) {
     if (aborted(abort)) return;
+    if (n_units_dev) n_units = *n_units_dev;  // sized on the device (the grid covers a capacity)
     extern __shared__ uint8_t lds[];
     // T[(out code << 3) | in code]: the xor terms of both strands for one step
     // (row 4, an N rolling out, adds nothing: the lane starts use it).
@@ -464,8 +468,10 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
                                                        const uint64_t *chunk_koff, uint32_t lcap,
                                                        const uint4 *stage, uint64_t *out_h, int32_t *out_r,
-                                                       uint8_t *out_s, const uint32_t *abort) {
+                                                       uint8_t *out_s, const uint32_t *abort,
+                                                       const uint64_t *n_units_dev) {
     if (aborted(abort)) return;
+    if (n_units_dev) n_chunks = (*n_units_dev + WAVE - 1) / WAVE;
     __shared__ uint32_t s_pre[NWAVES][WAVE];
     const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     uint32_t *pre_w = s_pre[w];
@@ -505,7 +511,9 @@ __device__ __forceinline__ uint64_t seq_koff_at(uint64_t u, const uint64_t *chun
 
 __global__ void __launch_bounds__(WG) k_seq_koff(int32_t n, const uint64_t *unit_start,
                                                  const uint64_t *chunk_koff, const uint32_t *lane_pre,
-                                                 uint64_t *koff, unsigned long long *max_m) {
+                                                 uint64_t *koff, unsigned long long *max_m,
+                                                 const uint32_t *abort) {
+    if (aborted(abort)) return;
     for (int32_t i0 = blockIdx.x * WG; i0 <= n; i0 += gridDim.x * WG) {
         const int32_t i = i0 + (int32_t)threadIdx.x;
         uint64_t m = 0;
@@ -2758,33 +2766,35 @@ uint32_t sketch_lane_cap(uint64_t thres) {
     return cap >= (double)LANE_W ? (uint32_t)LANE_W : (uint32_t)cap;
 }
 
-void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st) {
-    if (n > 0) k_unit_seq<<<flat_grid((uint64_t)n * WAVE, 4096), WG, 0, st>>>(n, unit_start, unit_seq);
+void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st,
+                     const uint32_t *abort) {
+    if (n > 0) k_unit_seq<<<flat_grid((uint64_t)n * WAVE, 4096), WG, 0, st>>>(n, unit_start, unit_seq, abort);
 }
 
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
                    const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                    uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                    uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st,
-                   const uint32_t *abort) {
+                   const uint32_t *abort, const uint64_t *n_units_dev) {
     const size_t lds = sketch_lds(k);
     const uint64_t want = (n_chunks + NWAVES - 1) / NWAVES;
     if (pass == 0)
         k_sketch<0><<<persistent_grid((const void *)k_sketch<0>, lds, want, max_grid), WG, lds, st>>>(
             S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
-            lcap, ovf_list, ovf_count, abort);
+            lcap, ovf_list, ovf_count, abort, n_units_dev);
     else
         k_sketch<1><<<persistent_grid((const void *)k_sketch<1>, lds, want, max_grid), WG, lds, st>>>(
             S, k, thres, unit_start, unit_seq, n_units, chunk_val, lane_cnt, lane_pre, stage, out_h, out_r, out_s,
-            lcap, ovf_list, ovf_count, abort);
+            lcap, ovf_list, ovf_count, abort, n_units_dev);
 }
 
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                           uint32_t max_grid, hipStream_t st, const uint32_t *abort) {
+                           uint32_t max_grid, hipStream_t st, const uint32_t *abort, const uint64_t *n_units_dev) {
     const uint32_t g =
         persistent_grid((const void *)k_sketch_compact, 0, (n_chunks + NWAVES - 1) / NWAVES, max_grid);
-    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s, abort);
+    k_sketch_compact<<<g, WG, 0, st>>>(n_chunks, lane_cnt, chunk_koff, lcap, stage, out_h, out_r, out_s, abort,
+                                       n_units_dev);
 }
 
 __global__ void __launch_bounds__(WG) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
@@ -2814,9 +2824,9 @@ void launch_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c
 }
 
 void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,
-                     uint64_t *koff, uint64_t *max_m, hipStream_t st) {
+                     uint64_t *koff, uint64_t *max_m, hipStream_t st, const uint32_t *abort) {
     k_seq_koff<<<flat_grid((uint64_t)n + 1, 1024), WG, 0, st>>>(n, unit_start, chunk_koff, lane_pre, koff,
-                                                                (unsigned long long *)max_m);
+                                                                (unsigned long long *)max_m, abort);
 }
 
 void launch_max_i32(const int32_t *a, int32_t n, uint64_t *out, hipStream_t st) {

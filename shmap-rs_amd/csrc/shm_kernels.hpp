@@ -192,25 +192,28 @@ void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint6
 void launch_unit_counts(SeqSet S, int32_t k, uint64_t *nunit, hipStream_t st);
 size_t sketch_lds(int32_t k);
 // unit_seq[u] = the sequence of unit u, from the scanned unit counts.
-void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st);
+void launch_unit_seq(int32_t n, const uint64_t *unit_start, uint32_t *unit_seq, hipStream_t st,
+                     const uint32_t *abort = nullptr);
 // A chunk is WAVE consecutive units (one per lane).  pass 0: roll every chunk
 // into lane-strided staging (lcap 16-byte records per lane), listing chunks with
 // a lane over lcap, each lane's count and its exclusive prefix in the chunk;
 // pass 1: re-roll the listed chunks straight to their final positions
-// (chunk_val = scanned chunk offsets).
+// (chunk_val = scanned chunk offsets).  With n_units_dev the unit count is
+// read from the device and n_units, n_chunks are capacities (grid sizes).
 void launch_sketch(int pass, SeqSet S, int32_t k, uint64_t thres, const uint64_t *unit_start,
                    const uint32_t *unit_seq, uint64_t n_units, uint64_t *chunk_val, uint32_t *lane_cnt,
                    uint32_t *lane_pre, uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s, uint32_t lcap,
                    uint32_t *ovf_list, uint32_t *ovf_count, uint64_t n_chunks, uint32_t max_grid, hipStream_t st,
-                   const uint32_t *abort = nullptr);
+                   const uint32_t *abort = nullptr, const uint64_t *n_units_dev = nullptr);
 void launch_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt, const uint64_t *chunk_koff,
                            uint32_t lcap, const uint4 *stage, uint64_t *out_h, int32_t *out_r, uint8_t *out_s,
-                           uint32_t max_grid, hipStream_t st, const uint32_t *abort = nullptr);
+                           uint32_t max_grid, hipStream_t st, const uint32_t *abort = nullptr,
+                           const uint64_t *n_units_dev = nullptr);
 // Staging slots per lane for a keep threshold: the expected kept count of
 // LANE_W windows plus five standard deviations and a margin, at most LANE_W.
 uint32_t sketch_lane_cap(uint64_t thres);
 void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,
-                     uint64_t *koff, uint64_t *max_m, hipStream_t st);
+                     uint64_t *koff, uint64_t *max_m, hipStream_t st, const uint32_t *abort = nullptr);
 void launch_max_i32(const int32_t *a, int32_t n, uint64_t *out, hipStream_t st);
 void launch_fill_pairs(const uint64_t *t_h, uint64_t n, uint64_t *keys, uint32_t *vals, hipStream_t st);
 void launch_radix_pass(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t n,

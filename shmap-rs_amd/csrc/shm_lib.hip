@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -65,6 +66,9 @@ extern "C" const char *shm_last_error(void) { return g_err; }
 // device buffers
 // ---------------------------------------------------------------------------
 
+// Buffer allocations made on this thread (a batch reports its own).
+static thread_local int64_t g_allocs = 0;
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -79,6 +83,7 @@ struct DevBuf {
         if (n <= cap && p) return SHM_OK;
         release();
         size_t c = n < 16 ? 16 : n + n / 4;
+        g_allocs++;
         hipError_t e = hipMalloc((void **)&p, c * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -107,6 +112,7 @@ struct PinnedBuf {
         if (n <= cap && p) return SHM_OK;
         release();
         size_t c = n < 16 ? 16 : n + n / 4;
+        g_allocs++;
         hipError_t e = hipHostMalloc((void **)&p, c * sizeof(T), hipHostMallocDefault);
         if (e != hipSuccess) {
             p = nullptr;
@@ -203,18 +209,20 @@ static uint64_t count_units(int32_t k, const int64_t *h_off, int32_t n) {
 }
 
 // Sketches a set of sequences already on the device: count pass, scan, write
-// pass.  koff (n+1) receives per-sequence offsets.  With the host's offsets
-// (h_off) the unit table is sized without a read-back; with an abort flag the
-// k-mer count is not read back either: the outputs hold m_cap + 1 entries,
-// and a larger count, or a sequence of more than mm_cap k-mers, sets *abort
-// and skips the writes (the caller reads so.koff[n] and so.stat later).
-// Otherwise so.M, so.max_m and so.n_overflow are set.  so.syncs counts the
-// host synchronisations.
+// pass.  koff (n+1) receives per-sequence offsets.  Sizes come one of three
+// ways.  With an abort flag nothing is read back: the unit table holds u_cap
+// units and the outputs m_cap + 1 k-mers, and more units, more k-mers or a
+// sequence of more than mm_cap k-mers set *abort and skip every later write
+// (the caller reads so.koff[n] and so.stat afterwards); the launches then
+// depend on the capacities alone, so they can be captured into a graph.
+// Otherwise the unit table is sized from the host's offsets (h_off) or from a
+// read-back, the k-mer count is read back, and so.M, so.max_m and
+// so.n_overflow are set.  so.syncs counts the host synchronisations.
 static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const int64_t *d_off, int32_t n,
                          SketchOut &so, DevBuf<uint64_t> &out_h, DevBuf<int32_t> &out_r, DevBuf<uint8_t> &out_s,
                          uint32_t max_grid, hipStream_t st, hipEvent_t *ev_count, hipEvent_t *ev_write,
                          bool full_koff, const int64_t *h_off = nullptr, uint32_t *abort = nullptr,
-                         uint64_t m_cap = 0, uint64_t mm_cap = 0) {
+                         uint64_t u_cap = 0, uint64_t m_cap = 0, uint64_t mm_cap = 0) {
     // unit table: units of LANE_W windows per sequence, scanned on the device;
     // a chunk is WAVE consecutive units
     SeqSet S{d_bases, d_off, n};
@@ -229,8 +237,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     launch_scan(so.nunit.p, so.unit_start.p, nullptr, n, so.part.p, st);
     launch_fill_u32(so.ovf_n.p, 1, 0, st);
     launch_fill_u64(so.stat.p, 1, 0, st);
-    uint64_t U;
-    if (h_off) {
+    uint64_t U;                         // units, or their capacity
+    const uint64_t *u_dev = nullptr;    // the unit count on the device, when U is a capacity
+    if (abort) {
+        U = u_cap;
+        u_dev = so.unit_start.p + n;
+        launch_gate(u_dev, u_cap, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
+    } else if (h_off) {
         U = count_units(sk->k, h_off, n);
     } else {
         HIPCHK(hipMemcpyAsync(so.h_sum.p, so.unit_start.p + n, 8, hipMemcpyDeviceToHost, st));
@@ -244,7 +257,7 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.lane_cnt.ensure(C * WAVE + 1));
     RCHK(so.lane_pre.ensure(C * WAVE + 1));
     RCHK(so.unit_seq.ensure(U + 1));
-    if (C > 0) launch_unit_seq(n, so.unit_start.p, so.unit_seq.p, st);
+    if (C > 0) launch_unit_seq(n, so.unit_start.p, so.unit_seq.p, st, abort);
     const uint32_t lcap = sketch_lane_cap(sk->thres);
     const uint64_t slots = C * WAVE * (uint64_t)lcap;
     RCHK(so.stage.ensure(slots + 1));
@@ -253,10 +266,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     if (ev_count) HIPCHK(hipEventRecord(ev_count[0], st));
     if (C > 0)
         launch_sketch(0, S, sk->k, sk->thres, so.unit_start.p, so.unit_seq.p, U, so.chunk_cnt.p, so.lane_cnt.p,
-                      so.lane_pre.p, so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st);
+                      so.lane_pre.p, so.stage.p, nullptr, nullptr, nullptr, lcap, ovf_list, ovf_count, C, max_grid, st,
+                      abort, u_dev);
     if (ev_count) HIPCHK(hipEventRecord(ev_count[1], st));
+    // (with U a capacity, the chunk offsets past the last chunk are stale;
+    // koff reads none of them)
     launch_scan(so.chunk_cnt.p, so.chunk_koff.p, nullptr, C, so.part.p, st);
-    launch_seq_koff(n, so.unit_start.p, so.chunk_koff.p, so.lane_pre.p, so.koff.p, so.stat.p, st);
+    launch_seq_koff(n, so.unit_start.p, so.chunk_koff.p, so.lane_pre.p, so.koff.p, so.stat.p, st, abort);
     if (full_koff) {
         so.h_koff.resize(n + 1);
         HIPCHK(hipMemcpyAsync(so.h_koff.data(), so.koff.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
@@ -284,13 +300,13 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     if (ev_write) HIPCHK(hipEventRecord(ev_write[0], st));
     if (C > 0)
         launch_sketch_compact(C, so.lane_cnt.p, so.chunk_koff.p, lcap, so.stage.p, out_h.p, out_r.p, out_s.p,
-                              max_grid, st, abort);
+                              max_grid, st, abort, u_dev);
     // the re-roll of overflowing chunks: their count is on the device when
     // the host has not read it back (the kernel takes it from there)
     if (C > 0 && (abort || so.n_overflow > 0))
         launch_sketch(1, S, sk->k, sk->thres, so.unit_start.p, so.unit_seq.p, U, so.chunk_koff.p, so.lane_cnt.p,
                       so.lane_pre.p, nullptr, out_h.p, out_r.p, out_s.p, lcap, ovf_list, ovf_count,
-                      abort ? C : so.n_overflow, max_grid, st, abort);
+                      abort ? C : so.n_overflow, max_grid, st, abort, u_dev);
     if (ev_write) HIPCHK(hipEventRecord(ev_write[1], st));
     HIPCHK(hipGetLastError());
     return SHM_OK;
@@ -954,26 +970,37 @@ extern "C" int shm_index_load(const char *path, int32_t device_id, shm_index **o
 static const char *kStageNames[ST_COUNT] = {"sketch", "sketch_compact", "seed_wave", "seed_wg", "tab_init",
                                             "contrib", "extract",        "prune",     "refine",  "final"};
 
-constexpr int SHM_N_COUNTS = 11;  // see shm_mapper_last_counts
+constexpr int SHM_N_COUNTS = 14;  // see shm_mapper_last_counts
 
 // The sizes batches have had so far, from which a batch's buffers are sized
-// before its own sizes are known (a speculative batch): the largest k-mers
-// per base, table slots and buckets per k-mer, and the capacities (in items)
-// the speculative buffers were last sized to.
+// before its own sizes are known (a speculative batch): the largest k-mers,
+// hits and buckets per base, k-mers of the largest sketch per base of the
+// longest read, and the capacities (in items) the buffers of speculative
+// batches were last sized to.  Ratios per base come from batches large enough
+// to show them (or the first batch), so that a batch of a few reads does not
+// inflate every later batch's buffers.
 struct BatchShape {
     bool seen = false;
-    double m_per_base = 0, t_per_m = 0, b_per_m = 0, mm_per_base = 0, e_per_m = 0;
-    uint64_t M = 0, T = 0, B = 0, max_m = 0;
-    void learn(uint64_t bases, uint64_t M_, uint64_t max_m_, int64_t max_len, uint64_t T_, uint64_t B_,
-               uint64_t E_) {
+    double m_per_base = 0, e_per_base = 0, b_per_base = 0, mm_per_base = 0;
+    uint64_t U = 0, M = 0, T = 0, B = 0, max_m = 0, max_len = 0;
+    void learn(uint64_t bases, uint64_t M_, uint64_t max_m_, int64_t max_len, uint64_t E_, uint64_t B_) {
         if (bases == 0 || M_ == 0) return;
-        seen = true;
-        e_per_m = fmax(e_per_m, (double)E_ / (double)M_);
-        m_per_base = fmax(m_per_base, (double)M_ / (double)bases);
+        if (!seen || M_ >= 4096) {
+            const double b = (double)bases;
+            m_per_base = fmax(m_per_base, (double)M_ / b);
+            e_per_base = fmax(e_per_base, (double)E_ / b);
+            b_per_base = fmax(b_per_base, (double)B_ / b);
+        }
         if (max_len > 0) mm_per_base = fmax(mm_per_base, (double)max_m_ / (double)max_len);
-        t_per_m = fmax(t_per_m, (double)T_ / (double)M_);
-        b_per_m = fmax(b_per_m, (double)B_ / (double)M_);
+        seen = true;
     }
+};
+
+// A batch's stream work captured for one read class, and what it was
+// captured with.
+struct MapGraph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
 };
 
 struct shm_mapper {
@@ -990,6 +1017,8 @@ struct shm_mapper {
     hipEvent_t ev_up = nullptr;              // the end of a batch's upload
     int32_t max_read_len = 0;  // reads longer than this fail alone (0: no limit)
     bool speculate = true;     // size batches from earlier ones and synchronise once
+    bool graphs_on = true;     // capture speculative batches per read class
+    std::map<int32_t, MapGraph> graphs;
     BatchShape shape;
     double stage_ms[ST_COUNT] = {};
     double total_ms = 0;
@@ -1032,6 +1061,13 @@ struct shm_mapper {
     DevBuf<shm_read_result> d_out;
     DevBuf<uint32_t> abort_flag;  // set by the capacity checks of a speculative batch
     PinnedBuf<uint64_t> h_spec;   // its sizes and flag, read back with its records
+    PinnedBuf<uint32_t> h_ntask;  // and its task counters
+
+    void drop_graphs() {
+        for (auto &g : graphs)
+            if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+        graphs.clear();
+    }
 
     // Frees every batch buffer (after a failed allocation, so that a retry
     // with fewer reads starts from the device memory a smaller batch needs).
@@ -1049,7 +1085,8 @@ struct shm_mapper {
         g_tr.release(), g_diff.release(), g_tails.release(), tasks.release(), sel.release(), big.release();
         ext.release(), ov.release(), ov_kpre.release(), mapped.release(), h_mapped.release();
         task_count.release(), d_out.release(), abort_flag.release();
-        shape.M = shape.T = shape.B = shape.max_m = 0;  // the speculative capacities went with them
+        drop_graphs();
+        shape.U = shape.M = shape.T = shape.B = shape.max_m = shape.max_len = 0;  // the capacities went with them
     }
 };
 
@@ -1092,6 +1129,7 @@ extern "C" int shm_mapper_create(const shm_index *idx, const shm_map_params *p, 
 
 extern "C" void shm_mapper_free(shm_mapper *mp) {
     if (!mp) return;
+    mp->drop_graphs();
     for (int s = 0; s < ST_COUNT; s++)
         for (int j = 0; j < 2; j++) (void)hipEventDestroy(mp->ev[s][j]);
     if (mp->t0) (void)hipEventDestroy(mp->t0);
@@ -1118,6 +1156,13 @@ extern "C" int shm_mapper_set_timing(shm_mapper *mp, int32_t on) {
 extern "C" int shm_mapper_set_speculation(shm_mapper *mp, int32_t on) {
     if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
     mp->speculate = on != 0;
+    return SHM_OK;
+}
+
+extern "C" int shm_mapper_set_graphs(shm_mapper *mp, int32_t on) {
+    if (!mp) return fail(SHM_ERR_ARG, "invalid argument");
+    mp->graphs_on = on != 0;
+    if (!mp->graphs_on) mp->drop_graphs();
     return SHM_OK;
 }
 
@@ -1486,62 +1531,112 @@ static void batch_extent(const int64_t *h_off, int32_t R, uint64_t *bases, int64
     *max_len = ml;
 }
 
-// Maps R reads with one host synchronisation, at the end.  Every buffer is
-// sized before the first launch from the shapes of earlier batches, with a
-// margin; device checks after the sketch (k-mers, the largest sketch), after
-// seeding (table slots) and after counting (buckets) set an abort flag when a
-// size exceeds its buffer, and every later kernel then returns at once.
-// *done is false when the batch stopped so (or would need table groups): the
-// caller maps it again by map_synced.
-static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R,
-                           shm_read_result *out, const int64_t *h_off, bool *done) {
-    *done = false;
-    hipStream_t st = mp->st;
-    const DevIndex I = mp->idx->view();
+// The capacities of a speculative batch of R reads, `bases` bases and longest
+// read max_len: its estimates from the shapes seen so far, with a margin, and
+// never below what the buffers were last sized to (buffers only grow, to
+// twice their size at least, so that a stream of slowly growing batches
+// reallocates rarely: a reallocation waits for the whole device).
+// A sequence of nw windows has ceil(nw / LANE_W) + halo sketch units, so
+// U <= bases / LANE_W + R (1 + halo).  A read's table has the smallest power
+// of two of at least twice its hits slots, 64 at least, so T <= 64 R + 4 E;
+// its hits give at most two buckets each.  False when the batch's tables
+// could exceed the table budget.
+static bool speculative_caps(shm_mapper *mp, int32_t R, uint64_t bases, int64_t max_len, double *E_est) {
     BatchShape &sh = mp->shape;
-    uint64_t bases;
-    int64_t max_len;
-    batch_extent(h_off, R, &bases, &max_len);
-    // capacities: this batch's estimates with a margin, never below what the
-    // buffers were last sized to (buffers only grow)
-    auto grow = [](uint64_t have, double want) {
-        const uint64_t w = (uint64_t)(want * 1.15) + 1024;
-        return w > have ? w : have;
+    // margins: k-mers per base vary little between batches, hits per k-mer
+    // and buckets per hit with the reads' repeat content
+    auto grow = [](uint64_t have, double want, double margin) {
+        const uint64_t w = (uint64_t)(want * margin) + 1024;
+        if (w <= have) return have;
+        return w > 2 * have ? w : 2 * have;
     };
     const double M_est = (double)bases * sh.m_per_base;
-    if (M_est * sh.t_per_m * 1.15 + 1024 > (double)mp->tab_budget) return SHM_OK;  // table groups
-    sh.M = grow(sh.M, M_est);
-    sh.max_m = grow(sh.max_m, (double)max_len * 2.0 * fmax(sh.mm_per_base, sh.m_per_base));
+    *E_est = (double)bases * sh.e_per_base;
+    const double T_est = 64.0 * R + 4.0 * *E_est;
+    if (T_est * 1.3 + 1024 > (double)mp->tab_budget) return false;
+    sh.U = grow(sh.U, (double)bases / LANE_W + (double)R * (1 + sketch_halo(mp->sk.k)), 1.0);
+    sh.M = grow(sh.M, M_est, 1.15);
+    sh.max_m = grow(sh.max_m, (double)max_len * 2.0 * fmax(sh.mm_per_base, sh.m_per_base), 1.15);
     if (sh.max_m > sh.M) sh.max_m = sh.M;
-    sh.T = std::min<uint64_t>(grow(sh.T, (double)sh.M * sh.t_per_m), mp->tab_budget);
-    sh.B = std::min<uint64_t>(grow(sh.B, (double)sh.M * sh.b_per_m), (uint64_t)UINT32_MAX - 1);
-    // a read's halflen is its k-mer count, or its length with absolute positions
-    const int32_t max_hl = mp->prm.abs_pos ? (int32_t)max_len : (int32_t)sh.max_m;
+    sh.max_len = grow(sh.max_len, (double)max_len, 1.15);
+    sh.T = std::min<uint64_t>(grow(sh.T, T_est, 1.3), mp->tab_budget);
+    sh.B = std::min<uint64_t>(grow(sh.B, fmin(2.0 * *E_est, (double)bases * sh.b_per_base), 1.5),
+                              (uint64_t)UINT32_MAX - 1);
+    return true;
+}
 
-    shm_read_result *res_dev;
-    bool direct;
-    RCHK(result_target(mp, out, R, &res_dev, &direct));
+// The largest halflen a speculative batch's buffers hold: a read's halflen is
+// its k-mer count, or its length with absolute positions.
+static int32_t speculative_hl(const shm_mapper *mp) {
+    const uint64_t hl = mp->prm.abs_pos ? mp->shape.max_len : mp->shape.max_m;
+    return hl < (uint64_t)INT32_MAX ? (int32_t)hl : INT32_MAX;
+}
+
+// Every buffer a speculative batch of R reads uses, at the capacities of
+// speculative_caps.
+static int reserve_speculative(shm_mapper *mp, int32_t R) {
+    const BatchShape &sh = mp->shape;
+    SketchOut &so = mp->so;
+    const uint64_t C = (sh.U + WAVE - 1) / WAVE;
+    RCHK(so.unit_start.ensure((size_t)R + 1));
+    RCHK(so.nunit.ensure((size_t)R + 1));
+    RCHK(so.koff.ensure((size_t)R + 1));
+    RCHK(so.part.ensure(SCAN_PARTS + 1));
+    RCHK(so.h_sum.ensure(3));
+    RCHK(so.ovf_n.ensure(1));
+    RCHK(so.stat.ensure(1));
+    RCHK(so.chunk_cnt.ensure(C + 1));
+    RCHK(so.chunk_koff.ensure(C + 1));
+    RCHK(so.lane_cnt.ensure(C * WAVE + 1));
+    RCHK(so.lane_pre.ensure(C * WAVE + 1));
+    RCHK(so.unit_seq.ensure(sh.U + 1));
+    RCHK(so.stage.ensure(C * WAVE * (uint64_t)sketch_lane_cap(mp->sk.thres) + 1));
+    RCHK(so.ovf.ensure(C + 1));
+    RCHK(mp->k_h.ensure(sh.M + 1));
+    RCHK(mp->k_r.ensure(sh.M + 1));
+    RCHK(mp->k_s.ensure(sh.M + 1));
     RCHK(ensure_read_buffers(mp, R, sh.M));
-    RCHK(ensure_group_buffers(mp, sh.T, sh.B, R, max_hl, sh.max_m));
+    RCHK(ensure_group_buffers(mp, sh.T, sh.B, R, speculative_hl(mp), sh.max_m));
     RCHK(mp->task_count.ensure(TASK_COUNTERS + 1));
     RCHK(mp->abort_flag.ensure(1));
     RCHK(mp->h_spec.ensure(8));
-    uint32_t *abort = mp->abort_flag.p;
+    RCHK(mp->h_ntask.ensure(TASK_COUNTERS));
+    RCHK(mp->d_out.ensure((size_t)R));  // the records of a captured batch
+    return SHM_OK;
+}
 
-    std::vector<int> pending;
-    HIPCHK(hipEventRecord(mp->t0, st));
+// Read counts of captured batches: powers of two and their halfway points
+// (1, 2, 3, 4, 6, 8, 12, ...); a batch is padded with empty reads up to its
+// class, so that one graph serves every batch of the class.
+static int32_t read_class(int32_t R) {
+    if (R <= 2) return R;
+    int32_t c = 2;
+    while (true) {
+        if (c >= R) return c;
+        if (c + c / 2 >= R) return c + c / 2;
+        if (c > INT32_MAX / 2) return R;
+        c *= 2;
+    }
+}
+
+// The stream work of a speculative batch of R reads (arguments A): every
+// kernel, then the copies of its sizes, counters and mapped list to the
+// mapper's page-locked buffers.  It depends on the capacities and buffers
+// alone, so it can be captured.
+static int enqueue_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R, MapArgs &A,
+                               shm_read_result *res_dev, std::vector<int> &pending) {
+    hipStream_t st = mp->st;
+    const DevIndex I = mp->idx->view();
+    const BatchShape &sh = mp->shape;
+    uint32_t *abort = mp->abort_flag.p;
     launch_fill_u32(abort, 1, 0, st);
-    mp->so.syncs = 0;
     RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
                        mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
-                       false, h_off, abort, sh.M, sh.max_m));
+                       false, nullptr, abort, sh.U, sh.M, sh.max_m));
     if (mp->timing) {
         pending.push_back(ST_SKETCH);
         pending.push_back(ST_SKETCH_COMPACT);
     }
-    // the sketch kernels grew the k-mer buffers to sh.M + 1 items already
-    MapArgs A = read_args(mp, R, d_off);
-    A.abort = abort;
     launch_fill_u32(A.big_count, 1, 0, st);
     for (Stage s : {ST_SEED_WAVE, ST_SEED_WG}) {
         TIME_BEGIN(s);
@@ -1555,11 +1650,7 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
     launch_gate(mp->grp_base.p + (R + 1) + R, sh.T, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
     launch_fill_u32(mp->mapped.p, 1, 0, st);
     launch_fill_u64(mp->u_total.p, 1, 0, st);
-    uint32_t *task_count = mp->task_count.p;
-    launch_fill_u32(task_count, TASK_COUNTERS, 0, st);
-    // the hit count only picks the contribution split, which records never
-    // depend on: the batch's estimate stands in for it
-    table_args(mp, A, 0, R, task_count, (uint64_t)(M_est * sh.e_per_m));
+    launch_fill_u32(A.task_count, TASK_COUNTERS, 0, st);
     TIME_BEGIN(ST_TAB_INIT);
     launch_stage(ST_TAB_INIT, A, I, nullptr, R, mp->max_grid, st);
     TIME_END(ST_TAB_INIT);
@@ -1569,39 +1660,141 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
     launch_scan(mp->grp_nb.p, mp->grp_base.p + 2 * (size_t)(R + 1), nullptr, R, mp->part.p, st);
     launch_gate(mp->grp_base.p + 2 * (size_t)(R + 1) + R, sh.B, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
     launch_fill_u32(mp->ext.p, 1, 0, st);
-    bucket_args(mp, A, sh.B, max_hl, sh.max_m);
     for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
         TIME_BEGIN(s);
         launch_stage(s, A, I, res_dev, sh.B, mp->max_grid, st);
         TIME_END(s);
     }
     HIPCHK(hipGetLastError());
-    // everything the host needs, then the batch's one synchronisation
-    std::vector<uint32_t> ntask(TASK_COUNTERS);
     uint64_t *hs = mp->h_spec.p;
-    hs[7] = 0;
     HIPCHK(hipMemcpyAsync(&hs[7], abort, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hs[0], mp->so.koff.p + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hs[1], mp->so.stat.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hs[2], mp->grp_base.p + R, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[3], mp->grp_base.p + (R + 1) + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hs[4], mp->grp_base.p + 2 * (size_t)(R + 1) + R, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hs[5], A.u_total, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(ntask.data(), task_count, TASK_COUNTERS * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mp->h_ntask.p, A.task_count, TASK_COUNTERS * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
-    if (!direct)
+    return SHM_OK;
+}
+
+// 64-bit FNV-1a over a byte range, chained.
+static uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+    const uint8_t *b = (const uint8_t *)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// What a captured batch depends on: its kernel arguments (every buffer
+// address and capacity among them) and the sketch's buffers and capacities.
+static uint64_t graph_key(const shm_mapper *mp, const MapArgs &A, const uint8_t *d_seq, const int64_t *d_off,
+                          const shm_read_result *res_dev) {
+    const SketchOut &so = mp->so;
+    const void *ptrs[] = {d_seq, d_off, res_dev, so.unit_start.p, so.nunit.p, so.koff.p, so.part.p, so.ovf_n.p,
+                          so.stat.p, so.chunk_cnt.p, so.chunk_koff.p, so.lane_cnt.p, so.lane_pre.p, so.unit_seq.p,
+                          so.stage.p, so.ovf.p, mp->abort_flag.p, mp->h_spec.p, mp->h_ntask.p, mp->h_mapped.p,
+                          mp->stat.p, mp->part.p, mp->grp_base.p, mp->grp_nb.p};
+    const BatchShape &sh = mp->shape;
+    const uint64_t caps[] = {sh.U, sh.M, sh.max_m, sh.T, sh.B, sh.max_len, (uint64_t)mp->max_grid};
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a(h, &A, sizeof A);
+    h = fnv1a(h, ptrs, sizeof ptrs);
+    return fnv1a(h, caps, sizeof caps);
+}
+
+// Maps R reads with one host synchronisation, at the end.  Every buffer is
+// sized before the first launch from the shapes of earlier batches, with a
+// margin; device checks after the sketch's unit scan (units), after the sketch
+// (k-mers, the largest sketch), after seeding (table slots) and after
+// counting (buckets) set an abort flag when a size exceeds its buffer, and
+// every later kernel then returns at once.  For reads in the mapper's own
+// upload buffers the batch is padded with empty reads to its read class and
+// its stream work captured into a graph once per class, then replayed.
+// *done is false when the batch stopped (or would need table groups): the
+// caller maps it again by map_synced.
+static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off, int32_t R,
+                           shm_read_result *out, const int64_t *h_off, bool *done) {
+    *done = false;
+    hipStream_t st = mp->st;
+    BatchShape &sh = mp->shape;
+    uint64_t bases;
+    int64_t max_len;
+    batch_extent(h_off, R, &bases, &max_len);
+    // a graph needs the mapper's upload buffers, with room for the padding
+    const int32_t Rc = read_class(R);
+    const bool graph = mp->graphs_on && !mp->timing && d_seq == mp->bases.p && d_off == mp->roff.p &&
+                       mp->roff.cap >= (size_t)Rc + 1;
+    const int32_t RL = graph ? Rc : R;  // reads launched
+    double E_est;
+    if (!speculative_caps(mp, RL, bases, max_len, &E_est)) return SHM_OK;  // table groups
+
+    shm_read_result *res_dev;
+    bool direct = false;
+    if (graph) {
+        RCHK(mp->d_out.ensure(RL));  // a fixed address for the graph; one copy to `out` after it
+        res_dev = mp->d_out.p;
+    } else {
+        RCHK(result_target(mp, out, R, &res_dev, &direct));
+    }
+    RCHK(reserve_speculative(mp, RL));
+    const int32_t max_hl = speculative_hl(mp);
+
+    MapArgs A = read_args(mp, RL, d_off);
+    A.abort = mp->abort_flag.p;
+    // the hit count only picks the contribution split, which records never
+    // depend on: the batch's estimate stands in for it
+    table_args(mp, A, 0, RL, mp->task_count.p, (uint64_t)E_est);
+    bucket_args(mp, A, sh.B, max_hl, sh.max_m);
+
+    std::vector<int> pending;
+    mp->h_spec.p[7] = 0;
+    HIPCHK(hipEventRecord(mp->t0, st));
+    if (RL > R) launch_fill_u64((uint64_t *)mp->roff.p + R + 1, (uint64_t)(RL - R), (uint64_t)h_off[R], st);
+    if (graph) {
+        MapGraph &g = mp->graphs[RL];
+        const uint64_t key = graph_key(mp, A, d_seq, d_off, res_dev);
+        if (!g.exec || g.key != key) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            const int64_t a0 = g_allocs;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            const int rc = enqueue_speculative(mp, d_seq, d_off, RL, A, res_dev, pending);
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &gr);
+            if (rc != SHM_OK || e != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                if (rc != SHM_OK) return rc;
+                HIPCHK(e);
+            }
+            const hipError_t ei = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) g.exec = nullptr;
+            HIPCHK(ei);
+            // an allocation during the capture changed an address the key read
+            g.key = g_allocs == a0 ? key : 0;
+            mp->counts[13] = 1;
+        }
+        HIPCHK(hipGraphLaunch(g.exec, st));
+        mp->counts[12] = 1;
         HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    } else {
+        RCHK(enqueue_speculative(mp, d_seq, d_off, RL, A, res_dev, pending));
+        if (!direct)
+            HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
+    }
     HIPCHK(hipEventRecord(mp->t1, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
+    const uint64_t *hs = mp->h_spec.p;
     if ((uint32_t)hs[7] != 0) return SHM_OK;  // a size outgrew its buffer: mapped again, synchronised
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, mp->t0, mp->t1));
     mp->total_ms = ms;
-    const uint64_t M = hs[0], E_all = hs[2], T_all = hs[3], B = hs[4];
+    const uint64_t M = hs[0], E_all = hs[2], B = hs[4];
+    const std::vector<uint32_t> ntask(mp->h_ntask.p, mp->h_ntask.p + TASK_COUNTERS);
     finish_batch(mp, out, ntask, 1, M, hs[5], (int64_t)B, E_all);
     mp->counts[9] = 1;
-    sh.learn(bases, M, hs[1], max_len, T_all, B, E_all);
+    sh.learn(bases, M, hs[1], max_len, E_all, B);
     *done = true;
     return SHM_OK;
 }
@@ -1753,7 +1946,13 @@ static int map_synced(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         uint64_t bases;
         int64_t max_len;
         batch_extent(h_off, R, &bases, &max_len);
-        mp->shape.learn(bases, M, mp->so.max_m, max_len, T_all, (uint64_t)nbuckets, E_all);
+        mp->shape.learn(bases, M, mp->so.max_m, max_len, E_all, (uint64_t)nbuckets);
+        // the buffers of the next batch, if it is like this one, while the
+        // stream is idle
+        double E_est;
+        const int32_t RL = mp->graphs_on && d_seq == mp->bases.p && d_off == mp->roff.p ? read_class(R) : R;
+        if (mp->speculate && mp->shape.seen && speculative_caps(mp, RL, bases, max_len, &E_est))
+            RCHK(reserve_speculative(mp, RL));
     }
     return SHM_OK;
 }
@@ -1770,6 +1969,12 @@ static int map_device(shm_mapper *mp, const uint8_t *d_seq, const int64_t *d_off
         mp->total_ms = 0;
         return SHM_OK;
     }
+    const int64_t allocs0 = g_allocs;
+    struct Allocs {  // the batch's allocations, on every return path
+        shm_mapper *mp;
+        int64_t a0;
+        ~Allocs() { mp->counts[11] = g_allocs - a0; }
+    } allocs{mp, allocs0};
     int64_t redone = 0;
     if (h_off && mp->speculate && mp->shape.seen) {
         bool done = false;
@@ -1804,7 +2009,7 @@ static int map_host_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_
     std::vector<int64_t> off(n + 1, 0);
     for (int32_t i = 0; i < n; i++) off[i + 1] = off[i] + reads[i].len;
     RCHK(mp->bases.ensure(off[n] + 64));
-    RCHK(mp->roff.ensure(n + 1));
+    RCHK(mp->roff.ensure((size_t)read_class(n) + 1));  // room to pad a captured batch
     // Reads back to back in one caller buffer go up in one copy; otherwise they
     // are gathered into one page-locked staging copy first.  (The previous call
     // ended with a stream synchronisation, so no copy still reads the staging
@@ -2062,7 +2267,7 @@ extern "C" int shm_map_batch_packed(shm_mapper *mp, const uint8_t *packed, const
     RCHK(mp->packed.ensure((uint64_t)(nbytes + 3) / 4 + 1));
     RCHK(mp->exc.ensure(2 * (uint64_t)n_exc + 2));
     RCHK(mp->bases.ensure(nb + 64));
-    RCHK(mp->roff.ensure(n + 1));
+    RCHK(mp->roff.ensure((size_t)read_class(n) + 1));  // room to pad a captured batch
     UploadTurn turn(mp, (uint64_t)nbytes);
     if (nbytes > 0) HIPCHK(hipMemcpyAsync(mp->packed.p, packed, (size_t)nbytes, hipMemcpyHostToDevice, mp->st));
     if (n_exc > 0) HIPCHK(hipMemcpyAsync(mp->exc.p, exc, (size_t)n_exc * 16, hipMemcpyHostToDevice, mp->st));
@@ -2085,6 +2290,8 @@ extern "C" int shm_map_batch_device(shm_mapper *mp, const uint8_t *d_seq, const 
                                     shm_read_result *out) {
     if (!mp || n < 0 || (n > 0 && (!d_seq || !d_off || !out))) return fail(SHM_ERR_ARG, "invalid argument");
     HIPCHK(hipSetDevice(mp->idx->device));
+    // no host copy of the offsets: the synchronised path (a read-back of the
+    // offsets first costs more than its four synchronisations on large batches)
     return map_device(mp, d_seq, d_off, n, out);
 }
 

@@ -123,6 +123,7 @@ def lib():
         L.shm_mapper_free.argtypes = [C.c_void_p]
         L.shm_mapper_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.shm_mapper_set_speculation.argtypes = [C.c_void_p, C.c_int32]
+        L.shm_mapper_set_graphs.argtypes = [C.c_void_p, C.c_int32]
         L.shm_device_compute_units.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
         L.shm_mapper_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.shm_mapper_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
@@ -341,6 +342,10 @@ class Mapper:
         """Speculative batches (one host synchronisation each) on or off."""
         _check(lib().shm_mapper_set_speculation(self._p, int(on)))
 
+    def set_graphs(self, on: bool) -> None:
+        """Graph capture of speculative batches, per read class, on or off."""
+        _check(lib().shm_mapper_set_graphs(self._p, int(on)))
+
     def set_tuning(self, **fields) -> None:
         """shm_mapper_set_tuning: contrib_wave_hits, contrib_chunk_hits,
         prune_thread_seeds (0: the built-in rule); unnamed fields keep their values."""
@@ -491,7 +496,7 @@ def host_array(n: int, dtype) -> np.ndarray:
 
 
 COUNT_NAMES = ("kmers", "seeds", "buckets", "refined", "groups", "first_pass_candidates", "second_pass_band",
-               "examined_hits", "global_table_reads", "host_syncs", "redone")
+               "examined_hits", "global_table_reads", "host_syncs", "redone", "allocations", "graph", "captured")
 
 
 class PackedReads:

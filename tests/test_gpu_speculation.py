@@ -65,7 +65,8 @@ def test_speculative_batches_give_the_synchronised_records(metric):
     for lo, hi in ((0, n), (0, 40), (7, 9), (0, n)):  # the same shape, smaller ones, the same again
         res = spec.map_batch(reads[lo:hi])
         c = spec.last_counts()
-        assert c["host_syncs"] == 1 and c["redone"] == 0, c
+        # the synchronised batch left buffers for batches like it
+        assert c["host_syncs"] == 1 and c["redone"] == 0 and c["allocations"] == 0, c
         bad = P.diff_results(ores[lo:hi], res)
         assert not bad, bad
     # counts other than the synchronisations are the synchronised path's
@@ -73,7 +74,7 @@ def test_speculative_batches_give_the_synchronised_records(metric):
     sync.map_batch(reads)
     a, b = spec.last_counts(), sync.last_counts()
     for k in G.COUNT_NAMES:
-        if k not in ("host_syncs", "redone"):
+        if k not in ("host_syncs", "redone", "allocations", "graph", "captured"):
             assert a[k] == b[k], (k, a[k], b[k])
     # results straight into page-locked memory
     out = G.host_array(len(reads), G.RESULT_DTYPE)
@@ -115,3 +116,44 @@ def test_a_batch_that_outgrows_the_learned_sizes_is_mapped_again():
     assert list(res["mapped"]) == [0, 0, 0] and list(res["m"]) == [0, 0, 0]
     res = mp.map_batch(rep_reads[:3])
     assert not P.diff_results(ores[len(uniq_reads):len(uniq_reads) + 3], res)
+
+
+@pytest.mark.parametrize("metric", ["Jaccard", "bucket_LCS"])
+def test_graph_replays_give_the_eager_records(metric):
+    """Batches captured into a graph per read class and replayed, with eager
+    work of other mappers and synchronisations between the replays: every
+    replay's records equal those of the same batch mapped without graphs (and
+    the oracle's), for classes the batch fills exactly and ones it is padded
+    to, down to a single read."""
+    segs, reads = T._workload(**T.WORKLOADS["uniform"])
+    prm = dict(PRM, metric=metric)
+    oi, op, ores = P.oracle_map(segs, reads, 15, 0.1, **prm)
+    idx = G.SketchIndex(G.FracMinHash(15, 0.1), segs)
+    p = G.make_params(k=15, h_frac=0.1, **prm)
+    gm = G.Mapper(idx, p)
+    eager = G.Mapper(idx, p)
+    eager.set_graphs(False)
+    sync = G.Mapper(idx, p)
+    sync.set_speculation(False)
+    gm.map_batch(reads)  # learns the shape: synchronised
+    eager.map_batch(reads)
+    ranges = [(0, 96), (10, 106), (0, 1), (5, 6), (0, 5), (20, 25), (0, 150), (3, 99), (7, 8)]
+    for rep in range(3):
+        for lo, hi in ranges:
+            want = eager.map_batch(reads[lo:hi])
+            assert eager.last_counts()["graph"] == 0
+            got = gm.map_batch(reads[lo:hi])
+            c = gm.last_counts()
+            assert c["graph"] == 1 and c["host_syncs"] == 1 and c["redone"] == 0, (lo, hi, c)
+            if rep == 2:  # every class captured, the capacities settled
+                assert c["captured"] == 0 and c["allocations"] == 0, (lo, hi, c)
+            bad = P.diff_results(want, got)
+            assert not bad, (lo, hi, bad)
+            bad = P.diff_results(ores[lo:hi], got)
+            assert not bad, (lo, hi, bad)
+            sync.map_batch(reads[hi:hi + 7])  # eager work and synchronisations between replays
+    # graphs off on the same mapper: the same records again
+    gm.set_graphs(False)
+    got = gm.map_batch(reads[:96])
+    assert gm.last_counts()["graph"] == 0
+    assert not P.diff_results(ores[:96], got)
