@@ -2744,8 +2744,31 @@ static inline uint32_t flat_grid(uint64_t n, uint32_t cap) {
     return (uint32_t)(g < cap ? g : cap);
 }
 
+// One workgroup's exclusive scan of a short array, out[n] = the total.
+__global__ void __launch_bounds__(WG) k_scan_single(const uint64_t *in, uint64_t *out, const uint64_t *n_ptr,
+                                                    uint64_t n_static) {
+    const uint64_t n = n_ptr ? *n_ptr : n_static;
+    __shared__ uint64_t tmp[NWAVES + 1];
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < n; base += WG) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = i < n ? in[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan64(v, tmp, &tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
 void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint,
                  uint64_t *part, hipStream_t st) {
+    // a short array in one workgroup and one launch (a few WG-wide steps
+    // cost less than two more launches)
+    if (n_hint <= SCAN_SINGLE_MAX) {
+        k_scan_single<<<1, WG, 0, st>>>(in, out, n_dev, n_hint);
+        return;
+    }
     // n_hint bounds the grid; the kernels read the true length from n_dev.
     const uint32_t G = flat_grid(n_hint, SCAN_PARTS);
     k_scan_reduce<<<G, WG, 0, st>>>(in, n_dev, n_hint, part);
@@ -2822,6 +2845,24 @@ void launch_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c
                  const uint64_t *v3, uint64_t c3, uint32_t *abort, hipStream_t st) {
     k_gate<<<1, 64, 0, st>>>(v0, c0, v1, c1, v2, c2, v3, c3, abort);
 }
+
+// Every listed run of 32-bit words set to 0, in one launch (a batch's
+// counters live in several buffers).
+__global__ void __launch_bounds__(WG) k_zero_list(ZeroList z) {
+    for (int32_t j = 0; j < z.count; j++)
+        for (uint32_t i = threadIdx.x; i < z.n[j]; i += WG) z.p[j][i] = 0u;
+}
+
+void launch_zero_list(const ZeroList &z, hipStream_t st) { k_zero_list<<<1, WG, 0, st>>>(z); }
+
+// dst[j] = the 8- or 4-byte value at src[j], one thread each: a batch's
+// scattered sizes and counters gathered into one (page-locked) block.
+__global__ void k_gather(GatherList g, uint64_t *dst) {
+    const int32_t j = threadIdx.x;
+    if (j < g.count) dst[j] = g.wide[j] ? *(const uint64_t *)g.src[j] : (uint64_t)*(const uint32_t *)g.src[j];
+}
+
+void launch_gather(const GatherList &g, uint64_t *dst, hipStream_t st) { k_gather<<<1, 64, 0, st>>>(g, dst); }
 
 void launch_seq_koff(int32_t n, const uint64_t *unit_start, const uint64_t *chunk_koff, const uint32_t *lane_pre,
                      uint64_t *koff, uint64_t *max_m, hipStream_t st, const uint32_t *abort) {

@@ -24,6 +24,7 @@ constexpr int RS_TILE = 256 * RS_ITEMS;   // radix sort tile
 constexpr uint32_t SORT_CAP = 4096;       // (u64,u32) pairs sorted in LDS
 constexpr int32_t MIN_HALFLEN = 5;        // src/buckets.rs:15
 constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
+constexpr uint64_t SCAN_SINGLE_MAX = 8192; // arrays up to this length scan in one workgroup
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
 constexpr uint32_t TASK_COUNTERS = 7;       // per group: first-pass, second band, 3 refine rounds, prune list,
@@ -186,6 +187,28 @@ void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 // capacity checks.
 void launch_gate(const uint64_t *v0, uint64_t c0, const uint64_t *v1, uint64_t c1, const uint64_t *v2, uint64_t c2,
                  const uint64_t *v3, uint64_t c3, uint32_t *abort, hipStream_t st);
+// Runs of 32-bit words to clear in one launch.
+struct ZeroList {
+    uint32_t *p[12];
+    uint32_t n[12];
+    int32_t count;
+    void add(void *ptr, uint32_t words) {
+        p[count] = (uint32_t *)ptr;
+        n[count++] = words;
+    }
+};
+void launch_zero_list(const ZeroList &z, hipStream_t st);
+// Scalars (8 bytes when wide, else 4) gathered into one array in one launch.
+struct GatherList {
+    const void *src[32];
+    uint8_t wide[32];
+    int32_t count;
+    void add(const void *ptr, bool w) {
+        src[count] = ptr;
+        wide[count++] = w;
+    }
+};
+void launch_gather(const GatherList &g, uint64_t *dst, hipStream_t st);
 void launch_scan(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_hint, uint64_t *part,
                  hipStream_t st);
 // Units of LANE_W windows per sequence (plus its halo units; none without windows).

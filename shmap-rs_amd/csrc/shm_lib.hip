@@ -210,7 +210,8 @@ static uint64_t count_units(int32_t k, const int64_t *h_off, int32_t n) {
 
 // Sketches a set of sequences already on the device: count pass, scan, write
 // pass.  koff (n+1) receives per-sequence offsets.  Sizes come one of three
-// ways.  With an abort flag nothing is read back: the unit table holds u_cap
+// ways.  With an abort flag (the caller then clears so.ovf_n and so.stat
+// beforehand) nothing is read back: the unit table holds u_cap
 // units and the outputs m_cap + 1 k-mers, and more units, more k-mers or a
 // sequence of more than mm_cap k-mers set *abort and skip every later write
 // (the caller reads so.koff[n] and so.stat afterwards); the launches then
@@ -235,8 +236,10 @@ static int sketch_device(const shm_sketcher *sk, const uint8_t *d_bases, const i
     RCHK(so.stat.ensure(1));
     launch_unit_counts(S, sk->k, so.nunit.p, st);
     launch_scan(so.nunit.p, so.unit_start.p, nullptr, n, so.part.p, st);
-    launch_fill_u32(so.ovf_n.p, 1, 0, st);
-    launch_fill_u64(so.stat.p, 1, 0, st);
+    if (!abort) {  // (with an abort flag the caller clears them with its own counters)
+        launch_fill_u32(so.ovf_n.p, 1, 0, st);
+        launch_fill_u64(so.stat.p, 1, 0, st);
+    }
     uint64_t U;                         // units, or their capacity
     const uint64_t *u_dev = nullptr;    // the unit count on the device, when U is a capacity
     if (abort) {
@@ -1060,8 +1063,7 @@ struct shm_mapper {
     DevBuf<uint32_t> task_count;
     DevBuf<shm_read_result> d_out;
     DevBuf<uint32_t> abort_flag;  // set by the capacity checks of a speculative batch
-    PinnedBuf<uint64_t> h_spec;   // its sizes and flag, read back with its records
-    PinnedBuf<uint32_t> h_ntask;  // and its task counters
+    PinnedBuf<uint64_t> h_spec;   // its sizes, flag and task counters, gathered there by the device
 
     void drop_graphs() {
         for (auto &g : graphs)
@@ -1531,6 +1533,9 @@ static void batch_extent(const int64_t *h_off, int32_t R, uint64_t *bases, int64
     *max_len = ml;
 }
 
+// Slots of a speculative batch's gathered sizes and counters.
+enum { SPEC_M, SPEC_MAX_M, SPEC_E, SPEC_B, SPEC_SEEDS, SPEC_ABORT, SPEC_TASKS, SPEC_SLOTS = SPEC_TASKS + TASK_COUNTERS };
+
 // The capacities of a speculative batch of R reads, `bases` bases and longest
 // read max_len: its estimates from the shapes seen so far, with a margin, and
 // never below what the buffers were last sized to (buffers only grow, to
@@ -1599,8 +1604,7 @@ static int reserve_speculative(shm_mapper *mp, int32_t R) {
     RCHK(ensure_group_buffers(mp, sh.T, sh.B, R, speculative_hl(mp), sh.max_m));
     RCHK(mp->task_count.ensure(TASK_COUNTERS + 1));
     RCHK(mp->abort_flag.ensure(1));
-    RCHK(mp->h_spec.ensure(8));
-    RCHK(mp->h_ntask.ensure(TASK_COUNTERS));
+    RCHK(mp->h_spec.ensure(SPEC_SLOTS));
     RCHK(mp->d_out.ensure((size_t)R));  // the records of a captured batch
     return SHM_OK;
 }
@@ -1629,7 +1633,18 @@ static int enqueue_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64
     const DevIndex I = mp->idx->view();
     const BatchShape &sh = mp->shape;
     uint32_t *abort = mp->abort_flag.p;
-    launch_fill_u32(abort, 1, 0, st);
+    // every counter of the batch cleared in one launch
+    ZeroList z{};
+    z.add(abort, 1);
+    z.add(mp->so.ovf_n.p, 1);
+    z.add(mp->so.stat.p, 2);
+    z.add(A.big_count, 1);
+    z.add(mp->stat.p, 2);
+    z.add(A.mapped_count, 1);
+    z.add(A.u_total, 2);
+    z.add(A.task_count, TASK_COUNTERS);
+    z.add(A.ext_count, 1);
+    launch_zero_list(z, st);
     RCHK(sketch_device(&mp->sk, d_seq, d_off, R, mp->so, mp->k_h, mp->k_r, mp->k_s, mp->max_grid, st,
                        mp->timing ? mp->ev[ST_SKETCH] : nullptr, mp->timing ? mp->ev[ST_SKETCH_COMPACT] : nullptr,
                        false, nullptr, abort, sh.U, sh.M, sh.max_m));
@@ -1637,20 +1652,15 @@ static int enqueue_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64
         pending.push_back(ST_SKETCH);
         pending.push_back(ST_SKETCH_COMPACT);
     }
-    launch_fill_u32(A.big_count, 1, 0, st);
     for (Stage s : {ST_SEED_WAVE, ST_SEED_WG}) {
         TIME_BEGIN(s);
         launch_stage(s, A, I, nullptr, 0, mp->max_grid, st);
         TIME_END(s);
     }
-    launch_fill_u64(mp->stat.p, 1, 0, st);
     launch_max_i32(A.rd_hl, R, mp->stat.p, st);
     launch_scan(A.rd_E, mp->grp_base.p, nullptr, R, mp->part.p, st);
     launch_scan(A.rd_cap, mp->grp_base.p + (R + 1), nullptr, R, mp->part.p, st);
     launch_gate(mp->grp_base.p + (R + 1) + R, sh.T, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
-    launch_fill_u32(mp->mapped.p, 1, 0, st);
-    launch_fill_u64(mp->u_total.p, 1, 0, st);
-    launch_fill_u32(A.task_count, TASK_COUNTERS, 0, st);
     TIME_BEGIN(ST_TAB_INIT);
     launch_stage(ST_TAB_INIT, A, I, nullptr, R, mp->max_grid, st);
     TIME_END(ST_TAB_INIT);
@@ -1659,21 +1669,25 @@ static int enqueue_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64
     TIME_END(ST_CONTRIB);
     launch_scan(mp->grp_nb.p, mp->grp_base.p + 2 * (size_t)(R + 1), nullptr, R, mp->part.p, st);
     launch_gate(mp->grp_base.p + 2 * (size_t)(R + 1) + R, sh.B, nullptr, 0, nullptr, 0, nullptr, 0, abort, st);
-    launch_fill_u32(mp->ext.p, 1, 0, st);
     for (Stage s : {ST_EXTRACT, ST_PRUNE, ST_REFINE, ST_FINAL}) {
         TIME_BEGIN(s);
         launch_stage(s, A, I, res_dev, sh.B, mp->max_grid, st);
         TIME_END(s);
     }
     HIPCHK(hipGetLastError());
-    uint64_t *hs = mp->h_spec.p;
-    HIPCHK(hipMemcpyAsync(&hs[7], abort, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[0], mp->so.koff.p + R, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[1], mp->so.stat.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[2], mp->grp_base.p + R, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[4], mp->grp_base.p + 2 * (size_t)(R + 1) + R, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hs[5], A.u_total, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(mp->h_ntask.p, A.task_count, TASK_COUNTERS * 4, hipMemcpyDeviceToHost, st));
+    // the batch's sizes and counters, gathered straight into page-locked
+    // memory (SPEC_* slots), then its mapped list
+    uint64_t *hs_dev = nullptr;
+    HIPCHK(hipHostGetDevicePointer((void **)&hs_dev, mp->h_spec.p, 0));
+    GatherList g{};
+    g.add(mp->so.koff.p + R, true);                            // SPEC_M
+    g.add(mp->so.stat.p, true);                                // SPEC_MAX_M
+    g.add(mp->grp_base.p + R, true);                           // SPEC_E
+    g.add(mp->grp_base.p + 2 * (size_t)(R + 1) + R, true);     // SPEC_B
+    g.add(A.u_total, true);                                    // SPEC_SEEDS
+    g.add(abort, false);                                       // SPEC_ABORT
+    for (uint32_t t = 0; t < TASK_COUNTERS; t++) g.add(A.task_count + t, false);  // SPEC_TASKS
+    launch_gather(g, hs_dev, st);
     HIPCHK(hipMemcpyAsync(mp->h_mapped.p, mp->mapped.p, ((size_t)R + 1) * 4, hipMemcpyDeviceToHost, st));
     return SHM_OK;
 }
@@ -1692,7 +1706,7 @@ static uint64_t graph_key(const shm_mapper *mp, const MapArgs &A, const uint8_t 
     const SketchOut &so = mp->so;
     const void *ptrs[] = {d_seq, d_off, res_dev, so.unit_start.p, so.nunit.p, so.koff.p, so.part.p, so.ovf_n.p,
                           so.stat.p, so.chunk_cnt.p, so.chunk_koff.p, so.lane_cnt.p, so.lane_pre.p, so.unit_seq.p,
-                          so.stage.p, so.ovf.p, mp->abort_flag.p, mp->h_spec.p, mp->h_ntask.p, mp->h_mapped.p,
+                          so.stage.p, so.ovf.p, mp->abort_flag.p, mp->h_spec.p, mp->h_mapped.p,
                           mp->stat.p, mp->part.p, mp->grp_base.p, mp->grp_nb.p};
     const BatchShape &sh = mp->shape;
     const uint64_t caps[] = {sh.U, sh.M, sh.max_m, sh.T, sh.B, sh.max_len, (uint64_t)mp->max_grid};
@@ -1747,7 +1761,6 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
     bucket_args(mp, A, sh.B, max_hl, sh.max_m);
 
     std::vector<int> pending;
-    mp->h_spec.p[7] = 0;
     HIPCHK(hipEventRecord(mp->t0, st));
     if (RL > R) launch_fill_u64((uint64_t *)mp->roff.p + R + 1, (uint64_t)(RL - R), (uint64_t)h_off[R], st);
     if (graph) {
@@ -1786,15 +1799,16 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));
     const uint64_t *hs = mp->h_spec.p;
-    if ((uint32_t)hs[7] != 0) return SHM_OK;  // a size outgrew its buffer: mapped again, synchronised
+    if (hs[SPEC_ABORT] != 0) return SHM_OK;  // a size outgrew its buffer: mapped again, synchronised
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, mp->t0, mp->t1));
     mp->total_ms = ms;
-    const uint64_t M = hs[0], E_all = hs[2], B = hs[4];
-    const std::vector<uint32_t> ntask(mp->h_ntask.p, mp->h_ntask.p + TASK_COUNTERS);
-    finish_batch(mp, out, ntask, 1, M, hs[5], (int64_t)B, E_all);
+    const uint64_t M = hs[SPEC_M], E_all = hs[SPEC_E], B = hs[SPEC_B];
+    std::vector<uint32_t> ntask(TASK_COUNTERS);
+    for (uint32_t t = 0; t < TASK_COUNTERS; t++) ntask[t] = (uint32_t)hs[SPEC_TASKS + t];
+    finish_batch(mp, out, ntask, 1, M, hs[SPEC_SEEDS], (int64_t)B, E_all);
     mp->counts[9] = 1;
-    sh.learn(bases, M, hs[1], max_len, E_all, B);
+    sh.learn(bases, M, hs[SPEC_MAX_M], max_len, E_all, B);
     *done = true;
     return SHM_OK;
 }
