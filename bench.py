@@ -14,9 +14,10 @@ overlap another's kernels (the serving loop's workers run the same way); the
 single-mapper rate is reported beside it (one_mapper).
 
 Also reported, outside the timed region: the plain one-read-per-launch form on
-a sample, per-stage kernel times with the dominant kernel's roofline, and the
-CPU baseline (the C oracle on the host cores, -@1 and all cores) on a bounded
-sample whose records are checked bit-for-bit against the GPU's.
+a sample (with and without its captured graph), per-stage kernel times with the
+dominant kernel's roofline, the serving loop at a sweep of offered rates, and
+the CPU baseline (the C oracle on the host cores, -@1 and all cores) on a
+bounded sample whose records are checked bit-for-bit against the GPU's.
 
 Prints one JSON line.
 """
@@ -201,20 +202,22 @@ def measure_issue(args, stage: str, kernel_ms: float) -> dict:
 def run_serving(idx, prm, lens, ref, args) -> dict:
     """Config 5: Poisson arrivals of reads (lognormal 1-100 kb, mean 15 kb, an
     even mix of 0.1% and 7% error profiles) into the serving loop over the
-    resident index; per-read latency from scheduled arrival to result."""
+    resident index; per-read latency from scheduled arrival to result.  One
+    warm-up run, then one run per offered rate of the sweep; the headline
+    figures are the run at --serve-rate."""
     import threading
     n = args.serve_reads
     bases, off, _ = W.serving_reads(lens, ref, n)
     hb = G.host_array(len(bases), np.uint8)  # page-locked: a batch of consecutive arrivals is one upload
     hb[:] = bases
-    rng = np.random.default_rng(11)
-    arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
     lg = C.CDLL(os.path.join(ROOT, "shmap-rs_amd", "lib", "libshm_probe.so"))
     lg.shm_loadgen_replay.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double]
     srv = G.Server(idx, prm, workers=args.serve_workers, max_batch_reads=args.serve_batch,
                    max_batch_bases=1 << 31)
-    runs = []
-    for rep in range(2):  # the first run warms every worker's device buffers
+    rates = [args.serve_rate] + [r for r in args.serve_sweep if r != args.serve_rate]
+
+    def one_run(rate):
+        arrival = np.cumsum(np.random.default_rng(11).exponential(1.0 / rate, n))
         t_begin = srv.now() + 0.005
         rc = []
         th = threading.Thread(target=lambda: rc.append(lg.shm_loadgen_replay(
@@ -232,29 +235,37 @@ def run_serving(idx, prm, lens, ref, args) -> dict:
             raise RuntimeError(f"serving run failed: rc={rc} got={have}/{n} {srv.error()}")
         s = np.concatenate(got)
         s = s[np.argsort(s["tag"])]
-        runs.append((t_begin, s))
-        log(f"[bench] serving run {rep} done: {n} reads")
-    t_begin, s = runs[-1]
+        lat = (s["done_s"] - (t_begin + arrival)) * 1e3
+        span = float(s["done_s"].max() - t_begin)
+        keys = s["worker"].astype(np.float64) * 1e6 + s["start_s"]
+        _, first = np.unique(keys, return_index=True)
+        bt = (s["done_s"][first] - s["start_s"][first]) * 1e3
+        stats = {"offered_reads_per_s": rate, "reads_per_s": round(n / span, 1),
+                 "gbp_per_s": round(float(off[-1]) / span / 1e9, 3),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                 "max_ms": round(float(lat.max()), 3), "batches": int(len(first)),
+                 "mean_batch_reads": round(float(s["batch_reads"].mean()), 1),
+                 "batch_ms_p50": round(float(np.percentile(bt, 50)), 3),
+                 "queue_wait_p50_ms": round(float(np.percentile((s["start_s"] - t_begin - arrival) * 1e3, 50)), 3)}
+        log(f"[bench] serving at {rate:.0f} reads/s offered: {stats}")
+        return s, stats
+
+    one_run(rates[0])  # warms every worker's device buffers and captured batches
+    runs = {r: one_run(r) for r in rates}
     srv.close()
-    lat = (s["done_s"] - (t_begin + arrival)) * 1e3
-    span = float(s["done_s"].max() - t_begin)
+    s, head = runs[args.serve_rate]
     # the same reads in one device-resident batch: results must not depend on batching
     mp = G.Mapper(idx, prm)
     whole = mp.map_batch((hb, off))
     same = all(np.array_equal(whole[f].view(np.int64) if whole.dtype[f] == np.float64 else whole[f],
                               s["r"][f].view(np.int64) if whole.dtype[f] == np.float64 else s["r"][f])
                for f in whole.dtype.names)
-    keys = s["worker"].astype(np.float64) * 1e6 + s["start_s"]
-    _, first = np.unique(keys, return_index=True)
-    bt = (s["done_s"][first] - s["start_s"][first]) * 1e3
-    log(f"[bench] serving batches: {len(first)}, batch ms p50 {np.percentile(bt, 50):.2f} "
-        f"max {bt.max():.2f}, queue wait p50 {np.percentile((s['start_s'] - t_begin - arrival) * 1e3, 50):.2f} ms")
-    return {"offered_reads_per_s": args.serve_rate, "reads": n, "reads_per_s": round(n / span, 1),
-            "gbp_per_s": round(float(off[-1]) / span / 1e9, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
-            "mean_batch_reads": round(float(s["batch_reads"].mean()), 1), "workers": args.serve_workers,
-            "mapped_reads": int(s["r"]["mapped"].sum()), "same_as_one_batch": same,
-            "workload": "lognormal 1-100 kb (mean 15 kb), 50/50 0.1% and 7% errors, Poisson arrivals"}
+    out = {"reads": n, **head, "workers": args.serve_workers, "mapped_reads": int(s["r"]["mapped"].sum()),
+           "same_as_one_batch": same,
+           "workload": "lognormal 1-100 kb (mean 15 kb), 50/50 0.1% and 7% errors, Poisson arrivals",
+           "batches_as": "speculative, graph-captured per read class",
+           "sweep": [runs[r][1] for r in sorted(runs)]}
+    return out
 
 
 def run_cli(args, segs, bases, off, res, prm, idx, h_frac) -> dict:
@@ -335,6 +346,8 @@ def main():
                     help="time the other scoring metrics on any workload (default: genome only)")
     ap.add_argument("--serve-reads", type=int, default=200_000)
     ap.add_argument("--serve-rate", type=float, default=1_000_000.0, help="offered reads/s (Poisson arrivals)")
+    ap.add_argument("--serve-sweep", type=lambda v: [float(x) for x in v.split(",") if x], default=[
+        500_000.0, 1_000_000.0, 2_000_000.0, 4_000_000.0], help="offered rates of the serving sweep (comma list)")
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=16384, help="admission budget: reads per batch")
     ap.add_argument("--no-autotune", action="store_true",
@@ -512,14 +525,34 @@ def main():
                  "frac": round(float(hit.sum()) / max(int(mk.sum()), 1), 5)}
     log(f"[bench] placement: {placement}")
 
-    # plain form: one read per launch
+    # plain form: one read per launch (a speculative batch of one read, its
+    # stream work captured once and replayed), then without the graph
     n_plain = min(args.plain_reads, n_reads)
     plain_reads = [bases[off[i]:off[i + 1]].tobytes() for i in range(n_plain)]
-    tp = time.time()
-    plain = np.concatenate([mp.map_batch([r]) for r in plain_reads]) if n_plain else res[:0]
-    plain_rate = n_plain / (time.time() - tp) if n_plain else None
-    plain_same = bool(np.array_equal(plain[["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]],
-                                     res[:n_plain][["mapped", "t_l", "t_r", "intersection", "seeded_buckets"]]))
+    plain_lat = {}
+    plain_rate = None
+    plain_same = True
+    for graphs in (True, False):
+        mp.set_graphs(graphs)
+        if n_plain:
+            mp.map_batch([plain_reads[0]])  # captures the one-read class
+        recs, lat_p = [], []
+        tp = time.time()
+        for r in plain_reads:
+            t1 = time.time()
+            recs.append(mp.map_batch([r]))
+            lat_p.append((time.time() - t1) * 1e3)
+        rate = n_plain / (time.time() - tp) if n_plain else None
+        plain = np.concatenate(recs) if n_plain else res[:0]
+        plain_same = plain_same and bool(np.array_equal(plain, res[:n_plain]))
+        key = "graph" if graphs else "no_graph"
+        plain_lat[key] = {"reads_per_s": round(rate, 1) if rate else None,
+                          "p50_ms": round(float(np.percentile(lat_p, 50)), 4) if n_plain else None,
+                          "p99_ms": round(float(np.percentile(lat_p, 99)), 4) if n_plain else None}
+        if graphs:
+            plain_rate = rate
+    mp.set_graphs(True)
+    log(f"[bench] plain form: {plain_lat}")
 
     # End to end from host memory: the reads start in page-locked host memory,
     # 2-bit packed as the reader delivers them (shm_pack_reads; its rate is
@@ -680,7 +713,7 @@ def main():
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "plain_gpu": {"reads_per_s": round(plain_rate, 2) if plain_rate else None, "reads": n_plain,
-                      "same_records": plain_same},
+                      "same_records": plain_same, **plain_lat},
         "batched_over_plain": round(value / plain_rate, 1) if plain_rate else None,
         "end_to_end": end_to_end,
         "pcie_inclusive_bytes": pcie,

@@ -337,8 +337,11 @@ typedef struct {
     int32_t max_batch_reads; /* admission budget per batch; 0 = 65536 */
     int64_t max_batch_bases; /* admission budget per batch; 0 = 2^31 */
     int32_t max_read_len;    /* longer reads fail alone (their status); 0 = no limit */
-    int32_t pad;
+    int32_t flags;           /* SHM_SERVER_*: the workers' mapper settings; 0 = defaults */
 } shm_server_config;
+
+#define SHM_SERVER_NO_SPECULATION 1 /* workers' batches synchronised (shm_mapper_set_speculation) */
+#define SHM_SERVER_NO_GRAPHS 2      /* workers' batches not captured (shm_mapper_set_graphs) */
 
 typedef struct {
     uint64_t tag;          /* the caller's id from shm_server_submit */
@@ -366,6 +369,11 @@ int shm_server_poll(shm_server *s, shm_served *out, int32_t cap, int32_t min_res
                     int32_t *n);
 /* Reads queued, being mapped, and finished but not yet polled. */
 int shm_server_counts(shm_server *s, int64_t *queued, int64_t *running, int64_t *finished);
+/* Batches the workers have mapped so far: [0] batches, [1] speculative
+ * batches mapped again, [2] batches that allocated device or page-locked
+ * memory, [3] graph captures (shm_mapper_last_counts [10], [11], [13]).
+ * Writes up to n values. */
+int shm_server_batch_stats(shm_server *s, int64_t *stats, int32_t n);
 double shm_server_now(const shm_server *s);
 /* The message of the first failed batch ("" if none). */
 const char *shm_server_error(shm_server *s);

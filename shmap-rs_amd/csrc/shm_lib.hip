@@ -1004,7 +1004,15 @@ struct BatchShape {
 struct MapGraph {
     hipGraphExec_t exec = nullptr;
     uint64_t key = 0;
+    uint64_t seen_key = 0;  // the key of the class's latest eager batches,
+    int32_t seen = 0;       // and how many ran in a row
 };
+
+// A class is captured once GRAPH_AFTER batches in a row had the same key: a
+// capture and instantiation cost several milliseconds, about what a hundred
+// replays save, so classes that come rarely, or while capacities still grow,
+// run eagerly.
+constexpr int32_t GRAPH_AFTER = 8;
 
 struct shm_mapper {
     const shm_index *idx = nullptr;
@@ -1763,10 +1771,20 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
     std::vector<int> pending;
     HIPCHK(hipEventRecord(mp->t0, st));
     if (RL > R) launch_fill_u64((uint64_t *)mp->roff.p + R + 1, (uint64_t)(RL - R), (uint64_t)h_off[R], st);
+    bool replay = false;
+    MapGraph *gp = nullptr;
     if (graph) {
         MapGraph &g = mp->graphs[RL];
+        gp = &g;
         const uint64_t key = graph_key(mp, A, d_seq, d_off, res_dev);
-        if (!g.exec || g.key != key) {
+        if (g.exec && g.key == key) {
+            replay = true;
+        } else {
+            g.seen = g.seen_key == key ? g.seen + 1 : 1;
+            g.seen_key = key;
+            replay = g.seen >= GRAPH_AFTER;
+        }
+        if (replay && (!g.exec || g.key != key)) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g.exec = nullptr;
             const int64_t a0 = g_allocs;
@@ -1787,14 +1805,15 @@ static int map_speculative(shm_mapper *mp, const uint8_t *d_seq, const int64_t *
             g.key = g_allocs == a0 ? key : 0;
             mp->counts[13] = 1;
         }
-        HIPCHK(hipGraphLaunch(g.exec, st));
+    }
+    if (replay) {
+        HIPCHK(hipGraphLaunch(gp->exec, st));
         mp->counts[12] = 1;
-        HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     } else {
         RCHK(enqueue_speculative(mp, d_seq, d_off, RL, A, res_dev, pending));
-        if (!direct)
-            HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     }
+    if (!direct)  // (a batch that may be captured writes its records to d_out)
+        HIPCHK(hipMemcpyAsync(out, res_dev, (size_t)R * sizeof(shm_read_result), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(mp->t1, st));
     HIPCHK(hipStreamSynchronize(st));
     RCHK(flush_timing(mp, pending));

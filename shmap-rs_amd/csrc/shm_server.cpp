@@ -37,6 +37,7 @@ struct shm_server {
     std::deque<Request> queue;
     std::deque<shm_served> done;
     int64_t running = 0;
+    int64_t bstats[4] = {};  // batches, redone, allocating, captures
     bool stop = false;
     std::string error;
 
@@ -77,11 +78,18 @@ void shm_server::worker(int wi) {
         const int rc = shm_map_batch(mp, reads.data(), n, res.data());
         const std::string msg = rc ? shm_last_error() : "";
         const double end = now();
+        int64_t wc[14] = {};
+        int32_t nw = 0;
+        (void)shm_mapper_last_counts(mp, wc, 14, &nw);
         bool marked = false;
         for (int32_t i = 0; i < n && !marked; i++) marked = res[i].status != 0;
         {
             std::lock_guard<std::mutex> lk(mu);
             if (rc && error.empty()) error = msg;
+            bstats[0]++;
+            bstats[1] += wc[10] != 0;
+            bstats[2] += wc[11] != 0;
+            bstats[3] += wc[13];
             for (int32_t i = 0; i < n; i++) {
                 shm_served d{};
                 d.tag = batch[i].tag;
@@ -119,6 +127,8 @@ extern "C" int shm_server_create(const shm_index *idx, const shm_map_params *par
         }
         s->mappers.push_back(mp);
         if (s->cfg.max_read_len > 0) (void)shm_mapper_set_read_limit(mp, s->cfg.max_read_len);
+        if (s->cfg.flags & SHM_SERVER_NO_SPECULATION) (void)shm_mapper_set_speculation(mp, 0);
+        if (s->cfg.flags & SHM_SERVER_NO_GRAPHS) (void)shm_mapper_set_graphs(mp, 0);
     }
     s->t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < s->cfg.workers; i++) s->threads.emplace_back(&shm_server::worker, s, i);
@@ -190,6 +200,13 @@ extern "C" int shm_server_counts(shm_server *s, int64_t *queued, int64_t *runnin
     if (queued) *queued = (int64_t)s->queue.size();
     if (running) *running = s->running;
     if (finished) *finished = (int64_t)s->done.size();
+    return SHM_OK;
+}
+
+extern "C" int shm_server_batch_stats(shm_server *s, int64_t *stats, int32_t n) {
+    if (!s || n < 0 || (n > 0 && !stats)) return SHM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int32_t i = 0; i < n && i < 4; i++) stats[i] = s->bstats[i];
     return SHM_OK;
 }
 

@@ -68,7 +68,9 @@ assert RESULT_DTYPE.itemsize == C.sizeof(ReadResult)
 
 class ServerConfig(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch_reads", C.c_int32), ("max_batch_bases", C.c_int64),
-                ("max_read_len", C.c_int32), ("pad", C.c_int32)]
+                ("max_read_len", C.c_int32), ("flags", C.c_int32)]
+
+SERVER_NO_SPECULATION, SERVER_NO_GRAPHS = 1, 2  # ServerConfig.flags
 
 
 class Served(C.Structure):
@@ -150,6 +152,7 @@ def lib():
         L.shm_server_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_int32)]
         L.shm_server_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 3
+        L.shm_server_batch_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]
         L.shm_server_now.argtypes = [C.c_void_p]
         L.shm_server_now.restype = C.c_double
         L.shm_server_error.argtypes = [C.c_void_p]
@@ -428,10 +431,10 @@ class Server:
     they arrive and mapped in batches by `workers` concurrent mappers."""
 
     def __init__(self, index: SketchIndex, params: MapParams, workers: int = 3, max_batch_reads: int = 0,
-                 max_batch_bases: int = 0, max_read_len: int = 0):
+                 max_batch_bases: int = 0, max_read_len: int = 0, flags: int = 0):
         self._index = index
         self._params = params
-        cfg = ServerConfig(workers, max_batch_reads, max_batch_bases, max_read_len, 0)
+        cfg = ServerConfig(workers, max_batch_reads, max_batch_bases, max_read_len, flags)
         self._p = C.c_void_p()
         _check(lib().shm_server_create(index._p, C.byref(params), C.byref(cfg), C.byref(self._p)))
 
@@ -460,6 +463,13 @@ class Server:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().shm_server_counts(self._p, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def batch_stats(self) -> dict:
+        """Batches so far: all, speculative ones mapped again, ones that
+        allocated memory, graph captures."""
+        a = (C.c_int64 * 4)()
+        _check(lib().shm_server_batch_stats(self._p, a, 4))
+        return dict(zip(("batches", "redone", "allocating", "captures"), (int(x) for x in a)))
 
     def error(self) -> str:
         return lib().shm_server_error(self._p).decode()
