@@ -624,14 +624,23 @@ def main():
 
     # whole-path algorithmic bytes per step: bases, one 16 B probe per distinct
     # k-mer, 8 B per examined hit, 2m reference k-mers of 12 B per refined
-    # bucket, 16 B per sketched k-mer (the prune pass's hit reads are not counted)
+    # bucket, 16 B per sketched k-mer, and 8 B per hit record the prune pass
+    # reads (P, counted by the oracle on the CPU sample below, scaled to the
+    # batch by reads)
     Lr = (off[1:] - off[:-1]).astype(np.int64)
     pb = int((Lr + 16 * res["u"].astype(np.int64) + 8 * res["seed_matches"].astype(np.int64).clip(min=0)
               + 24 * res["refined_buckets"].astype(np.int64) * res["m"].astype(np.int64)
               + 16 * res["m"].astype(np.int64)).sum())
-    path_gbs = pb * args.steps / elapsed / 1e9
-    path_roof = {"bytes_per_step": pb, "achieved": round(path_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": round(path_gbs / HBM_PEAK_GBS, 5)}
+
+    def path_roofline(prune_hits):
+        b = pb + 8 * (prune_hits or 0)
+        gbs = b * args.steps / elapsed / 1e9
+        return {"bytes_per_step": b, "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "prune_hits": prune_hits,
+                "prune_hits_by": "oracle on the CPU sample, scaled to the batch" if prune_hits is not None
+                else "not counted (no CPU sample)"}
+
+    path_roof = path_roofline(None)
 
     cli = None
     if not args.no_cli:
@@ -672,6 +681,7 @@ def main():
         cpu1 = n_s / spent
         ores_all, spent_all = oidx.map_reads_timed(op, sample, threads=ncores)
         cpun = n_s / spent_all
+        path_roof = path_roofline(int(round(oidx.last_prune_hits * n_reads / n_s)))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import parity_util  # tests/parity_util.py: field-by-field record diff
         bad = parity_util.diff_results(ores, res[:n_s])

@@ -81,6 +81,8 @@ def lib():
         L.ok_map_read.argtypes = [C.c_void_p, C.POINTER(OkParams), C.c_void_p, C.c_int32, C.POINTER(OkResult)]
         L.ok_map_reads.argtypes = [C.c_void_p, C.POINTER(OkParams), C.c_int32, C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_int32), C.c_void_p, C.c_int32]
+        L.ok_map_reads_counted.argtypes = [C.c_void_p, C.POINTER(OkParams), C.c_int32, C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_int32), C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
         L.ok_format_paf.argtypes = [C.c_void_p, C.POINTER(OkParams), C.c_char_p, C.POINTER(OkResult), C.c_double,
                                     C.c_char_p, C.c_size_t]
         L.ok_hseed.restype = C.c_double
@@ -167,9 +169,11 @@ class OracleIndex:
             ptrs = (C.c_void_p * max(n, 1))(*[a.ctypes.data if a.size else None for a in keep])
             lens = (C.c_int32 * max(n, 1))(*[len(a) for a in keep])
         out = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        ph = C.c_int64(0)
         t0 = time.perf_counter()
-        lib().ok_map_reads(self._p, C.byref(prm), n, ptrs, lens, out.ctypes.data, threads)
+        lib().ok_map_reads_counted(self._p, C.byref(prm), n, ptrs, lens, out.ctypes.data, threads, C.byref(ph))
         dt = time.perf_counter() - t0
+        self.last_prune_hits = ph.value  # hit records the prune pass read
         del keep
         return out[:n], dt
 

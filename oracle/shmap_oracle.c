@@ -672,6 +672,7 @@ typedef struct {
     int32_t *ppos;    int64_t ppos_cap;
     struct { int32_t idx; mapping_t m; } *memo; int64_t memo_cap; int64_t n_memo;
     uint64_t ht_mask;
+    int64_t prune_hits;  /* hit records the prune pass read (a measurement, not a result) */
 } ws_t;
 
 static int cmp_kmer_h_rdesc(const void *a, const void *b) {
@@ -712,6 +713,7 @@ typedef struct {
     const ok_index *idx;
     const ok_params *prm;
     int32_t halflen;
+    int64_t *prune_hits;  /* the workspace's count of hit records the prune pass reads */
 } ctx_t;
 
 static inline int32_t bucket_begin(const ctx_t *c, int32_t b) { return b * c->halflen; }
@@ -865,6 +867,7 @@ static void matches_in_bucket(const ctx_t *c, int32_t bsegm, int32_t bb, bconten
     const hslot_t *slot = index_find(idx, s->h);
     const int32_t begin = bucket_begin(c, bb), end = bucket_end(c, bb);
     if (s->hits_in_t == 1) {
+        *c->prune_hits += 1;
         const hit_t *hit = &idx->hits[slot->start];
         const int32_t pos = ap ? hit->r : hit->tpos;
         if (hit->segm_id == bsegm && begin <= pos && pos < end) {
@@ -882,6 +885,7 @@ static void matches_in_bucket(const ctx_t *c, int32_t bsegm, int32_t bb, bconten
     while (lo < hi) {
         const int64_t mid = (lo + hi) / 2;
         const hit_t *h = &hits[mid];
+        *c->prune_hits += 1;
         const int32_t pos = ap ? h->r : h->tpos;
         const int before = h->segm_id != bsegm ? h->segm_id < bsegm : pos < begin;
         if (before) lo = mid + 1; else hi = mid;
@@ -889,6 +893,7 @@ static void matches_in_bucket(const ctx_t *c, int32_t bsegm, int32_t bb, bconten
     int32_t matches = 0;
     for (int64_t t = lo; t < n; t++) {
         const hit_t *h = &hits[t];
+        *c->prune_hits += 1;
         const int32_t pos = ap ? h->r : h->tpos;
         if (!(h->segm_id == bsegm && pos < end)) break;
         matches++;
@@ -1148,7 +1153,7 @@ static int map_read_ws(ws_t *w, const ok_index *idx, const ok_params *prm, const
     const double theta2 = theta - prm->min_diff;                  /* :728 */
     const int32_t S = prm->one_sweep ? u : (int32_t)((1.0 - theta2) * (double)m) + 1; /* :732-736 */
     out->S = S;
-    ctx_t c = {idx, prm, prm->abs_pos ? len : m};                 /* :738-742 */
+    ctx_t c = {idx, prm, prm->abs_pos ? len : m, &w->prune_hits};  /* :738-742 */
     if (c.halflen < MIN_HALFLEN) {                                /* :772-783 */
         out->unmappable_short = 1;
         return 0;
@@ -1258,6 +1263,7 @@ typedef struct {
     ok_result *out;
     int32_t next;
     pthread_mutex_t mu;
+    int64_t prune_hits;
 } pool_t;
 
 static void *map_worker(void *arg) {
@@ -1274,23 +1280,32 @@ static void *map_worker(void *arg) {
         for (int32_t j = i; j < e; j++)
             map_read_ws(&w, pool->idx, pool->p, pool->seqs[j], pool->lens[j], &pool->out[j]);
     }
+    pthread_mutex_lock(&pool->mu);
+    pool->prune_hits += w.prune_hits;
+    pthread_mutex_unlock(&pool->mu);
     ws_free(&w);
     return NULL;
 }
 
-int ok_map_reads(const ok_index *idx, const ok_params *p, int32_t n, const uint8_t *const *seqs,
-                 const int32_t *lens, ok_result *out, int32_t n_threads) {
+int ok_map_reads_counted(const ok_index *idx, const ok_params *p, int32_t n, const uint8_t *const *seqs,
+                         const int32_t *lens, ok_result *out, int32_t n_threads, int64_t *prune_hits) {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > 256) n_threads = 256;
-    pool_t pool = {idx, p, n, seqs, lens, out, 0, PTHREAD_MUTEX_INITIALIZER};
+    pool_t pool = {idx, p, n, seqs, lens, out, 0, PTHREAD_MUTEX_INITIALIZER, 0};
     if (n_threads == 1) {
         map_worker(&pool);
-        return 0;
+    } else {
+        pthread_t th[256];
+        for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, map_worker, &pool);
+        for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
     }
-    pthread_t th[256];
-    for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, map_worker, &pool);
-    for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+    if (prune_hits) *prune_hits = pool.prune_hits;
     return 0;
+}
+
+int ok_map_reads(const ok_index *idx, const ok_params *p, int32_t n, const uint8_t *const *seqs,
+                 const int32_t *lens, ok_result *out, int32_t n_threads) {
+    return ok_map_reads_counted(idx, p, n, seqs, lens, out, n_threads, NULL);
 }
 
 /* Rust `{:.N}` on f64: exact value rounded half-to-even (glibc printf agrees in

@@ -120,6 +120,11 @@ int ok_map_read(const ok_index *idx, const ok_params *p, const uint8_t *seq, int
  * n_threads threads; results land at out[i] in input order. */
 int ok_map_reads(const ok_index *idx, const ok_params *p, int32_t n, const uint8_t *const *seqs,
                  const int32_t *lens, ok_result *out, int32_t n_threads);
+/* The same, also returning the hit records the seed-heuristic prune pass
+ * (matches_in_bucket, src/shmap/pruning.rs:18-80) read in total: the P term of
+ * the path's algorithmic bytes.  A measurement, not part of any result. */
+int ok_map_reads_counted(const ok_index *idx, const ok_params *p, int32_t n, const uint8_t *const *seqs,
+                         const int32_t *lens, ok_result *out, int32_t n_threads, int64_t *prune_hits);
 
 /* src/mapping.rs:87-107, 161-182, 228-247: PAF line (no newline), t:f: printed
  * from map_time.  Returns bytes written (snprintf semantics). */

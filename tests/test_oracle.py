@@ -565,6 +565,29 @@ def test_pruning_is_sound(metric):
     assert np.all(a["refined_buckets"] <= b["refined_buckets"])
 
 
+def test_prune_hit_count_is_a_per_read_measurement():
+    """The hit records the prune pass reads (the P term of the path's
+    algorithmic bytes) are counted per read: none under -P, which skips the
+    pass; the same on 1 and 4 threads; additive over disjoint read sets."""
+    segs = _dup_reference()
+    reads = _dup_reads(segs, n=40, seed=23)
+    k, hf = 15, 0.1
+    oi = O.OracleIndex(k, hf, segs)
+    prm = dict(k=k, h_frac=hf, theta=0.3, min_diff=0.075, max_overlap=0.3)
+    oi.map_reads(O.params(no_bucket_pruning=True, **prm), reads)
+    assert oi.last_prune_hits == 0
+    p = O.params(**prm)
+    oi.map_reads(p, reads, threads=1)
+    total = oi.last_prune_hits
+    assert total > 0
+    oi.map_reads(p, reads, threads=4)
+    assert oi.last_prune_hits == total
+    oi.map_reads(p, reads[:17])
+    a = oi.last_prune_hits
+    oi.map_reads(p, reads[17:])
+    assert a + oi.last_prune_hits == total
+
+
 def test_sigma_and_mapq_at_hand_computed_points():
     """src/mapping.rs:331-361: sigma = |I2 - I| / sqrt(I2 + I) with -1 read
     as 0; J2 below theta2 is raised to theta2 and printed so; MAPQ is 60 (5
