@@ -299,21 +299,27 @@ int shm_mapper_set_graphs(shm_mapper *mp, int32_t on);
  * shape to another.  No reference counterpart.  Any values give the same
  * records, only the speed changes; 0 in a field means the built-in rule.
  *   contrib_wave_hits   examined hits up to which one wave counts a read's
- *                       buckets whole; larger reads go to chunks (built-in:
- *                       1024 for batches under 2^24 examined hits, else 4096)
- *   contrib_chunk_hits  chunk length of those reads, a power of two from 64
- *                       to 4096 (built-in: 512-4096, about 8192 chunks)
+ *                       buckets whole; larger reads go to a workgroup
+ *                       (built-in: 1024 for batches under 2^24 examined
+ *                       hits, else 4096)
+ *   contrib_chunk_hits  chunk length of the reads past a workgroup, counted
+ *                       through global tables, a power of two from 64 to
+ *                       4096 (built-in: 512-4096, about 8192 chunks)
  *   prune_thread_seeds  seeds one thread extends a bucket with before a wave
- *                       takes it over (built-in: 16) */
+ *                       takes it over (built-in: 16)
+ *   contrib_wg_hits     examined hits up to which one workgroup counts a
+ *                       read's buckets whole in a shared table of 1,024
+ *                       slots (built-in: 8192) */
 typedef struct {
     int32_t contrib_wave_hits;
     int32_t contrib_chunk_hits;
     int32_t prune_thread_seeds;
+    int32_t contrib_wg_hits;
 } shm_tuning;
 int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t);
 int shm_mapper_get_tuning(const shm_mapper *mp, shm_tuning *t);
 /* Autotuner: maps the sample batch repeatedly under candidate tunings (a
- * coordinate search over the three fields, each candidate timed twice by the
+ * coordinate search over the four fields, each candidate timed twice by the
  * batch's GPU time), keeps the fastest on the mapper and writes it to *out
  * (optional).  The sample should look like the batches to come; a few
  * thousand reads suffice. */

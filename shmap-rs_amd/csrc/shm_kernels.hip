@@ -1187,12 +1187,22 @@ constexpr uint32_t LTAB_MAX = LTAB * 7 / 8;
 constexpr uint32_t CONTRIB_CHUNK_MIN = 512, CONTRIB_CHUNK_MAX = 4096;
 constexpr uint64_t CONTRIB_CHUNKS_WANT = 8192;
 
-enum ContribMode { CM_LDS, CM_LDS_FLUSH };
+// A workgroup takes a read past one wave whole, its waves splitting the read's
+// examined hits into quarters, with a table of WTAB slots in LDS shared by its
+// waves, when the read has at most A.contrib_wg_e examined hits and its
+// buckets fit (waves reserve a window's worst case of new slots before adding,
+// so the table never passes WTAB_MAX); other reads go through global tables.
+constexpr uint32_t WTAB_BITS = 10;
+constexpr uint32_t WTAB = 1u << WTAB_BITS;
+constexpr uint32_t WTAB_MAX = WTAB * 7 / 8;
+
+enum ContribMode { CM_LDS, CM_LDS_FLUSH, CM_WG };
 
 struct LTab {
     uint32_t *key, *mark, *used;
     uint64_t *mc;
     int32_t *mn, *mx;
+    uint32_t *over;  // CM_WG: set when the shared table would pass WTAB_MAX
 };
 
 __device__ __forceinline__ void ltab_clear(const LTab &T, uint32_t lane) {
@@ -1204,10 +1214,11 @@ __device__ __forceinline__ void ltab_clear(const LTab &T, uint32_t lane) {
     }
 }
 
-// Adds a contribution to the LDS table; returns 1 when it claimed a new slot
-// (the caller counts claims into *T.used).
+// Adds a contribution to an LDS table of 2^BITS slots; returns 1 when it
+// claimed a new slot (the caller counts claims into *T.used).
+template <uint32_t BITS>
 __device__ __forceinline__ uint32_t ltab_add(const LTab &T, uint32_t g, uint64_t mcv, int32_t rmin, int32_t rmax) {
-    uint32_t pos = (g * 0x9E3779B1u) >> (32 - LTAB_BITS), claimed = 0;
+    uint32_t pos = (g * 0x9E3779B1u) >> (32 - BITS), claimed = 0;
     for (;;) {
         const uint32_t old = atomicCAS(&T.key[pos], EMPTY_KEY, g);
         if (old == EMPTY_KEY) {
@@ -1215,7 +1226,7 @@ __device__ __forceinline__ uint32_t ltab_add(const LTab &T, uint32_t g, uint64_t
             break;
         }
         if (old == g) break;
-        pos = (pos + 1u) & (LTAB - 1);
+        pos = (pos + 1u) & ((1u << BITS) - 1);
     }
     atomicAdd((unsigned long long *)&T.mc[pos], (unsigned long long)mcv);
     atomicMin(&T.mn[pos], rmin);
@@ -1258,10 +1269,14 @@ __device__ __forceinline__ void ltab_flush(const MapArgs &A, const LTab &T, uint
 // run before it), settled by one ballot in the window where its run ends, or
 // by binary search at the end of the range.  CM_LDS returns false, the table
 // partly filled, when the table would pass LTAB_MAX; CM_LDS_FLUSH then
-// empties it into the global table and goes on.
+// empties it into the global table and goes on; CM_WG (a table of WTAB slots
+// shared by a workgroup's waves) reserves each window's worst case in *T.used
+// and returns false, with *T.over set, when that would pass WTAB_MAX or
+// another wave has set it.
 template <int MODE>
 __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab &T, int32_t lr, uint32_t pa,
                                 uint32_t pb, uint32_t E, uint32_t lane, uint32_t &stamp) {
+    constexpr uint32_t BITS = MODE == CM_WG ? WTAB_BITS : LTAB_BITS;
     const int ap = A.abs_pos;
     const uint64_t upto = (((1ull << lane) - 1) << 1) | 1ull;
     const int32_t rd = A.r0 + lr;
@@ -1285,7 +1300,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         const uint32_t same = (ocv >> 31) ? ones : n - ones;
         const uint32_t matches = n < occs ? n : occs;
         const int32_t codir = (int32_t)(2 * same) - (int32_t)n;
-        claims += ltab_add(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
+        claims += ltab_add<BITS>(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
                            rmin0, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
     };
     // pending contributions: lane i < 3 holds item i (pmask, uniform): its
@@ -1295,6 +1310,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
     uint32_t pq = 0, psg = 0, ppos = 0, pidx = 0, pee = 0, poc = 0;
     int32_t ptb = 0, pbb = 0, prmin = 0;
     for (uint32_t p0 = pa; p0 < pb; p0 += WAVE) {
+        if (MODE == CM_WG && *(volatile uint32_t *)T.over) return false;
         ++stamp;
         const uint32_t qi = qc + lane;
         const bool qv = qi < c;
@@ -1361,7 +1377,19 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         }
         const bool emit = head && act;
         const uint32_t nh = (uint32_t)__popcll(__ballot(emit));
-        if (*T.used + 2 * nh + 6 > LTAB_MAX) {  // heads, settled and closing pending contributions
+        const uint32_t need = 2 * nh + 6;  // heads, settled and closing pending contributions
+        if (MODE == CM_WG) {
+            uint32_t ok = 1;
+            if (lane == 0) {
+                const uint32_t had = atomicAdd(T.used, need);
+                if (had + need > WTAB_MAX) {
+                    atomicSub(T.used, need);
+                    atomicExch(T.over, 1u);
+                    ok = 0;
+                }
+            }
+            if (!__shfl(ok, 0)) return false;
+        } else if (*T.used + need > LTAB_MAX) {
             if (MODE == CM_LDS) return false;
             ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[rd], lane);
         }
@@ -1425,7 +1453,10 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
                                 4u * (uint32_t)__popcll(__ballot(claims & 4u));
         claims = 0;
         wave_sync();
-        if (lane == 0) *T.used += nclaim;
+        if (lane == 0) {
+            if (MODE == CM_WG) atomicSub(T.used, need - nclaim);  // the unused part of the reservation
+            else *T.used += nclaim;
+        }
         // the seed holding p0 + 64: every seed starting at or before it
         uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= p0 + WAVE));
         if (adv == WAVE - 1 && qc + WAVE < c && A.sd_epre[o + qc + WAVE] <= p0 + WAVE) adv++;
@@ -1442,7 +1473,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
     __shared__ uint32_t s_mark[NWAVES][WAVE];                                                  \
     __shared__ uint32_t s_used[NWAVES];                                                        \
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); \
-    const LTab T{s_key[w], s_mark[w], &s_used[w], s_mc[w], s_mn[w], s_mx[w]};                  \
+    const LTab T{s_key[w], s_mark[w], &s_used[w], s_mc[w], s_mn[w], s_mx[w], nullptr};         \
     ltab_clear(T, lane);                                                                       \
     T.mark[lane] = 0;                                                                          \
     if (lane == 0) *T.used = 0;                                                                \
@@ -1452,7 +1483,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
 // One wave per read with at most A.contrib_wave_e examined hits: its buckets end
 // as the first grp_nb rows of its table region, listed in order.  Larger
 // reads, and reads with more than LTAB_MAX buckets, are listed for
-// k_contrib_chunks.
+// k_contrib_wg.
 __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
     if (aborted(A.abort)) return;
     LTAB_DECLARE();
@@ -1503,15 +1534,80 @@ __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
     }
 }
 
+// One workgroup per read listed by k_contrib_wave: its examined hits split
+// into WG quarters, one per wave, counted into one LDS table of WTAB slots;
+// its buckets end as the first grp_nb rows of its table region, as
+// k_contrib_wave leaves them.  Reads with more than A.contrib_wg_e examined
+// hits, or whose buckets would overfill the table, are listed for the global
+// tables (k_contrib_chunks).
+__global__ void __launch_bounds__(WG) k_contrib_wg(MapArgs A, DevIndex I) {
+    if (aborted(A.abort)) return;
+    __shared__ uint32_t s_key[WTAB];
+    __shared__ uint64_t s_mc[WTAB];
+    __shared__ int32_t s_mn[WTAB], s_mx[WTAB];
+    __shared__ uint32_t s_mark[NWAVES][WAVE];
+    __shared__ uint32_t s_used, s_over;
+    __shared__ uint64_t tmp[NWAVES + 1];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const LTab T{s_key, s_mark[w], &s_used, s_mc, s_mn, s_mx, &s_over};
+    s_mark[w][lane] = 0;
+    uint32_t stamp = 0;
+    const uint32_t n = *A.ov_count;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+        const int32_t lr = (int32_t)A.ov_list[t];
+        const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
+        if (E > A.contrib_wg_e) {
+            if (threadIdx.x == 0) A.gl_list[atomicAdd(A.gl_count, 1u)] = (uint32_t)lr;
+            continue;
+        }
+        for (uint32_t i = threadIdx.x; i < WTAB; i += WG) {
+            s_key[i] = EMPTY_KEY;
+            s_mc[i] = 0;
+            s_mn[i] = INT32_MAX;
+            s_mx[i] = -1;
+        }
+        if (threadIdx.x == 0) s_used = s_over = 0;
+        __syncthreads();
+        const uint32_t pa = (uint32_t)((uint64_t)E * w / NWAVES), pb = (uint32_t)((uint64_t)E * (w + 1) / NWAVES);
+        if (pa < pb) (void)contrib_windows<CM_WG>(A, I, T, lr, pa, pb, E, lane, stamp);
+        __syncthreads();
+        if (s_over) {
+            if (threadIdx.x == 0) A.gl_list[atomicAdd(A.gl_count, 1u)] = (uint32_t)lr;
+            __syncthreads();
+            continue;
+        }
+        // the used slots become rows 0.. of the read's table region
+        const uint64_t tb = A.tab_base[lr];
+        uint32_t nb = 0;
+        for (uint32_t s0 = 0; s0 < WTAB; s0 += WG) {
+            const uint32_t sl = s0 + threadIdx.x;
+            const uint32_t k = s_key[sl];
+            const bool used = k != EMPTY_KEY;
+            uint64_t tot;
+            const uint32_t at = nb + (uint32_t)wg_excl_scan64(used ? 1u : 0u, tmp, &tot);
+            if (used) {
+                A.tb_key[tb + at] = k;
+                A.tb_mc[tb + at] = s_mc[sl];
+                A.tb_min[tb + at] = s_mn[sl];
+                A.tb_max[tb + at] = s_mx[sl];
+                A.tb_list[tb + at] = at;
+            }
+            nb += (uint32_t)tot;
+        }
+        if (threadIdx.x == 0) A.grp_nb[lr] = nb;
+        __syncthreads();
+    }
+}
+
 // The chunk length for the listed reads (kept at ov_kpre[n + 1]) and their
 // chunk prefix (one workgroup).
 __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
     if (aborted(A.abort)) return;
     __shared__ uint64_t tmp[NWAVES + 1];
-    const uint32_t n = *A.ov_count;
+    const uint32_t n = *A.gl_count;
     uint64_t part = 0;
     for (uint32_t i = threadIdx.x; i < n; i += WG) {
-        const uint32_t lr = A.ov_list[i];
+        const uint32_t lr = A.gl_list[i];
         part += A.hit_base[lr + 1] - A.hit_base[lr];
     }
     uint64_t tot_e;
@@ -1526,7 +1622,7 @@ __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
         const uint32_t i = b0 + threadIdx.x;
         uint64_t k = 0;
         if (i < n) {
-            const uint32_t lr = A.ov_list[i];
+            const uint32_t lr = A.gl_list[i];
             k = (A.hit_base[lr + 1] - A.hit_base[lr] + cs - 1) / cs;
         }
         uint64_t tot;
@@ -1543,9 +1639,9 @@ __global__ void __launch_bounds__(WG) k_contrib_over_scan(MapArgs A) {
 // Empty global tables for the listed reads, one workgroup per read.
 __global__ void __launch_bounds__(WG) k_contrib_over_clear(MapArgs A) {
     if (aborted(A.abort)) return;
-    const uint32_t n = *A.ov_count;
+    const uint32_t n = *A.gl_count;
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
-        const uint32_t lr = A.ov_list[t];
+        const uint32_t lr = A.gl_list[t];
         const uint64_t tb = A.tab_base[lr], cap = A.rd_cap[A.r0 + (int32_t)lr];
         for (uint64_t i = threadIdx.x; i < cap; i += WG) {
             A.tb_key[tb + i] = EMPTY_KEY;
@@ -1562,12 +1658,12 @@ __global__ void __launch_bounds__(WG) k_contrib_over_clear(MapArgs A) {
 __global__ void __launch_bounds__(WG) k_contrib_chunks(MapArgs A, DevIndex I) {
     if (aborted(A.abort)) return;
     LTAB_DECLARE();
-    const uint32_t nl = *A.ov_count;
+    const uint32_t nl = *A.gl_count;
     const uint64_t nk = A.ov_kpre[nl];
     const uint32_t cs = (uint32_t)A.ov_kpre[nl + 1];
     for (uint64_t k = (uint64_t)blockIdx.x * NWAVES + w; k < nk; k += (uint64_t)gridDim.x * NWAVES) {
         const uint32_t i = upper_bound_u64(A.ov_kpre, nl + 1, k) - 1;
-        const int32_t lr = (int32_t)A.ov_list[i];
+        const int32_t lr = (int32_t)A.gl_list[i];
         const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
         const uint32_t pa = (uint32_t)(k - A.ov_kpre[i]) * cs;
         const uint32_t pb = pa + cs < E ? pa + cs : E;
@@ -2937,8 +3033,10 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
     case ST_TAB_INIT: k_tab_init<<<flat_grid((uint64_t)A.RG, 1024), WG, 0, st>>>(A); break;
     case ST_CONTRIB:
         // one wave per read (the grid covers every read: reads differ widely
-        // in cost), then the listed reads in chunks through global tables
+        // in cost), a workgroup per read past one wave, then the reads past
+        // that in chunks through global tables
         k_contrib_wave<<<(uint32_t)((A.RG + NWAVES - 1) / NWAVES), WG, 0, st>>>(A, I);
+        k_contrib_wg<<<max_grid, WG, 0, st>>>(A, I);
         k_contrib_over_scan<<<1, WG, 0, st>>>(A);
         k_contrib_over_clear<<<max_grid, WG, 0, st>>>(A);
         k_contrib_chunks<<<PGRID(k_contrib_chunks, 0, max_grid), WG, 0, st>>>(A, I);

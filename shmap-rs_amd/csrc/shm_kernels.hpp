@@ -27,8 +27,8 @@ constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint64_t SCAN_SINGLE_MAX = 8192; // arrays up to this length scan in one workgroup
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
-constexpr uint32_t TASK_COUNTERS = 7;       // per group: first-pass, second band, 3 refine rounds, prune list,
-                                            // reads on the global-table bucket path
+constexpr uint32_t TASK_COUNTERS = 8;       // per group: first-pass, second band, 3 refine rounds, prune list,
+                                            // reads on the global-table bucket path, reads past one wave
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
 
 enum { METRIC_CONTAINMENT = 0, METRIC_JACCARD = 1, METRIC_BUCKET_SH = 2, METRIC_BUCKET_LCS = 3 };
@@ -129,13 +129,16 @@ struct MapArgs {
     uint32_t *big_list, *big_count;
     // group reads whose bucket tables exceed one wave's slice, for k_extract
     uint32_t *ext_list, *ext_count;
-    // group reads whose buckets go through their global tables in chunks
-    // (k_contrib_chunks), listed by k_contrib_wave; ov_kpre holds their chunk
-    // prefix and then the chunk length
+    // group reads past one wave's bucket count (listed by k_contrib_wave for
+    // k_contrib_wg), and those whose buckets go through their global tables in
+    // chunks (k_contrib_chunks, listed by k_contrib_wg); ov_kpre holds the
+    // latter's chunk prefix and then the chunk length
     uint32_t *ov_list, *ov_count;
+    uint32_t *gl_list, *gl_count;
     uint64_t *ov_kpre;
     uint32_t contrib_wave_e;  // examined hits one k_contrib_wave wave takes whole
     uint32_t contrib_chunk;   // chunk length of the listed reads (0: chosen by k_contrib_over_scan)
+    uint32_t contrib_wg_e;    // examined hits one k_contrib_wg workgroup takes whole
     uint32_t prune_seq;       // seeds one k_prune thread extends a bucket with
     // k_final's batch totals: the mapped reads (for the host's finalization)
     // and the sum of distinct k-mers

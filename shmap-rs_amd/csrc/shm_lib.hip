@@ -1064,6 +1064,7 @@ struct shm_mapper {
     DevBuf<uint32_t> big;  // [0] count, then the reads seeded by a workgroup
     DevBuf<uint32_t> ext;  // [0] count, then the group reads extracted by a workgroup
     DevBuf<uint32_t> ov;       // the group reads bucketed through their global tables
+    DevBuf<uint32_t> gl;       // the group reads counted through global tables
     DevBuf<uint64_t> ov_kpre;  // their chunk prefix, then the chunk length
     DevBuf<uint32_t> mapped;   // [0] count, then the mapped reads
     DevBuf<uint64_t> u_total;  // sum of distinct k-mers
@@ -1093,7 +1094,7 @@ struct shm_mapper {
         tb_min.release(), tb_max.release(), bk_key.release(), bk_val.release(), bs_g.release();
         bk_i32.release(), bs_alive.release(), bs_sh.release(), sc_j.release(), g_sid.release();
         g_tr.release(), g_diff.release(), g_tails.release(), tasks.release(), sel.release(), big.release();
-        ext.release(), ov.release(), ov_kpre.release(), mapped.release(), h_mapped.release();
+        ext.release(), ov.release(), gl.release(), ov_kpre.release(), mapped.release(), h_mapped.release();
         task_count.release(), d_out.release(), abort_flag.release();
         drop_graphs();
         shape.U = shape.M = shape.T = shape.B = shape.max_m = shape.max_len = 0;  // the capacities went with them
@@ -1178,7 +1179,7 @@ extern "C" int shm_mapper_set_graphs(shm_mapper *mp, int32_t on) {
 
 extern "C" int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t) {
     if (!mp || !t) return fail(SHM_ERR_ARG, "invalid argument");
-    if (t->contrib_wave_hits < 0 || t->prune_thread_seeds < 0 || t->contrib_chunk_hits < 0 ||
+    if (t->contrib_wave_hits < 0 || t->prune_thread_seeds < 0 || t->contrib_chunk_hits < 0 || t->contrib_wg_hits < 0 ||
         (t->contrib_chunk_hits &&
          (t->contrib_chunk_hits < 64 || t->contrib_chunk_hits > 4096 || (t->contrib_chunk_hits & (t->contrib_chunk_hits - 1)))))
         return fail(SHM_ERR_ARG, "tuning out of range (contrib_chunk_hits: 0 or a power of two from 64 to 4096; "
@@ -1218,12 +1219,16 @@ extern "C" int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_
     static const int32_t cand_prune[] = {4, 8, 16, 32, 64, 128};
     static const int32_t cand_wave[] = {512, 1024, 2048, 4096, 8192};
     static const int32_t cand_chunk[] = {0, 512, 1024, 2048, 4096};
-    for (int field = 0; field < 3; field++) {
-        const int32_t *cand = field == 0 ? cand_prune : field == 1 ? cand_wave : cand_chunk;
-        const int nc = field == 0 ? 6 : 5;
+    static const int32_t cand_wg[] = {1, 8192, 32768, 1 << 20};  // 1: no read takes a workgroup
+    for (int field = 0; field < 4; field++) {
+        const int32_t *cand = field == 0 ? cand_prune : field == 1 ? cand_wave : field == 2 ? cand_chunk : cand_wg;
+        const int nc = field == 0 ? 6 : field == 3 ? 4 : 5;
         for (int i = 0; i < nc; i++) {
             shm_tuning t = best;
-            int32_t &f = field == 0 ? t.prune_thread_seeds : field == 1 ? t.contrib_wave_hits : t.contrib_chunk_hits;
+            int32_t &f = field == 0   ? t.prune_thread_seeds
+                         : field == 1 ? t.contrib_wave_hits
+                         : field == 2 ? t.contrib_chunk_hits
+                                      : t.contrib_wg_hits;
             if (f == cand[i]) continue;
             f = cand[i];
             float ms;
@@ -1353,6 +1358,7 @@ static int ensure_read_buffers(shm_mapper *mp, int32_t R, uint64_t M) {
     RCHK(mp->h_mapped.ensure((size_t)R + 1));
     RCHK(mp->h_sum.ensure(6));
     RCHK(mp->ov.ensure((size_t)R + 1));
+    RCHK(mp->gl.ensure((size_t)R + 1));
     RCHK(mp->ov_kpre.ensure((size_t)R + 2));
     RCHK(mp->ext.ensure((size_t)R + 2));
     return SHM_OK;
@@ -1462,14 +1468,17 @@ static void table_args(shm_mapper *mp, MapArgs &A, int32_t r0, int32_t RG, uint3
     // the group's task counters ([6] counts the reads k_contrib_wave hands
     // to the global tables)
     A.task_count = task_count;
-    A.ov_count = task_count + 6;
+    A.ov_count = task_count + 7;
     A.ov_list = mp->ov.p;
+    A.gl_count = task_count + 6;
+    A.gl_list = mp->gl.p;
     A.ov_kpre = mp->ov_kpre.p;
     // a big batch fills the device with whole reads; a small one splits
     // long reads so no read's serial walk sets the batch's time
     A.contrib_wave_e = mp->tune.contrib_wave_hits > 0 ? (uint32_t)mp->tune.contrib_wave_hits
                                                       : (E_grp >= (1ull << 24) ? 4096u : 1024u);
     A.contrib_chunk = (uint32_t)mp->tune.contrib_chunk_hits;
+    A.contrib_wg_e = mp->tune.contrib_wg_hits > 0 ? (uint32_t)mp->tune.contrib_wg_hits : 8192u;
     A.prune_seq = mp->tune.prune_thread_seeds > 0 ? (uint32_t)mp->tune.prune_thread_seeds : 16u;
 }
 

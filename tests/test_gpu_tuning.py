@@ -1,6 +1,7 @@
 """A mapper's batching choices (shm_tuning) change where per-read work runs --
-whole reads or chunks for the bucket counts, tiny or long chunks, threads or
-waves for the prune -- and never the records.  Extreme settings force every
+whole reads by a wave or a workgroup, or chunks through global tables, for the
+bucket counts, tiny or long chunks, threads or waves for the prune -- and
+never the records.  Extreme settings force every
 read down one path; each run must equal the built-in choice and the oracle
 bit for bit.  The autotuner keeps a valid tuning with the same records."""
 import numpy as np
@@ -30,9 +31,13 @@ def setup():
 
 
 TUNINGS = [
-    dict(contrib_wave_hits=1, contrib_chunk_hits=64, prune_thread_seeds=1),      # chunks of one window, waves
-    dict(contrib_wave_hits=100_000, contrib_chunk_hits=4096, prune_thread_seeds=100_000),  # whole reads, threads
-    dict(contrib_wave_hits=200, contrib_chunk_hits=128, prune_thread_seeds=3),
+    # chunks of one window through global tables, waves
+    dict(contrib_wave_hits=1, contrib_chunk_hits=64, prune_thread_seeds=1, contrib_wg_hits=1),
+    # whole reads by one wave, threads
+    dict(contrib_wave_hits=100_000, contrib_chunk_hits=4096, prune_thread_seeds=100_000, contrib_wg_hits=0),
+    # every read with hits by a workgroup
+    dict(contrib_wave_hits=1, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=1_000_000),
+    dict(contrib_wave_hits=200, contrib_chunk_hits=128, prune_thread_seeds=3, contrib_wg_hits=300),
 ]
 
 
@@ -47,10 +52,13 @@ def test_tunings_give_the_same_records(setup):
         got = mp.map_batch(reads)
         bad = P.diff_results(want, got)
         assert not bad, (t, bad)
-        if t["contrib_wave_hits"] == 1:
+        if t["contrib_wg_hits"] == 1:
             # every read with hits took the chunked path
             assert mp.last_counts()["global_table_reads"] >= int((got["seed_matches"] > 1).sum())
-    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0)
+        if t["contrib_wg_hits"] == 1_000_000:
+            # every read took a workgroup's table (none has 896 buckets)
+            assert mp.last_counts()["global_table_reads"] == 0
+    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=0)
     assert counts["examined_hits"] > 0
     _, _, ores = P.oracle_map(segs, reads, K, H_FRAC, **PRM)
     bad = P.diff_results(ores, want)
@@ -73,6 +81,7 @@ def test_autotune_keeps_a_valid_tuning(setup):
     assert t["prune_thread_seeds"] in (0, 4, 8, 16, 32, 64, 128)
     assert t["contrib_wave_hits"] in (0, 512, 1024, 2048, 4096, 8192)
     assert t["contrib_chunk_hits"] in (0, 512, 1024, 2048, 4096)
+    assert t["contrib_wg_hits"] in (0, 1, 8192, 32768, 1 << 20)
     got = mp.map_batch(reads)
     bad = P.diff_results(want, got)
     assert not bad, bad
