@@ -15,9 +15,11 @@ single-mapper rate is reported beside it (one_mapper).
 
 Also reported, outside the timed region: the plain one-read-per-launch form on
 a sample (with and without its captured graph), per-stage kernel times with the
-dominant kernel's roofline, the serving loop at a sweep of offered rates, and
-the CPU baseline (the C oracle on the host cores, -@1 and all cores) on a
-bounded sample whose records are checked bit-for-bit against the GPU's.
+dominant kernel's roofline, the other configurations (plain_small with its
+one-read-per-launch form, batched_uneven, dense: `other_configs`), the serving
+loop at a sweep of offered rates, and the CPU baseline (the C oracle on the
+host cores, -@1 and all cores) on a bounded sample whose records are checked
+bit-for-bit against the GPU's.
 
 Prints one JSON line.
 """
@@ -333,6 +335,79 @@ def run_cli(args, segs, bases, off, res, prm, idx, h_frac) -> dict:
         shutil.rmtree(td, ignore_errors=True)
 
 
+def run_other_workload(name, args, hip) -> dict:
+    """Another configuration of the same path, timed the same way as the
+    headline (index resident, reads in HBM, tuning from a sample drawn with
+    another seed, three mappers taking the steps in turn; one mapper beside)."""
+    t0 = time.time()
+    lens, ref, bases, off, truth, w = make_workload(name)
+    n = len(off) - 1
+    k, h_frac = W.PARAMS["k"], w["r"]
+    idx = G.SketchIndex(G.FracMinHash(k, h_frac), W.segments(lens, ref), None, device=0)
+    hip.sync()
+    t_index = time.time() - t0
+    prm = G.make_params(k=k, h_frac=h_frac, theta=W.PARAMS["theta"], min_diff=W.PARAMS["min_diff"],
+                        max_overlap=W.PARAMS["max_overlap"], metric=args.metric)
+    # the sample: half the batch's size (a few heavy reads would set a small
+    # sample's time and steer the tuning away from what a full batch needs)
+    tb, to, _ = W.reads(name, lens, ref, seed=7, n_reads=max(min(n, 20_000), n // 2))
+    mps = [G.Mapper(idx, prm) for _ in range(3)]
+    tuning = mps[0].autotune((tb, to))
+    del tb, to
+    for m_ in mps[1:]:
+        m_.set_tuning(**tuning)
+    d_b, d_o = hip.upload(bases), hip.upload(off)
+    outs = [G.host_array(n, G.RESULT_DTYPE) for _ in mps]
+    for m_, o_ in zip(mps, outs):
+        m_.map_device(d_b, d_o, n, out=o_)
+    hip.sync()
+    t1 = time.time()
+    for _ in range(2):
+        mps[0].map_device(d_b, d_o, n, out=outs[0])
+    one_ms = (time.time() - t1) / 2 * 1e3
+    steps = 6
+    import threading
+
+    def worker(wi):
+        for _ in range(wi, steps, len(mps)):
+            mps[wi].map_device(d_b, d_o, n, out=outs[wi])
+            log(f"[bench] {name} step (mapper {wi}) done")
+
+    t2 = time.time()
+    ths = [threading.Thread(target=worker, args=(wi,)) for wi in range(len(mps))]
+    for t_ in ths:
+        t_.start()
+    for t_ in ths:
+        t_.join()
+    hip.sync()
+    el = time.time() - t2
+    same = all(np.array_equal(o_, outs[0]) for o_ in outs[1:])
+    mps[0].set_timing(True)
+    mps[0].map_device(d_b, d_o, n, out=outs[0])
+    _, stage = mps[0].last_timing()
+    res = outs[0]
+    mk = res["mapped"] == 1
+    hit = mk & (res["segm_id"] == truth[:n, 0]) & (res["t_l"] < truth[:n, 2]) & (res["t_r"] > truth[:n, 1])
+    plain = None
+    if name == "plain_small":  # config 1's own form: one read per launch (a captured one-read batch)
+        m1 = mps[1]
+        m1.map_batch([bases[off[0]:off[1]].tobytes()])
+        tp = time.time()
+        recs = [m1.map_batch([bases[off[i]:off[i + 1]].tobytes()]) for i in range(n)]
+        dt = time.time() - tp
+        plain = {"reads_per_s": round(n / dt, 1), "same_records": bool(np.array_equal(np.concatenate(recs), res))}
+    hip.free(d_b)
+    hip.free(d_o)
+    out = {"reads": n, "read_gbp": round(float(off[-1]) / 1e9, 3), "r": h_frac, "plain_form": plain,
+           "reads_per_s": round(n * steps / el, 1), "ms_per_step": round(el / steps * 1e3, 3),
+           "one_mapper": {"reads_per_s": round(n / one_ms * 1e3, 1), "ms_per_step": round(one_ms, 3)},
+           "gbp_per_s": round(float(off[-1]) * steps / el / 1e9, 2), "mapped_reads": int(mk.sum()),
+           "overlapping_true_origin": int(hit.sum()), "mappers_agree": bool(same), "tuning": tuning,
+           "stage_ms": {s_: round(v, 3) for s_, v in stage.items()}, "index_build_s": round(t_index, 2)}
+    log(f"[bench] {name}: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,6 +424,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
     ap.add_argument("--no-cli", action="store_true", help="skip the command-line end-to-end run")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the other configurations (plain_small, batched_uneven, dense)")
     ap.add_argument("--no-metrics", action="store_true", help="skip the other scoring metrics' timings")
     ap.add_argument("--all-metrics", action="store_true",
                     help="time the other scoring metrics on any workload (default: genome only)")
@@ -650,6 +727,13 @@ def main():
             cli = {"error": str(e)[:300]}
         log(f"[bench] CLI: {cli}")
 
+    # the other configurations of the same path (after the genome's own legs)
+    others = None
+    if not args.no_others and args.workload == "genome":
+        others = {}
+        for name in ("plain_small", "batched_uneven", "dense"):
+            others[name] = run_other_workload(name, args, hip)
+
     serving = None
     if not args.no_serve:
         mp = None  # the serving workers get the device's hardware queues to themselves
@@ -738,6 +822,7 @@ def main():
         "per_metric": per_metric,
         "placement": placement,
         "serving": serving,
+        "other_configs": others,
         "cli": cli,
         "path_roofline": path_roof,
         "index_build_s": round(t_index, 3),
