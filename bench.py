@@ -440,6 +440,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM traffic")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--probe-tuning", default="", help=argparse.SUPPRESS)  # field=value,... for --pmc-probe
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
@@ -467,6 +468,8 @@ def main():
         assert probe.shm_probe_copy(PROBE_COPY_BYTES) == 0
         prm = G.make_params(k=k, h_frac=h_frac, theta=theta, min_diff=d, max_overlap=o, metric=args.metric)
         mp = G.Mapper(idx, prm)
+        if args.probe_tuning:
+            mp.set_tuning(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.probe_tuning.split(",")})
         d_bases, d_off = hip.upload(bases), hip.upload(off)
         mp.map_device(d_bases, d_off, n_reads)
         hip.sync()
