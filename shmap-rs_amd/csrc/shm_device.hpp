@@ -459,6 +459,90 @@ __device__ void wave_bitonic_keys(K *k, uint32_t n) {
     }
 }
 
+// Cross-wave step of a workgroup's register sort (x = threadIdx.x * E + e):
+// every element meets x ^ mask through LDS (element-major, so a step's reads
+// and writes are conflict-free); HB is mask's highest bit, and the lower
+// index keeps the smaller element.  mask >= WAVE * E.
+template <int E, bool PAIRS>
+__device__ __forceinline__ void wg_cross_step(uint64_t (&k)[E], uint32_t (&v)[E], uint64_t *sk, uint32_t *sv,
+                                              uint32_t mask, uint32_t hb) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        sk[(uint32_t)e * WG + t] = k[e];
+        if (PAIRS) sv[(uint32_t)e * WG + t] = v[e];
+    }
+    __syncthreads();
+    const uint32_t pt = t ^ (mask / (uint32_t)E);
+    const bool lower = (t & (hb / (uint32_t)E)) == 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t pe = (uint32_t)e ^ (mask & (uint32_t)(E - 1));
+        const uint64_t pk = sk[pe * WG + pt];
+        const uint32_t pv = PAIRS ? sv[pe * WG + pt] : 0u;
+        const bool take = sort_less<PAIRS>(pk, pv, k[e], v[e]) == lower;
+        k[e] = take ? pk : k[e];
+        if (PAIRS) v[e] = take ? pv : v[e];
+    }
+    __syncthreads();
+}
+
+// The strides below one wave's block (WAVE * E) of a bitonic merge, within the
+// wave: lane steps for strides of E and more, then register steps.
+template <int E, bool PAIRS>
+__device__ __forceinline__ void wave_merge_tail(uint64_t (&k)[E], uint32_t (&v)[E]) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+#pragma unroll 1
+    for (uint32_t ls = WAVE >> 1; ls > 0; ls >>= 1)
+        sort_lane_step<E, PAIRS, false>(k, v, (int)((lane ^ ls) << 2), (lane & ls) == 0);
+    sort_reg_strides<E, PAIRS, E / 2>(k, v);
+}
+
+// A workgroup sorts n <= WG * E elements held in registers between load(x,
+// key, val) and store(x, key, val) over x < n (thread-major: thread t holds x
+// = t*E .. t*E+E-1): each wave sorts its block in registers, then bitonic
+// merges of 2 and 4 waves' blocks take their cross-wave steps through LDS
+// (sk, sv: WG * E entries) and the rest within each wave.  Ascending by (key,
+// val); ends with a barrier.
+template <int E, bool PAIRS, typename LoadF, typename StoreF>
+__device__ __forceinline__ void wg_sort_e(uint32_t n, LoadF load, StoreF store, uint64_t *sk, uint32_t *sv) {
+    uint64_t k[E];
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = threadIdx.x * E + (uint32_t)e;
+        k[e] = ~0ull;
+        v[e] = ~0u;
+        if (x < n) load(x, k[e], v[e]);
+    }
+    wave_sort_regs<E, PAIRS>(k, v);
+    constexpr uint32_t WB = WAVE * E;  // one wave's block
+#pragma unroll 1
+    for (uint32_t size = 2 * WB; size <= (uint32_t)WG * E; size <<= 1) {
+        wg_cross_step<E, PAIRS>(k, v, sk, sv, size - 1, size >> 1);
+#pragma unroll 1
+        for (uint32_t st = size >> 2; st >= WB; st >>= 1) wg_cross_step<E, PAIRS>(k, v, sk, sv, st, st);
+        wave_merge_tail<E, PAIRS>(k, v);
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = threadIdx.x * E + (uint32_t)e;
+        if (x < n) store(x, k[e], v[e]);
+    }
+    __syncthreads();
+}
+
+// n <= WG * 16: a workgroup's register sort; returns false (nothing done) for
+// larger n.
+template <bool PAIRS, typename LoadF, typename StoreF>
+__device__ bool wg_sort_n(uint32_t n, LoadF load, StoreF store, uint64_t *sk, uint32_t *sv) {
+    if (n <= (uint32_t)WG * 4) wg_sort_e<4, PAIRS>(n, load, store, sk, sv);
+    else if (n <= (uint32_t)WG * 8) wg_sort_e<8, PAIRS>(n, load, store, sk, sv);
+    else if (n <= (uint32_t)WG * 16) wg_sort_e<16, PAIRS>(n, load, store, sk, sv);
+    else return false;
+    return true;
+}
+
 // Sorts global (gk, gv)[0, n) using LDS scratch (sk, sv) of `cap` entries when
 // it fits.  Ends with a barrier; results are visible to the whole workgroup.
 template <typename K, typename V>

@@ -945,10 +945,6 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
         const uint64_t o = A.koff[rd];
         const uint32_t m = (uint32_t)(A.koff[rd + 1] - o);
         // 1. (hash, position) order
-        for (uint32_t i = threadIdx.x; i < m; i += WG) {
-            A.ps_h[o + i] = A.k_h[o + i];
-            A.ps_v[o + i] = i;
-        }
         if (threadIdx.x == 0) {
             s_u = 0;
             s_pm = 0;
@@ -959,7 +955,16 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
             s_cnt_carry = 0;
         }
         __syncthreads();
-        wg_sort_pairs(A.ps_h + o, A.ps_v + o, m, sk, sv, SORT_CAP);
+        if (!wg_sort_n<true>(
+                m, [&](uint32_t x, uint64_t &key, uint32_t &val) { key = A.k_h[o + x], val = x; },
+                [&](uint32_t x, uint64_t key, uint32_t val) { A.ps_h[o + x] = key, A.ps_v[o + x] = val; }, sk, sv)) {
+            for (uint32_t i = threadIdx.x; i < m; i += WG) {
+                A.ps_h[o + i] = A.k_h[o + i];
+                A.ps_v[o + i] = i;
+            }
+            __syncthreads();
+            wg_sort_pairs(A.ps_h + o, A.ps_v + o, m, sk, sv, SORT_CAP);
+        }
         // 2. runs -> seeds in hash order, one probe each
         for (uint32_t base = 0; base < m; base += WG) {
             const uint32_t i = base + threadIdx.x;
@@ -1030,7 +1035,12 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
             }
         }
         __syncthreads();
-        if (u - zs > 1) wg_sort_pairs(A.sd_key + o + zs, A.sd_ord + o + zs, u - zs, sk, sv, SORT_CAP);
+        if (u - zs > 1 &&
+            !wg_sort_n<true>(
+                u - zs, [&](uint32_t x, uint64_t &key, uint32_t &val) { key = A.sd_key[o + zs + x], val = A.sd_ord[o + zs + x]; },
+                [&](uint32_t x, uint64_t key, uint32_t val) { A.sd_key[o + zs + x] = key, A.sd_ord[o + zs + x] = val; },
+                sk, sv))
+            wg_sort_pairs(A.sd_key + o + zs, A.sd_ord + o + zs, u - zs, sk, sv, SORT_CAP);
         // 4. budget: consume seeds while fewer than S occurrences are consumed
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
@@ -1729,17 +1739,24 @@ __global__ void __launch_bounds__(WG) k_extract(MapArgs A) {
         const int32_t lr = (int32_t)A.ext_list[t];
         const uint64_t tb = A.tab_base[lr], base = A.bk_base[lr];
         const uint32_t nb = (uint32_t)A.grp_nb[lr];
-        for (uint32_t i = threadIdx.x; i < nb; i += WG) {
-            const uint32_t slot = A.tb_list[tb + i];
+        // key ((~matches) << 32 | bucket id), value the slot; sorted in
+        // registers when the read's buckets fit, else through global memory
+        auto key_of = [&](uint32_t i, uint64_t &key, uint32_t &slot) {
+            slot = A.tb_list[tb + i];
             int32_t codir;
             const uint32_t matches = mc_matches(A.tb_mc[tb + slot], codir);
-            A.bk_key[base + i] = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
-            A.bk_val[base + i] = slot;
+            key = ((uint64_t)(0xffffffffu - matches) << 32) | A.tb_key[tb + slot];
+        };
+        if (!wg_sort_n<true>(
+                nb, key_of, [&](uint32_t p, uint64_t key, uint32_t slot) { emit_bucket(A, base, p, key, tb, slot); },
+                sk, sv)) {
+            for (uint32_t i = threadIdx.x; i < nb; i += WG) key_of(i, A.bk_key[base + i], A.bk_val[base + i]);
+            __syncthreads();
+            wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
+            for (uint32_t p = threadIdx.x; p < nb; p += WG)
+                emit_bucket(A, base, p, A.bk_key[base + p], tb, A.bk_val[base + p]);
+            __syncthreads();
         }
-        __syncthreads();
-        wg_sort_pairs(A.bk_key + base, A.bk_val + base, nb, sk, sv, SORT_CAP);
-        for (uint32_t p = threadIdx.x; p < nb; p += WG) emit_bucket(A, base, p, A.bk_key[base + p], tb, A.bk_val[base + p]);
-        __syncthreads();
     }
 }
 
