@@ -860,6 +860,10 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
         }
         wave_sync();
         // budget: consume seeds while fewer than S occurrences are consumed
+        // (the rows read back the hit offsets and strands this wave's lanes
+        // stored above)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int32_t len = (int32_t)(A.roff[rd + 1] - A.roff[rd]);
         const int32_t S = A.one_sweep ? (int32_t)u : (int32_t)((1.0 - theta2) * (double)m) + 1;
         const int32_t hl = A.abs_pos ? len : (int32_t)m;
@@ -873,13 +877,21 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
                 sid = (uint32_t)key;
                 occ = so[sid];
                 cnt = (uint32_t)(key >> 32);
-                A.sd_ord[o + q] = sid;
             }
             const uint32_t oinc = wave_incl_scan(occ);
             const bool consumed = q < u && (int64_t)(occ_carry + oinc - occ) < (int64_t)S;
             const uint32_t cv = consumed ? cnt : 0u;
             const uint32_t cinc = wave_incl_scan(cv);
-            if (consumed) A.sd_epre[o + q] = cnt_carry + cinc - cv;
+            const uint32_t epre = cnt_carry + cinc - cv;
+            if (q < u) {
+                // the seed's row (a seed without hits needs no offset or strand)
+                uint32_t hoff = 0, str = 0;
+                if (cnt) {
+                    hoff = A.sd_hoff[o + sid];
+                    str = A.sd_strand[o + sid];
+                }
+                A.sd_row[o + q] = make_uint4(hoff, cnt, occ | (str << 31), consumed ? epre : 0u);
+            }
             const uint64_t cb = __ballot(consumed);
             c += __popcll(cb);
             z += __popcll(__ballot(consumed && cnt == 0));  // hitless seeds sort first
@@ -894,8 +906,6 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             occ_carry += __shfl(oinc, WAVE - 1);
             cnt_carry += __shfl(cinc, WAVE - 1);
         }
-        if (!ok)
-            for (uint32_t q = lane; q < u; q += WAVE) A.sd_ord[o + q] = (uint32_t)kh[q];
         if (lane == 0) {
             const uint32_t E = cnt_carry;
             A.rd_u[rd] = (int32_t)u;
@@ -1049,9 +1059,9 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
         if (ok) {
             for (int32_t base = 0; base < (int32_t)u; base += WG) {
                 const int32_t q = base + threadIdx.x;
-                uint32_t occs = 0, cnt = 0;
+                uint32_t occs = 0, cnt = 0, sid = 0;
                 if (q < (int32_t)u) {
-                    const uint32_t sid = A.sd_ord[o + q];
+                    sid = A.sd_ord[o + q];
                     occs = A.sd_occs[o + sid];
                     cnt = A.sd_cnt[o + sid];
                 }
@@ -1061,7 +1071,14 @@ __global__ void __launch_bounds__(WG) k_seed_wg(MapArgs A, DevIndex I) {
                 const bool consumed = q < (int32_t)u && (int64_t)pre < (int64_t)S;
                 uint32_t ctot;
                 const uint32_t cex = wg_excl_scan(consumed ? cnt : 0u, tmp, &ctot);
-                if (consumed) A.sd_epre[o + q] = s_cnt_carry + cex;
+                if (q < (int32_t)u) {
+                    uint32_t hoff = 0, str = 0;
+                    if (cnt) {
+                        hoff = A.sd_hoff[o + sid];
+                        str = A.sd_strand[o + sid];
+                    }
+                    A.sd_row[o + q] = make_uint4(hoff, cnt, occs | (str << 31), consumed ? s_cnt_carry + cex : 0u);
+                }
                 // the consumed seeds' count, occurrences and largest hit count:
                 // one LDS atomic per wave
                 const uint64_t cb = __ballot(consumed);
@@ -1264,7 +1281,7 @@ __device__ __forceinline__ void ltab_flush(const MapArgs &A, const LTab &T, uint
 // match_seeds (src/shmap/seeding.rs:98-212) over positions [pa, pb) of read
 // lr's examined hits, by one wave into its LDS table.  The examined hits are
 // each consumed seed's hit run in seed order, flattened: seed q covers
-// positions [sd_epre[q], sd_epre[q] + cnt).  They are taken 64 at a time, lane
+// positions [w, w + y) of its row sd_row[q].  They are taken 64 at a time, lane
 // j at position p0 + j.  From the first seed with hits on (rd_z) the seeds'
 // first positions strictly increase, so a window's seeds are the current one
 // and those whose first position falls inside it: each lane marks its seed's
@@ -1298,7 +1315,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         uint32_t lo = z, hi = c;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (A.sd_epre[o + mid] <= pa) lo = mid + 1; else hi = mid;
+            if (A.sd_row[o + mid].w <= pa) lo = mid + 1; else hi = mid;
         }
         qc = lo - 1;
     }
@@ -1324,8 +1341,8 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         ++stamp;
         const uint32_t qi = qc + lane;
         const bool qv = qi < c;
-        const uint32_t st = qv ? A.sd_epre[o + qi] : 0xffffffffu;
-        const uint32_t sid = qv ? A.sd_ord[o + qi] : 0u;
+        const uint4 row = qv ? A.sd_row[o + qi] : make_uint4(0, 0, 0, 0xffffffffu);
+        const uint32_t st = row.w;
         if (lane > 0 && st > p0 && st < p0 + WAVE) T.mark[st - p0] = stamp;
         wave_sync();
         const uint64_t M = __ballot(T.mark[lane] == stamp);
@@ -1333,10 +1350,9 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         // lanes 0..ns hold the window's seeds
         uint32_t sa = 0, send = 0, socc = 0;
         if (lane <= ns && qv) {
-            const uint32_t hoff = A.sd_hoff[o + sid];
-            sa = hoff - st;  // hit index = sa + position
-            send = hoff + A.sd_cnt[o + sid];
-            socc = A.sd_occs[o + sid] | ((uint32_t)A.sd_strand[o + sid] << 31);
+            sa = row.x - st;  // hit index = sa + position
+            send = row.x + row.y;
+            socc = row.z;
         }
         // every hit of the window is read (run heads past pb end runs before
         // it); only those before pb emit
@@ -1469,7 +1485,7 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         }
         // the seed holding p0 + 64: every seed starting at or before it
         uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= p0 + WAVE));
-        if (adv == WAVE - 1 && qc + WAVE < c && A.sd_epre[o + qc + WAVE] <= p0 + WAVE) adv++;
+        if (adv == WAVE - 1 && qc + WAVE < c && A.sd_row[o + qc + WAVE].w <= p0 + WAVE) adv++;
         qc += adv;
         wave_sync();
     }
@@ -1847,18 +1863,16 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                         break;
                     }
                     if (q >= qend) break;
-                    const uint32_t sid = A.sd_ord[o + q];
-                    const uint32_t occs = A.sd_occs[o + sid];
-                    const uint32_t cnt = A.sd_cnt[o + sid];
+                    const uint4 row = A.sd_row[o + q];
+                    const uint32_t occs = row.z & 0x7fffffffu, cnt = row.y, hoff = row.x;
                     seeds += (int32_t)occs;
                     if (cnt == 0) continue;
-                    const uint32_t hoff = A.sd_hoff[o + sid];
                     const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
                     const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
                     const uint32_t n = e - lo;
                     if (n == 0) continue;
                     const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
-                    const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
+                    const uint32_t same = (row.z >> 31) ? ones : n - ones;
                     codir += (int32_t)(2 * same) - (int32_t)n;
                     const int32_t x0 = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
                     const int32_t x1 = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);
@@ -1910,18 +1924,17 @@ __global__ void __launch_bounds__(WG) k_prune_ext(MapArgs A, DevIndex I) {
             const int32_t q = q0 + (int32_t)lane;
             int32_t d_seeds = 0, d_matches = 0, d_codir = 0, d_min = INT32_MAX, d_max = -1;
             if (q < u) {
-                const uint32_t sid = A.sd_ord[o + q];
-                const uint32_t occs = A.sd_occs[o + sid];
-                const uint32_t cnt = A.sd_cnt[o + sid];
+                const uint4 row = A.sd_row[o + q];
+                const uint32_t occs = row.z & 0x7fffffffu, cnt = row.y;
                 d_seeds = (int32_t)occs;
                 if (cnt) {
-                    const uint32_t hoff = A.sd_hoff[o + sid];
+                    const uint32_t hoff = row.x;
                     const uint32_t lo = hits_lower_bound(I, hoff, hoff + cnt, seg, begin, ap);
                     const uint32_t e = hits_lower_bound(I, lo, hoff + cnt, seg, end, ap);
                     const uint32_t n = e - lo;
                     if (n) {
                         const uint32_t ones = I.strand_pre[e] - I.strand_pre[lo];
-                        const uint32_t same = A.sd_strand[o + sid] ? ones : n - ones;
+                        const uint32_t same = (row.z >> 31) ? ones : n - ones;
                         d_codir = (int32_t)(2 * same) - (int32_t)n;
                         d_min = (int32_t)(I.hit_r[lo] & 0x7fffffffu);
                         d_max = (int32_t)(I.hit_r[e - 1] & 0x7fffffffu);

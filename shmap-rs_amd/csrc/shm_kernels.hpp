@@ -84,9 +84,12 @@ struct MapArgs {
     uint32_t *ps_v;
     // seeds, hash order, at koff[rd]
     uint64_t *sd_h;
-    uint32_t *sd_first, *sd_occs, *sd_cnt, *sd_hoff, *sd_ord, *sd_epre;
+    uint32_t *sd_first, *sd_occs, *sd_cnt, *sd_hoff, *sd_ord;
     uint8_t *sd_strand;
     uint64_t *sd_key;
+    // seeds in rarest-first order, at koff[rd]: {hit offset, hit count,
+    // occurrences | strand << 31, first examined position (consumed seeds)}
+    uint4 *sd_row;
     // per read (whole batch)
     int32_t *rd_u, *rd_possible, *rd_S, *rd_hl, *rd_ok, *rd_c, *rd_bseeds, *rd_max_seed,
         *rd_seed_matches;

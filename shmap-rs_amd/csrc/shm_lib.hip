@@ -1045,7 +1045,8 @@ struct shm_mapper {
     DevBuf<uint64_t> k_h, ps_h, sd_h, sd_key;
     DevBuf<int32_t> k_r;
     DevBuf<uint8_t> k_s, sd_strand;
-    DevBuf<uint32_t> ps_v, sd_first, sd_occs, sd_cnt, sd_hoff, sd_ord, sd_epre;
+    DevBuf<uint32_t> ps_v, sd_first, sd_occs, sd_cnt, sd_hoff, sd_ord;
+    DevBuf<uint4> sd_row;  // seeds in rarest-first order: offset, count, occs | strand, first position
     DevBuf<int32_t> rd_i32;  // 12 per-read int arrays
     DevBuf<uint64_t> rd_u64; // 2 per-read u64 arrays
     DevBuf<uint64_t> grp_base; // 3 x (RG+1): hit, table and bucket offsets of a group's reads
@@ -1089,7 +1090,7 @@ struct shm_mapper {
         so.ovf.release(), so.stage.release();
         k_h.release(), ps_h.release(), sd_h.release(), sd_key.release(), k_r.release(), k_s.release();
         sd_strand.release(), ps_v.release(), sd_first.release(), sd_occs.release(), sd_cnt.release();
-        sd_hoff.release(), sd_ord.release(), sd_epre.release(), rd_i32.release(), rd_u64.release();
+        sd_hoff.release(), sd_ord.release(), sd_row.release(), rd_i32.release(), rd_u64.release();
         grp_base.release(), grp_nb.release(), tb_key.release(), tb_list.release(), tb_mc.release();
         tb_min.release(), tb_max.release(), bk_key.release(), bk_val.release(), bs_g.release();
         bk_i32.release(), bs_alive.release(), bs_sh.release(), sc_j.release(), g_sid.release();
@@ -1345,7 +1346,7 @@ static int ensure_read_buffers(shm_mapper *mp, int32_t R, uint64_t M) {
     RCHK(mp->sd_cnt.ensure(M + 1));
     RCHK(mp->sd_hoff.ensure(M + 1));
     RCHK(mp->sd_ord.ensure(M + 1));
-    RCHK(mp->sd_epre.ensure(M + 1));
+    RCHK(mp->sd_row.ensure(M + 1));
     RCHK(mp->rd_i32.ensure((size_t)R * 12 + 1));
     RCHK(mp->rd_u64.ensure((size_t)R * 2 + 1));
     RCHK(mp->big.ensure((size_t)R + 2));
@@ -1394,9 +1395,9 @@ static MapArgs read_args(shm_mapper *mp, int32_t R, const int64_t *d_off) {
     A.sd_cnt = mp->sd_cnt.p;
     A.sd_hoff = mp->sd_hoff.p;
     A.sd_ord = mp->sd_ord.p;
-    A.sd_epre = mp->sd_epre.p;
     A.sd_strand = mp->sd_strand.p;
     A.sd_key = mp->sd_key.p;
+    A.sd_row = mp->sd_row.p;
     int32_t *ri = mp->rd_i32.p;
     A.rd_u = ri + 0 * (size_t)R;
     A.rd_possible = ri + 1 * (size_t)R;
