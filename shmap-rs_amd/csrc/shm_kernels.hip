@@ -2254,15 +2254,16 @@ __device__ int32_t wave_lcs(const MapArgs &A, const int32_t *sid, int64_t n, uin
 // LDS while they fit, else in the workgroup's global scratch.
 // LCS: the bucket_LCS instance (its patience sweep needs registers the other
 // metrics' instance is better off without).
-constexpr uint32_t REFINE_LDS_BYTES = 60 * 1024;
+constexpr uint32_t REFINE_LDS_BYTES = 60 * 1024;  // the most a workgroup takes
 
-// Takes LDS bytes (8-aligned) while they last; nullptr once they do not fit.
+// Takes LDS bytes (8-aligned) while the launch's `cap` bytes last; nullptr
+// once they do not fit.
 struct LdsCarve {
     uint8_t *base;
-    uint32_t used;
+    uint32_t used, cap;
     __device__ void *take(uint64_t bytes) {
         const uint64_t b = (bytes + 7) & ~7ull;
-        if ((uint64_t)used + b > REFINE_LDS_BYTES) return nullptr;
+        if ((uint64_t)used + b > cap) return nullptr;
         void *p = base + used;
         used += (uint32_t)b;
         return p;
@@ -2338,7 +2339,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         int32_t *const wtr = A.g_tr + (uint64_t)blockIdx.x * A.sid_stride;
         bool parallel = n > 0 && !ap && !LCS && m < 32768;
         if (parallel) {
-            LdsCarve C{lds, 0};
+            LdsCarve C{lds, 0, A.refine_lds};
             int32_t *pk = (int32_t *)C.take((uint64_t)(n + 1) * 4);
             if (!pk) pk = wsid;  // sid_stride >= 2 * hl + 2 > n
             // seeds whose P(x) needs their hit run (more hits than occs), then
@@ -2468,7 +2469,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
             }
         }
         if (parallel) {
-            LdsCarve C{lds, 0};
+            LdsCarve C{lds, 0, A.refine_lds};
             int32_t *pk = (int32_t *)C.take((uint64_t)(n + 1) * 4);
             if (!pk) pk = wsid;
             wg_scan_inplace(pk, n, tmp);
@@ -2527,7 +2528,7 @@ __global__ void __launch_bounds__(WG) k_refine(MapArgs A, DevIndex I) {
         // sequential forms: LCS, abs_pos windows, seeds dropped by -M, m >= 2^15.
         // sid[x]: the seed matching span position x (| 0x40000000 when its
         // strand disagrees), -1 if none.
-        LdsCarve C{lds, 0};
+        LdsCarve C{lds, 0, A.refine_lds};
         // LCS: the tails (at most m, the read's distinct positions) and the
         // append slot; a sketch too large for them takes thread 0's sweep
         int32_t *Ts = LCS ? (int32_t *)C.take((uint64_t)((m > LCS_TAILS ? m : LCS_TAILS) + 2) * 4) : nullptr;
@@ -3048,7 +3049,26 @@ static inline uint32_t read_grid(int32_t n, uint32_t max_grid) {
 }
 
 static const size_t SORT_LDS = (size_t)SORT_CAP * 12;
-static const size_t REFINE_LDS = REFINE_LDS_BYTES;
+
+uint32_t refine_lds_bytes(uint64_t max_hl, uint64_t max_m, bool lcs) {
+    // the carves of k_refine, each rounded up to 8 bytes: span n <= 2 max_hl,
+    // seeds u <= max_m.  Parallel form: pk (n+1) int32, need u uint32, s_h u
+    // uint64, s_kind u bytes; sequential form: sid and tr n int32 each, diff u
+    // int32, s_h; LCS: tails max(m, LCS_TAILS) + 2 int32, sid, s_h.
+    const uint64_t n = 2 * max_hl, u = max_m;
+    uint64_t need;
+    if (lcs) {
+        need = 4 * ((max_m > (uint64_t)LCS_TAILS ? max_m : (uint64_t)LCS_TAILS) + 2) + 4 * n + 8 * u;
+    } else {
+        const uint64_t par = 4 * (n + 1) + 4 * u + 8 * u + u;
+        const uint64_t seq = 8 * n + 4 * u + 8 * u;
+        need = par > seq ? par : seq;
+    }
+    need += 64;                          // alignment of the carves
+    need = (need + 1023) & ~1023ull;     // whole KiB
+    if (need < 4096) need = 4096;
+    return (uint32_t)(need < REFINE_LDS_BYTES ? need : REFINE_LDS_BYTES);
+}
 static const size_t SEED_WAVE_LDS = (size_t)NWAVES * WAVE_SEED_LDS;
 
 // One wrapper per per-read stage kernel so the driver can time each launch.
@@ -3088,9 +3108,9 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
             P.refine_pass = round;
             k_refine_select<<<max_grid, WG, 0, st>>>(P);
             if (A.metric == METRIC_BUCKET_LCS)
-                k_refine<true><<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+                k_refine<true><<<max_grid, WG, A.refine_lds, st>>>(P, I);
             else
-                k_refine<false><<<max_grid, WG, REFINE_LDS, st>>>(P, I);
+                k_refine<false><<<max_grid, WG, A.refine_lds, st>>>(P, I);
         }
         break;
     }

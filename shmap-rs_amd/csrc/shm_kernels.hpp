@@ -127,6 +127,10 @@ struct MapArgs {
     uint32_t *task_count;
     uint32_t task_cap;
     int32_t refine_pass;
+    // dynamic LDS bytes of a refinement workgroup: what the batch's largest
+    // span and sketch need (refine_lds_bytes), so that more workgroups share a
+    // CU when the reads are short; arrays past it go to global scratch
+    uint32_t refine_lds;
     int32_t *rd_flag, *rd_first;
     // reads too large for one wave's seeding slice, for k_seed_wg
     uint32_t *big_list, *big_count;
@@ -258,6 +262,10 @@ void launch_shard_count(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t n
                         unsigned long long *shard_n, unsigned long long *stats, hipStream_t st);
 void launch_insert(const uint64_t *uk_h, const uint64_t *uk_off, uint64_t nk, int32_t max_matches,
                    uint4 *table, ShardGeom geom, hipStream_t st);
+// Dynamic LDS of a refinement workgroup for spans of at most 2 * max_hl
+// positions and sketches of at most max_m k-mers (bounded by the largest a
+// workgroup may take).
+uint32_t refine_lds_bytes(uint64_t max_hl, uint64_t max_m, bool lcs);
 void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_result *d_out, uint64_t flat_n,
                   uint32_t max_grid, hipStream_t st);
 void launch_lookup(const DevIndex &I, const uint64_t *hs, uint32_t n, uint32_t *cnt, uint32_t *off,

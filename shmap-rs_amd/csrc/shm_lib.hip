@@ -1515,6 +1515,7 @@ static void bucket_args(shm_mapper *mp, MapArgs &A, uint64_t B, int32_t max_hl, 
     A.tasks = mp->tasks.p;
     A.sel = mp->sel.p;
     A.task_cap = (uint32_t)(B + 1);
+    A.refine_lds = refine_lds_bytes((uint64_t)max_hl, max_m, mp->prm.metric == METRIC_BUCKET_LCS);
 }
 
 // What every batch ends with once its records and counters are on the host:
