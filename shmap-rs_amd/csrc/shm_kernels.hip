@@ -1225,8 +1225,20 @@ constexpr uint32_t WTAB_MAX = WTAB * 7 / 8;
 
 enum ContribMode { CM_LDS, CM_LDS_FLUSH, CM_WG };
 
+// Rows of 64 positions a wave takes per step of the window walk, and rows
+// whose run-end loads are issued together.
+constexpr int CROWS = 4;
+constexpr int CPREP = 1;
+// Reads past one wave are counted by a workgroup of NWAVES waves, those with
+// more than CWG_SPLIT examined hits by one of CWG waves.
+constexpr int CWG = 16;
+constexpr uint32_t CWG_SPLIT = 16384;
+static_assert(CROWS % CPREP == 0, "rows are prepared in whole groups");
+
 struct LTab {
-    uint32_t *key, *mark, *used;
+    uint32_t *key;
+    uint64_t *mark;  // seed starts of the current block, CROWS words
+    uint32_t *used;
     uint64_t *mc;
     int32_t *mn, *mx;
     uint32_t *over;  // CM_WG: set when the shared table would pass WTAB_MAX
@@ -1243,10 +1255,17 @@ __device__ __forceinline__ void ltab_clear(const LTab &T, uint32_t lane) {
 
 // Adds a contribution to an LDS table of 2^BITS slots; returns 1 when it
 // claimed a new slot (the caller counts claims into *T.used).
-template <uint32_t BITS>
+// BOUNDED (a table shared by a workgroup's waves, filled without reserving
+// room): a probe that finds no slot in the whole table sets *T.over and adds
+// nothing.
+template <uint32_t BITS, bool BOUNDED = false>
 __device__ __forceinline__ uint32_t ltab_add(const LTab &T, uint32_t g, uint64_t mcv, int32_t rmin, int32_t rmax) {
     uint32_t pos = (g * 0x9E3779B1u) >> (32 - BITS), claimed = 0;
-    for (;;) {
+    for (uint32_t tries = 0;; tries++) {
+        if (BOUNDED && tries == (1u << BITS)) {
+            atomicExch(T.over, 1u);
+            return 0;
+        }
         const uint32_t old = atomicCAS(&T.key[pos], EMPTY_KEY, g);
         if (old == EMPTY_KEY) {
             claimed = 1;
@@ -1281,36 +1300,43 @@ __device__ __forceinline__ void ltab_flush(const MapArgs &A, const LTab &T, uint
 // match_seeds (src/shmap/seeding.rs:98-212) over positions [pa, pb) of read
 // lr's examined hits, by one wave into its LDS table.  The examined hits are
 // each consumed seed's hit run in seed order, flattened: seed q covers
-// positions [w, w + y) of its row sd_row[q].  They are taken 64 at a time, lane
-// j at position p0 + j.  From the first seed with hits on (rd_z) the seeds'
-// first positions strictly increase, so a window's seeds are the current one
-// and those whose first position falls inside it: each lane marks its seed's
-// start, and a lane's seed is the current one plus the marks at or below it.
+// positions [w, w + y) of its row sd_row[q].  A step takes a block of up to
+// CROWS rows of 64 positions (lane j of row i at position P0 + 64 i + j) and
+// issues the hit loads of all its rows together, so one chain of dependent
+// loads (seed rows, hits, run ends) serves up to 64 * CROWS positions.  From
+// the first seed with hits on (rd_z) the seeds' first positions strictly
+// increase; a block holds positions of the 64 seeds whose rows are loaded (the
+// whole first row, then the positions before the last loaded seed's start),
+// and a bitmap of the seed starts in the block gives each position its seed.
 // A seed's hits are in (segm_id, pos) order; the first hit of each (segment,
-// half-bucket) run emits the clamped per-(seed, bucket) contribution of bucket
-// b (the hits up to the first at half-bucket b+2 or in another segment or
-// seed) and -- unless the previous run, at b-1, already did -- of bucket b-1
-// (up to the first at half-bucket b+1), with strand agreement from the strand
-// prefix.  An end inside the window is the next run head; an end past it
+// half-bucket) run of a seed -- a run head -- emits the clamped per-(seed,
+// bucket) contribution of bucket b (the hits up to the first at half-bucket
+// b+2 or in another segment or seed) and -- unless the previous run, at b-1,
+// already did -- of bucket b-1 (up to the first at half-bucket b+1).  Run ends
+// come from the block's head masks: the next head ends bucket b-1, and ends
+// bucket b too unless it starts the run at b+1 (its previous hit is at b, which
+// the second mask marks), when the head after it does.  An end past the block
 // leaves the contribution pending (at most three: the last run's two and the
-// run before it), settled by one ballot in the window where its run ends, or
-// by binary search at the end of the range.  CM_LDS returns false, the table
-// partly filled, when the table would pass LTAB_MAX; CM_LDS_FLUSH then
-// empties it into the global table and goes on; CM_WG (a table of WTAB slots
-// shared by a workgroup's waves) reserves each window's worst case in *T.used
-// and returns false, with *T.over set, when that would pass WTAB_MAX or
-// another wave has set it.
+// run before it), settled by ballots in the block where its run ends, or by
+// binary search at the end of the range.  A wave's own table is given room
+// per row, one new slot per contribution the row can add: CM_LDS returns
+// false, the table partly filled, when the table would pass LTAB_MAX;
+// CM_LDS_FLUSH then empties it into the global table and goes on.  CM_WG (a
+// table of WTAB slots shared by a workgroup's waves) adds without reserving:
+// a row that takes the table past WTAB_MAX slots, or a probe that finds it
+// full, sets *T.over, and every wave returns false at its next row.
 template <int MODE>
 __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab &T, int32_t lr, uint32_t pa,
-                                uint32_t pb, uint32_t E, uint32_t lane, uint32_t &stamp) {
+                                uint32_t pb, uint32_t E, uint32_t lane) {
     constexpr uint32_t BITS = MODE == CM_WG ? WTAB_BITS : LTAB_BITS;
+    constexpr int J = CROWS;
     const int ap = A.abs_pos;
-    const uint64_t upto = (((1ull << lane) - 1) << 1) | 1ull;
+    const uint64_t upto = (((1ull << lane) - 1) << 1) | 1ull;  // lanes 0..lane
     const int32_t rd = A.r0 + lr;
     const uint64_t o = A.koff[rd];
     const uint32_t c = (uint32_t)A.rd_c[rd], z = (uint32_t)A.rd_z[rd];
     const int32_t hl = A.rd_hl[rd];
-    uint32_t qc = z;  // the seed holding position p0
+    uint32_t qc = z;  // the seed holding P0
     if (pa > 0) {
         uint32_t lo = z, hi = c;
         while (lo < hi) {
@@ -1320,15 +1346,51 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
         qc = lo - 1;
     }
     // one contribution: hits [idx0, e) of a seed (occs | strand << 31) to bucket bb
-    uint32_t claims = 0;  // this lane's new slots in the window (at most six)
+    uint32_t claims = 0;  // this lane's new slots in the row (at most three)
     auto add = [&](uint32_t e, uint32_t idx0, uint32_t ocv, int32_t rmin0, uint32_t sg, int32_t bb) {
         const uint32_t n = e - idx0, occs = ocv & 0x7fffffffu;
         const uint32_t ones = I.strand_pre[e] - I.strand_pre[idx0];
         const uint32_t same = (ocv >> 31) ? ones : n - ones;
         const uint32_t matches = n < occs ? n : occs;
         const int32_t codir = (int32_t)(2 * same) - (int32_t)n;
-        claims += ltab_add<BITS>(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
+        claims += ltab_add<BITS, MODE == CM_WG>(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
                            rmin0, (int32_t)(I.hit_r[e - 1] & 0x7fffffffu));
+    };
+    // the same from values already loaded: n hits, `ones` of them on the
+    // reverse strand, the last hit's r | strand << 31
+    auto put = [&](uint32_t n, uint32_t ones, uint32_t ocv, int32_t rmin0, uint32_t rlast, uint32_t sg, int32_t bb) {
+        const uint32_t occs = ocv & 0x7fffffffu;
+        const uint32_t same = (ocv >> 31) ? ones : n - ones;
+        const uint32_t matches = n < occs ? n : occs;
+        const int32_t codir = (int32_t)(2 * same) - (int32_t)n;
+        claims += ltab_add<BITS, MODE == CM_WG>(T, (uint32_t)bucket_id(A, sg, bb, hl), ((uint64_t)matches << 32) + (uint64_t)(int64_t)codir,
+                           rmin0, (int32_t)(rlast & 0x7fffffffu));
+    };
+    // room for `need` new slots before a row's adds: false when the caller
+    // must return false (CM_WG: the table has overflowed)
+    auto reserve = [&](uint32_t need) -> bool {
+        if (MODE == CM_WG) return *(volatile uint32_t *)T.over == 0;
+        if (*T.used + need > LTAB_MAX) {
+            if (MODE == CM_LDS) return false;
+            ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[rd], lane);
+        }
+        return true;
+    };
+    // after the adds: the new slots counted (for CM_WG, *T.over set past
+    // WTAB_MAX)
+    auto account = [&](uint32_t need) {
+        const uint32_t nclaim = (uint32_t)__popcll(__ballot(claims & 1u)) + 2u * (uint32_t)__popcll(__ballot(claims & 2u)) +
+                                4u * (uint32_t)__popcll(__ballot(claims & 4u));
+        claims = 0;
+        wave_sync();
+        if (lane == 0) {
+            if (MODE == CM_WG) {
+                if (nclaim && atomicAdd(T.used, nclaim) + nclaim > WTAB_MAX) atomicExch(T.over, 1u);
+            } else {
+                *T.used += nclaim;
+            }
+        }
+        wave_sync();
     };
     // pending contributions: lane i < 3 holds item i (pmask, uniform): its
     // seed, segment and first position/hit, the half-bucket that ends it, the
@@ -1336,157 +1398,257 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
     uint32_t pmask = 0;
     uint32_t pq = 0, psg = 0, ppos = 0, pidx = 0, pee = 0, poc = 0;
     int32_t ptb = 0, pbb = 0, prmin = 0;
-    for (uint32_t p0 = pa; p0 < pb; p0 += WAVE) {
+    for (uint32_t P0 = pa; P0 < pb;) {
         if (MODE == CM_WG && *(volatile uint32_t *)T.over) return false;
-        ++stamp;
         const uint32_t qi = qc + lane;
         const bool qv = qi < c;
         const uint4 row = qv ? A.sd_row[o + qi] : make_uint4(0, 0, 0, 0xffffffffu);
         const uint32_t st = row.w;
-        if (lane > 0 && st > p0 && st < p0 + WAVE) T.mark[st - p0] = stamp;
+        // the block [P0, P1): the whole first row (at most 64 seeds start in
+        // it), beyond it only positions before the last loaded seed's start
+        uint32_t P1 = P0 + (uint32_t)J * WAVE;
+        if (J > 1) {
+            const uint32_t st_last = __builtin_amdgcn_readlane(st, WAVE - 1);  // ~0 when not loaded
+            if (st_last < P1) P1 = st_last > P0 + WAVE ? st_last : P0 + WAVE;
+        }
+        // the read's end lies in the block: positions from E on end runs
+        const bool tail = P1 >= E;
+        const uint32_t vis_end = tail ? E + 1 : P1;  // rows holding a position below this are visited
+        if (lane > 0 && st > P0 && st < P1)
+            atomicOr((unsigned long long *)&T.mark[(st - P0) >> 6], 1ull << ((st - P0) & 63u));
         wave_sync();
-        const uint64_t M = __ballot(T.mark[lane] == stamp);
-        const uint32_t ns = (uint32_t)__popcll(M), d = (uint32_t)__popcll(M & upto);
-        // lanes 0..ns hold the window's seeds
-        uint32_t sa = 0, send = 0, socc = 0;
-        if (lane <= ns && qv) {
-            sa = row.x - st;  // hit index = sa + position
-            send = row.x + row.y;
-            socc = row.z;
+        uint64_t M[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            // (readfirstlane returns int: widen each half as unsigned)
+            const uint64_t v = T.mark[j];
+            M[j] = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
         }
-        // every hit of the window is read (run heads past pb end runs before
-        // it); only those before pb emit
-        const uint32_t p = p0 + lane, q = qc + d;
-        const bool live = p < E, act = p < pb;
-        const uint32_t a = __shfl(sa, (int)d), ee = __shfl(send, (int)d), oc = __shfl(socc, (int)d);
-        const uint32_t s0 = __shfl(st, (int)d);
-        const uint32_t idx = a + p;
-        uint32_t seg = 0xffffffffu;
-        int32_t bk = -1;
-        if (live) {
-            seg = I.hit_seg[idx];
-            bk = hit_pos(I, idx, seg, ap) / hl;
+        wave_sync();
+        if (lane < (uint32_t)J) T.mark[lane] = 0;
+        const bool s0start = __builtin_amdgcn_readlane(st, 0) == P0;  // seed qc starts at P0
+        const uint32_t sa = row.x - st, send = row.x + row.y, socc = row.z;
+        // each row's seed (qc + dq), hit, segment and half-bucket
+        uint32_t dq[J], idx[J], seg[J];
+        int32_t bk[J];
+        uint32_t pre = 0;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t p = P0 + (uint32_t)j * WAVE + lane;
+            dq[j] = pre + (uint32_t)__popcll(M[j] & upto);
+            pre += (uint32_t)__popcll(M[j]);
+            idx[j] = __shfl(sa, (int)dq[j]) + p;
+            seg[j] = 0xffffffffu;
+            if (P0 + (uint32_t)j * WAVE < vis_end && p < P1 && p < E) seg[j] = I.hit_seg[idx[j]];
         }
-        const uint32_t pseg = __shfl_up(seg, 1);
-        const int32_t pbk = __shfl_up(bk, 1);
-        bool head = false, prev_bm1 = false;
-        if (live) {
-            if (p == s0) {
-                head = true;
-            } else {
-                uint32_t sg0 = pseg;
-                int32_t b0 = pbk;
+        uint32_t sg_prev = 0xffffffffu;  // the hit before P0 (same seed)
+        int32_t b_prev = -1;
+        if (lane == 0 && !s0start) {
+            sg_prev = I.hit_seg[idx[0] - 1];
+            b_prev = hit_pos(I, idx[0] - 1, sg_prev, ap) / hl;
+        }
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t p = P0 + (uint32_t)j * WAVE + lane;
+            bk[j] = -1;
+            if (P0 + (uint32_t)j * WAVE < vis_end && p < P1 && p < E) bk[j] = hit_pos(I, idx[j], seg[j], ap) / hl;
+        }
+        // head masks: H (heads, and positions from E on in the last block),
+        // B1 (heads whose previous hit is in the same seed and segment at the
+        // previous half-bucket)
+        uint64_t H[J], B1[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            H[j] = B1[j] = 0;
+            if (P0 + (uint32_t)j * WAVE >= vis_end) continue;
+            const uint32_t p = P0 + (uint32_t)j * WAVE + lane;
+            const bool vis = p < P1 || tail, live = p < P1 && p < E;
+            uint32_t ps = __shfl_up(seg[j], 1);
+            int32_t pbk = __shfl_up(bk[j], 1);
+            if (j == 0) {
                 if (lane == 0) {
-                    sg0 = I.hit_seg[idx - 1];
-                    b0 = hit_pos(I, idx - 1, sg0, ap) / hl;
+                    ps = sg_prev;
+                    pbk = b_prev;
                 }
-                head = sg0 != seg || b0 != bk;
-                prev_bm1 = sg0 == seg && b0 == bk - 1;
+            } else {
+                const uint32_t ls = __builtin_amdgcn_readlane(seg[j - 1], WAVE - 1);
+                const int32_t lb = __builtin_amdgcn_readlane(bk[j - 1], WAVE - 1);
+                if (lane == 0) {
+                    ps = ls;
+                    pbk = lb;
+                }
             }
+            const bool sstart = (j == 0 && lane == 0) ? s0start : ((M[j] >> lane) & 1ull) != 0;
+            const bool head = live && (sstart || ps != seg[j] || pbk != bk[j]);
+            const bool pbm1 = live && !sstart && ps == seg[j] && pbk == bk[j] - 1;
+            H[j] = __ballot(head || (vis && !live));
+            B1[j] = __ballot(head && pbm1);
         }
-        // pending contributions whose run ends in this window
+        // pending contributions whose run ends in this block
         bool settle = false;
-        uint32_t e_set = 0;
+        uint32_t e_set = 0, nset = 0;
         for (uint32_t i = 0; i < 3; i++) {
             if (!((pmask >> i) & 1u)) continue;
             const uint32_t fq = __builtin_amdgcn_readlane(pq, i), fsg = __builtin_amdgcn_readlane(psg, i);
             const int32_t ftb = __builtin_amdgcn_readlane(ptb, i);
-            const uint64_t m = __ballot(!live || q != fq || seg != fsg || bk >= ftb);
-            if (m) {
-                const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1;
-                if (lane == i) {
-                    settle = true;
-                    e_set = pidx + (p0 + l - ppos);
+            bool found = false;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                if (found || P0 + (uint32_t)j * WAVE >= vis_end) continue;
+                const uint32_t p = P0 + (uint32_t)j * WAVE + lane;
+                const bool vis = p < P1 || tail, live = p < P1 && p < E;
+                const uint64_t m = __ballot(vis && (!live || qc + dq[j] != fq || seg[j] != fsg || bk[j] >= ftb));
+                if (m) {
+                    const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                    if (lane == i) {
+                        settle = true;
+                        e_set = pidx + (P0 + (uint32_t)j * WAVE + l - ppos);
+                    }
+                    found = true;
                 }
+            }
+            if (found) {
                 pmask &= ~(1u << i);
+                nset++;
             }
         }
-        const bool emit = head && act;
-        const uint32_t nh = (uint32_t)__popcll(__ballot(emit));
-        const uint32_t need = 2 * nh + 6;  // heads, settled and closing pending contributions
-        if (MODE == CM_WG) {
-            uint32_t ok = 1;
-            if (lane == 0) {
-                const uint32_t had = atomicAdd(T.used, need);
-                if (had + need > WTAB_MAX) {
-                    atomicSub(T.used, need);
-                    atomicExch(T.over, 1u);
-                    ok = 0;
+        // the first row after each row whose head mask is not empty (J: none)
+        uint32_t nzr[J];
+        {
+            uint32_t nx = J;
+#pragma unroll
+            for (int j = J - 1; j >= 0; j--) {
+                nzr[j] = nx;
+                if (H[j]) nx = (uint32_t)j;
+            }
+        }
+        auto pick = [&](const uint64_t (&a)[J], uint32_t r) -> uint64_t {  // a[r] for a uniform r < J, else 0
+            uint64_t v = 0;
+#pragma unroll
+            for (int t = 0; t < J; t++)
+                if (r == (uint32_t)t) v = a[t];
+            return v;
+        };
+        auto pick_nz = [&](uint32_t r) -> uint32_t {
+            uint32_t v = J;
+#pragma unroll
+            for (int t = 0; t < J; t++)
+                if (r == (uint32_t)t) v = nzr[t];
+            return v;
+        };
+        // Rows are taken CPREP at a time: each row's run ends found and the
+        // hits they need (strand prefix at both ends, r of the first and last
+        // hit) loaded for all CPREP rows, then the table updates row by row,
+        // so the loads of the rows overlap.
+#pragma unroll
+        for (int j0 = 0; j0 < J; j0 += CPREP) {
+            uint32_t nA[CPREP], nB[CPREP], sA[CPREP], sB[CPREP], s0[CPREP], xA[CPREP], xB[CPREP], rmn[CPREP];
+            uint32_t pf[CPREP];  // bit 0: bucket b's contribution pending, bit 1: bucket b-1's
+            uint32_t nh[CPREP];
+#pragma unroll
+            for (int t = 0; t < CPREP; t++) {
+                const int j = j0 + t;
+                nA[t] = nB[t] = sA[t] = sB[t] = s0[t] = xA[t] = xB[t] = rmn[t] = pf[t] = nh[t] = 0;
+                if (P0 + (uint32_t)j * WAVE >= vis_end) continue;
+                const uint32_t p = P0 + (uint32_t)j * WAVE + lane;
+                const bool emit = ((H[j] >> lane) & 1ull) && p < P1 && p < E && p < pb;
+                nh[t] = (uint32_t)__popcll(__ballot(emit));
+                if (emit) {
+                    // h1: the next head; h2: the head after it (block positions)
+                    const uint32_t ra = nzr[j];  // uniform
+                    const uint64_t ha = pick(H, ra), ba = pick(B1, ra);
+                    const uint64_t mj = H[j] & ~upto;
+                    const bool f1 = mj != 0 || ra < (uint32_t)J;
+                    const uint64_t h1m = mj ? H[j] : ha;
+                    const uint32_t r1 = mj ? (uint32_t)j : ra;
+                    const uint32_t b1 = f1 ? (uint32_t)__ffsll((unsigned long long)(mj ? mj : ha)) - 1 : 0;
+                    const bool h1b1 = f1 && (((mj ? B1[j] : ba) >> b1) & 1ull);
+                    const uint64_t m2 = b1 < 63 ? (h1m & (~0ull << (b1 + 1))) : 0ull;
+                    const uint32_t rb = mj ? ra : pick_nz(ra);  // row after r1 holding a head
+                    const uint64_t hb = mj ? ha : pick(H, pick_nz(ra));
+                    const bool f2 = f1 && (m2 != 0 || rb < (uint32_t)J);
+                    const uint32_t r2 = m2 ? r1 : rb;
+                    const uint32_t b2 = m2 ? (uint32_t)__ffsll((unsigned long long)m2) - 1
+                                           : (f2 ? (uint32_t)__ffsll((unsigned long long)hb) - 1 : 0);
+                    const uint32_t here = (uint32_t)j * WAVE + lane;
+                    const uint32_t e1 = r1 * WAVE + b1, e2 = r2 * WAVE + b2;
+                    const uint32_t ix = idx[j];
+                    s0[t] = I.strand_pre[ix];
+                    rmn[t] = I.hit_r[ix] & 0x7fffffffu;
+                    if (h1b1 ? f2 : f1) {
+                        nA[t] = (h1b1 ? e2 : e1) - here;
+                        sA[t] = I.strand_pre[ix + nA[t]];
+                        xA[t] = I.hit_r[ix + nA[t] - 1];
+                    } else {
+                        pf[t] |= 1u;
+                    }
+                    if (bk[j] > 0 && !((B1[j] >> lane) & 1ull)) {
+                        if (f1) {
+                            nB[t] = e1 - here;
+                            sB[t] = I.strand_pre[ix + nB[t]];
+                            xB[t] = I.hit_r[ix + nB[t] - 1];
+                        } else {
+                            pf[t] |= 2u;
+                        }
+                    }
                 }
             }
-            if (!__shfl(ok, 0)) return false;
-        } else if (*T.used + need > LTAB_MAX) {
-            if (MODE == CM_LDS) return false;
-            ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[rd], lane);
-        }
-        if (settle) add(e_set, pidx, poc, prmin, psg, pbb);
-        // next run heads (positions past the read's end count as heads)
-        const uint64_t H = __ballot(head || !live);
-        const uint64_t h1 = H & ~upto;
-        const uint32_t nh1 = h1 ? (uint32_t)__ffsll((unsigned long long)h1) - 1 : WAVE;
-        const int src1 = (int)(nh1 < WAVE ? nh1 : WAVE - 1);
-        const uint32_t q1 = __shfl(q, src1), sg1 = __shfl(seg, src1);
-        const int32_t b1 = __shfl(bk, src1);
-        bool pend0 = false, pend1 = false;
-        int32_t rmin = 0;
-        if (emit) {
-            const bool next_b1 = nh1 < WAVE && p0 + nh1 < E && q1 == q && sg1 == seg && b1 == bk + 1;
-            const uint64_t h2 = nh1 + 1 < WAVE ? (H >> (nh1 + 1)) : 0ull;
-            const uint32_t nh2 = h2 ? nh1 + (uint32_t)__ffsll((unsigned long long)h2) : WAVE;
-            const uint32_t nb = next_b1 ? nh2 : nh1;  // the run head ending bucket bk
-            rmin = (int32_t)(I.hit_r[idx] & 0x7fffffffu);
-            if (nb < WAVE) add(idx + (nb - lane), idx, oc, rmin, seg, bk);
-            else pend0 = true;
-            if (bk > 0 && !prev_bm1) {
-                if (nh1 < WAVE) add(idx + (nh1 - lane), idx, oc, rmin, seg, bk - 1);
-                else pend1 = true;
+#pragma unroll
+            for (int t = 0; t < CPREP; t++) {
+                const int j = j0 + t;
+                if (P0 + (uint32_t)j * WAVE >= vis_end) continue;
+                const uint32_t need = 2 * nh[t] + (j == 0 ? nset : 0);
+                if (need == 0) continue;
+                if (!reserve(need)) return false;
+                if (j == 0 && settle) add(e_set, pidx, poc, prmin, psg, pbb);
+                const uint32_t oc = __shfl(socc, (int)dq[j]);
+                if (nA[t]) put(nA[t], sA[t] - s0[t], oc, (int32_t)rmn[t], xA[t], seg[j], bk[j]);
+                if (nB[t]) put(nB[t], sB[t] - s0[t], oc, (int32_t)rmn[t], xB[t], seg[j], bk[j] - 1);
+                // this row's open contributions join the pending ones
+                uint64_t m0 = __ballot(pf[t] & 1u), m1 = __ballot(pf[t] & 2u);
+                while (m0 | m1) {
+                    const bool s1 = m0 == 0;
+                    const uint32_t L = (uint32_t)__ffsll((unsigned long long)(s1 ? m1 : m0)) - 1;
+                    if (s1) m1 &= m1 - 1; else m0 &= m0 - 1;
+                    const uint32_t slot = (uint32_t)__ffs(~pmask & 7u) - 1;
+                    pmask |= 1u << slot;
+                    const uint32_t vd = __builtin_amdgcn_readlane(dq[j], L);
+                    const uint32_t vsg = __builtin_amdgcn_readlane(seg[j], L), vidx = __builtin_amdgcn_readlane(idx[j], L);
+                    const uint32_t vee = __builtin_amdgcn_readlane(send, vd), voc = __builtin_amdgcn_readlane(socc, vd);
+                    const int32_t vbk = __builtin_amdgcn_readlane(bk[j], L);
+                    const int32_t vrmin = (int32_t)__builtin_amdgcn_readlane(rmn[t], L);
+                    if (lane == slot) {
+                        pq = qc + vd;
+                        psg = vsg;
+                        ppos = P0 + (uint32_t)j * WAVE + L;
+                        pidx = vidx;
+                        pee = vee;
+                        poc = voc;
+                        ptb = vbk + (s1 ? 1 : 2);
+                        pbb = vbk - (s1 ? 1 : 0);
+                        prmin = vrmin;
+                    }
+                }
+                account(need);
             }
         }
-        // this window's open contributions join the pending ones
-        uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
-        while (m0 | m1) {
-            const bool s1 = m0 == 0;
-            const uint32_t L = (uint32_t)__ffsll((unsigned long long)(s1 ? m1 : m0)) - 1;
-            if (s1) m1 &= m1 - 1; else m0 &= m0 - 1;
-            const uint32_t slot = (uint32_t)__ffs(~pmask & 7u) - 1;
-            pmask |= 1u << slot;
-            const uint32_t vq = __builtin_amdgcn_readlane(q, L), vsg = __builtin_amdgcn_readlane(seg, L);
-            const uint32_t vpos = __builtin_amdgcn_readlane(p, L), vidx = __builtin_amdgcn_readlane(idx, L);
-            const uint32_t vee = __builtin_amdgcn_readlane(ee, L), voc = __builtin_amdgcn_readlane(oc, L);
-            const int32_t vbk = __builtin_amdgcn_readlane(bk, L), vrmin = __builtin_amdgcn_readlane(rmin, L);
-            if (lane == slot) {
-                pq = vq;
-                psg = vsg;
-                ppos = vpos;
-                pidx = vidx;
-                pee = vee;
-                poc = voc;
-                ptb = vbk + (s1 ? 1 : 2);
-                pbb = vbk - (s1 ? 1 : 0);
-                prmin = vrmin;
-            }
-        }
-        // the range ends: what is still open ends past it
-        if (p0 + WAVE >= pb && pmask) {
+        // the range ends: what is still open ends past the block
+        if (P1 >= pb && pmask) {
+            const uint32_t need = (uint32_t)__popc(pmask);
+            if (!reserve(need)) return false;
             if (lane < 3 && ((pmask >> lane) & 1u)) {
-                const uint32_t from = pidx + (p0 + WAVE - ppos);
+                const uint32_t from = pidx + (P1 - ppos);
                 add(hits_lower_bound(I, from, pee, psg, (int64_t)ptb * hl, ap), pidx, poc, prmin, psg, pbb);
             }
             pmask = 0;
+            account(need);
         }
-        // the window's new slots
-        const uint32_t nclaim = (uint32_t)__popcll(__ballot(claims & 1u)) + 2u * (uint32_t)__popcll(__ballot(claims & 2u)) +
-                                4u * (uint32_t)__popcll(__ballot(claims & 4u));
-        claims = 0;
-        wave_sync();
-        if (lane == 0) {
-            if (MODE == CM_WG) atomicSub(T.used, need - nclaim);  // the unused part of the reservation
-            else *T.used += nclaim;
-        }
-        // the seed holding p0 + 64: every seed starting at or before it
-        uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= p0 + WAVE));
-        if (adv == WAVE - 1 && qc + WAVE < c && A.sd_row[o + qc + WAVE].w <= p0 + WAVE) adv++;
+        // the seed holding P1: every seed starting at or before it
+        uint32_t adv = (uint32_t)__popcll(__ballot(lane > 0 && st <= P1));
+        if (adv == WAVE - 1 && qc + WAVE < c && A.sd_row[o + qc + WAVE].w <= P1) adv++;
         qc += adv;
+        P0 = P1;
         wave_sync();
     }
     return true;
@@ -1496,15 +1658,14 @@ __device__ bool contrib_windows(const MapArgs &A, const DevIndex &I, const LTab 
     __shared__ uint32_t s_key[NWAVES][LTAB];                                                   \
     __shared__ uint64_t s_mc[NWAVES][LTAB];                                                    \
     __shared__ int32_t s_mn[NWAVES][LTAB], s_mx[NWAVES][LTAB];                                 \
-    __shared__ uint32_t s_mark[NWAVES][WAVE];                                                  \
+    __shared__ uint64_t s_mark[NWAVES][CROWS];                                                 \
     __shared__ uint32_t s_used[NWAVES];                                                        \
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); \
     const LTab T{s_key[w], s_mark[w], &s_used[w], s_mc[w], s_mn[w], s_mx[w], nullptr};         \
     ltab_clear(T, lane);                                                                       \
-    T.mark[lane] = 0;                                                                          \
+    if (lane < (uint32_t)CROWS) T.mark[lane] = 0;                                              \
     if (lane == 0) *T.used = 0;                                                                \
-    wave_sync();                                                                               \
-    uint32_t stamp = 0
+    wave_sync()
 
 // One wave per read with at most A.contrib_wave_e examined hits: its buckets end
 // as the first grp_nb rows of its table region, listed in order.  Larger
@@ -1521,7 +1682,7 @@ __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
             if (lane == 0) A.ov_list[atomicAdd(A.ov_count, 1u)] = (uint32_t)lr;
             continue;
         }
-        if (!contrib_windows<CM_LDS>(A, I, T, lr, 0, E, E, lane, stamp)) {  // too many buckets
+        if (!contrib_windows<CM_LDS>(A, I, T, lr, 0, E, E, lane)) {  // too many buckets
             ltab_clear(T, lane);
             if (lane == 0) {
                 *T.used = 0;
@@ -1560,33 +1721,36 @@ __global__ void __launch_bounds__(WG) k_contrib_wave(MapArgs A, DevIndex I) {
     }
 }
 
-// One workgroup per read listed by k_contrib_wave: its examined hits split
-// into WG quarters, one per wave, counted into one LDS table of WTAB slots;
-// its buckets end as the first grp_nb rows of its table region, as
-// k_contrib_wave leaves them.  Reads with more than A.contrib_wg_e examined
-// hits, or whose buckets would overfill the table, are listed for the global
-// tables (k_contrib_chunks).
-__global__ void __launch_bounds__(WG) k_contrib_wg(MapArgs A, DevIndex I) {
+// One workgroup of CW waves per read listed by k_contrib_wave whose examined
+// hits E satisfy e_lo < E <= e_hi: the hits split into CW parts, one per wave,
+// counted into one LDS table of WTAB slots; its buckets end as the first
+// grp_nb rows of its table region, as k_contrib_wave leaves them.  Reads with
+// more than A.contrib_wg_e examined hits, or whose buckets overfill the table,
+// are listed for the global tables (k_contrib_chunks).  Satellite reads with
+// hundreds of thousands of hits take 16 waves, so a few of them do not
+// outlast the rest of the launch.
+template <int CW>
+__global__ void __launch_bounds__(CW * WAVE) k_contrib_wg(MapArgs A, DevIndex I, uint32_t e_lo, uint32_t e_hi) {
     if (aborted(A.abort)) return;
+    constexpr uint32_t NT = (uint32_t)CW * WAVE;
     __shared__ uint32_t s_key[WTAB];
     __shared__ uint64_t s_mc[WTAB];
     __shared__ int32_t s_mn[WTAB], s_mx[WTAB];
-    __shared__ uint32_t s_mark[NWAVES][WAVE];
+    __shared__ uint64_t s_mark[CW][CROWS];
     __shared__ uint32_t s_used, s_over;
-    __shared__ uint64_t tmp[NWAVES + 1];
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const LTab T{s_key, s_mark[w], &s_used, s_mc, s_mn, s_mx, &s_over};
-    s_mark[w][lane] = 0;
-    uint32_t stamp = 0;
+    if (lane < (uint32_t)CROWS) s_mark[w][lane] = 0;
     const uint32_t n = *A.ov_count;
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
         const int32_t lr = (int32_t)A.ov_list[t];
         const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
+        if (E <= e_lo || E > e_hi) continue;  // the other width's read
         if (E > A.contrib_wg_e) {
             if (threadIdx.x == 0) A.gl_list[atomicAdd(A.gl_count, 1u)] = (uint32_t)lr;
             continue;
         }
-        for (uint32_t i = threadIdx.x; i < WTAB; i += WG) {
+        for (uint32_t i = threadIdx.x; i < WTAB; i += NT) {
             s_key[i] = EMPTY_KEY;
             s_mc[i] = 0;
             s_mn[i] = INT32_MAX;
@@ -1594,33 +1758,36 @@ __global__ void __launch_bounds__(WG) k_contrib_wg(MapArgs A, DevIndex I) {
         }
         if (threadIdx.x == 0) s_used = s_over = 0;
         __syncthreads();
-        const uint32_t pa = (uint32_t)((uint64_t)E * w / NWAVES), pb = (uint32_t)((uint64_t)E * (w + 1) / NWAVES);
-        if (pa < pb) (void)contrib_windows<CM_WG>(A, I, T, lr, pa, pb, E, lane, stamp);
+        const uint32_t pa = (uint32_t)((uint64_t)E * w / CW), pb = (uint32_t)((uint64_t)E * (w + 1) / CW);
+        if (pa < pb) (void)contrib_windows<CM_WG>(A, I, T, lr, pa, pb, E, lane);
         __syncthreads();
         if (s_over) {
             if (threadIdx.x == 0) A.gl_list[atomicAdd(A.gl_count, 1u)] = (uint32_t)lr;
             __syncthreads();
             continue;
         }
-        // the used slots become rows 0.. of the read's table region
-        const uint64_t tb = A.tab_base[lr];
-        uint32_t nb = 0;
-        for (uint32_t s0 = 0; s0 < WTAB; s0 += WG) {
-            const uint32_t sl = s0 + threadIdx.x;
-            const uint32_t k = s_key[sl];
-            const bool used = k != EMPTY_KEY;
-            uint64_t tot;
-            const uint32_t at = nb + (uint32_t)wg_excl_scan64(used ? 1u : 0u, tmp, &tot);
-            if (used) {
-                A.tb_key[tb + at] = k;
-                A.tb_mc[tb + at] = s_mc[sl];
-                A.tb_min[tb + at] = s_mn[sl];
-                A.tb_max[tb + at] = s_mx[sl];
-                A.tb_list[tb + at] = at;
+        // the used slots become rows 0.. of the read's table region, in slot
+        // order (the first wave)
+        if (w == 0) {
+            const uint64_t tb = A.tab_base[lr], below = (1ull << lane) - 1;
+            uint32_t nb = 0;
+            for (uint32_t s0 = 0; s0 < WTAB; s0 += WAVE) {
+                const uint32_t sl = s0 + lane;
+                const uint32_t k = s_key[sl];
+                const bool used = k != EMPTY_KEY;
+                const uint64_t bm = __ballot(used);
+                if (used) {
+                    const uint32_t at = nb + (uint32_t)__popcll(bm & below);
+                    A.tb_key[tb + at] = k;
+                    A.tb_mc[tb + at] = s_mc[sl];
+                    A.tb_min[tb + at] = s_mn[sl];
+                    A.tb_max[tb + at] = s_mx[sl];
+                    A.tb_list[tb + at] = at;
+                }
+                nb += (uint32_t)__popcll(bm);
             }
-            nb += (uint32_t)tot;
+            if (lane == 0) A.grp_nb[lr] = nb;
         }
-        if (threadIdx.x == 0) A.grp_nb[lr] = nb;
         __syncthreads();
     }
 }
@@ -1693,7 +1860,7 @@ __global__ void __launch_bounds__(WG) k_contrib_chunks(MapArgs A, DevIndex I) {
         const uint32_t E = (uint32_t)(A.hit_base[lr + 1] - A.hit_base[lr]);
         const uint32_t pa = (uint32_t)(k - A.ov_kpre[i]) * cs;
         const uint32_t pb = pa + cs < E ? pa + cs : E;
-        contrib_windows<CM_LDS_FLUSH>(A, I, T, lr, pa, pb, E, lane, stamp);
+        contrib_windows<CM_LDS_FLUSH>(A, I, T, lr, pa, pb, E, lane);
         ltab_flush(A, T, (uint32_t)lr, A.tab_base[lr], A.rd_cap[A.r0 + lr], lane);
     }
 }
@@ -3086,7 +3253,8 @@ void launch_stage(Stage stage, const MapArgs &A, const DevIndex &I, shm_read_res
         // in cost), a workgroup per read past one wave, then the reads past
         // that in chunks through global tables
         k_contrib_wave<<<(uint32_t)((A.RG + NWAVES - 1) / NWAVES), WG, 0, st>>>(A, I);
-        k_contrib_wg<<<max_grid, WG, 0, st>>>(A, I);
+        k_contrib_wg<NWAVES><<<max_grid, WG, 0, st>>>(A, I, 0u, CWG_SPLIT);
+        k_contrib_wg<CWG><<<(max_grid * NWAVES + CWG - 1) / CWG, CWG * WAVE, 0, st>>>(A, I, CWG_SPLIT, UINT32_MAX);
         k_contrib_over_scan<<<1, WG, 0, st>>>(A);
         k_contrib_over_clear<<<max_grid, WG, 0, st>>>(A);
         k_contrib_chunks<<<PGRID(k_contrib_chunks, 0, max_grid), WG, 0, st>>>(A, I);
