@@ -1160,9 +1160,22 @@ __device__ __forceinline__ void decode_bucket(const MapArgs &A, uint32_t g, int3
 }
 
 // First hit index in [lo, hi) at or after (seg, pos) -- the partition_point of
-// src/shmap/pruning.rs:52-61.
+// src/shmap/pruning.rs:52-61.  With sketch positions (not abs_pos) a hit's
+// global k-mer id is kstart[seg] + pos, below kstart[seg + 1]: ids increase in
+// (segment, position) order, so the search compares ids alone against
+// kstart[seg] + pos clamped to the segment (one array and no kstart load per
+// step).  With abs_pos (positions in bases) it compares segment, then r.
 __device__ __forceinline__ uint32_t hits_lower_bound(const DevIndex &I, uint32_t lo, uint32_t hi,
                                                      uint32_t seg, int64_t pos, int ap) {
+    if (!ap) {
+        const int64_t ks = I.kstart[seg], len = I.kstart[seg + 1] - ks;
+        const uint32_t g = (uint32_t)(ks + (pos < 0 ? 0 : (pos < len ? pos : len)));
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (I.hit_g[mid] < g) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    }
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         const uint32_t s = I.hit_seg[mid];
