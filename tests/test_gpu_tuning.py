@@ -86,3 +86,30 @@ def test_autotune_keeps_a_valid_tuning(setup):
     bad = P.diff_results(want, got)
     assert not bad, bad
     mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0)
+
+
+def test_satellite_reads_take_every_counting_path():
+    """Reads inside tandem arrays examine 10^4 - 10^5 hits each: with the
+    workgroup bound raised, reads up to 16,384 examined hits are counted by
+    four-wave workgroups and the longer ones by sixteen-wave workgroups (whose
+    waves fill one shared table without reservations); with it at 1 they all
+    go through chunks and global tables.  Every run equals the oracle."""
+    lens = [300_000, 300_000]
+    ref = G.synth_reference(71, lens, repeat_frac=0.7, unit_len=171, div_min=0.005, div_max=0.01, n_units=1,
+                            array_len=100_000)
+    bases, off, _ = G.synth_reads(72, ref, lens, 40, (12_000, 5_000), lognormal=True, clip=(1_000, 30_000),
+                                  errors=((0.001, 0.6, 0.2, 0.2),))
+    segs, reads = P.contigs(ref, lens), P.split_reads(bases, off)
+    _, _, ores = P.oracle_map(segs, reads, K, H_FRAC, **PRM)
+    E = ores["seed_matches"].astype(np.int64)
+    assert (E > 16_384).sum() >= 3 and ((E > 1_024) & (E <= 16_384)).sum() >= 3, np.sort(E)
+    idx = G.SketchIndex(G.FracMinHash(K, H_FRAC), segs, None)
+    mp = G.Mapper(idx, G.make_params(k=K, h_frac=H_FRAC, **PRM))
+    for t in (dict(contrib_wave_hits=1024, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=1_000_000),
+              dict(contrib_wave_hits=1, contrib_chunk_hits=512, prune_thread_seeds=0, contrib_wg_hits=1),
+              dict(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=0)):
+        mp.set_tuning(**t)
+        got = mp.map_batch(reads)
+        bad = P.diff_results(ores, got)
+        assert not bad, (t, bad)
+    assert ores["mapped"].sum() > 0
