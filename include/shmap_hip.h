@@ -309,12 +309,18 @@ int shm_mapper_set_graphs(shm_mapper *mp, int32_t on);
  *                       takes it over (built-in: 16)
  *   contrib_wg_hits     examined hits up to which one workgroup counts a
  *                       read's buckets whole in a shared table of 1,024
- *                       slots (built-in: 8192) */
+ *                       slots (built-in: 8192)
+ *   seed_key_bits       leading bits of each k-mer hash (from the keep
+ *                       threshold's highest bit down) in the one-word keys a
+ *                       wave sorts a read's k-mers by, 1 to 55; k-mers whose
+ *                       distinct hashes share them are put in order after
+ *                       the sort (built-in: 55) */
 typedef struct {
     int32_t contrib_wave_hits;
     int32_t contrib_chunk_hits;
     int32_t prune_thread_seeds;
     int32_t contrib_wg_hits;
+    int32_t seed_key_bits;
 } shm_tuning;
 int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t);
 int shm_mapper_get_tuning(const shm_mapper *mp, shm_tuning *t);

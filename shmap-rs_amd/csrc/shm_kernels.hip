@@ -731,6 +731,70 @@ __global__ void __launch_bounds__(WG) k_insert(const uint64_t *uk_h, const uint6
 // whose tables fit the device budget.
 // ============================================================================
 
+// A read's k-mers (m <= E * WAVE) in (hash, position) order, as hashes in kh
+// and k-mer indices in kv (the wave's LDS slice).  The wave sorts one-word keys
+// (hash >> A.seed_sh) << 9 | index: the shifted hash keeps the hashes' order
+// and the index breaks ties, so two keys are out of order only if their
+// distinct hashes agree in the kept bits.  Such neighbours are looked for
+// after the sort, and if there are any, one lane puts the array in order by
+// insertion (few and short runs are out of place, so this is close to one
+// pass).
+template <int E>
+__device__ __forceinline__ void seed_sort_e(const MapArgs &A, uint64_t o, uint32_t m, uint64_t *kh, uint32_t *kv,
+                                            uint32_t lane) {
+    const uint32_t ssh = A.seed_sh;
+    uint64_t k[E];
+    uint32_t v[E];  // (unused: the keys carry the index)
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = lane * E + (uint32_t)e;
+        k[e] = ~0ull;
+        v[e] = 0;
+        if (x < m) {
+            const uint64_t h = A.k_h[o + x];
+            kh[x] = h;
+            k[e] = ((h >> ssh) << 9) | x;
+        }
+    }
+    wave_sync();
+    wave_sort_regs<E, false>(k, v);
+    uint64_t hv[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = lane * E + (uint32_t)e;
+        hv[e] = x < m ? kh[(uint32_t)k[e] & 511u] : 0ull;
+    }
+    wave_sync();  // every hash read before any is overwritten
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t x = lane * E + (uint32_t)e;
+        if (x < m) {
+            kh[x] = hv[e];
+            kv[x] = (uint32_t)k[e] & 511u;
+        }
+    }
+    wave_sync();
+    bool mixed = false;
+    for (uint32_t x = lane + 1; x < m; x += WAVE)
+        mixed |= kh[x] != kh[x - 1] && (kh[x] >> ssh) == (kh[x - 1] >> ssh);
+    if (__ballot(mixed) == 0) return;
+    if (lane == 0) {
+        for (uint32_t i = 1; i < m; i++) {
+            const uint64_t h = kh[i];
+            const uint32_t p = kv[i];
+            uint32_t j = i;
+            while (j > 0 && (kh[j - 1] > h || (kh[j - 1] == h && kv[j - 1] > p))) {
+                kh[j] = kh[j - 1];
+                kv[j] = kv[j - 1];
+                j--;
+            }
+            kh[j] = h;
+            kv[j] = p;
+        }
+    }
+    wave_sync();
+}
+
 // The seeding stages for one read per wave (reads with at most WAVE_SEED_CAP
 // sketched k-mers; larger reads are listed for k_seed_wg):
 //   unique_elements_with_info (src/shmap/seeding.rs:26-77): sort by (hash,
@@ -761,17 +825,11 @@ __global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
             if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)rd;
             continue;
         }
-        // (hash, position) order: in registers up to 8*WAVE k-mers, else in LDS
-        if (!wave_sort_n<true>(
-                m, [&](uint32_t x, uint64_t &key, uint32_t &val) { key = A.k_h[o + x]; val = x; },
-                [&](uint32_t x, uint64_t key, uint32_t val) { kh[x] = key; kv[x] = val; })) {
-            for (uint32_t i = lane; i < m; i += WAVE) {
-                kh[i] = A.k_h[o + i];
-                kv[i] = i;
-            }
-            wave_sync();
-            wave_bitonic(kh, kv, m);
-        }
+        // (hash, position) order, in registers (m <= WAVE_SEED_CAP = 8 * WAVE)
+        if (m <= WAVE) seed_sort_e<1>(A, o, m, kh, kv, lane);
+        else if (m <= 2 * WAVE) seed_sort_e<2>(A, o, m, kh, kv, lane);
+        else if (m <= 4 * WAVE) seed_sort_e<4>(A, o, m, kh, kv, lane);
+        else seed_sort_e<8>(A, o, m, kh, kv, lane);
         wave_sync();
         // runs of equal hash -> seeds in hash order.  A seed id never exceeds
         // the position of its run head (and equals it only while every earlier

@@ -27,6 +27,8 @@ constexpr uint32_t SCAN_PARTS = 1024;     // workgroups of the device-wide scan
 constexpr uint64_t SCAN_SINGLE_MAX = 8192; // arrays up to this length scan in one workgroup
 constexpr uint32_t WAVE_SEED_CAP = 512;   // k-mers of a read seeded by one wave
 constexpr uint32_t WAVE_SEED_LDS = WAVE_SEED_CAP * 14;  // per-wave LDS of k_seed_wave
+constexpr int SEED_KEY_BITS_MAX = 55;     // hash bits in k_seed_wave's sort keys (with 9 bits of k-mer index)
+static_assert(WAVE_SEED_CAP <= 512, "k_seed_wave's sort keys hold a k-mer index in 9 bits");
 constexpr uint32_t TASK_COUNTERS = 8;       // per group: first-pass, second band, 3 refine rounds, prune list,
                                             // reads on the global-table bucket path, reads past one wave
 constexpr uint32_t EXTRACT_WAVE_CAP = 512;  // buckets of a read sorted by one wave (in registers)
@@ -144,6 +146,7 @@ struct MapArgs {
     uint32_t *gl_list, *gl_count;
     uint64_t *ov_kpre;
     uint32_t contrib_wave_e;  // examined hits one k_contrib_wave wave takes whole
+    uint32_t seed_sh;         // k_seed_wave's sort keys: hash >> seed_sh (< 2^SEED_KEY_BITS_MAX) above the k-mer index
     uint32_t contrib_chunk;   // chunk length of the listed reads (0: chosen by k_contrib_over_scan)
     uint32_t contrib_wg_e;    // examined hits one k_contrib_wg workgroup takes whole
     uint32_t prune_seq;       // seeds one k_prune thread extends a bucket with

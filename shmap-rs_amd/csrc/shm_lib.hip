@@ -1182,9 +1182,10 @@ extern "C" int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t) {
     if (!mp || !t) return fail(SHM_ERR_ARG, "invalid argument");
     if (t->contrib_wave_hits < 0 || t->prune_thread_seeds < 0 || t->contrib_chunk_hits < 0 || t->contrib_wg_hits < 0 ||
         (t->contrib_chunk_hits &&
-         (t->contrib_chunk_hits < 64 || t->contrib_chunk_hits > 4096 || (t->contrib_chunk_hits & (t->contrib_chunk_hits - 1)))))
+         (t->contrib_chunk_hits < 64 || t->contrib_chunk_hits > 4096 || (t->contrib_chunk_hits & (t->contrib_chunk_hits - 1)))) ||
+        t->seed_key_bits < 0 || t->seed_key_bits > SEED_KEY_BITS_MAX)
         return fail(SHM_ERR_ARG, "tuning out of range (contrib_chunk_hits: 0 or a power of two from 64 to 4096; "
-                                 "the others >= 0)");
+                                 "seed_key_bits: 0 to 55; the others >= 0)");
     mp->tune = *t;
     return SHM_OK;
 }
@@ -1365,6 +1366,15 @@ static int ensure_read_buffers(shm_mapper *mp, int32_t R, uint64_t M) {
     return SHM_OK;
 }
 
+// The shift that leaves a hash's leading key bits (from the keep threshold's
+// highest bit down) for the one-word keys of k_seed_wave's sort: every kept
+// hash is at most the threshold, so hash >> shift < 2^key_bits.
+static uint32_t seed_key_shift(const shm_mapper *mp) {
+    const int kb = mp->tune.seed_key_bits > 0 ? mp->tune.seed_key_bits : SEED_KEY_BITS_MAX;
+    const int b = mp->sk.thres ? 64 - __builtin_clzll(mp->sk.thres) : 0;
+    return b > kb ? (uint32_t)(b - kb) : 0u;
+}
+
 // The read-level arguments of the mapping kernels.
 static MapArgs read_args(shm_mapper *mp, int32_t R, const int64_t *d_off) {
     const shm_index *ix = mp->idx;
@@ -1378,6 +1388,7 @@ static MapArgs read_args(shm_mapper *mp, int32_t R, const int64_t *d_off) {
     A.metric = mp->prm.metric;
     A.no_bucket_pruning = mp->prm.no_bucket_pruning;
     A.one_sweep = mp->prm.one_sweep;
+    A.seed_sh = seed_key_shift(mp);
     A.abs_pos = mp->prm.abs_pos;
     A.roff = d_off;
     A.kstart = ix->kstart.p;
@@ -1482,6 +1493,7 @@ static void table_args(shm_mapper *mp, MapArgs &A, int32_t r0, int32_t RG, uint3
     A.contrib_wg_e = mp->tune.contrib_wg_hits > 0 ? (uint32_t)mp->tune.contrib_wg_hits : 8192u;
     A.prune_seq = mp->tune.prune_thread_seeds > 0 ? (uint32_t)mp->tune.prune_thread_seeds : 16u;
 }
+
 
 // The bucket arguments of a group whose buffers hold B buckets.
 static void bucket_args(shm_mapper *mp, MapArgs &A, uint64_t B, int32_t max_hl, uint64_t max_m) {

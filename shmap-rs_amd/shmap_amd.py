@@ -173,7 +173,7 @@ def device_cus(device: int = 0) -> int:
 class Tuning(C.Structure):
     """shm_tuning: a mapper's batching choices (0: the built-in rule)."""
     _fields_ = [("contrib_wave_hits", C.c_int32), ("contrib_chunk_hits", C.c_int32),
-                ("prune_thread_seeds", C.c_int32), ("contrib_wg_hits", C.c_int32)]
+                ("prune_thread_seeds", C.c_int32), ("contrib_wg_hits", C.c_int32), ("seed_key_bits", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -351,8 +351,8 @@ class Mapper:
 
     def set_tuning(self, **fields) -> None:
         """shm_mapper_set_tuning: contrib_wave_hits, contrib_chunk_hits,
-        prune_thread_seeds, contrib_wg_hits (0: the built-in rule); unnamed
-        fields keep their values."""
+        prune_thread_seeds, contrib_wg_hits, seed_key_bits (0: the built-in
+        rule); unnamed fields keep their values."""
         t = Tuning()
         _check(lib().shm_mapper_get_tuning(self._p, C.byref(t)))
         for k, v in fields.items():

@@ -1,9 +1,11 @@
 """A mapper's batching choices (shm_tuning) change where per-read work runs --
 whole reads by a wave or a workgroup, or chunks through global tables, for the
-bucket counts, tiny or long chunks, threads or waves for the prune -- and
-never the records.  Extreme settings force every
-read down one path; each run must equal the built-in choice and the oracle
-bit for bit.  The autotuner keeps a valid tuning with the same records."""
+bucket counts, tiny or long chunks, threads or waves for the prune, and how
+many hash bits the seeding sort's one-word keys keep (with one or eight bits
+nearly every read's k-mers need the ordering pass after the sort) -- and
+never the records.  Extreme settings force every read down one path; each run
+must equal the built-in choice and the oracle bit for bit.  The autotuner
+keeps a valid tuning with the same records."""
 import numpy as np
 import pytest
 
@@ -31,19 +33,23 @@ def setup():
 
 
 TUNINGS = [
-    # chunks of one window through global tables, waves
-    dict(contrib_wave_hits=1, contrib_chunk_hits=64, prune_thread_seeds=1, contrib_wg_hits=1),
+    # chunks of one window through global tables, waves; one hash bit in the
+    # seeding keys (nearly every read is ordered after its sort)
+    dict(contrib_wave_hits=1, contrib_chunk_hits=64, prune_thread_seeds=1, contrib_wg_hits=1, seed_key_bits=1),
     # whole reads by one wave, threads
-    dict(contrib_wave_hits=100_000, contrib_chunk_hits=4096, prune_thread_seeds=100_000, contrib_wg_hits=0),
+    dict(contrib_wave_hits=100_000, contrib_chunk_hits=4096, prune_thread_seeds=100_000, contrib_wg_hits=0,
+         seed_key_bits=8),
     # every read with hits by a workgroup
-    dict(contrib_wave_hits=1, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=1_000_000),
-    dict(contrib_wave_hits=200, contrib_chunk_hits=128, prune_thread_seeds=3, contrib_wg_hits=300),
+    dict(contrib_wave_hits=1, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=1_000_000,
+         seed_key_bits=55),
+    dict(contrib_wave_hits=200, contrib_chunk_hits=128, prune_thread_seeds=3, contrib_wg_hits=300, seed_key_bits=20),
 ]
+BUILT_IN = dict(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=0, seed_key_bits=0)
 
 
 def test_tunings_give_the_same_records(setup):
     segs, reads, mp = setup
-    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0)
+    mp.set_tuning(**BUILT_IN)
     want = mp.map_batch(reads)
     assert want["mapped"].sum() > 0
     counts = mp.last_counts()
@@ -58,7 +64,7 @@ def test_tunings_give_the_same_records(setup):
         if t["contrib_wg_hits"] == 1_000_000:
             # every read took a workgroup's table (none has 896 buckets)
             assert mp.last_counts()["global_table_reads"] == 0
-    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0, contrib_wg_hits=0)
+    mp.set_tuning(**BUILT_IN)
     assert counts["examined_hits"] > 0
     _, _, ores = P.oracle_map(segs, reads, K, H_FRAC, **PRM)
     bad = P.diff_results(ores, want)
@@ -67,10 +73,11 @@ def test_tunings_give_the_same_records(setup):
 
 def test_tuning_rejects_out_of_range_values(setup):
     _, _, mp = setup
-    for bad in (dict(contrib_chunk_hits=100), dict(contrib_chunk_hits=8192), dict(prune_thread_seeds=-1)):
+    for bad in (dict(contrib_chunk_hits=100), dict(contrib_chunk_hits=8192), dict(prune_thread_seeds=-1),
+                dict(seed_key_bits=56), dict(seed_key_bits=-1)):
         with pytest.raises(G.ShmError, match="tuning out of range"):
             mp.set_tuning(**bad)
-    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0)
+    mp.set_tuning(**BUILT_IN)
 
 
 def test_autotune_keeps_a_valid_tuning(setup):
@@ -82,10 +89,11 @@ def test_autotune_keeps_a_valid_tuning(setup):
     assert t["contrib_wave_hits"] in (0, 512, 1024, 2048, 4096, 8192)
     assert t["contrib_chunk_hits"] in (0, 512, 1024, 2048, 4096)
     assert t["contrib_wg_hits"] in (0, 1, 8192, 32768, 1 << 20)
+    assert t["seed_key_bits"] == 0  # not searched: the built-in keys are the fastest
     got = mp.map_batch(reads)
     bad = P.diff_results(want, got)
     assert not bad, bad
-    mp.set_tuning(contrib_wave_hits=0, contrib_chunk_hits=0, prune_thread_seeds=0)
+    mp.set_tuning(**BUILT_IN)
 
 
 def test_satellite_reads_take_every_counting_path():
