@@ -803,7 +803,9 @@ __device__ __forceinline__ void seed_sort_e(const MapArgs &A, uint64_t o, uint32
 //   the seed budget S, halflen and the consumed prefix of match_seeds
 //     (src/shmap/mod.rs:726-742, src/shmap/seeding.rs:101-106, 207-209).
 // Everything stays in the wave's LDS slice; seeds are written out once.
-__global__ void __launch_bounds__(WG) k_seed_wave(MapArgs A, DevIndex I) {
+// (five waves per SIMD: 96 VGPRs without spills since the one-word sort keys;
+// its waves mostly wait, so the fifth pays: 1.0 -> 0.97 ms on genome)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5))) k_seed_wave(MapArgs A, DevIndex I) {
     if (aborted(A.abort)) return;
     extern __shared__ uint8_t lds[];
     const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
