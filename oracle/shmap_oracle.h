@@ -24,10 +24,16 @@
  *   mapping     -- update/overlap/calc_mapq fixtures (src/mapping.rs:445-495)
  *   end to end  -- golden PAF (tests/fixtures/tiny.golden.paf), short-read
  *                  floor (tests/short_read_sketch_floor.rs)
- * Parity unpinned: the ratcheting two-pass match_rest on realistic reads and
- * get_sorted_buckets tie order on real data have no executable reference
- * vector (the Rust reference cannot be built here); they follow the source
- * text line by line.
+ *   match_rest  -- best / second-best buckets, scores, sigma, MAPQ,
+ *                  final_buckets, refined_buckets and refine_memo_hits
+ *                  against an independent brute-force mapper
+ *                  (tests/brute_map.py: every bucket scored in closed form,
+ *                  the two passes' threshold sequences replayed), four
+ *                  metrics x {default, -P, -B}; -P and the default agree on
+ *                  the mappings and sh >= J
+ * Parity unpinned (no executable reference vector; they follow the source
+ * text line by line): -F (abs_pos) bucket geometry; needletail parsing of
+ * FASTQ / compressed input; Rust float spelling on exact decimal ties.
  */
 #ifndef SHMAP_ORACLE_H
 #define SHMAP_ORACLE_H
