@@ -462,9 +462,14 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) k_
 
 // Moves the staged k-mers of every chunk without a lane overflow to their
 // final positions (the chunk's scanned offset plus the lane's exclusive
-// prefix); the listed overflow chunks are written by k_sketch<1>.  Output
-// element t of a chunk is found from the lane prefix (in LDS), so each wave
-// writes its chunk's k-mers contiguously and gathers from the staging slots.
+// prefix); the listed overflow chunks are written by k_sketch<1>.  A chunk of
+// at most COMPACT_TILE k-mers goes row by row: row i holds each lane's i-th
+// record side by side, so the wave reads the rows whole (lanes past their
+// count stay idle) and places every record at its output index in LDS, then
+// writes the chunk's k-mers out contiguously.  A larger chunk (sketches at
+// high ratios) gathers each output element from its slot instead, the lane
+// found by a search of the lane prefix.
+constexpr uint32_t COMPACT_TILE = 512;
 __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const uint32_t *lane_cnt,
                                                        const uint64_t *chunk_koff, uint32_t lcap,
                                                        const uint4 *stage, uint64_t *out_h, int32_t *out_r,
@@ -473,6 +478,9 @@ __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const 
     if (aborted(abort)) return;
     if (n_units_dev) n_chunks = (*n_units_dev + WAVE - 1) / WAVE;
     __shared__ uint32_t s_pre[NWAVES][WAVE];
+    __shared__ uint64_t s_h[NWAVES][COMPACT_TILE];
+    __shared__ int32_t s_r[NWAVES][COMPACT_TILE];
+    __shared__ uint8_t s_s[NWAVES][COMPACT_TILE];
     const int lane = threadIdx.x & (WAVE - 1), w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     uint32_t *pre_w = s_pre[w];
     for (uint64_t c = (uint64_t)blockIdx.x * NWAVES + w; c < n_chunks; c += (uint64_t)gridDim.x * NWAVES) {
@@ -480,23 +488,52 @@ __global__ void __launch_bounds__(WG) k_sketch_compact(uint64_t n_chunks, const 
         if (__ballot(cnt > lcap) != 0) continue;
         const uint32_t incl = wave_incl_scan(cnt);
         const uint32_t total = __shfl(incl, WAVE - 1);
-        pre_w[lane] = incl - cnt;
-        wave_sync();
+        const uint32_t pre = incl - cnt;
         const uint64_t ob = chunk_koff[c];
         const uint64_t sb = (uint64_t)c * lcap * WAVE;
-        for (uint32_t t = lane; t < total; t += WAVE) {
-            uint32_t lo = 0, hi = WAVE;  // the last lane whose prefix is <= t
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (pre_w[mid] <= t) lo = mid; else hi = mid;
+        if (total <= COMPACT_TILE) {
+            uint32_t rows = cnt;
+            for (int d = WAVE / 2; d > 0; d >>= 1) {
+                const uint32_t o = __shfl_xor(rows, d);
+                rows = o > rows ? o : rows;
             }
-            const uint64_t si = sb + (uint64_t)(t - pre_w[lo]) * WAVE + lo;
-            const uint4 rec = stage[si];
-            out_h[ob + t] = ((uint64_t)rec.y << 32) | rec.x;
-            out_r[ob + t] = (int32_t)rec.z;
-            out_s[ob + t] = (uint8_t)rec.w;
+            for (uint32_t i0 = 0; i0 < rows; i0 += 4) {
+                uint4 rec[4];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++)
+                    if (i0 + j < cnt) rec[j] = stage[sb + (uint64_t)(i0 + j) * WAVE + lane];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++)
+                    if (i0 + j < cnt) {
+                        const uint32_t p = pre + i0 + j;  // < pre + cnt <= total
+                        s_h[w][p] = ((uint64_t)rec[j].y << 32) | rec[j].x;
+                        s_r[w][p] = (int32_t)rec[j].z;
+                        s_s[w][p] = (uint8_t)rec[j].w;
+                    }
+            }
+            wave_sync();
+            for (uint32_t t = lane; t < total; t += WAVE) {
+                out_h[ob + t] = s_h[w][t];
+                out_r[ob + t] = s_r[w][t];
+                out_s[ob + t] = s_s[w][t];
+            }
+        } else {
+            pre_w[lane] = pre;
+            wave_sync();
+            for (uint32_t t = lane; t < total; t += WAVE) {
+                uint32_t lo = 0, hi = WAVE;  // the last lane whose prefix is <= t
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pre_w[mid] <= t) lo = mid; else hi = mid;
+                }
+                const uint64_t si = sb + (uint64_t)(t - pre_w[lo]) * WAVE + lo;
+                const uint4 rec = stage[si];
+                out_h[ob + t] = ((uint64_t)rec.y << 32) | rec.x;
+                out_r[ob + t] = (int32_t)rec.z;
+                out_s[ob + t] = (uint8_t)rec.w;
+            }
         }
-        wave_sync();  // the prefix slice is rewritten by the wave's next chunk
+        wave_sync();  // the wave's LDS slices are rewritten by its next chunk
     }
 }
 
