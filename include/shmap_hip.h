@@ -325,10 +325,11 @@ typedef struct {
 int shm_mapper_set_tuning(shm_mapper *mp, const shm_tuning *t);
 int shm_mapper_get_tuning(const shm_mapper *mp, shm_tuning *t);
 /* Autotuner: maps the sample batch repeatedly under candidate tunings (a
- * coordinate search over the four fields, each candidate timed twice by the
- * batch's GPU time), keeps the fastest on the mapper and writes it to *out
- * (optional).  The sample should look like the batches to come; a few
- * thousand reads suffice. */
+ * coordinate search over the four batching fields, workgroup bound first,
+ * each candidate timed twice by the batch's GPU time and taken only when 2%
+ * faster than the best so far; seed_key_bits is left as set), keeps the
+ * fastest on the mapper and writes it to *out (optional).  The sample should
+ * look like the batches to come; a few thousand reads suffice. */
 int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_t n_reads, shm_tuning *out);
 
 /* ---- Serving loop: continuous batching over one resident index ----------

@@ -1199,8 +1199,11 @@ extern "C" int shm_mapper_get_tuning(const shm_mapper *mp, shm_tuning *t) {
 extern "C" int shm_map_batch(shm_mapper *mp, const shm_read *reads, int32_t n, shm_read_result *out);
 
 // Coordinate search: each field in turn over its candidates, the others held
-// at the best so far; a candidate's cost is the better of two timed batches
-// (GPU time from the first kernel to the last record).
+// at the best so far, the workgroup bound first (it moves the most work, and
+// the one-wave bound and chunk length act on what it leaves); a candidate's
+// cost is the better of two timed batches (GPU time from the first kernel to
+// the last record), and it replaces the best only when 2% faster, so timing
+// noise does not move the choice.
 extern "C" int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_t n, shm_tuning *out) {
     if (!mp || n < 0 || (n > 0 && !reads)) return fail(SHM_ERR_ARG, "invalid argument");
     std::vector<shm_read_result> res((size_t)(n > 0 ? n : 1));
@@ -1223,14 +1226,14 @@ extern "C" int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_
     static const int32_t cand_chunk[] = {0, 512, 1024, 2048, 4096};
     static const int32_t cand_wg[] = {1, 8192, 32768, 1 << 20};  // 1: no read takes a workgroup
     for (int field = 0; field < 4; field++) {
-        const int32_t *cand = field == 0 ? cand_prune : field == 1 ? cand_wave : field == 2 ? cand_chunk : cand_wg;
-        const int nc = field == 0 ? 6 : field == 3 ? 4 : 5;
+        const int32_t *cand = field == 0 ? cand_wg : field == 1 ? cand_wave : field == 2 ? cand_chunk : cand_prune;
+        const int nc = field == 0 ? 4 : field == 3 ? 6 : 5;
         for (int i = 0; i < nc; i++) {
             shm_tuning t = best;
-            int32_t &f = field == 0   ? t.prune_thread_seeds
+            int32_t &f = field == 0   ? t.contrib_wg_hits
                          : field == 1 ? t.contrib_wave_hits
                          : field == 2 ? t.contrib_chunk_hits
-                                      : t.contrib_wg_hits;
+                                      : t.prune_thread_seeds;
             if (f == cand[i]) continue;
             f = cand[i];
             float ms;
@@ -1239,7 +1242,7 @@ extern "C" int shm_mapper_autotune(shm_mapper *mp, const shm_read *reads, int32_
                 mp->tune = best;
                 return rc;
             }
-            if (ms < best_ms) {
+            if (ms < 0.98f * best_ms) {
                 best_ms = ms;
                 best = t;
             }
