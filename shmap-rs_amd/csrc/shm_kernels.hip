@@ -2117,12 +2117,18 @@ __global__ void __launch_bounds__(WG) k_prune(MapArgs A, DevIndex I) {
                 const uint32_t t = atomicAdd(&A.task_count[0], 1u);
                 A.tasks[t] = make_uint2((uint32_t)lr, (uint32_t)p);
                 atomicMin(&A.rd_first[rd], p);
+            } else if (const int32_t m0 = A.bs_m[base + p];
+                       1.0 - (double)(seeds0 - m0) / (double)m < A.thr_min) {
+                // dead before its first seed (the loop below would stop at
+                // once and leave its sums as they are): only the verdict
+                A.bs_alive[base + p] = 0;
+                A.bs_sh[base + p] = 1.0 - (double)(seeds0 - m0) / (double)m;
             } else {
                 // this thread extends its bucket seed by seed, stopping as soon
                 // as it dies, for up to A.prune_seq seeds; a bucket still
                 // alive with seeds left goes to k_prune_ext with its partial
                 // sums, its next seed (sc_i) and its seeds so far (sc_same)
-                int32_t matches = A.bs_m[base + p], codir = A.bs_c[base + p];
+                int32_t matches = m0, codir = A.bs_c[base + p];
                 int32_t rmin = A.bs_min[base + p], rmax = A.bs_max[base + p];
                 int32_t seeds = seeds0;
                 uint32_t seg;
